@@ -1,0 +1,105 @@
+/*
+ * CPU restatement of SurfATT's forward + adjoint hot path ("the oracle").
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (libsatt.so) never links or calls it.
+ *
+ * Every function follows the reference's statement order and floating-point
+ * evaluation order (Fortran left-to-right, no FMA contraction; compiled with
+ * -ffp-contract=off), including the reference's single-precision quirks:
+ *   - pi is a default-real literal widened to double
+ *     (src/eikon_solver_2d.f90:785,1297);
+ *   - the bilinear weights r1, r2 are implicitly REAL(4)
+ *     (src/eikon_solver_2d.f90:828-841,1329-1335);
+ *   - the tolerances are REAL(4) constant expressions
+ *     (src/eikon_solver_2d.f90:775,1294).
+ *
+ * Array layout follows the reference: a field f(nx,ny) is column-major with
+ * x fastest, f[i + nx*j]; period-major model arrays svel(nperiod,nx,ny) are
+ * v[p + nperiod*(i + nx*j)].  All indices in this API are 0-based.
+ */
+#ifndef SATT_ORACLE_H
+#define SATT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (same values as include/satt.h) */
+#define ORC_OK 0
+#define ORC_W_MAXITER 1
+#define ORC_E_ARG -1
+#define ORC_E_SRC_ON_BOUNDARY -2
+#define ORC_E_REC_OUT_OF_GRID -3
+#define ORC_E_NOMEM -4
+
+/* src/eikon_solver_2d.f90:764-1126  FSM_UW_PS_lonlat_2d */
+int orc_fwd_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                   const double *spha, const double *sphb, const double *sphc,
+                   const double *fun, double x0_deg, double y0_deg, double *T,
+                   int *iters);
+
+/* Same solver with T0/T0x/T0y supplied by the caller instead of computed
+ * (src/eikon_solver_2d.f90:845-857 replaced by the inputs).  Used to compare
+ * a device solve bit-for-bit when the device's libm differs from the host's.
+ * tau_init/ischange are still derived exactly as :859-872. */
+int orc_fwd_lonlat_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                      const double *spha, const double *sphb, const double *sphc,
+                      const double *fun, double x0_deg, double y0_deg,
+                      const double *T0, const double *T0x, const double *T0y,
+                      double *T, int *iters);
+
+/* The analytic factor alone (src/eikon_solver_2d.f90:801-857). */
+int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
+               const double *fun, double x0_deg, double y0_deg, double *T0,
+               double *T0x, double *T0y);
+
+/* src/eikon_solver_2d.f90:1276-1446  FSM_O1_JSE_lonlat_2d */
+int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                   const double *spha, const double *sphb, const double *sphc,
+                   const double *T, const double *xrec_deg,
+                   const double *yrec_deg, const double *src_adj, int nr,
+                   double *Ta, int *iters);
+
+/* src/eikon_solver_2d.f90:1449-1467  mask */
+void orc_mask(const double *xx, const double *yy, int nx, int ny,
+              double *table, double x0, double y0);
+
+/* src/shared/utils.f90:917-956  interp2_1_dp */
+int orc_interp2(const double *x, const double *y, int nx, int ny,
+                const double *v, const double *xq, const double *yq, int nq,
+                double *vq);
+
+/* The per-(source, period) loop: src/tomo2d.f90:158-201 (dim==2) and
+ * src/acqui.f90:91-161 (dim==3), with att_measadj (src/measadj.f90:62-114)
+ * and post_proc_eikokernel (src/tomo2d.f90:145-156, src/acqui.f90:307-316).
+ *
+ * items i = 0..nitems-1 in isrcs order; item_pidx[i] is the 0-based period
+ * index; receivers of item i are rec_off[i]..rec_off[i+1]-1.
+ * Outputs (any may be NULL when the matching flag is off):
+ *   tt_fwd[npath]            overwritten (zeros where no receiver maps)
+ *   adj_s[nperiod*nx*ny]      period-major, overwritten
+ *   adj_density[...]          dim==3 only
+ *   item_chi[nitems], iters[3*nitems] (fwd, adj, density-adj)
+ * nthreads > 1 splits the item list into the reference's static contiguous
+ * blocks (src/shared/parallel.f90:1877-1903) and sums the per-block kernels
+ * in block order, as MPI_Reduce over ranks would.
+ */
+int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
+                         const double *ygrids, int nperiod, const double *svel,
+                         const double *m11, const double *m12,
+                         const double *m22, int nitems, const int *item_pidx,
+                         const double *item_srcx, const double *item_srcy,
+                         const int *rec_off, const double *recx,
+                         const double *recy, const double *tt,
+                         const double *weight, const int *idx_path, int npath,
+                         int istotable, int isadj, int nthreads,
+                         double *tt_fwd, double *chi, double *adj_s,
+                         double *adj_density, double *item_chi, int *iters,
+                         double *weight_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
