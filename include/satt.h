@@ -1,0 +1,193 @@
+/*
+ * satt.h -- C-ABI of the MI355X-native SurfATT forward + adjoint hot path.
+ *
+ * libsatt.so replaces, behind a plain C ABI (ISO_C_BINDING-callable: plain
+ * pointers, C ints and doubles, no C++ types), the reference's per-solve
+ * external subroutines and the body of its per-(source, period) loop:
+ *
+ *   satt_fwd_lonlat        FSM_UW_PS_lonlat_2d   src/eikon_solver_2d.f90:764
+ *   satt_adj_lonlat        FSM_O1_JSE_lonlat_2d  src/eikon_solver_2d.f90:1276
+ *   satt_mask              mask                  src/eikon_solver_2d.f90:1449
+ *   satt_interp2           interp2_1_dp          src/shared/utils.f90:944
+ *   satt_forward_simulate  loop body of att_tomo_2d%forward_simulate
+ *                          (src/tomo2d.f90:168-195) and att_acqui%forward_simulate
+ *                          (src/acqui.f90:121-151), with att_measadj
+ *                          (src/measadj.f90:40-114) and post_proc_eikokernel
+ *                          (src/tomo2d.f90:145-156, src/acqui.f90:307-316)
+ *   satt_plan_*            the same loop with the model, gathers and fields
+ *                          resident in device memory across calls
+ *
+ * Conventions
+ *   - A grid field f(nx,ny) is column-major with x fastest: f[i + nx*j], as
+ *     the reference's Fortran arrays.  Period-major model arrays
+ *     svel/m11/m12/m22(nperiod,nx,ny) are v[p + nperiod*(i + nx*j)], the
+ *     reference's att_grid layout (src/grid.f90:25-34).
+ *   - Coordinates are in degrees, like the reference's call sites
+ *     (src/measadj.f90:68-71,85-89).  Indices passed in are 0-based.
+ *   - All host arrays are caller-owned; outputs are fully overwritten.
+ *   - Every entry point returns a status code.  Fatal codes are negative.
+ *     SATT_W_MAXITER is a warning: the fields are returned, as the reference
+ *     returns them silently (src/eikon_solver_2d.f90:1117-1119,1436-1438).
+ *   - A context owns one HIP stream on one device and is not thread-safe;
+ *     use one context per host thread.
+ */
+#ifndef SATT_H
+#define SATT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define SATT_OK 0
+#define SATT_W_MAXITER 1          /* MaxIter reached without convergence     */
+#define SATT_E_ARG (-1)           /* invalid sizes / pointers                */
+#define SATT_E_SRC_ON_BOUNDARY (-2) /* src/eikon_solver_2d.f90:863-867 'stop' */
+#define SATT_E_REC_OUT_OF_GRID (-3) /* receiver bracket leaves the grid; the
+                                     reference reads out of bounds
+                                     (:1329-1335, src/shared/utils.f90:927) */
+#define SATT_E_NOMEM (-4)         /* device or host allocation failed        */
+#define SATT_E_HIP (-5)           /* HIP runtime error                       */
+#define SATT_E_NODEV (-6)         /* no usable GPU                           */
+
+/* precision modes for the batched path */
+#define SATT_PREC_FP64 0  /* the reference's FP64 arithmetic, sweep for sweep */
+#define SATT_PREC_MIXED 1 /* FP32 sweeps, then FP64 sweeps to the reference's
+                             stopping rule; T0, residuals, interpolation and
+                             kernel accumulation in FP64 */
+
+/* launch forms for satt_plan_run */
+#define SATT_LAUNCH_BATCHED 0   /* all items of a chunk in one launch per stage */
+#define SATT_LAUNCH_PER_SOLVE 1 /* one solve per launch (the plain form)     */
+
+typedef struct satt_ctx satt_ctx;
+typedef struct satt_plan satt_plan;
+
+/* Create a context on HIP device `device` (its own stream). */
+int satt_create(satt_ctx **ctx, int device);
+void satt_destroy(satt_ctx *ctx);
+/* Message of the last failing call on this context (static storage). */
+const char *satt_last_error(const satt_ctx *ctx);
+/* Library version string. */
+const char *satt_version(void);
+
+/* ---------------------------------------------------- per-solve (plain) form
+ * One solve per launch; host arrays in and out.  `iters` (may be NULL)
+ * receives the iteration count at exit. */
+
+/* FSM_UW_PS_lonlat_2d(xx_deg,yy_deg,nx,ny,spha,sphb,sphc,T,fun,x0_deg,y0_deg,u)
+ * src/eikon_solver_2d.f90:764.  `u` only feeds an unused error norm in the
+ * reference (:1099-1104) and is not taken. */
+int satt_fwd_lonlat(satt_ctx *ctx, const double *xx_deg, const double *yy_deg,
+                    int nx, int ny, const double *spha, const double *sphb,
+                    const double *sphc, const double *fun, double x0_deg,
+                    double y0_deg, double *T, int *iters);
+
+/* FSM_O1_JSE_lonlat_2d(xx_deg,yy_deg,nx,ny,spha,sphb,sphc,T,Ta,xrec_deg,
+ * yrec_deg,sourceAdj,nr)  src/eikon_solver_2d.f90:1276. */
+int satt_adj_lonlat(satt_ctx *ctx, const double *xx_deg, const double *yy_deg,
+                    int nx, int ny, const double *spha, const double *sphb,
+                    const double *sphc, const double *T, const double *xrec_deg,
+                    const double *yrec_deg, const double *src_adj, int nr,
+                    double *Ta, int *iters);
+
+/* mask(xx,yy,nx,ny,TableAdj,x0,y0)  src/eikon_solver_2d.f90:1449 (in place). */
+int satt_mask(satt_ctx *ctx, const double *xx, const double *yy, int nx, int ny,
+              double *table, double x0, double y0);
+
+/* vq = interp2(x, y, v, xq, yq)  src/shared/utils.f90:944 (interp2_1_dp). */
+int satt_interp2(satt_ctx *ctx, const double *x, const double *y, int nx,
+                 int ny, const double *v, const double *xq, const double *yq,
+                 int nq, double *vq);
+
+/* ------------------------------------------------------------ batched form */
+
+/* One forward_simulate call's inputs, in the reference's shapes.
+ *   dim 2: att_tomo_2d path (src/tomo2d.f90:158): kernel adj/svel**2.
+ *   dim 3: att_acqui path (src/acqui.f90:91): adds the density adjoint
+ *          (RHS -weight, src/measadj.f90:92-104); kernels adj/svel**3 and
+ *          kden/svel**3 (src/acqui.f90:314-315).
+ * Items are the reference's isrcs list in order (src/src_rec.f90:338-369);
+ * item_pidx is 0-based (isrcs(:,1)-1); the receiver gather of item i
+ * (get_evt_gather, src/src_rec.f90:252-283) is rec_off[i]..rec_off[i+1]-1 of
+ * recx/recy/tt/weight/idx_path (idx_path 0-based). */
+typedef struct {
+  int dim;
+  int nx, ny, nperiod;
+  const double *xgrids, *ygrids;               /* am%xgrids, am%ygrids (deg) */
+  const double *svel, *m11, *m12, *m22;        /* (nperiod,nx,ny) */
+  int nitems;
+  const int *item_pidx;
+  const double *item_srcx, *item_srcy;         /* source lon/lat (deg) */
+  const int *rec_off;                          /* nitems+1 */
+  const double *recx, *recy, *tt, *weight;     /* rec_off[nitems] each */
+  const int *idx_path;
+  int npath;
+} satt_batch;
+
+/* Outputs; any pointer may be NULL to skip it. */
+typedef struct {
+  double *tt_fwd;      /* npath: sum_all(local_tt) (src/tomo2d.f90:200)     */
+  double *chi;         /* 1: sum of 0.5*sum(w*(t-tobs)**2) over items       */
+  double *adj_s;       /* (nperiod,nx,ny) stacked kernel (isadj)            */
+  double *adj_density; /* (nperiod,nx,ny) dim 3 only (isadj)                */
+  double *item_chi;    /* nitems                                            */
+  int *iters;          /* 3*nitems: forward, adjoint, density-adjoint       */
+  int *item_status;    /* nitems: per-item status                           */
+  double *weight_out;  /* rec_off[nitems]: weights after outlier rejection  */
+} satt_result;
+
+/* One call of forward_simulate(chi, istotable, isadj): upload, run, download.
+ * Equivalent to satt_plan_create + satt_plan_run + satt_plan_results. */
+int satt_forward_simulate(satt_ctx *ctx, const satt_batch *batch,
+                          int istotable, int isadj, int precision,
+                          satt_result *result);
+
+/* Resident form: the plan keeps the grid, the model, the gathers and every
+ * field in device memory.  satt_plan_run is asynchronous on the context's
+ * stream; satt_plan_results waits for it and downloads. */
+int satt_plan_create(satt_ctx *ctx, const satt_batch *batch, int precision,
+                     satt_plan **plan);
+/* Replace the model (same shapes) for the next run, e.g. after a model
+ * update or inside a line search (src/tomo2d.f90:272). */
+int satt_plan_set_model(satt_plan *plan, const double *svel, const double *m11,
+                        const double *m12, const double *m22);
+int satt_plan_run(satt_plan *plan, int istotable, int isadj, int launch_form);
+int satt_plan_results(satt_plan *plan, satt_result *result);
+void satt_plan_destroy(satt_plan *plan);
+
+/* Sizing and timing hooks of a plan (for measurement harnesses). */
+typedef struct {
+  int chunk;                /* items per launch                              */
+  long long device_bytes;   /* device memory held by the plan                */
+  double ms_forward;        /* device time of the last run's forward kernels */
+  double ms_adjoint;        /* ... adjoint kernels                           */
+  double ms_total;          /* ... whole run                                 */
+  long long fwd_sweeps;     /* sum over items of 4*forward iterations        */
+  long long adj_sweeps;     /* sum over items of 4*adjoint iterations (both) */
+} satt_plan_stats;
+int satt_plan_get_stats(satt_plan *plan, satt_plan_stats *stats);
+
+/* ------------------------------------------------------------ inspection
+ * The per-solve setup fields as the device computes them, for bit-level
+ * comparison against a CPU restatement that takes them as inputs (the
+ * device's libm differs from the host's in the last bits).
+ *   forward: a, b, c (metric, :817-821), T0, T0x, T0y (:845-857)
+ *   adjoint: a, b, c, delta (:1320-1344) */
+int satt_fwd_setup_fields(satt_ctx *ctx, const double *xx_deg,
+                          const double *yy_deg, int nx, int ny,
+                          const double *spha, const double *sphb,
+                          const double *sphc, const double *fun, double x0_deg,
+                          double y0_deg, double *a, double *b, double *c,
+                          double *T0, double *T0x, double *T0y);
+int satt_adj_setup_fields(satt_ctx *ctx, const double *xx_deg,
+                          const double *yy_deg, int nx, int ny,
+                          const double *spha, const double *sphb,
+                          const double *sphc, const double *xrec_deg,
+                          const double *yrec_deg, const double *src_adj, int nr,
+                          double *a, double *b, double *c, double *delta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SATT_H */

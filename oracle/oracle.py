@@ -46,8 +46,9 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         _lib = ctypes.CDLL(LIB_PATH)
-        for name in ("orc_fwd_lonlat", "orc_fwd_lonlat_t0", "orc_fwd_t0",
-                     "orc_adj_lonlat", "orc_interp2", "orc_forward_simulate"):
+        for name in ("orc_fwd_lonlat", "orc_fwd_lonlat_pre", "orc_fwd_t0",
+                     "orc_adj_lonlat", "orc_adj_lonlat_pre", "orc_interp2",
+                     "orc_forward_simulate"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.orc_mask.restype = None
     return _lib
@@ -68,10 +69,10 @@ def _f(a, shape=None):
     return a
 
 
-def fwd(xx, yy, spha, sphb, sphc, fun, x0, y0, t0=None):
+def fwd(xx, yy, spha, sphb, sphc, fun, x0, y0, pre=None):
     """FSM_UW_PS_lonlat_2d (src/eikon_solver_2d.f90:764).  Returns (T, iters,
-    status).  With ``t0=(T0, T0x, T0y)`` the analytic factor is taken from the
-    caller."""
+    status).  With ``pre=(a, b, c, T0, T0x, T0y)`` the converted metric and
+    the analytic factor are taken from the caller (spha/sphb/sphc ignored)."""
     xx = np.ascontiguousarray(xx, dtype=np.float64)
     yy = np.ascontiguousarray(yy, dtype=np.float64)
     nx, ny = len(xx), len(yy)
@@ -79,15 +80,15 @@ def fwd(xx, yy, spha, sphb, sphc, fun, x0, y0, t0=None):
     spha, sphb, sphc, fun = (_f(v, sh) for v in (spha, sphb, sphc, fun))
     T = np.zeros(sh, order="F")
     it = ctypes.c_int(0)
-    if t0 is None:
+    if pre is None:
         st = lib().orc_fwd_lonlat(_d(xx), _d(yy), nx, ny, _d(spha), _d(sphb), _d(sphc),
                                   _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
                                   _d(T), ctypes.byref(it))
     else:
-        T0, T0x, T0y = (_f(v, sh) for v in t0)
-        st = lib().orc_fwd_lonlat_t0(_d(xx), _d(yy), nx, ny, _d(spha), _d(sphb), _d(sphc),
-                                     _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
-                                     _d(T0), _d(T0x), _d(T0y), _d(T), ctypes.byref(it))
+        a, b, c, T0, T0x, T0y = (_f(v, sh) for v in pre)
+        st = lib().orc_fwd_lonlat_pre(_d(xx), _d(yy), nx, ny, _d(a), _d(b), _d(c),
+                                      _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
+                                      _d(T0), _d(T0x), _d(T0y), _d(T), ctypes.byref(it))
     if st < 0:
         raise OracleError(st, "orc_fwd_lonlat")
     return T, it.value, st
@@ -104,6 +105,23 @@ def t0_fields(xx, yy, fun, x0, y0):
     if st < 0:
         raise OracleError(st, "orc_fwd_t0")
     return tuple(out)
+
+
+def adj_pre(xx, yy, a, b, c, T, delta):
+    """FSM_O1_JSE_lonlat_2d with the converted metric and the delta source
+    supplied by the caller.  Returns (Ta, iters, status)."""
+    xx = np.ascontiguousarray(xx, dtype=np.float64)
+    yy = np.ascontiguousarray(yy, dtype=np.float64)
+    nx, ny = len(xx), len(yy)
+    sh = (nx, ny)
+    a, b, c, T, delta = (_f(v, sh) for v in (a, b, c, T, delta))
+    Ta = np.zeros(sh, order="F")
+    it = ctypes.c_int(0)
+    st = lib().orc_adj_lonlat_pre(_d(xx), _d(yy), nx, ny, _d(a), _d(b), _d(c), _d(T),
+                                  _d(delta), _d(Ta), ctypes.byref(it))
+    if st < 0:
+        raise OracleError(st, "orc_adj_lonlat_pre")
+    return Ta, it.value, st
 
 
 def adj(xx, yy, spha, sphb, sphc, T, xrec, yrec, src_adj):

@@ -269,8 +269,9 @@ static int fwd_iterate(int nx, int ny, double dx, double dy, const double *a,
 static int fwd_common(const double *xx_deg, const double *yy_deg, int nx,
                       int ny, const double *spha, const double *sphb,
                       const double *sphc, const double *fun, double x0_deg,
-                      double y0_deg, const double *T0_in, const double *T0x_in,
-                      const double *T0y_in, double *T, int *iters) {
+                      double y0_deg, const double *abc_in, const double *T0_in,
+                      const double *T0x_in, const double *T0y_in, double *T,
+                      int *iters) {
   if (nx < 2 || ny < 2) return ORC_E_ARG;
   const size_t n = (size_t)nx * ny;
   double *xx = malloc(sizeof(double) * nx), *yy = malloc(sizeof(double) * ny);
@@ -287,7 +288,13 @@ static int fwd_common(const double *xx_deg, const double *yy_deg, int nx,
   for (int j = 0; j < ny; ++j) yy[j] = deg2rad_ref(yy_deg[j]);
   const double x0 = deg2rad_ref(x0_deg), y0 = deg2rad_ref(y0_deg);
   const double dx = xx[1] - xx[0], dy = yy[1] - yy[0];
-  metric_abc(nx, ny, yy, spha, sphb, sphc, a, b, c);
+  if (abc_in) {
+    memcpy(a, abc_in, n * sizeof(double));
+    memcpy(b, abc_in + n, n * sizeof(double));
+    memcpy(c, abc_in + 2 * n, n * sizeof(double));
+  } else {
+    metric_abc(nx, ny, yy, spha, sphb, sphc, a, b, c);
+  }
   int st = ORC_OK;
   if (T0_in) {
     memcpy(T0, T0_in, n * sizeof(double));
@@ -312,16 +319,24 @@ int orc_fwd_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
                    const double *fun, double x0_deg, double y0_deg, double *T,
                    int *iters) {
   return fwd_common(xx_deg, yy_deg, nx, ny, spha, sphb, sphc, fun, x0_deg,
-                    y0_deg, NULL, NULL, NULL, T, iters);
+                    y0_deg, NULL, NULL, NULL, NULL, T, iters);
 }
 
-int orc_fwd_lonlat_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
-                      const double *spha, const double *sphb, const double *sphc,
-                      const double *fun, double x0_deg, double y0_deg,
-                      const double *T0, const double *T0x, const double *T0y,
-                      double *T, int *iters) {
-  return fwd_common(xx_deg, yy_deg, nx, ny, spha, sphb, sphc, fun, x0_deg,
-                    y0_deg, T0, T0x, T0y, T, iters);
+int orc_fwd_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
+                       int ny, const double *a, const double *b,
+                       const double *c, const double *fun, double x0_deg,
+                       double y0_deg, const double *T0, const double *T0x,
+                       const double *T0y, double *T, int *iters) {
+  const size_t n = (size_t)nx * ny;
+  double *abc = malloc(sizeof(double) * 3 * n);
+  if (!abc) return ORC_E_NOMEM;
+  memcpy(abc, a, n * sizeof(double));
+  memcpy(abc + n, b, n * sizeof(double));
+  memcpy(abc + 2 * n, c, n * sizeof(double));
+  const int st = fwd_common(xx_deg, yy_deg, nx, ny, NULL, NULL, NULL, fun,
+                            x0_deg, y0_deg, abc, T0, T0x, T0y, T, iters);
+  free(abc);
+  return st;
 }
 
 int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
@@ -343,11 +358,12 @@ int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
 
 /* ------------------------------------------------------------------ adjoint */
 
-int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
-                   const double *spha, const double *sphb, const double *sphc,
-                   const double *T, const double *xrec_deg,
-                   const double *yrec_deg, const double *src_adj, int nr,
-                   double *Ta, int *iters) {
+static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
+                      int ny, const double *spha, const double *sphb,
+                      const double *sphc, const double *T,
+                      const double *xrec_deg, const double *yrec_deg,
+                      const double *src_adj, int nr, const double *pre,
+                      double *Ta, int *iters) {
   if (nx < 3 || ny < 3 || nr < 0) return ORC_E_ARG;
   const size_t n = (size_t)nx * ny;
   double *xx = malloc(sizeof(double) * nx), *yy = malloc(sizeof(double) * ny);
@@ -372,6 +388,10 @@ int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
 
   /* delta source (:1320-1336) */
   memset(delta, 0, n * sizeof(double));
+  if (pre) {
+    memcpy(delta, pre + 3 * n, n * sizeof(double));
+    nr = 0;
+  }
   for (int ir = 0; ir < nr; ++ir) {
     const double xr = deg2rad_ref(xrec_deg[ir]), yr = deg2rad_ref(yrec_deg[ir]);
     const int idx0 = (int)floor((xr - xx[0]) / dx + 1);
@@ -395,7 +415,13 @@ int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
     return st;
   }
 
-  metric_abc(nx, ny, yy, spha, sphb, sphc, a, b, c);
+  if (pre) {
+    memcpy(a, pre, n * sizeof(double));
+    memcpy(b, pre + n, n * sizeof(double));
+    memcpy(c, pre + 2 * n, n * sizeof(double));
+  } else {
+    metric_abc(nx, ny, yy, spha, sphb, sphc, a, b, c);
+  }
 
   /* face coefficients (:1348-1370) */
   for (int i = 1; i < nx - 1; ++i) {
@@ -472,6 +498,32 @@ int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
   if (iters) *iters = iter;
   free(xx); free(yy); free(buf); free(bb);
   return converged ? ORC_OK : ORC_W_MAXITER;
+}
+
+int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                   const double *spha, const double *sphb, const double *sphc,
+                   const double *T, const double *xrec_deg,
+                   const double *yrec_deg, const double *src_adj, int nr,
+                   double *Ta, int *iters) {
+  return adj_common(xx_deg, yy_deg, nx, ny, spha, sphb, sphc, T, xrec_deg,
+                    yrec_deg, src_adj, nr, NULL, Ta, iters);
+}
+
+int orc_adj_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
+                       int ny, const double *a, const double *b,
+                       const double *c, const double *T, const double *delta,
+                       double *Ta, int *iters) {
+  const size_t n = (size_t)nx * ny;
+  double *pre = malloc(sizeof(double) * 4 * n);
+  if (!pre) return ORC_E_NOMEM;
+  memcpy(pre, a, n * sizeof(double));
+  memcpy(pre + n, b, n * sizeof(double));
+  memcpy(pre + 2 * n, c, n * sizeof(double));
+  memcpy(pre + 3 * n, delta, n * sizeof(double));
+  const int st = adj_common(xx_deg, yy_deg, nx, ny, NULL, NULL, NULL, T, NULL,
+                            NULL, NULL, 0, pre, Ta, iters);
+  free(pre);
+  return st;
 }
 
 /* --------------------------------------------------------------- mask */

@@ -40,15 +40,16 @@ int orc_fwd_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
                    const double *fun, double x0_deg, double y0_deg, double *T,
                    int *iters);
 
-/* Same solver with T0/T0x/T0y supplied by the caller instead of computed
- * (src/eikon_solver_2d.f90:845-857 replaced by the inputs).  Used to compare
- * a device solve bit-for-bit when the device's libm differs from the host's.
- * tau_init/ischange are still derived exactly as :859-872. */
-int orc_fwd_lonlat_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
-                      const double *spha, const double *sphb, const double *sphc,
-                      const double *fun, double x0_deg, double y0_deg,
-                      const double *T0, const double *T0x, const double *T0y,
-                      double *T, int *iters);
+/* Same solver with the libm-dependent setup supplied by the caller: the
+ * converted metric a, b, c (src/eikon_solver_2d.f90:817-821) and T0, T0x,
+ * T0y (:845-857).  Used to compare a device solve bit-for-bit when the
+ * device's libm differs from the host's in the last bits.  tau init and the
+ * frozen set are still derived as :859-872. */
+int orc_fwd_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
+                       int ny, const double *a, const double *b,
+                       const double *c, const double *fun, double x0_deg,
+                       double y0_deg, const double *T0, const double *T0x,
+                       const double *T0y, double *T, int *iters);
 
 /* The analytic factor alone (src/eikon_solver_2d.f90:801-857). */
 int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
@@ -61,6 +62,13 @@ int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
                    const double *T, const double *xrec_deg,
                    const double *yrec_deg, const double *src_adj, int nr,
                    double *Ta, int *iters);
+
+/* Adjoint with a, b, c (:1338-1344) and the delta source (:1320-1336)
+ * supplied by the caller (same purpose as orc_fwd_lonlat_pre). */
+int orc_adj_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
+                       int ny, const double *a, const double *b,
+                       const double *c, const double *T, const double *delta,
+                       double *Ta, int *iters);
 
 /* src/eikon_solver_2d.f90:1449-1467  mask */
 void orc_mask(const double *xx, const double *yy, int nx, int ny,

@@ -1,0 +1,108 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace satt {
+
+// Constants of the reference solvers, with their REAL(4) rounding kept:
+// src/eikon_solver_2d.f90:774-776,785 (forward) and :1293-1297 (adjoint).
+constexpr double kPiF = (double)3.14159265358979323846264338327950288f;
+constexpr double kREarth = 6371.0;
+constexpr double kFwdTol = (double)1.0e-4f;
+constexpr double kAdjTol = (double)1.0e-9f;
+constexpr double kAdjEps = (double)1.0e-14f;
+constexpr int kFwdMaxIter = 2000;
+constexpr int kAdjMaxIter = 1000;
+
+// status codes (include/satt.h)
+constexpr int kOk = 0, kWMaxIter = 1, kEArg = -1, kESrcOnBoundary = -2,
+              kERecOutOfGrid = -3, kENoMem = -4, kEHip = -5;
+
+// The lon/lat grid shared by every item of a batch.  Radian axes are made on
+// the host with the reference's expression xx_deg/180.0*pi.
+struct Grid {
+  int nx, ny;
+  double dx, dy;          // radians, xx(2)-xx(1), yy(2)-yy(1)
+  const double *xx, *yy;  // radians
+  const double *xd, *yd;  // degrees (interp2 and mask work in degrees)
+};
+
+// Per-period fields in natural layout, period p at offset p*nx*ny.
+struct PeriodFields {
+  const double *a, *b, *c;  // converted metric (src/eikon_solver_2d.f90:817-821)
+  const double *fun;        // 1/svel (src/measadj.f90:67)
+  const double *vel;        // svel (kernel stacking)
+};
+
+// One chunk of items: item k of the chunk is global item item0+k and uses
+// scratch slot k.
+struct Items {
+  int item0, count;
+  const int *pidx;                  // per global item
+  const double *srcx, *srcy;        // degrees, per global item
+  const int *rec_off;               // per global item (+1)
+  const double *recx, *recy;        // degrees, per receiver (global index)
+  const double *tt, *weight;        // per receiver
+  const int *idx_path;              // per receiver
+};
+
+struct FwdWork {
+  double *T;                        // out [slot][N]
+  double *tau, *tau_old, *T0, *T0x, *T0y;  // scratch [slot][N]
+};
+
+struct AdjWork {
+  const double *T;                  // forward travel times [slot][N]
+  double *Ta;                       // out [slot][N]
+  double *coef;                     // scratch [slot][5][N]: a1,a2,b1,b2,d
+  double *delta, *old;              // scratch [slot][N]
+  const double *rhs;                // per receiver (global index)
+};
+
+struct RecWork {
+  const double *T;                  // [slot][N]
+  double *tekio, *weight_out, *rhs_adj, *rhs_den;  // per receiver
+  double *item_chi;                 // per global item
+  double *tt_fwd;                   // per path (scatter), may be null
+};
+
+// ---- launches (satt_kernels.hip); all asynchronous on `s`
+void launch_period_fields(hipStream_t s, const Grid &g, int nperiod,
+                          const double *svel_pm, const double *m11_pm,
+                          const double *m12_pm, const double *m22_pm,
+                          double *a, double *b, double *c, double *fun,
+                          double *vel);
+void launch_metric_fields(hipStream_t s, const Grid &g, const double *spha,
+                          const double *sphb, const double *sphc, double *a,
+                          double *b, double *c);
+void launch_forward(hipStream_t s, const Grid &g, const PeriodFields &pf,
+                    const Items &it, const FwdWork &w, int *iters, int *status,
+                    int iter_stride);
+void launch_receivers(hipStream_t s, const Grid &g, const Items &it,
+                      const RecWork &w, int *status);
+void launch_adjoint(hipStream_t s, const Grid &g, const PeriodFields &pf,
+                    const Items &it, const AdjWork &w, int *iters, int *status,
+                    int iter_stride, int apply_mask);
+void launch_stack(hipStream_t s, const Grid &g, const PeriodFields &pf,
+                  int nperiod, const int *per_off, const int *per_slot,
+                  const double *fields, double *acc, int power);
+void launch_stack_item(hipStream_t s, const Grid &g, const PeriodFields &pf,
+                       const int *pidx, int item, const double *field,
+                       double *acc, int power);
+void launch_to_period_major(hipStream_t s, int N, int nperiod,
+                            const double *nat, double *pm);
+void launch_mask(hipStream_t s, const Grid &g, double *table, double x0_deg,
+                 double y0_deg);
+void launch_interp2(hipStream_t s, const Grid &g, const double *v,
+                    const double *xq, const double *yq, int nq, double *vq,
+                    int *status);
+void launch_fwd_setup_fields(hipStream_t s, const Grid &g, const double *a,
+                             const double *fun, double x0_deg, double y0_deg,
+                             double *T0, double *T0x, double *T0y, int *status);
+void launch_adj_delta(hipStream_t s, const Grid &g, const double *xr,
+                      const double *yr, const double *rhs, int nr,
+                      double *delta, int *status);
+
+}  // namespace satt

@@ -1,0 +1,291 @@
+"""Python binding of libsatt.so (the C-ABI in include/satt.h) via ctypes.
+
+The product is the HIP library; this module only marshals numpy arrays into
+its entry points, for tests and measurement.  It never falls back to a CPU
+path: when libsatt.so is missing or no GPU is present, every call raises.
+
+Field convention: numpy arrays of shape (nx, ny) in Fortran order, so that
+``f[i, j]`` is the reference's ``f(i+1, j+1)``; model arrays have shape
+(nperiod, nx, ny) in Fortran order (period fastest), the reference's
+att_grid layout (src/grid.f90:25-34).
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsatt.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "satt.h")
+
+OK, W_MAXITER = 0, 1
+E_ARG, E_SRC_ON_BOUNDARY, E_REC_OUT_OF_GRID, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
+PREC_FP64, PREC_MIXED = 0, 1
+LAUNCH_BATCHED, LAUNCH_PER_SOLVE = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class SattError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("nperiod", ctypes.c_int), ("xgrids", _dp), ("ygrids", _dp),
+        ("svel", _dp), ("m11", _dp), ("m12", _dp), ("m22", _dp),
+        ("nitems", ctypes.c_int), ("item_pidx", _ip), ("item_srcx", _dp),
+        ("item_srcy", _dp), ("rec_off", _ip), ("recx", _dp), ("recy", _dp),
+        ("tt", _dp), ("weight", _dp), ("idx_path", _ip), ("npath", ctypes.c_int),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("tt_fwd", _dp), ("chi", _dp), ("adj_s", _dp), ("adj_density", _dp),
+        ("item_chi", _dp), ("iters", _ip), ("item_status", _ip), ("weight_out", _dp),
+    ]
+
+
+class PlanStats(ctypes.Structure):
+    _fields_ = [
+        ("chunk", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
+        ("ms_forward", ctypes.c_double), ("ms_adjoint", ctypes.c_double),
+        ("ms_total", ctypes.c_double), ("fwd_sweeps", ctypes.c_longlong),
+        ("adj_sweeps", ctypes.c_longlong),
+    ]
+
+
+def header_symbols():
+    """Every function name declared in include/satt.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(satt_\w+)\s*\(", text, re.M)))
+
+
+def load(path=LIB_PATH):
+    """Load libsatt.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SattError(E_ARG, f"{path} not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    for name in header_symbols():
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+    L.satt_destroy.restype = None
+    L.satt_plan_destroy.restype = None
+    L.satt_last_error.restype = ctypes.c_char_p
+    L.satt_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f(a, shape=None):
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Context:
+    """A satt_ctx: one HIP stream on one device."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        p = ctypes.c_void_p()
+        st = self.lib.satt_create(ctypes.byref(p), int(device))
+        if st != OK:
+            raise SattError(st, "satt_create failed (no usable GPU?)")
+        self.ptr = p
+
+    def close(self):
+        if self.ptr:
+            self.lib.satt_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        return self.lib.satt_last_error(self.ptr).decode()
+
+    def _check(self, st, allow_warn=True):
+        if st < 0 or (st > 0 and not allow_warn):
+            raise SattError(st, self.error())
+        return st
+
+    # ---- per-solve entry points
+    def fwd(self, xx, yy, spha, sphb, sphc, fun, x0, y0):
+        """satt_fwd_lonlat -> (T, iters, status)."""
+        xx, yy = _c(xx), _c(yy)
+        sh = (len(xx), len(yy))
+        spha, sphb, sphc, fun = (_f(v, sh) for v in (spha, sphb, sphc, fun))
+        T = np.zeros(sh, order="F")
+        it = ctypes.c_int(0)
+        st = self.lib.satt_fwd_lonlat(self.ptr, _d(xx), _d(yy), sh[0], sh[1], _d(spha), _d(sphb),
+                                      _d(sphc), _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
+                                      _d(T), ctypes.byref(it))
+        self._check(st)
+        return T, it.value, st
+
+    def adj(self, xx, yy, spha, sphb, sphc, T, xrec, yrec, src_adj):
+        """satt_adj_lonlat -> (Ta, iters, status)."""
+        xx, yy = _c(xx), _c(yy)
+        sh = (len(xx), len(yy))
+        spha, sphb, sphc, T = (_f(v, sh) for v in (spha, sphb, sphc, T))
+        xrec, yrec, src_adj = _c(xrec), _c(yrec), _c(src_adj)
+        Ta = np.zeros(sh, order="F")
+        it = ctypes.c_int(0)
+        st = self.lib.satt_adj_lonlat(self.ptr, _d(xx), _d(yy), sh[0], sh[1], _d(spha), _d(sphb),
+                                      _d(sphc), _d(T), _d(xrec), _d(yrec), _d(src_adj),
+                                      len(xrec), _d(Ta), ctypes.byref(it))
+        self._check(st)
+        return Ta, it.value, st
+
+    def mask(self, xx, yy, table, x0, y0):
+        xx, yy = _c(xx), _c(yy)
+        t = _f(table, (len(xx), len(yy))).copy(order="F")
+        self._check(self.lib.satt_mask(self.ptr, _d(xx), _d(yy), len(xx), len(yy), _d(t),
+                                       ctypes.c_double(x0), ctypes.c_double(y0)))
+        return t
+
+    def interp2(self, x, y, v, xq, yq):
+        x, y = _c(x), _c(y)
+        v = _f(v, (len(x), len(y)))
+        xq, yq = _c(xq), _c(yq)
+        out = np.zeros(len(xq))
+        self._check(self.lib.satt_interp2(self.ptr, _d(x), _d(y), len(x), len(y), _d(v), _d(xq),
+                                          _d(yq), len(xq), _d(out)))
+        return out
+
+    def fwd_setup_fields(self, xx, yy, spha, sphb, sphc, fun, x0, y0):
+        xx, yy = _c(xx), _c(yy)
+        sh = (len(xx), len(yy))
+        spha, sphb, sphc, fun = (_f(v, sh) for v in (spha, sphb, sphc, fun))
+        out = [np.zeros(sh, order="F") for _ in range(6)]
+        self._check(self.lib.satt_fwd_setup_fields(
+            self.ptr, _d(xx), _d(yy), sh[0], sh[1], _d(spha), _d(sphb), _d(sphc), _d(fun),
+            ctypes.c_double(x0), ctypes.c_double(y0), *(_d(o) for o in out)))
+        return tuple(out)
+
+    def adj_setup_fields(self, xx, yy, spha, sphb, sphc, xrec, yrec, src_adj):
+        xx, yy = _c(xx), _c(yy)
+        sh = (len(xx), len(yy))
+        spha, sphb, sphc = (_f(v, sh) for v in (spha, sphb, sphc))
+        xrec, yrec, src_adj = _c(xrec), _c(yrec), _c(src_adj)
+        out = [np.zeros(sh, order="F") for _ in range(4)]
+        self._check(self.lib.satt_adj_setup_fields(
+            self.ptr, _d(xx), _d(yy), sh[0], sh[1], _d(spha), _d(sphb), _d(sphc), _d(xrec),
+            _d(yrec), _d(src_adj), len(xrec), *(_d(o) for o in out)))
+        return tuple(out)
+
+    # ---- batched
+    def plan(self, batch, precision=PREC_FP64):
+        return Plan(self, batch, precision)
+
+    def forward_simulate(self, batch, istotable=True, isadj=True, precision=PREC_FP64):
+        with Plan(self, batch, precision) as p:
+            p.run(istotable, isadj)
+            return p.results()
+
+
+def _batch_struct(b):
+    s = Batch()
+    s.dim, s.nx, s.ny, s.nperiod = b["dim"], b["nx"], b["ny"], b["nperiod"]
+    s.xgrids, s.ygrids = _d(b["xgrids"]), _d(b["ygrids"])
+    s.svel, s.m11, s.m12, s.m22 = _d(b["svel"]), _d(b["m11"]), _d(b["m12"]), _d(b["m22"])
+    s.nitems = len(b["item_pidx"])
+    s.item_pidx, s.item_srcx, s.item_srcy = _i(b["item_pidx"]), _d(b["item_srcx"]), _d(b["item_srcy"])
+    s.rec_off, s.recx, s.recy = _i(b["rec_off"]), _d(b["recx"]), _d(b["recy"])
+    s.tt, s.weight, s.idx_path, s.npath = _d(b["tt"]), _d(b["weight"]), _i(b["idx_path"]), b["npath"]
+    return s
+
+
+class Plan:
+    """satt_plan: model, gathers and fields resident on the device."""
+
+    def __init__(self, ctx, batch, precision=PREC_FP64):
+        self.ctx, self.lib, self.b = ctx, ctx.lib, batch
+        self._bs = _batch_struct(batch)
+        p = ctypes.c_void_p()
+        st = self.lib.satt_plan_create(ctx.ptr, ctypes.byref(self._bs), int(precision),
+                                       ctypes.byref(p))
+        ctx._check(st)
+        self.ptr = p
+
+    def close(self):
+        if self.ptr:
+            self.lib.satt_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_model(self, svel, m11, m12, m22):
+        arrs = [_f(v) for v in (svel, m11, m12, m22)]
+        self.ctx._check(self.lib.satt_plan_set_model(self.ptr, *(_d(a) for a in arrs)))
+
+    def run(self, istotable=True, isadj=True, launch=LAUNCH_BATCHED):
+        self.flags = (istotable, isadj)
+        self.ctx._check(self.lib.satt_plan_run(self.ptr, int(istotable), int(isadj), int(launch)))
+
+    def results(self):
+        b = self.b
+        nit, npd, nx, ny = len(b["item_pidx"]), b["nperiod"], b["nx"], b["ny"]
+        out = {
+            "tt_fwd": np.zeros(b["npath"]),
+            "chi": np.zeros(1),
+            "adj_s": np.zeros((npd, nx, ny), order="F"),
+            "adj_density": np.zeros((npd, nx, ny), order="F") if b["dim"] == 3 else None,
+            "item_chi": np.zeros(nit),
+            "iters": np.zeros((nit, 3), dtype=np.int32),
+            "item_status": np.zeros(nit, dtype=np.int32),
+            "weight_out": np.zeros(len(b["recx"])),
+        }
+        r = Result()
+        r.tt_fwd, r.chi, r.adj_s = _d(out["tt_fwd"]), _d(out["chi"]), _d(out["adj_s"])
+        r.adj_density = _d(out["adj_density"])
+        r.item_chi, r.iters, r.item_status = _d(out["item_chi"]), _i(out["iters"]), _i(out["item_status"])
+        r.weight_out = _d(out["weight_out"])
+        st = self.lib.satt_plan_results(self.ptr, ctypes.byref(r))
+        out["status"] = st
+        out["chi"] = float(out["chi"][0])
+        if st < 0:
+            raise SattError(st, self.ctx.error())
+        return out
+
+    def stats(self):
+        s = PlanStats()
+        self.ctx._check(self.lib.satt_plan_get_stats(self.ptr, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in PlanStats._fields_}

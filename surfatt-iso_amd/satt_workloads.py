@@ -1,0 +1,267 @@
+"""Host-side bookkeeping of the reference's source-receiver table, and the
+synthetic workloads of the measurement plan.
+
+``make_batch`` turns per-path arrays (the columns of the reference's src_rec
+CSV) into the batch the C-ABI takes, with the reference's orders:
+
+* stations: unique receiver names in order of first appearance, then event
+  names not yet listed (``get_sta``, src/src_rec.f90:180-217);
+* periods: sorted unique periods (``get_periods``, src/src_rec.f90:130-160);
+* items: for each station in station order that is an event, its periods in
+  order of first appearance (``scatter_src_gather`` / ``get_periods_by_src``,
+  src/src_rec.f90:338-369, :162-179); ``item_pidx`` is ``isrcs(:,1)-1``;
+* gather: the item's paths in path order, receiver positions from the station
+  list (``get_evt_gather``, src/src_rec.f90:252-283); source position from
+  ``get_sta_pos`` (src/src_rec.f90:327-336).
+
+The generators follow the reference's own model builders where one exists:
+``arange`` (src/shared/utils.f90:645-653), the 2-D checkerboard of
+``att_acqui_2d%add_pert`` (src/acqui_2d.f90:101-147) and the topographic
+metric of ``att_grid%get_topo`` (src/grid.f90:119-122 with
+``gradient_2_geo``, src/shared/utils.f90:1118-1160).
+"""
+import numpy as np
+
+# default-real literals of the reference, rounded to REAL(4) then widened
+PI_F = float(np.float32(3.14159265359))  # constants module pi (src/shared/shared_par.f90:11)
+RADIUS = 6371.0
+DEG2RAD = PI_F / 180.0
+
+
+def arange(first, last, step):
+    """arange_r8 (src/shared/utils.f90:645-653)."""
+    n = int((last - first) / step) + 1
+    return np.array([first + i * step for i in range(n)], dtype=np.float64)
+
+
+def checkerboard_2d(nx, ny, ncb_x, ncb_y, pert):
+    """Amplitude field of add_pert with hmarg=0 (src/acqui_2d.f90:129-141):
+    pert * sin(ncb_x*pi*i/nx) * sin(ncb_y*pi*j/ny), i, j = 1..n."""
+    xp = np.sin(ncb_x * PI_F * np.arange(1, nx + 1) / nx)
+    yp = np.sin(ncb_y * PI_F * np.arange(1, ny + 1) / ny)
+    return pert * np.outer(xp, yp)
+
+
+def gradient_2_geo(f, lon, lat):
+    """gradient_2_geo (src/shared/utils.f90:1118-1160): d f / d(km) along
+    lon and lat, central inside, one-sided at the edges."""
+    nx, ny = f.shape
+    dx1 = np.zeros(nx); dx1[1:] = np.diff(lon)
+    dy1 = np.zeros(ny); dy1[1:] = np.diff(lat)
+    dx = RADIUS * np.repeat(dx1[:, None], ny, 1) * DEG2RAD
+    dy = RADIUS * np.repeat(dy1[None, :], nx, 0) * DEG2RAD
+    dx = dx * np.cos(np.deg2rad(lat))[None, :]
+    gfx = np.zeros_like(f); gbx = np.zeros_like(f); gfy = np.zeros_like(f); gby = np.zeros_like(f)
+    gfx[1:, :] = f[1:, :] - f[:-1, :]
+    gbx[:-1, :] = f[1:, :] - f[:-1, :]
+    gfy[:, 1:] = f[:, 1:] - f[:, :-1]
+    gby[:, :-1] = f[:, 1:] - f[:, :-1]
+    tx = np.zeros_like(f); ty = np.zeros_like(f)
+    tx[1:-1, :] = (gfx[1:-1, :] + gbx[1:-1, :]) / (2 * dx[1:-1, :])
+    tx[0, :] = gbx[0, :] / dx[1, :]
+    tx[-1, :] = gfx[-1, :] / dx[-1, :]
+    ty[:, 1:-1] = (gfy[:, 1:-1] + gby[:, 1:-1]) / (2 * dy[:, 1:-1])
+    ty[:, 0] = gby[:, 0] / dy[:, 1]
+    ty[:, -1] = gfy[:, -1] / dy[:, -1]
+    return tx, ty
+
+
+def topo_metric(topo, lon, lat):
+    """m11, m12, m22 of get_topo (src/grid.f90:119-122,140-144):
+    a=(1+fy^2)/(1+fx^2+fy^2), b=(1+fx^2)/(..), c=fx*fy/(..); m11=a, m22=b,
+    m12=-c."""
+    fx, fy = gradient_2_geo(topo, lon, lat)
+    den = 1 + fx ** 2 + fy ** 2
+    a = (1 + fy ** 2) / den
+    b = (1 + fx ** 2) / den
+    c = fx * fy / den
+    return a, -c, b
+
+
+def gaussian_hills(lon, lat, nhill, rng, h=(0.5, 4.0), w=(5.0, 30.0)):
+    """Sum of Gaussian hills (heights in km, half-widths in km)."""
+    X, Y = np.meshgrid(lon, lat, indexing="ij")
+    lat0 = np.deg2rad(lat.mean())
+    topo = np.zeros(X.shape)
+    for _ in range(nhill):
+        cx = rng.uniform(lon[0], lon[-1]); cy = rng.uniform(lat[0], lat[-1])
+        hh = rng.uniform(*h); ww = rng.uniform(*w)
+        ddx = (X - cx) * DEG2RAD * RADIUS * np.cos(lat0)
+        ddy = (Y - cy) * DEG2RAD * RADIUS
+        topo += hh * np.exp(-(ddx ** 2 + ddy ** 2) / (2 * ww ** 2))
+    return topo
+
+
+def smooth_random_field(nx, ny, corr_cells, rng, lo, hi):
+    """Gaussian random field with a Gaussian correlation of corr_cells cells,
+    scaled to [lo, hi]."""
+    z = rng.standard_normal((nx, ny))
+    kx = np.fft.fftfreq(nx)[:, None]; ky = np.fft.fftfreq(ny)[None, :]
+    filt = np.exp(-2 * (np.pi * corr_cells) ** 2 * (kx ** 2 + ky ** 2))
+    f = np.real(np.fft.ifft2(np.fft.fft2(z) * filt))
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+# ------------------------------------------------------------------ gather
+
+def make_batch(dim, xgrids, ygrids, periods_sorted, svel, m11, m12, m22, paths):
+    """Build the C-ABI batch from per-path columns.
+
+    ``paths``: dict of equal-length arrays ``evtname, staname`` (str),
+    ``evla, evlo, stla, stlo, period, tt, weight`` (float).  Model arrays are
+    (nperiod, nx, ny) over ``periods_sorted``; when ``periods_sorted`` is None
+    the sorted unique periods of the table are used (get_periods).
+    """
+    evt = list(paths["evtname"]); sta = list(paths["staname"])
+    per = np.asarray(paths["period"], dtype=np.float64)
+    npath = len(evt)
+    # get_sta (src/src_rec.f90:180-217)
+    names, slat, slon, seen = [], [], [], {}
+    for i in range(npath):
+        if sta[i] not in seen:
+            seen[sta[i]] = len(names); names.append(sta[i])
+            slat.append(paths["stla"][i]); slon.append(paths["stlo"][i])
+    for i in range(npath):
+        if evt[i] not in seen:
+            seen[evt[i]] = len(names); names.append(evt[i])
+            slat.append(paths["evla"][i]); slon.append(paths["evlo"][i])
+    slat = np.asarray(slat, dtype=np.float64); slon = np.asarray(slon, dtype=np.float64)
+    # get_periods (src/src_rec.f90:130-160)
+    if periods_sorted is None:
+        periods_sorted = np.array(sorted(set(per.tolist())), dtype=np.float64)
+    pindex = {float(p): k for k, p in enumerate(periods_sorted)}
+    # paths per event in path order
+    by_evt = {}
+    for i in range(npath):
+        by_evt.setdefault(evt[i], []).append(i)
+    items, gathers = [], []
+    for j, nm in enumerate(names):           # scatter_src_gather (:345-357)
+        if nm not in by_evt:
+            continue
+        plist = []
+        for i in by_evt[nm]:                  # get_periods_by_src (:171-178)
+            if per[i] not in plist:
+                plist.append(per[i])
+        for p in plist:
+            items.append((pindex[float(p)], j))
+            gathers.append([i for i in by_evt[nm] if per[i] == p])   # get_evt_gather
+    nit = len(items)
+    rec_off = np.zeros(nit + 1, dtype=np.int32)
+    for k, g in enumerate(gathers):
+        rec_off[k + 1] = rec_off[k] + len(g)
+    flat = np.array([i for g in gathers for i in g], dtype=np.int64)
+    rx = np.array([slon[seen[sta[i]]] for i in flat], dtype=np.float64)
+    ry = np.array([slat[seen[sta[i]]] for i in flat], dtype=np.float64)
+    return {
+        "dim": int(dim), "nx": len(xgrids), "ny": len(ygrids),
+        "nperiod": len(periods_sorted), "periods": np.asarray(periods_sorted, dtype=np.float64),
+        "xgrids": np.ascontiguousarray(xgrids, dtype=np.float64),
+        "ygrids": np.ascontiguousarray(ygrids, dtype=np.float64),
+        "svel": np.asfortranarray(svel, dtype=np.float64),
+        "m11": np.asfortranarray(m11, dtype=np.float64),
+        "m12": np.asfortranarray(m12, dtype=np.float64),
+        "m22": np.asfortranarray(m22, dtype=np.float64),
+        "item_pidx": np.array([p for p, _ in items], dtype=np.int32),
+        "item_sta": np.array([j for _, j in items], dtype=np.int32),
+        "item_srcx": np.array([slon[j] for _, j in items], dtype=np.float64),
+        "item_srcy": np.array([slat[j] for _, j in items], dtype=np.float64),
+        "rec_off": rec_off,
+        "recx": rx, "recy": ry,
+        "tt": np.ascontiguousarray(np.asarray(paths["tt"], dtype=np.float64)[flat]),
+        "weight": np.ascontiguousarray(np.asarray(paths["weight"], dtype=np.float64)[flat]),
+        "idx_path": flat.astype(np.int32),
+        "npath": npath,
+        "station_names": names,
+    }
+
+
+def stations_uniform(rng, n, xg, yg, frac=0.9, min_cells=2):
+    """n station positions uniform in the interior `frac` of the grid and at
+    least `min_cells` cells from every edge."""
+    dx, dy = xg[1] - xg[0], yg[1] - yg[0]
+    x_lo = max(xg[0] + (1 - frac) / 2 * (xg[-1] - xg[0]), xg[0] + min_cells * dx)
+    x_hi = min(xg[-1] - (1 - frac) / 2 * (xg[-1] - xg[0]), xg[-1] - min_cells * dx)
+    y_lo = max(yg[0] + (1 - frac) / 2 * (yg[-1] - yg[0]), yg[0] + min_cells * dy)
+    y_hi = min(yg[-1] - (1 - frac) / 2 * (yg[-1] - yg[0]), yg[-1] - min_cells * dy)
+    return rng.uniform(x_lo, x_hi, n), rng.uniform(y_lo, y_hi, n)
+
+
+def all_pairs_paths(sx, sy, periods, tt=None):
+    """Every station is a source and every other station a receiver, for
+    every period (path order: source, period, receiver)."""
+    n = len(sx)
+    cols = {k: [] for k in ("evtname", "staname", "evla", "evlo", "stla", "stlo",
+                            "period", "tt", "weight")}
+    for s in range(n):
+        for p in periods:
+            for r in range(n):
+                if r == s:
+                    continue
+                cols["evtname"].append(f"S{s:04d}"); cols["staname"].append(f"S{r:04d}")
+                cols["evla"].append(sy[s]); cols["evlo"].append(sx[s])
+                cols["stla"].append(sy[r]); cols["stlo"].append(sx[r])
+                cols["period"].append(float(p)); cols["tt"].append(1.0); cols["weight"].append(1.0)
+    out = {k: (np.array(v) if k not in ("evtname", "staname") else v) for k, v in cols.items()}
+    if tt is not None:
+        out["tt"] = np.asarray(tt, dtype=np.float64)
+    return out
+
+
+def set_observed(batch, tt_fwd):
+    """Replace the batch's observed times by a forward result (per path)."""
+    b = dict(batch)
+    b["tt"] = np.ascontiguousarray(np.asarray(tt_fwd)[batch["idx_path"]], dtype=np.float64)
+    return b
+
+
+def with_model(batch, svel, m11=None, m12=None, m22=None):
+    b = dict(batch)
+    b["svel"] = np.asfortranarray(svel, dtype=np.float64)
+    if m11 is not None:
+        b["m11"] = np.asfortranarray(m11, dtype=np.float64)
+        b["m12"] = np.asfortranarray(m12, dtype=np.float64)
+        b["m22"] = np.asfortranarray(m22, dtype=np.float64)
+    return b
+
+
+# -------------------------------------------------------------- workloads
+
+def tomo2d_checkerboard(n=81, d=0.025, nsta=25, seed=0, pert=0.05, ncb=2, v0=3.0,
+                        period=10.0):
+    """The plain 2-D tomography step: n x n grid at d degrees from
+    (100E, 20N), one period; true model v0*(1 + checkerboard), initial model
+    v0.  Returns (batch_true, batch_init): run a forward on batch_true to get
+    the observed times, then set them on batch_init."""
+    xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
+    rng = np.random.default_rng(seed)
+    sx, sy = stations_uniform(rng, nsta, xg, yg)
+    paths = all_pairs_paths(sx, sy, [period])
+    amp = checkerboard_2d(n, n, ncb, ncb, pert)
+    vt = (v0 * (1 + amp))[None, :, :]
+    ones = np.ones((1, n, n)); zeros = np.zeros((1, n, n))
+    bt = make_batch(2, xg, yg, np.array([period]), vt, ones, zeros, ones, paths)
+    bi = with_model(bt, v0 * ones)
+    return bt, bi
+
+
+def topo_checkerboard(n=401, d=0.005, nsrc=200, nper=20, seed=3, pert=0.08,
+                      nhill=8, ncb=4, dim=2):
+    """Curved-surface topography workload: n x n grid at d degrees, `nhill`
+    Gaussian hills entering the metric, nper periods log-spaced 3-30 s,
+    checkerboard +-pert over a period-dependent reference velocity; nsrc
+    stations, each a source with the others as receivers."""
+    xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
+    rng = np.random.default_rng(seed)
+    topo = gaussian_hills(xg, yg, nhill, rng)
+    m11, m12, m22 = topo_metric(topo, xg, yg)
+    periods = np.geomspace(3.0, 30.0, nper)
+    vref = 2.8 + 0.04 * periods                     # a smooth dispersion curve
+    amp = checkerboard_2d(n, n, ncb, ncb, pert)
+    svel = vref[:, None, None] * (1 + amp)[None, :, :]
+    sx, sy = stations_uniform(rng, nsrc, xg, yg)
+    paths = all_pairs_paths(sx, sy, periods)
+    rep = lambda f: np.repeat(f[None, :, :], nper, 0)
+    bt = make_batch(dim, xg, yg, periods, svel, rep(m11), rep(m12), rep(m22), paths)
+    bi = with_model(bt, vref[:, None, None] * np.ones((nper, n, n)))
+    return bt, bi
