@@ -1,0 +1,283 @@
+#!/usr/bin/env python3
+"""Throughput of the SurfATT forward + adjoint hot path on one MI355X.
+
+A step is one forward_simulate(chi, istotable=.true., isadj=.true.) over the
+whole synthetic batch with the model, the gathers and every field resident
+in device memory: for each (source, period) item a factored-eikonal forward
+solve, receiver interpolation and misfit, an adjoint solve, the source mask
+and the kernel stacking.  Default workload: the 401 x 401 curved-surface
+topography grid at 0.005 deg, 200 sources x 20 periods = 4000 fwd+adj solves.
+
+Prints one JSON line (rank 0).  `value` is fwd+adj solves per second of the
+batched path; the plain FP64 one-solve-per-launch rate, the roofline of the
+dominant kernel and a host-core baseline are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "surfatt-iso_amd"), os.path.join(ROOT, "oracle")]
+
+import satt  # noqa: E402
+import satt_workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (builder, kwargs, description)
+    "topo401": (W.topo_checkerboard, dict(n=401, d=0.005, nsrc=200, nper=20, seed=3),
+                "401x401 @0.005deg, 8-hill topographic metric, +-8% checkerboard, "
+                "200 sources x 20 periods (3-30 s), 199 receivers each"),
+    "tomo81": (W.tomo2d_checkerboard, dict(n=81, d=0.025, nsta=25, seed=0),
+               "81x81 @0.025deg, 2x2 checkerboard +-5%, 25 stations, 1 period"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes(N, iters_f, iters_a, nitems):
+    """Algorithmic HBM bytes of the forward and adjoint kernels (FP64):
+    forward 72 B per cell-sweep + 24 B per cell-iteration (convergence) +
+    48 B per cell once (T0 build, T = tau*T0); adjoint 64 B per cell-sweep +
+    16 B per cell-iteration + 80 B per cell once (coefficient build)."""
+    fwd = N * (72 * 4 * iters_f.sum() + 24 * iters_f.sum() + 48 * nitems)
+    adj = N * (64 * 4 * iters_a.sum() + 16 * iters_a.sum() + 80 * nitems)
+    return float(fwd), float(adj)
+
+
+def cpu_baseline(batch, nsample, threads):
+    """The reference's own solver file (oracle/_ref, built from
+    src/eikon_solver_2d.f90) driving the loop body over a sample of items on
+    `threads` host threads; interpolation, misfit and stacking by the
+    oracle.  Falls back to the oracle alone when the reference build is
+    absent."""
+    import oracle as O
+    nit = len(batch["item_pidx"])
+    pick = np.linspace(0, nit - 1, nsample).round().astype(int)
+    use_ref = O.RefSolver.available()
+    if not use_ref:
+        sub = _subset(batch, pick)
+        t0 = time.perf_counter()
+        O.forward_simulate(sub, nthreads=threads)
+        dt = time.perf_counter() - t0
+        return dict(value=nsample / dt, unit="solves/s", cores=threads, kind="port",
+                    sample=f"{nsample} items of the workload, oracle loop, {dt:.1f} s wall")
+    ref = O.RefSolver()
+    b = batch
+    xg, yg = b["xgrids"], b["ygrids"]
+    blocks = np.array_split(pick, threads)
+    err = []
+
+    def work(items):
+        try:
+            acc = {}
+            for it in items:
+                p = b["item_pidx"][it]
+                r0, r1 = b["rec_off"][it], b["rec_off"][it + 1]
+                v = b["svel"][p]
+                a, c, bb = b["m11"][p], -b["m12"][p], b["m22"][p]
+                T = _ref_fwd(ref, xg, yg, a, bb, c, 1.0 / v, b["item_srcx"][it], b["item_srcy"][it])
+                tek = O.interp2(xg, yg, T, b["recx"][r0:r1], b["recy"][r0:r1])
+                tt, w = b["tt"][r0:r1], b["weight"][r0:r1].copy()
+                w[np.abs(tek - tt) / tt > 0.5] = 0
+                Ta = _ref_adj(ref, xg, yg, a, bb, c, T, b["recx"][r0:r1], b["recy"][r0:r1], w * (tek - tt))
+                Ta = ref.mask(xg, yg, Ta, b["item_srcx"][it], b["item_srcy"][it])
+                acc[p] = acc.get(p, 0) + Ta / (v * v)
+        except BaseException as e:  # reported below
+            err.append(e)
+
+    old = threading.stack_size()
+    threading.stack_size(ref.STACK)
+    try:
+        ths = [threading.Thread(target=work, args=(blk,)) for blk in blocks]
+        t0 = time.perf_counter()
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        dt = time.perf_counter() - t0
+    finally:
+        threading.stack_size(old)
+    if err:
+        raise err[0]
+    return dict(value=nsample / dt, unit="solves/s", cores=threads, kind="reference",
+                sample=f"{nsample} of the workload's items (evenly spaced), reference "
+                       f"src/eikon_solver_2d.f90 solvers built -O2, {threads} threads, "
+                       f"{dt:.2f} s wall, {dt * threads:.1f} thread-s")
+
+
+def _ref_fwd(ref, xg, yg, a, b, c, fun, x0, y0):
+    import ctypes
+    import oracle as O
+    nx, ny = len(xg), len(yg)
+    a, b, c, fun = (np.asfortranarray(x) for x in (a, b, c, fun))
+    T = np.zeros((nx, ny), order="F"); u = np.zeros((nx, ny), order="F")
+    ref.lib.fsm_uw_ps_lonlat_2d_(O._d(xg), O._d(yg), ctypes.byref(ctypes.c_int(nx)),
+                                 ctypes.byref(ctypes.c_int(ny)), O._d(a), O._d(b), O._d(c),
+                                 O._d(T), O._d(fun), ctypes.byref(ctypes.c_double(x0)),
+                                 ctypes.byref(ctypes.c_double(y0)), O._d(u))
+    return T
+
+
+def _ref_adj(ref, xg, yg, a, b, c, T, xr, yr, sa):
+    import ctypes
+    import oracle as O
+    nx, ny = len(xg), len(yg)
+    a, b, c, T = (np.asfortranarray(x) for x in (a, b, c, T))
+    xr, yr, sa = (np.ascontiguousarray(x) for x in (xr, yr, sa))
+    Ta = np.zeros((nx, ny), order="F")
+    ref.lib.fsm_o1_jse_lonlat_2d_(O._d(xg), O._d(yg), ctypes.byref(ctypes.c_int(nx)),
+                                  ctypes.byref(ctypes.c_int(ny)), O._d(a), O._d(b), O._d(c),
+                                  O._d(T), O._d(Ta), O._d(xr), O._d(yr), O._d(sa),
+                                  ctypes.byref(ctypes.c_int(len(xr))))
+    return Ta
+
+
+def _subset(b, items):
+    """A batch with only `items` (same grid, model and path numbering)."""
+    s = dict(b)
+    s["item_pidx"] = np.ascontiguousarray(b["item_pidx"][items])
+    s["item_srcx"] = np.ascontiguousarray(b["item_srcx"][items])
+    s["item_srcy"] = np.ascontiguousarray(b["item_srcy"][items])
+    offs = [0]
+    sel = []
+    for it in items:
+        r0, r1 = b["rec_off"][it], b["rec_off"][it + 1]
+        sel.append(np.arange(r0, r1))
+        offs.append(offs[-1] + r1 - r0)
+    sel = np.concatenate(sel)
+    s["rec_off"] = np.array(offs, dtype=np.int32)
+    for k in ("recx", "recy", "tt", "weight", "idx_path"):
+        s[k] = np.ascontiguousarray(b[k][sel])
+    return s
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n)), 16))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="topo401", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--plain-items", type=int, default=48,
+                    help="items timed in the one-solve-per-launch FP64 form")
+    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the (source, period) items shard across GPUs with one kernel all-reduce, "
+              "which is not built here)", file=sys.stderr)
+        sys.exit(2)
+
+    build, kw, desc = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    bt, bi = build(**kw)
+    nit, N = len(bi["item_pidx"]), bi["nx"] * bi["ny"]
+    log(f"[bench] workload {args.workload}: {nit} items, grid {bi['nx']}x{bi['ny']}, "
+        f"{bi['npath']} paths (built in {time.perf_counter() - t0:.1f}s)")
+    prec = satt.PREC_MIXED if args.precision == "mixed" else satt.PREC_FP64
+    ctx = satt.Context(0)
+    with ctx.plan(bt) as p:                     # observed times: FP64 forward, true model
+        p.run(True, False)
+        obs = p.results()["tt_fwd"]
+    b = W.set_observed(bi, obs)
+    log("[bench] observed times computed")
+
+    with ctx.plan(b, prec) as plan:
+        for _ in range(args.warmup):
+            plan.run(True, True)
+            plan.wait()
+        log(f"[bench] warmup done ({args.warmup})")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run(True, True)
+            plan.wait()
+        dt = time.perf_counter() - t0
+        out = plan.results()
+        st = plan.stats()
+    log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
+    value = args.steps * nit / dt
+    iters = out["iters"]
+    fwd_b, adj_b = alg_bytes(N, iters[:, 0].astype(np.float64), iters[:, 1].astype(np.float64), nit)
+    launches = max(1, st["launches"])
+    if st["ms_forward"] >= st["ms_adjoint"]:
+        kname, kbytes, kms = "k_forward", fwd_b, st["ms_forward"]
+    else:
+        kname, kbytes, kms = "k_adjoint", adj_b, st["ms_adjoint"]
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # plain FP64 form: one solve per launch, resident inputs
+    plain = None
+    if args.plain_items > 0:
+        sub = _subset(b, np.linspace(0, nit - 1, args.plain_items).round().astype(int))
+        with ctx.plan(sub, satt.PREC_FP64) as pp:
+            pp.run(True, True, satt.LAUNCH_PER_SOLVE)
+            pp.wait()
+            t1 = time.perf_counter()
+            pp.run(True, True, satt.LAUNCH_PER_SOLVE)
+            pp.wait()
+            pdt = time.perf_counter() - t1
+        plain = args.plain_items / pdt
+        log(f"[bench] plain per-solve FP64: {plain:.1f} solves/s")
+
+    cpu = None
+    if not args.no_cpu:
+        th = host_threads()
+        cpu = cpu_baseline(b, max(args.cpu_sample, th), th)
+        log(f"[bench] cpu baseline: {cpu['value']:.2f} solves/s on {th} threads")
+
+    line = {
+        "metric": "fwd+adj solves/s at 401^2 grid, batched vs plain FP64 one-solve-per-launch",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == satt.PREC_FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "items": nit,
+                   "grid": [bi["nx"], bi["ny"]], "precision": args.precision,
+                   "launch": "batched", "chunk": st["chunk"],
+                   "mean_iters_fwd": float(iters[:, 0].mean()),
+                   "mean_iters_adj": float(iters[:, 1].mean())},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "alg_bytes_per_launch": kbytes / launches,
+                     "avg_launch_ms": kms / launches},
+        "plain_fp64_per_solve": None if plain is None else round(plain, 3),
+        "batched_vs_plain": None if plain is None else round(value / plain, 2),
+        "cpu_baseline": cpu,
+        "vs_cpu": None if cpu is None else round(value / cpu["value"], 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,12 +153,15 @@ int satt_plan_create(satt_ctx *ctx, const satt_batch *batch, int precision,
 int satt_plan_set_model(satt_plan *plan, const double *svel, const double *m11,
                         const double *m12, const double *m22);
 int satt_plan_run(satt_plan *plan, int istotable, int isadj, int launch_form);
+/* Wait for the plan's last run (no download). */
+int satt_plan_wait(satt_plan *plan);
 int satt_plan_results(satt_plan *plan, satt_result *result);
 void satt_plan_destroy(satt_plan *plan);
 
 /* Sizing and timing hooks of a plan (for measurement harnesses). */
 typedef struct {
   int chunk;                /* items per launch                              */
+  int launches;             /* launch groups of the last run (per stage)     */
   long long device_bytes;   /* device memory held by the plan                */
   double ms_forward;        /* device time of the last run's forward kernels */
   double ms_adjoint;        /* ... adjoint kernels                           */

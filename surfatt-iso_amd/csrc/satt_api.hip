@@ -626,9 +626,11 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   hipStream_t s = ctx->stream;
   const size_t N = P->N, M = (size_t)P->np * N;
   const int ni = P->nitems;
-  // events: start, then (after forward, after adjoint) per launch group
+  // events: start; per launch group: forward start/end, adjoint start/end;
+  // end.  They bracket the solver kernels only (not the receiver and
+  // stacking kernels), on the stream the kernels run on.
   const int groups = launch_form == SATT_LAUNCH_BATCHED ? P->nchunks : ni;
-  const size_t nev = 1 + 2 * (size_t)groups + 1;
+  const size_t nev = 2 + 4 * (size_t)groups;
   while (P->ev.size() < nev) {
     hipEvent_t e;
     HIPCHK(ctx, hipEventCreate(&e));
@@ -668,7 +670,9 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
       per_off = nullptr;
       per_slot = nullptr;
     }
+    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, 3);
+    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     RecWork rw{fwk.T, P->tekio, P->weight_out, P->rhs_adj, P->rhs_den, P->item_chi,
                istotable ? P->tt_fwd.as<double>() : nullptr};
     launch_receivers(s, P->dg.g, it, rw, P->status);
@@ -680,6 +684,9 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
         AdjWork dwk{fwk.T, Td, coef, delta, old, P->rhs_den};
         launch_adjoint(s, P->dg.g, P->pf, it, dwk, P->iters + 2, P->status, 3, 1);
       }
+    }
+    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
+    if (isadj) {
       const int power = P->dim == 3 ? 3 : 2;
       if (per_off) {
         launch_stack(s, P->dg.g, P->pf, P->np, per_off, per_slot, Ta, acc_s, power);
@@ -689,7 +696,6 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
         if (P->dim == 3) launch_stack_item(s, P->dg.g, P->pf, P->it.pidx, g0, Td, acc_d, power);
       }
     }
-    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
   }
   if (isadj) {
     double *pm = P->acc.as<double>() + (P->dim == 3 ? 2 : 1) * M;
@@ -702,7 +708,14 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   P->last_isadj = isadj;
   P->ran = groups;
   P->stats.chunk = cap;
+  P->stats.launches = groups;
   return SATT_OK;
+}
+
+extern "C" int satt_plan_wait(satt_plan *P) {
+  if (!P) return SATT_E_ARG;
+  HIPCHK(P->ctx, hipSetDevice(P->ctx->device));
+  return sync_status(P->ctx);
 }
 
 extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
@@ -717,12 +730,12 @@ extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
     float ms = 0;
     double f = 0, a = 0;
     for (int g0 = 0; g0 < P->ran; ++g0) {
-      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[2 * g0], P->ev[2 * g0 + 1]));
+      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[1 + 4 * g0], P->ev[2 + 4 * g0]));
       f += ms;
-      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[2 * g0 + 1], P->ev[2 * g0 + 2]));
+      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[3 + 4 * g0], P->ev[4 + 4 * g0]));
       a += ms;
     }
-    HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[0], P->ev[2 * P->ran + 1]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[0], P->ev[1 + 4 * P->ran]));
     P->stats.ms_forward = f;
     P->stats.ms_adjoint = a;
     P->stats.ms_total = ms;

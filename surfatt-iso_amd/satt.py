@@ -55,7 +55,7 @@ class Result(ctypes.Structure):
 
 class PlanStats(ctypes.Structure):
     _fields_ = [
-        ("chunk", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
+        ("chunk", ctypes.c_int), ("launches", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
         ("ms_forward", ctypes.c_double), ("ms_adjoint", ctypes.c_double),
         ("ms_total", ctypes.c_double), ("fwd_sweeps", ctypes.c_longlong),
         ("adj_sweeps", ctypes.c_longlong),
@@ -259,6 +259,9 @@ class Plan:
     def run(self, istotable=True, isadj=True, launch=LAUNCH_BATCHED):
         self.flags = (istotable, isadj)
         self.ctx._check(self.lib.satt_plan_run(self.ptr, int(istotable), int(isadj), int(launch)))
+
+    def wait(self):
+        self.ctx._check(self.lib.satt_plan_wait(self.ptr))
 
     def results(self):
         b = self.b

@@ -132,27 +132,29 @@ def make_batch(dim, xgrids, ygrids, periods_sorted, svel, m11, m12, m22, paths):
         periods_sorted = np.array(sorted(set(per.tolist())), dtype=np.float64)
     pindex = {float(p): k for k, p in enumerate(periods_sorted)}
     # paths per event in path order
+    evt_arr = np.array(evt, dtype=object)
     by_evt = {}
-    for i in range(npath):
-        by_evt.setdefault(evt[i], []).append(i)
+    for i, e in enumerate(evt):
+        by_evt.setdefault(e, []).append(i)
     items, gathers = [], []
     for j, nm in enumerate(names):           # scatter_src_gather (:345-357)
         if nm not in by_evt:
             continue
-        plist = []
-        for i in by_evt[nm]:                  # get_periods_by_src (:171-178)
-            if per[i] not in plist:
-                plist.append(per[i])
-        for p in plist:
+        idx = np.asarray(by_evt[nm], dtype=np.int64)
+        pe = per[idx]
+        _, first = np.unique(pe, return_index=True)   # get_periods_by_src (:171-178)
+        for p in pe[np.sort(first)]:
             items.append((pindex[float(p)], j))
-            gathers.append([i for i in by_evt[nm] if per[i] == p])   # get_evt_gather
+            gathers.append(idx[pe == p])                # get_evt_gather
+    del evt_arr
     nit = len(items)
     rec_off = np.zeros(nit + 1, dtype=np.int32)
     for k, g in enumerate(gathers):
         rec_off[k + 1] = rec_off[k] + len(g)
-    flat = np.array([i for g in gathers for i in g], dtype=np.int64)
-    rx = np.array([slon[seen[sta[i]]] for i in flat], dtype=np.float64)
-    ry = np.array([slat[seen[sta[i]]] for i in flat], dtype=np.float64)
+    flat = np.concatenate(gathers) if gathers else np.zeros(0, dtype=np.int64)
+    sta_idx = np.array([seen[s_] for s_ in sta], dtype=np.int64)
+    rx = slon[sta_idx[flat]]
+    ry = slat[sta_idx[flat]]
     return {
         "dim": int(dim), "nx": len(xgrids), "ny": len(ygrids),
         "nperiod": len(periods_sorted), "periods": np.asarray(periods_sorted, dtype=np.float64),
