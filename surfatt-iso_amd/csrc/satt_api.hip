@@ -15,6 +15,7 @@
 
 #include "../../include/satt.h"
 #include "satt_internal.h"
+#include "satt_strip.h"
 
 using namespace satt;
 
@@ -426,7 +427,10 @@ struct satt_plan {
   DBuf tt_fwd;
   DBuf acc;     // stacked kernels, natural then period-major staging
   DBuf lists;   // per chunk: per_off (np+1), per_slot (chunk)
-  DBuf fwdw, adjw;
+  DBuf fwdw, adjw;  // one-solve-per-launch scratch (one slot)
+  DBuf pskew;   // per-period skewed fields (strip pipeline)
+  DBuf strip;   // per-slot strip scratch: skew fields, tau, snap, T, Ta, Td, delta
+  SkewGeom sg{};
   PeriodFields pf{};
   Items it{};
   double *tekio = nullptr, *weight_out = nullptr, *rhs_adj = nullptr, *rhs_den = nullptr;
@@ -446,6 +450,11 @@ namespace {
 
 size_t fwd_item_bytes(size_t N) { return 6 * N * sizeof(double); }
 size_t adj_item_bytes(size_t N, int dim) { return (dim == 3 ? 10 : 9) * N * sizeof(double); }
+// strip scratch of one slot: adjoint skew fields (the forward's share the
+// same storage), then tau, snap, T, Ta, Td, delta in natural layout
+size_t strip_slot_doubles(size_t N, const SkewGeom &sg) {
+  return 2 * (size_t)kAF * sg.skew_elems() + 6 * N;
+}
 
 int plan_period_fields(satt_plan *P) {
   satt_ctx *ctx = P->ctx;
@@ -460,6 +469,7 @@ int plan_period_fields(satt_plan *P) {
   P->pf.c = f + 2 * M;
   P->pf.fun = f + 3 * M;
   P->pf.vel = f + 4 * M;
+  if (P->pskew.p) launch_period_skew(ctx->stream, P->dg.g, P->sg, P->np, P->pf, P->pskew.as<double>());
   HIPCHK(ctx, hipGetLastError());
   return SATT_OK;
 }
@@ -504,7 +514,9 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
   const size_t N = P->N, M = (size_t)P->np * N;
   int st;
   if ((st = make_grid(ctx, B->xgrids, B->ygrids, B->nx, B->ny, P->dg)) < 0) return bail(st);
-  if (P->model.alloc(4 * M * 8) != hipSuccess || P->pfields.alloc(5 * M * 8) != hipSuccess)
+  P->sg = make_skew_geom(B->nx, B->ny);
+  if (P->model.alloc(4 * M * 8) != hipSuccess || P->pfields.alloc(5 * M * 8) != hipSuccess ||
+      P->pskew.alloc((size_t)P->np * 2 * kPF * P->sg.skew_elems() * 8) != hipSuccess)
     return bail(fail(ctx, SATT_E_NOMEM, "model allocation failed"));
   if ((st = satt_plan_set_model(P, B->svel, B->m11, B->m12, B->m22)) < 0) return bail(st);
 
@@ -561,15 +573,21 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
   // chunk size from free device memory
   size_t freeb = 0, totalb = 0;
   HIPCHK(ctx, hipMemGetInfo(&freeb, &totalb));
-  const size_t per_item = fwd_item_bytes(N) + adj_item_bytes(N, P->dim);
-  size_t budget = std::min(freeb / 3, (size_t)48 << 30);
+  // one-solve-per-launch scratch: one slot
+  if (P->fwdw.alloc(fwd_item_bytes(N)) != hipSuccess ||
+      P->adjw.alloc(adj_item_bytes(N, P->dim)) != hipSuccess)
+    return bail(fail(ctx, SATT_E_NOMEM, "scratch allocation failed"));
+  // batched: as many slots as 60% of the free memory holds (every item of a
+  // 4000-solve 401x401 batch in one launch on a 288 GB device)
+  HIPCHK(ctx, hipMemGetInfo(&freeb, &totalb));
+  const size_t per_item = strip_slot_doubles(N, P->sg) * 8;
+  size_t budget = std::min(freeb / 10 * 6, (size_t)180 << 30);
   size_t ch = std::max<size_t>(1, budget / per_item);
-  ch = std::min<size_t>(ch, 4096);
+  ch = std::min<size_t>(ch, 8192);
   ch = std::min<size_t>(ch, (size_t)std::max(ni, 1));
   P->chunk = (int)ch;
   P->nchunks = ni > 0 ? (ni + P->chunk - 1) / P->chunk : 0;
-  if (P->fwdw.alloc(fwd_item_bytes(N) * ch) != hipSuccess ||
-      P->adjw.alloc(adj_item_bytes(N, P->dim) * ch) != hipSuccess)
+  if (P->strip.alloc(per_item * ch) != hipSuccess)
     return bail(fail(ctx, SATT_E_NOMEM, "scratch allocation failed (chunk %zu)", ch));
 
   // per chunk: the chunk's slots of each period, in item order
@@ -597,7 +615,8 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
   P->stats.device_bytes =
       (long long)(P->dg.buf.bytes + P->model.bytes + P->pfields.bytes + P->items.bytes +
                   P->recs.bytes + P->rec_out.bytes + P->item_out.bytes + P->tt_fwd.bytes +
-                  P->acc.bytes + P->lists.bytes + P->fwdw.bytes + P->adjw.bytes);
+                  P->acc.bytes + P->lists.bytes + P->fwdw.bytes + P->adjw.bytes +
+                  P->pskew.bytes + P->strip.bytes);
   *out = P;
   return SATT_OK;
 }
@@ -643,16 +662,34 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   if (isadj) HIPCHK(ctx, hipMemsetAsync(P->acc.p, 0, P->acc.bytes, s));
   HIPCHK(ctx, hipEventRecord(P->ev[0], s));
 
+  const bool batched = launch_form == SATT_LAUNCH_BATCHED;
+  const int cap = batched ? P->chunk : 1;
+  // one-solve-per-launch buffers (workgroup wavefront kernels)
   double *fw = P->fwdw.as<double>(), *aw = P->adjw.as<double>();
-  const int cap = launch_form == SATT_LAUNCH_BATCHED ? P->chunk : 1;
-  FwdWork fwk{fw, fw + N * cap, fw + 2 * N * cap, fw + 3 * N * cap, fw + 4 * N * cap,
-              fw + 5 * N * cap};
-  double *Ta = aw, *Td = aw + N * cap, *coef = aw + 2 * N * cap,
-         *delta = aw + 7 * N * cap, *old = aw + 8 * N * cap;
+  FwdWork fwk{fw, fw + N, fw + 2 * N, fw + 3 * N, fw + 4 * N, fw + 5 * N};
+  double *Ta = aw, *Td = aw + N, *coef = aw + 2 * N, *delta = aw + 7 * N, *old = aw + 8 * N;
   if (P->dim == 2) {  // no density field: shift scratch down by one slot
-    coef = aw + N * cap;
-    delta = aw + 6 * N * cap;
-    old = aw + 7 * N * cap;
+    coef = aw + N;
+    delta = aw + 6 * N;
+    old = aw + 7 * N;
+  }
+  // batched buffers (strip-pipeline kernels)
+  StripFwdWork sfw{};
+  StripAdjWork saw{};
+  if (batched) {
+    const size_t SE = P->sg.skew_elems();
+    double *b0 = P->strip.as<double>();
+    double *sk = b0, *nat = b0 + 2 * (size_t)kAF * SE * cap;
+    sfw.iskew = sk;
+    saw.iskew = sk;
+    sfw.tau = nat;
+    sfw.snap = saw.snap = nat + N * cap;
+    sfw.T = nat + 2 * N * cap;
+    saw.T = sfw.T;
+    Ta = nat + 3 * N * cap;
+    Td = nat + 4 * N * cap;
+    saw.delta = nat + 5 * N * cap;
+    fwk.T = sfw.T;
   }
   double *acc_s = P->acc.as<double>(), *acc_d = acc_s + M;
   int ei = 1;
@@ -671,18 +708,35 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
       per_slot = nullptr;
     }
     HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
-    launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, 3);
+    if (batched)
+      launch_fwd_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), P->pf, it, sfw, P->iters + 0,
+                       P->status, 3);
+    else
+      launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, 3);
     HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     RecWork rw{fwk.T, P->tekio, P->weight_out, P->rhs_adj, P->rhs_den, P->item_chi,
                istotable ? P->tt_fwd.as<double>() : nullptr};
     launch_receivers(s, P->dg.g, it, rw, P->status);
     HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     if (isadj) {
-      AdjWork awk{fwk.T, Ta, coef, delta, old, P->rhs_adj};
-      launch_adjoint(s, P->dg.g, P->pf, it, awk, P->iters + 1, P->status, 3, 1);
-      if (P->dim == 3) {
-        AdjWork dwk{fwk.T, Td, coef, delta, old, P->rhs_den};
-        launch_adjoint(s, P->dg.g, P->pf, it, dwk, P->iters + 2, P->status, 3, 1);
+      if (batched) {
+        StripAdjWork a1 = saw;
+        a1.Ta = Ta;
+        a1.rhs = P->rhs_adj;
+        launch_adj_strip(s, P->dg.g, P->sg, P->pf, it, a1, P->iters + 1, P->status, 3, 1);
+        if (P->dim == 3) {
+          StripAdjWork a2 = saw;
+          a2.Ta = Td;
+          a2.rhs = P->rhs_den;
+          launch_adj_strip(s, P->dg.g, P->sg, P->pf, it, a2, P->iters + 2, P->status, 3, 1);
+        }
+      } else {
+        AdjWork awk{fwk.T, Ta, coef, delta, old, P->rhs_adj};
+        launch_adjoint(s, P->dg.g, P->pf, it, awk, P->iters + 1, P->status, 3, 1);
+        if (P->dim == 3) {
+          AdjWork dwk{fwk.T, Td, coef, delta, old, P->rhs_den};
+          launch_adjoint(s, P->dg.g, P->pf, it, dwk, P->iters + 2, P->status, 3, 1);
+        }
       }
     }
     HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
