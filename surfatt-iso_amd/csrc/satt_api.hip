@@ -450,10 +450,10 @@ namespace {
 
 size_t fwd_item_bytes(size_t N) { return 6 * N * sizeof(double); }
 size_t adj_item_bytes(size_t N, int dim) { return (dim == 3 ? 10 : 9) * N * sizeof(double); }
-// strip scratch of one slot: adjoint skew fields (the forward's share the
-// same storage), then tau, snap, T, Ta, Td, delta in natural layout
+// strip scratch of one slot: kSlotSkews skewed fields and kSlotNat natural
+// fields (see satt_strip.h)
 size_t strip_slot_doubles(size_t N, const SkewGeom &sg) {
-  return 2 * (size_t)kAF * sg.skew_elems() + 6 * N;
+  return (size_t)kSlotSkews * sg.skew_elems() + (size_t)kSlotNat * N;
 }
 
 int plan_period_fields(satt_plan *P) {
@@ -674,22 +674,19 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
     old = aw + 7 * N;
   }
   // batched buffers (strip-pipeline kernels)
-  StripFwdWork sfw{};
-  StripAdjWork saw{};
+  StripWork sw{};
   if (batched) {
     const size_t SE = P->sg.skew_elems();
     double *b0 = P->strip.as<double>();
-    double *sk = b0, *nat = b0 + 2 * (size_t)kAF * SE * cap;
-    sfw.iskew = sk;
-    saw.iskew = sk;
-    sfw.tau = nat;
-    sfw.snap = saw.snap = nat + N * cap;
-    sfw.T = nat + 2 * N * cap;
-    saw.T = sfw.T;
-    Ta = nat + 3 * N * cap;
-    Td = nat + 4 * N * cap;
-    saw.delta = nat + 5 * N * cap;
-    fwk.T = sfw.T;
+    double *nat = b0 + (size_t)kSlotSkews * SE * cap;
+    sw.skew = b0;
+    sw.T = nat + (size_t)kNatT * N * cap;
+    sw.Ta = nat + (size_t)kNatTa * N * cap;
+    sw.Td = nat + (size_t)kNatTd * N * cap;
+    sw.delta = nat + (size_t)kNatDelta * N * cap;
+    Ta = sw.Ta;
+    Td = sw.Td;
+    fwk.T = sw.T;
   }
   double *acc_s = P->acc.as<double>(), *acc_d = acc_s + M;
   int ei = 1;
@@ -709,7 +706,7 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
     }
     HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     if (batched)
-      launch_fwd_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), P->pf, it, sfw, P->iters + 0,
+      launch_fwd_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), P->pf, it, sw, P->iters + 0,
                        P->status, 3);
     else
       launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, 3);
@@ -720,16 +717,11 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
     HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     if (isadj) {
       if (batched) {
-        StripAdjWork a1 = saw;
-        a1.Ta = Ta;
-        a1.rhs = P->rhs_adj;
-        launch_adj_strip(s, P->dg.g, P->sg, P->pf, it, a1, P->iters + 1, P->status, 3, 1);
-        if (P->dim == 3) {
-          StripAdjWork a2 = saw;
-          a2.Ta = Td;
-          a2.rhs = P->rhs_den;
-          launch_adj_strip(s, P->dg.g, P->sg, P->pf, it, a2, P->iters + 2, P->status, 3, 1);
-        }
+        launch_adj_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), it, sw, P->rhs_adj, kNatTa,
+                         P->iters + 1, P->status, 3, 1);
+        if (P->dim == 3)
+          launch_adj_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), it, sw, P->rhs_den, kNatTd,
+                           P->iters + 2, P->status, 3, 1);
       } else {
         AdjWork awk{fwk.T, Ta, coef, delta, old, P->rhs_adj};
         launch_adjoint(s, P->dg.g, P->pf, it, awk, P->iters + 1, P->status, 3, 1);

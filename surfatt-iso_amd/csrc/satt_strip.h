@@ -23,29 +23,33 @@ inline SkewGeom make_skew_geom(int nx, int ny) {
   return s;
 }
 
-struct StripFwdWork {
-  double *iskew;  // [slot][2 skews][kFF][SE]  (shares the adjoint's skew scratch)
-  double *tau;    // [slot][N] natural
-  double *snap;   // [slot][N] tau at the start of the iteration
-  double *T;      // [slot][N] output travel times
-};
+// Per-slot scratch of the batched path, in units of one skewed field (SE
+// doubles) and one natural field (N doubles).
+//   skew fields 0..11 : adjoint coefficients (a1,a2,b1,b2,d,delta) x 2 skews;
+//                       the forward uses them first for T0,T0x,T0y x 2 skews
+//                       (0..5), tau plus (6), tau minus (7), snapshot (8)
+//   skew fields 12,13 : travel time T, plus and minus skew (forward output)
+//   skew fields 14..16: Ta plus, Ta minus, snapshot (adjoint)
+//   natural 0..3      : T, Ta, Td (density adjoint), delta source
+constexpr int kSlotSkews = 17;
+constexpr int kSlotNat = 4;
+enum : int { kSkTauP = 6, kSkTauM = 7, kSkSnapF = 8, kSkTP = 12, kSkTM = 13,
+             kSkTaP = 14, kSkTaM = 15, kSkSnapA = 16 };
+enum : int { kNatT = 0, kNatTa = 1, kNatTd = 2, kNatDelta = 3 };
 
-struct StripAdjWork {
-  const double *T;  // [slot][N] forward travel times
-  double *iskew;    // [slot][2 skews][kAF][SE]
-  double *Ta;       // [slot][N] output
-  double *snap;     // [slot][N]
-  double *delta;    // [slot][N] natural delta source
-  const double *rhs;  // per receiver (global index)
+struct StripWork {
+  double *skew;                // [slot][kSlotSkews][SE]
+  double *T, *Ta, *Td, *delta; // natural layout, [slot][N] each
 };
 
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
                         const PeriodFields &pf, double *out);
 void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const PeriodFields &pf, const Items &it, const StripFwdWork &w,
+                      const PeriodFields &pf, const Items &it, const StripWork &w,
                       int *iters, int *status, int iter_stride);
-void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
-                      const Items &it, const StripAdjWork &w, int *iters, int *status,
-                      int iter_stride, int apply_mask);
+// out_nat: kNatTa (adjoint) or kNatTd (density adjoint); rhs per receiver.
+void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                      const Items &it, const StripWork &w, const double *rhs, int out_nat,
+                      int *iters, int *status, int iter_stride, int apply_mask);
 
 }  // namespace satt
