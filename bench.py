@@ -160,6 +160,24 @@ def _subset(b, items):
     return s
 
 
+def copy_calibration(nbytes):
+    """One hipMemcpy device-to-device of nbytes (the runtime's copy kernel):
+    under rocprofv3 --pmc its FETCH_SIZE and WRITE_SIZE give the counters'
+    scale for a transfer of known size."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+
+
 def host_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -179,7 +197,13 @@ def main():
                     help="items timed in the one-solve-per-launch FP64 form")
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (a known-size "
+                         "reference for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the (source, period) items shard across GPUs with one kernel all-reduce, "
@@ -217,10 +241,11 @@ def main():
     iters = out["iters"]
     fwd_b, adj_b = alg_bytes(N, iters[:, 0].astype(np.float64), iters[:, 1].astype(np.float64), nit)
     launches = max(1, st["launches"])
+    # the batched launch runs the strip-pipeline kernels
     if st["ms_forward"] >= st["ms_adjoint"]:
-        kname, kbytes, kms = "k_forward", fwd_b, st["ms_forward"]
+        kname, kbytes, kms = "k_fwd_strip", fwd_b, st["ms_forward"]
     else:
-        kname, kbytes, kms = "k_adjoint", adj_b, st["ms_adjoint"]
+        kname, kbytes, kms = "k_adj_strip", adj_b, st["ms_adjoint"]
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")

@@ -1,0 +1,36 @@
+!=====================================================================
+! C-callable wrapper of satt_forward_simulate_host for harnesses that cannot
+! pass Fortran CHARACTER arrays: names are given as integer ids and turned
+! into the reference's fixed-length names ('S' followed by six digits).
+!=====================================================================
+subroutine satt_fhost_forward_simulate(dim, nx, ny, nperiod, xgrids, ygrids, periods, svel, &
+                                       m11, m12, m22, npath, evt_id, sta_id, evla, evlo, &
+                                       stla, stlo, periods_all, tt, weight, istotable, isadj, &
+                                       tt_fwd, chi, adj_s, adj_density, status) &
+                                       bind(C, name="satt_fhost_forward_simulate")
+  use, intrinsic :: iso_c_binding
+  use satt_host, only: satt_forward_simulate_host
+  implicit none
+  integer(c_int), value :: dim, nx, ny, nperiod, npath, istotable, isadj
+  real(c_double), intent(in) :: xgrids(nx), ygrids(ny), periods(nperiod)
+  real(c_double), intent(in) :: svel(nperiod,nx,ny), m11(nperiod,nx,ny), m12(nperiod,nx,ny)
+  real(c_double), intent(in) :: m22(nperiod,nx,ny)
+  integer(c_int), intent(in) :: evt_id(npath), sta_id(npath)
+  real(c_double), intent(in) :: evla(npath), evlo(npath), stla(npath), stlo(npath)
+  real(c_double), intent(in) :: periods_all(npath), tt(npath), weight(npath)
+  real(c_double), intent(out) :: tt_fwd(npath), chi
+  real(c_double), intent(out) :: adj_s(nperiod,nx,ny), adj_density(nperiod,nx,ny)
+  integer(c_int), intent(out) :: status
+  character(len=30), allocatable :: evtname(:), staname(:)
+  integer :: i, st
+  allocate(evtname(npath), staname(npath))
+  do i = 1, npath
+    write(evtname(i), '(a,i6.6)') 'S', evt_id(i)
+    write(staname(i), '(a,i6.6)') 'S', sta_id(i)
+  end do
+  call satt_forward_simulate_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, &
+                                  evtname, staname, evla, evlo, stla, stlo, periods_all, tt, &
+                                  weight, istotable /= 0, isadj /= 0, tt_fwd, chi, adj_s, &
+                                  adj_density, st)
+  status = st
+end subroutine satt_fhost_forward_simulate

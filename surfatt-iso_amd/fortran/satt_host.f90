@@ -1,0 +1,217 @@
+!=====================================================================
+! satt_host: the Fortran host side of the MI355X path.
+!
+!  - satt_host_ctx()   one device context per process (device from the
+!                      SATT_DEVICE environment variable, default 0)
+!  - satt_host_check() maps status codes to the reference's behaviour:
+!                      "source on the boundary" stops the run as
+!                      src/eikon_solver_2d.f90:863-867 does; reaching MaxIter
+!                      is silent as in :1117-1119,1436-1438
+!  - satt_forward_simulate_host()  the per-(source, period) loop of
+!                      att_tomo_2d%forward_simulate (src/tomo2d.f90:158-201)
+!                      and att_acqui%forward_simulate (src/acqui.f90:91-161)
+!                      over the reference's own arrays: the grid axes, the
+!                      period-major model svel/m11/m12/m22(nperiod,nx,ny) and
+!                      the src_rec path columns.  It builds the station list,
+!                      the isrcs work list and the receiver gathers in the
+!                      reference's orders (src/src_rec.f90:162-217,252-283,
+!                      338-369) and runs every item in one device call.
+!=====================================================================
+module satt_host
+  use, intrinsic :: iso_c_binding
+  use satt_cabi
+  implicit none
+  private
+  public :: satt_host_ctx, satt_host_check, satt_host_release, satt_forward_simulate_host
+
+  type(c_ptr), save :: ctx = c_null_ptr
+
+contains
+
+  function satt_host_ctx() result(c)
+    type(c_ptr) :: c
+    integer(c_int) :: st, dev
+    character(len=16) :: env
+    integer :: l, ios
+    if (.not. c_associated(ctx)) then
+      dev = 0
+      call get_environment_variable('SATT_DEVICE', env, length=l)
+      if (l > 0) then
+        read(env(1:l), *, iostat=ios) dev
+        if (ios /= 0) dev = 0
+      end if
+      st = satt_create(ctx, dev)
+      if (st /= SATT_OK) then
+        write(*,*) 'satt: no usable GPU (status ', st, ')'
+        stop 1
+      end if
+    end if
+    c = ctx
+  end function satt_host_ctx
+
+  subroutine satt_host_release()
+    if (c_associated(ctx)) call satt_destroy(ctx)
+    ctx = c_null_ptr
+  end subroutine satt_host_release
+
+  subroutine satt_host_check(st, where)
+    integer(c_int), intent(in) :: st
+    character(len=*), intent(in) :: where
+    if (st >= 0) return
+    if (st == SATT_E_SRC_ON_BOUNDARY) then
+      write(*,*) 'source on the boundary, mesh error'
+    else
+      write(*,*) 'satt: ', where, ': ', satt_error_message(ctx)
+    end if
+    stop 1
+  end subroutine satt_host_check
+
+  !> forward_simulate over the reference's arrays.  Names are CHARACTER
+  !> arrays like SrcRec%evtname/staname; periods(nperiod) is the sorted unique
+  !> list (SrcRec%periods).  dim = 2 (tomo2d) or 3 (acqui).  Outputs are fully
+  !> overwritten: tt_fwd(npath) (zero where no item maps, as local_tt), chi,
+  !> adj_s(nperiod,nx,ny) and, for dim 3, adj_density(nperiod,nx,ny).
+  subroutine satt_forward_simulate_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, &
+                                        evtname, staname, evla, evlo, stla, stlo, &
+                                        periods_all, tt, weight, istotable, isadj, &
+                                        tt_fwd, chi, adj_s, adj_density, status)
+    integer, intent(in) :: dim
+    real(c_double), intent(in), target, contiguous :: xgrids(:), ygrids(:), periods(:)
+    real(c_double), intent(in), target, contiguous :: svel(:,:,:), m11(:,:,:), m12(:,:,:)
+    real(c_double), intent(in), target, contiguous :: m22(:,:,:)
+    character(len=*), intent(in) :: evtname(:), staname(:)
+    real(c_double), intent(in) :: evla(:), evlo(:), stla(:), stlo(:), periods_all(:)
+    real(c_double), intent(in) :: tt(:), weight(:)
+    logical, intent(in) :: istotable, isadj
+    real(c_double), intent(out), target, contiguous :: tt_fwd(:)
+    real(c_double), intent(out) :: chi
+    real(c_double), intent(out), target, contiguous :: adj_s(:,:,:), adj_density(:,:,:)
+    integer, intent(out), optional :: status
+
+    integer :: npath, nsta, i, j, k, np, nitems, n, p
+    character(len=len(staname)), allocatable :: names(:)
+    real(c_double), allocatable :: slat(:), slon(:)
+    integer, allocatable :: evt_sta(:), rec_sta(:), item_pidx_f(:), item_sta(:), plist(:)
+    integer(c_int), allocatable, target :: item_pidx(:), rec_off(:), idx_path(:)
+    real(c_double), allocatable, target :: item_srcx(:), item_srcy(:), recx(:), recy(:)
+    real(c_double), allocatable, target :: rtt(:), rweight(:)
+    real(c_double), target :: chi_c(1)
+    type(satt_batch) :: b
+    type(satt_result) :: r
+    integer(c_int) :: st
+    logical :: seen
+
+    npath = size(evtname)
+    np = size(periods)
+    ! station list: receivers in order of first appearance, then events not
+    ! yet listed (get_sta, src/src_rec.f90:180-217)
+    allocate(names(2 * npath), slat(2 * npath), slon(2 * npath))
+    nsta = 0
+    do i = 1, npath
+      if (find_name(names, nsta, staname(i)) == 0) then
+        nsta = nsta + 1; names(nsta) = staname(i); slat(nsta) = stla(i); slon(nsta) = stlo(i)
+      end if
+    end do
+    do i = 1, npath
+      if (find_name(names, nsta, evtname(i)) == 0) then
+        nsta = nsta + 1; names(nsta) = evtname(i); slat(nsta) = evla(i); slon(nsta) = evlo(i)
+      end if
+    end do
+    allocate(evt_sta(npath), rec_sta(npath))
+    do i = 1, npath
+      evt_sta(i) = find_name(names, nsta, evtname(i))
+      rec_sta(i) = find_name(names, nsta, staname(i))
+    end do
+    ! work list isrcs (scatter_src_gather / get_periods_by_src,
+    ! src/src_rec.f90:338-369, :162-179): stations in order, each event's
+    ! periods in order of first appearance
+    allocate(item_pidx_f(npath), item_sta(npath), plist(np))
+    nitems = 0
+    do j = 1, nsta
+      n = 0
+      do i = 1, npath
+        if (evt_sta(i) /= j) cycle
+        seen = .false.
+        do k = 1, n
+          if (periods(plist(k)) == periods_all(i)) seen = .true.
+        end do
+        if (seen) cycle
+        n = n + 1
+        plist(n) = period_index(periods, periods_all(i))
+      end do
+      do k = 1, n
+        nitems = nitems + 1
+        item_pidx_f(nitems) = plist(k)
+        item_sta(nitems) = j
+      end do
+    end do
+    ! receiver gathers in path order (get_evt_gather, src/src_rec.f90:252-283)
+    allocate(item_pidx(nitems), item_srcx(nitems), item_srcy(nitems), rec_off(nitems + 1))
+    allocate(recx(npath), recy(npath), rtt(npath), rweight(npath), idx_path(npath))
+    rec_off(1) = 0
+    n = 0
+    do k = 1, nitems
+      p = item_pidx_f(k)
+      item_pidx(k) = p - 1
+      item_srcx(k) = slon(item_sta(k))
+      item_srcy(k) = slat(item_sta(k))
+      do i = 1, npath
+        if (evt_sta(i) == item_sta(k) .and. periods_all(i) == periods(p)) then
+          n = n + 1
+          recx(n) = slon(rec_sta(i)); recy(n) = slat(rec_sta(i))
+          rtt(n) = tt(i); rweight(n) = weight(i); idx_path(n) = i - 1
+        end if
+      end do
+      rec_off(k + 1) = n
+    end do
+
+    b%dim = dim
+    b%nx = size(xgrids); b%ny = size(ygrids); b%nperiod = np
+    b%xgrids = c_loc(xgrids); b%ygrids = c_loc(ygrids)
+    b%svel = c_loc(svel); b%m11 = c_loc(m11); b%m12 = c_loc(m12); b%m22 = c_loc(m22)
+    b%nitems = nitems
+    b%item_pidx = c_loc(item_pidx); b%item_srcx = c_loc(item_srcx); b%item_srcy = c_loc(item_srcy)
+    b%rec_off = c_loc(rec_off)
+    b%recx = c_loc(recx); b%recy = c_loc(recy); b%tt = c_loc(rtt); b%weight = c_loc(rweight)
+    b%idx_path = c_loc(idx_path)
+    b%npath = npath
+    r%tt_fwd = c_loc(tt_fwd)
+    r%chi = c_loc(chi_c)
+    r%adj_s = c_loc(adj_s)
+    if (dim == 3) r%adj_density = c_loc(adj_density)
+    st = satt_forward_simulate(satt_host_ctx(), b, merge(1_c_int, 0_c_int, istotable), &
+                               merge(1_c_int, 0_c_int, isadj), SATT_PREC_FP64, r)
+    if (present(status)) then
+      status = st
+    else
+      call satt_host_check(st, 'forward_simulate')
+    end if
+    chi = chi_c(1)
+  end subroutine satt_forward_simulate_host
+
+  integer function find_name(names, n, s)
+    character(len=*), intent(in) :: names(:), s
+    integer, intent(in) :: n
+    integer :: k
+    find_name = 0
+    do k = 1, n
+      if (names(k) == s) then
+        find_name = k
+        return
+      end if
+    end do
+  end function find_name
+
+  integer function period_index(periods, p)
+    real(c_double), intent(in) :: periods(:), p
+    integer :: k
+    period_index = 0
+    do k = 1, size(periods)
+      if (periods(k) == p) then
+        period_index = k
+        return
+      end if
+    end do
+  end function period_index
+
+end module satt_host

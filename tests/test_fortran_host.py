@@ -1,0 +1,110 @@
+"""The Fortran host layer (libsatt_fhost.so, built with amdflang): link-level
+drop-ins with the reference's subroutine names and argument lists, and
+forward_simulate over the reference's src_rec arrays."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import satt
+import satt_workloads as W
+from cases import solver_case
+from conftest import rel_err
+
+FHOST = os.path.join(os.path.dirname(satt.LIB_PATH), "libsatt_fhost.so")
+_d = O._d
+
+
+def test_fhost_exports_reference_names():
+    assert os.path.exists(FHOST), "libsatt_fhost.so not built"
+    lib = ctypes.CDLL(FHOST)
+    for name in ("fsm_uw_ps_lonlat_2d_", "fsm_o1_jse_lonlat_2d_", "mask_",
+                 "satt_fhost_forward_simulate"):
+        assert hasattr(lib, name), name
+
+
+def _byref_i(v):
+    return ctypes.byref(ctypes.c_int(v))
+
+
+def _byref_d(v):
+    return ctypes.byref(ctypes.c_double(v))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["chk41", "topo33x29", "cb81"])
+def test_dropin_solvers_called_like_the_reference(ctx, golden, name):
+    """Call fsm_uw_ps_lonlat_2d_ / fsm_o1_jse_lonlat_2d_ / mask_ exactly as
+    src/measadj.f90:68-70,85-89 does (Fortran by-reference arguments)."""
+    lib = ctypes.CDLL(FHOST)
+    c = solver_case(name)
+    nx, ny = len(c["xx"]), len(c["yy"])
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)
+    spha, sphb, sphc, fun = f(c["spha"]), f(c["sphb"]), f(c["sphc"]), f(c["fun"])
+    T = np.zeros((nx, ny), order="F"); u = np.zeros((nx, ny), order="F")
+    lib.fsm_uw_ps_lonlat_2d_(_d(c["xx"]), _d(c["yy"]), _byref_i(nx), _byref_i(ny), _d(spha),
+                             _d(sphb), _d(sphc), _d(T), _d(fun), _byref_d(c["x0"]),
+                             _byref_d(c["y0"]), _d(u))
+    Tc, _, _ = ctx.fwd(c["xx"], c["yy"], spha, sphb, sphc, fun, c["x0"], c["y0"])
+    assert np.array_equal(T, Tc)
+    assert rel_err(T, golden[f"{name}/T"]) < 1e-9
+    Ta = np.zeros((nx, ny), order="F")
+    xr, yr, sa = (np.ascontiguousarray(c[k]) for k in ("xr", "yr", "sa"))
+    lib.fsm_o1_jse_lonlat_2d_(_d(c["xx"]), _d(c["yy"]), _byref_i(nx), _byref_i(ny), _d(spha),
+                              _d(sphb), _d(sphc), _d(T), _d(Ta), _d(xr), _d(yr), _d(sa),
+                              _byref_i(len(xr)))
+    Tac, _, _ = ctx.adj(c["xx"], c["yy"], spha, sphb, sphc, T, xr, yr, sa)
+    assert np.array_equal(Ta, Tac)
+    lib.mask_(_d(c["xx"]), _d(c["yy"]), _byref_i(nx), _byref_i(ny), _d(Ta), _byref_d(c["x0"]),
+              _byref_d(c["y0"]))
+    assert np.array_equal(Ta, O.mask(c["xx"], c["yy"], Tac, c["x0"], c["y0"]))
+
+
+def fhost_forward_simulate(b, paths, dim):
+    lib = ctypes.CDLL(FHOST)
+    ids = lambda names: np.array([int(s[1:]) for s in names], dtype=np.int32)
+    npd, nx, ny = b["nperiod"], b["nx"], b["ny"]
+    npath = len(paths["evtname"])
+    tt_fwd = np.zeros(npath); chi = ctypes.c_double(0); st = ctypes.c_int(0)
+    adj_s = np.zeros((npd, nx, ny), order="F"); adj_d = np.zeros((npd, nx, ny), order="F")
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    evt, sta = ids(paths["evtname"]), ids(paths["staname"])
+    args = [c(paths[k]) for k in ("evla", "evlo", "stla", "stlo", "period", "tt", "weight")]
+    lib.satt_fhost_forward_simulate(
+        dim, nx, ny, npd, _d(b["xgrids"]), _d(b["ygrids"]), _d(c(b["periods"])), _d(b["svel"]),
+        _d(b["m11"]), _d(b["m12"]), _d(b["m22"]), npath, O._i(evt), O._i(sta),
+        *(_d(a) for a in args), 1, 1, _d(tt_fwd), ctypes.byref(chi), _d(adj_s), _d(adj_d),
+        ctypes.byref(st))
+    return st.value, tt_fwd, chi.value, adj_s, adj_d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fortran_forward_simulate_matches_c_abi(ctx, dim):
+    """The Fortran host builds isrcs and the gathers from the path columns
+    itself; its result equals the C-ABI batch built by the Python host bit for
+    bit, and the oracle within the FP64 bar."""
+    n, d = 37, 0.03
+    xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
+    rng = np.random.default_rng(21)
+    sx, sy = W.stations_uniform(rng, 5, xg, yg)
+    periods = np.array([7.0, 12.0])
+    paths = W.all_pairs_paths(sx, sy, periods)
+    vt = np.stack([(2.9 + 0.03 * p) * (1 + W.checkerboard_2d(n, n, 2, 2, 0.06)) for p in periods])
+    one = np.ones((2, n, n)); zero = np.zeros((2, n, n))
+    bt = W.make_batch(dim, xg, yg, periods, vt, one, zero, one, paths)
+    obs = O.forward_simulate(bt, isadj=False)["tt_fwd"]
+    paths["tt"] = obs
+    b = W.make_batch(dim, xg, yg, periods, np.stack([(2.9 + 0.03 * p) * np.ones((n, n)) for p in periods]),
+                     one, zero, one, paths)
+    st, tt_fwd, chi, adj_s, adj_d = fhost_forward_simulate(b, paths, dim)
+    assert st >= 0
+    out = ctx.forward_simulate(b)
+    assert np.array_equal(tt_fwd, out["tt_fwd"]) and chi == out["chi"]
+    assert np.array_equal(adj_s, out["adj_s"])
+    if dim == 3:
+        assert np.array_equal(adj_d, out["adj_density"])
+    ref = O.forward_simulate(b)
+    assert rel_err(tt_fwd, ref["tt_fwd"]) < 1e-9
