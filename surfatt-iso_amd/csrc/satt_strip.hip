@@ -17,12 +17,15 @@
 // column from memory (final values upstream, old values downstream).
 // There is no barrier: one wave, in-order lanes.
 //
-// Layout.  tau / Ta live in natural layout (i + nx*j).  Read-only per-cell
-// coefficients are stored twice, skewed so that the 64 cells of one step are
-// one contiguous 512-byte row:
+// Layout.  Every per-cell field a sweep touches is stored skewed so that the
+// 64 cells of one step are one contiguous 512-byte row:
 //   plus skew  (sweeps with xd == yd):  row r = j + l
 //   minus skew (sweeps with xd != yd):  row r = j - l + 63
-// each as [strip][row 0..ny+62][lane 0..63], zero padded.
+// each as [strip][row 0..ny+62][lane 0..63], zero padded.  Read-only
+// coefficients exist in both skews.  tau / Ta also exist in both: a sweep of
+// the same family as the previous one updates its skew in place; the first
+// sweep after a family change reads one skew and writes the other.  Natural
+// layout (i + nx*j) is used only for the solve's inputs and outputs.
 #include <climits>
 
 #include "satt_internal.h"
