@@ -43,14 +43,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def alg_bytes(N, iters_f, iters_a, nitems):
-    """Algorithmic HBM bytes of the forward and adjoint kernels (FP64):
-    forward 72 B per cell-sweep + 24 B per cell-iteration (convergence) +
-    48 B per cell once (T0 build, T = tau*T0); adjoint 64 B per cell-sweep +
-    16 B per cell-iteration + 80 B per cell once (coefficient build)."""
-    fwd = N * (72 * 4 * iters_f.sum() + 24 * iters_f.sum() + 48 * nitems)
-    adj = N * (64 * 4 * iters_a.sum() + 16 * iters_a.sum() + 80 * nitems)
-    return float(fwd), float(adj)
+def alg_bytes(N, out, nitems, mixed):
+    """Algorithmic HBM bytes per kernel, summed over the run's items.
+
+    FP64 forward: 72 B per cell-sweep (7 coefficient fields read, tau read and
+    written) + 24 B per cell-iteration (convergence) + 48 B per cell once
+    (T0 build, T = tau*T0).  FP64 adjoint: 64 B per cell-sweep + 16 B per
+    cell-iteration + 80 B per cell once (coefficient build).  FP32 phase:
+    half the per-sweep and per-iteration bytes (36 / 12 forward, 32 / 8
+    adjoint), T0 build 24 + 12 B (FP64 and float), coefficient build 80 B.
+    Polish: the FP64 figures plus 12 B per cell for the float-to-double
+    conversion."""
+    i64, i32 = out["iters"].astype(np.float64), out["iters_fp32"].astype(np.float64)
+    f64, a64 = i64[:, 0].sum(), i64[:, 1].sum() + i64[:, 2].sum()
+    nadj = nitems * (1 + int(np.any(i64[:, 2] > 0)))
+    if not mixed:
+        return {"k_fwd_strip": N * (72 * 4 * f64 + 24 * f64 + 48 * nitems),
+                "k_adj_strip": N * (64 * 4 * a64 + 16 * a64 + 80 * nadj)}
+    f32, a32 = i32[:, 0].sum(), i32[:, 1].sum() + i32[:, 2].sum()
+    return {"k_fwd_mixed32": N * (36 * 4 * f32 + 12 * f32 + 36 * nitems),
+            "k_fwd_polish": N * (72 * 4 * f64 + 24 * f64 + 36 * nitems),
+            "k_adj_mixed32": N * (32 * 4 * a32 + 8 * a32 + 80 * nadj),
+            "k_adj_polish": N * (64 * 4 * a64 + 92 * nadj)}
+
+
+STAGE_OF = {"k_fwd_strip": "ms_fwd_fp64", "k_adj_strip": "ms_adj_fp64",
+            "k_fwd_mixed32": "ms_fwd_fp32", "k_fwd_polish": "ms_fwd_fp64",
+            "k_adj_mixed32": "ms_adj_fp32", "k_adj_polish": "ms_adj_fp64"}
 
 
 def cpu_baseline(batch, nsample, threads):
@@ -192,7 +211,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="topo401", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--oracle-items", type=int, default=4,
+                    help="items of the workload checked against the FP64 oracle (mixed)")
+    ap.add_argument("--no-fp64-compare", action="store_true",
+                    help="mixed mode: skip the FP64 batched run and the error report")
     ap.add_argument("--plain-items", type=int, default=48,
                     help="items timed in the one-solve-per-launch FP64 form")
     ap.add_argument("--cpu-sample", type=int, default=32)
@@ -238,20 +261,54 @@ def main():
         st = plan.stats()
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
     value = args.steps * nit / dt
-    iters = out["iters"]
-    fwd_b, adj_b = alg_bytes(N, iters[:, 0].astype(np.float64), iters[:, 1].astype(np.float64), nit)
+    mixed = prec == satt.PREC_MIXED
+    abytes = alg_bytes(N, out, nit, mixed)
     launches = max(1, st["launches"])
-    # the batched launch runs the strip-pipeline kernels
-    if st["ms_forward"] >= st["ms_adjoint"]:
-        kname, kbytes, kms = "k_fwd_strip", fwd_b, st["ms_forward"]
-    else:
-        kname, kbytes, kms = "k_adj_strip", adj_b, st["ms_adjoint"]
+    kname = max(abytes, key=lambda k: st[STAGE_OF[k]])      # the dominant kernel
+    kbytes, kms = abytes[kname], st[STAGE_OF[kname]]
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    kernels = {k: {"ms": round(st[STAGE_OF[k]] / launches, 3),
+                   "alg_GBs": round(abytes[k] / (st[STAGE_OF[k]] * 1e-3) / 1e9, 1)}
+               for k in abytes if st[STAGE_OF[k]] > 0}
+
+    # mixed mode: the same step in FP64 (throughput and the error of the mixed
+    # results against it), and a sample of items against the FP64 oracle
+    fp64_value, mixed_err = None, None
+    if mixed and not args.no_fp64_compare:
+        with ctx.plan(b, satt.PREC_FP64) as p64:
+            p64.run(True, True)
+            p64.wait()
+            t1 = time.perf_counter()
+            p64.run(True, True)
+            p64.wait()
+            fp64_value = nit / (time.perf_counter() - t1)
+            o64 = p64.results()
+        log(f"[bench] batched FP64: {fp64_value:.1f} solves/s")
+        d = np.abs(out["tt_fwd"] - o64["tt_fwd"])
+        mixed_err = {
+            "vs": "batched FP64 on the device (parity-checked against the oracle)",
+            "tt_max_ms": float(d.max() * 1e3), "tt_mean_ms": float(d.mean() * 1e3),
+            "kernel_rel_l2": float(np.linalg.norm((out["adj_s"] - o64["adj_s"]).ravel())
+                                   / np.linalg.norm(o64["adj_s"].ravel())),
+            "chi_rel": float(abs(out["chi"] - o64["chi"]) / abs(o64["chi"])),
+        }
+        if args.oracle_items > 0:
+            import oracle as O
+            pick = np.linspace(0, nit - 1, args.oracle_items).round().astype(int)
+            sub = _subset(b, pick)
+            ref = O.forward_simulate(sub, istotable=True, isadj=False, nthreads=len(pick))
+            paths = sub["idx_path"]
+            mixed_err["oracle_sample"] = {
+                "items": int(len(pick)),
+                "tt_max_ms": float(np.abs(out["tt_fwd"][paths] - ref["tt_fwd"][paths]).max() * 1e3),
+                "fp64_tt_max_rel": float(np.abs(o64["tt_fwd"][paths] - ref["tt_fwd"][paths]).max()
+                                         / np.abs(ref["tt_fwd"][paths]).max())}
+        log(f"[bench] mixed error: {mixed_err}")
 
     # plain FP64 form: one solve per launch, resident inputs
     plain = None
@@ -273,8 +330,9 @@ def main():
         cpu = cpu_baseline(b, max(args.cpu_sample, th), th)
         log(f"[bench] cpu baseline: {cpu['value']:.2f} solves/s on {th} threads")
 
+    iters = out["iters"]
     line = {
-        "metric": "fwd+adj solves/s at 401^2 grid, batched vs plain FP64 one-solve-per-launch",
+        "metric": "fwd+adj solves/s at 401^2 grid, batched mixed vs plain FP64 one-solve-per-launch",
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -290,12 +348,17 @@ def main():
                    "grid": [bi["nx"], bi["ny"]], "precision": args.precision,
                    "launch": "batched", "chunk": st["chunk"],
                    "mean_iters_fwd": float(iters[:, 0].mean()),
-                   "mean_iters_adj": float(iters[:, 1].mean())},
+                   "mean_iters_adj": float(iters[:, 1].mean()),
+                   "mean_iters_fwd_fp32": float(out["iters_fp32"][:, 0].mean()),
+                   "mean_iters_adj_fp32": float(out["iters_fp32"][:, 1].mean())},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "alg_bytes_per_launch": kbytes / launches,
                      "avg_launch_ms": kms / launches},
+        "kernels": kernels,
+        "batched_fp64": None if fp64_value is None else round(fp64_value, 3),
+        "mixed_error": mixed_err,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),
         "batched_vs_plain": None if plain is None else round(value / plain, 2),
         "cpu_baseline": cpu,

@@ -52,9 +52,10 @@ extern "C" {
 
 /* precision modes for the batched path */
 #define SATT_PREC_FP64 0  /* the reference's FP64 arithmetic, sweep for sweep */
-#define SATT_PREC_MIXED 1 /* FP32 sweeps, then FP64 sweeps to the reference's
-                             stopping rule; T0, residuals, interpolation and
-                             kernel accumulation in FP64 */
+#define SATT_PREC_MIXED 1 /* FP32 sweeps, then FP64 polish: forward to the
+                             reference's stopping rule, adjoint one iteration;
+                             T0, residuals, interpolation and kernel
+                             accumulation in FP64.  Batched form only. */
 
 /* launch forms for satt_plan_run */
 #define SATT_LAUNCH_BATCHED 0   /* all items of a chunk in one launch per stage */
@@ -135,6 +136,8 @@ typedef struct {
   int *iters;          /* 3*nitems: forward, adjoint, density-adjoint       */
   int *item_status;    /* nitems: per-item status                           */
   double *weight_out;  /* rec_off[nitems]: weights after outlier rejection  */
+  int *iters_fp32;     /* 3*nitems: FP32-phase iterations (mixed mode; 0 in
+                          FP64 mode); `iters` then counts the FP64 polish  */
 } satt_result;
 
 /* One call of forward_simulate(chi, istotable, isadj): upload, run, download.
@@ -166,10 +169,27 @@ typedef struct {
   double ms_forward;        /* device time of the last run's forward kernels */
   double ms_adjoint;        /* ... adjoint kernels                           */
   double ms_total;          /* ... whole run                                 */
-  long long fwd_sweeps;     /* sum over items of 4*forward iterations        */
+  long long fwd_sweeps;     /* sum over items of 4*forward iterations (FP64) */
   long long adj_sweeps;     /* sum over items of 4*adjoint iterations (both) */
+  long long fwd_sweeps32;   /* the same for the FP32 phase of the mixed mode */
+  long long adj_sweeps32;
+  double ms_fwd_fp64;       /* forward FP64 sweep kernels                    */
+  double ms_fwd_fp32;       /* forward FP32-phase kernels (mixed mode)       */
+  double ms_adj_fp64;       /* adjoint FP64 kernels (both adjoints)          */
+  double ms_adj_fp32;       /* adjoint FP32-phase kernels (mixed mode)       */
+  double ms_fwd_setup;      /* forward setup (T0 fields, tau init)           */
+  double ms_adj_setup;      /* adjoint setup (delta source, coefficients)    */
 } satt_plan_stats;
 int satt_plan_get_stats(satt_plan *plan, satt_plan_stats *stats);
+
+/* A field of one item of the plan's last batched run, in natural layout
+ * (nx,ny): SATT_FIELD_T travel times, SATT_FIELD_TA adjoint field (masked),
+ * SATT_FIELD_TD density adjoint field (dim 3).  Only items of the run's last
+ * launch group are resident; others give SATT_E_ARG. */
+#define SATT_FIELD_T 0
+#define SATT_FIELD_TA 1
+#define SATT_FIELD_TD 2
+int satt_plan_item_field(satt_plan *plan, int item, int which, double *out);
 
 /* ------------------------------------------------------------ inspection
  * The per-solve setup fields as the device computes them, for bit-level

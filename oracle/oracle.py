@@ -48,7 +48,8 @@ def lib():
         _lib = ctypes.CDLL(LIB_PATH)
         for name in ("orc_fwd_lonlat", "orc_fwd_lonlat_pre", "orc_fwd_t0",
                      "orc_adj_lonlat", "orc_adj_lonlat_pre", "orc_interp2",
-                     "orc_forward_simulate"):
+                     "orc_forward_simulate", "orc_fwd_mixed", "orc_fwd_mixed_pre",
+                     "orc_adj_mixed", "orc_adj_mixed_pre"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.orc_mask.restype = None
     return _lib
@@ -69,29 +70,39 @@ def _f(a, shape=None):
     return a
 
 
-def fwd(xx, yy, spha, sphb, sphc, fun, x0, y0, pre=None):
+def _iters(mixed):
+    return (ctypes.c_int * 2)(0, 0) if mixed else ctypes.c_int(0)
+
+
+def _iters_value(it, mixed):
+    return (it[0], it[1]) if mixed else it.value
+
+
+def fwd(xx, yy, spha, sphb, sphc, fun, x0, y0, pre=None, mixed=False):
     """FSM_UW_PS_lonlat_2d (src/eikon_solver_2d.f90:764).  Returns (T, iters,
     status).  With ``pre=(a, b, c, T0, T0x, T0y)`` the converted metric and
-    the analytic factor are taken from the caller (spha/sphb/sphc ignored)."""
+    the analytic factor are taken from the caller (spha/sphb/sphc ignored).
+    ``mixed=True`` runs the restatement of the device's mixed-precision mode;
+    iters is then (FP32 iterations, FP64 polish iterations)."""
     xx = np.ascontiguousarray(xx, dtype=np.float64)
     yy = np.ascontiguousarray(yy, dtype=np.float64)
     nx, ny = len(xx), len(yy)
     sh = (nx, ny)
     spha, sphb, sphc, fun = (_f(v, sh) for v in (spha, sphb, sphc, fun))
     T = np.zeros(sh, order="F")
-    it = ctypes.c_int(0)
+    it = _iters(mixed)
     if pre is None:
-        st = lib().orc_fwd_lonlat(_d(xx), _d(yy), nx, ny, _d(spha), _d(sphb), _d(sphc),
-                                  _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
-                                  _d(T), ctypes.byref(it))
+        f = lib().orc_fwd_mixed if mixed else lib().orc_fwd_lonlat
+        st = f(_d(xx), _d(yy), nx, ny, _d(spha), _d(sphb), _d(sphc), _d(fun),
+               ctypes.c_double(x0), ctypes.c_double(y0), _d(T), ctypes.byref(it))
     else:
         a, b, c, T0, T0x, T0y = (_f(v, sh) for v in pre)
-        st = lib().orc_fwd_lonlat_pre(_d(xx), _d(yy), nx, ny, _d(a), _d(b), _d(c),
-                                      _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
-                                      _d(T0), _d(T0x), _d(T0y), _d(T), ctypes.byref(it))
+        f = lib().orc_fwd_mixed_pre if mixed else lib().orc_fwd_lonlat_pre
+        st = f(_d(xx), _d(yy), nx, ny, _d(a), _d(b), _d(c), _d(fun), ctypes.c_double(x0),
+               ctypes.c_double(y0), _d(T0), _d(T0x), _d(T0y), _d(T), ctypes.byref(it))
     if st < 0:
         raise OracleError(st, "orc_fwd_lonlat")
-    return T, it.value, st
+    return T, _iters_value(it, mixed), st
 
 
 def t0_fields(xx, yy, fun, x0, y0):
@@ -107,7 +118,7 @@ def t0_fields(xx, yy, fun, x0, y0):
     return tuple(out)
 
 
-def adj_pre(xx, yy, a, b, c, T, delta):
+def adj_pre(xx, yy, a, b, c, T, delta, mixed=False):
     """FSM_O1_JSE_lonlat_2d with the converted metric and the delta source
     supplied by the caller.  Returns (Ta, iters, status)."""
     xx = np.ascontiguousarray(xx, dtype=np.float64)
@@ -116,15 +127,16 @@ def adj_pre(xx, yy, a, b, c, T, delta):
     sh = (nx, ny)
     a, b, c, T, delta = (_f(v, sh) for v in (a, b, c, T, delta))
     Ta = np.zeros(sh, order="F")
-    it = ctypes.c_int(0)
-    st = lib().orc_adj_lonlat_pre(_d(xx), _d(yy), nx, ny, _d(a), _d(b), _d(c), _d(T),
-                                  _d(delta), _d(Ta), ctypes.byref(it))
+    it = _iters(mixed)
+    f = lib().orc_adj_mixed_pre if mixed else lib().orc_adj_lonlat_pre
+    st = f(_d(xx), _d(yy), nx, ny, _d(a), _d(b), _d(c), _d(T), _d(delta), _d(Ta),
+           ctypes.byref(it))
     if st < 0:
         raise OracleError(st, "orc_adj_lonlat_pre")
-    return Ta, it.value, st
+    return Ta, _iters_value(it, mixed), st
 
 
-def adj(xx, yy, spha, sphb, sphc, T, xrec, yrec, src_adj):
+def adj(xx, yy, spha, sphb, sphc, T, xrec, yrec, src_adj, mixed=False):
     """FSM_O1_JSE_lonlat_2d (src/eikon_solver_2d.f90:1276).  Returns
     (Ta, iters, status)."""
     xx = np.ascontiguousarray(xx, dtype=np.float64)
@@ -134,13 +146,13 @@ def adj(xx, yy, spha, sphb, sphc, T, xrec, yrec, src_adj):
     spha, sphb, sphc, T = (_f(v, sh) for v in (spha, sphb, sphc, T))
     xrec, yrec, src_adj = (np.ascontiguousarray(v, dtype=np.float64) for v in (xrec, yrec, src_adj))
     Ta = np.zeros(sh, order="F")
-    it = ctypes.c_int(0)
-    st = lib().orc_adj_lonlat(_d(xx), _d(yy), nx, ny, _d(spha), _d(sphb), _d(sphc), _d(T),
-                              _d(xrec), _d(yrec), _d(src_adj), len(xrec), _d(Ta),
-                              ctypes.byref(it))
+    it = _iters(mixed)
+    f = lib().orc_adj_mixed if mixed else lib().orc_adj_lonlat
+    st = f(_d(xx), _d(yy), nx, ny, _d(spha), _d(sphb), _d(sphc), _d(T), _d(xrec), _d(yrec),
+           _d(src_adj), len(xrec), _d(Ta), ctypes.byref(it))
     if st < 0:
         raise OracleError(st, "orc_adj_lonlat")
-    return Ta, it.value, st
+    return Ta, _iters_value(it, mixed), st
 
 
 def mask(xx, yy, table, x0, y0):

@@ -358,12 +358,17 @@ int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
 
 /* ------------------------------------------------------------------ adjoint */
 
+static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
+                       const double *a1p, const double *a2m, const double *a2p,
+                       const double *b1m, const double *b1p, const double *b2m,
+                       const double *b2p, const double *delta, double *Ta, int *done);
+
 static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
                       int ny, const double *spha, const double *sphb,
                       const double *sphc, const double *T,
                       const double *xrec_deg, const double *yrec_deg,
                       const double *src_adj, int nr, const double *pre,
-                      double *Ta, int *iters) {
+                      int mixed, double *Ta, int *iters) {
   if (nx < 3 || ny < 3 || nr < 0) return ORC_E_ARG;
   const size_t n = (size_t)nx * ny;
   double *xx = malloc(sizeof(double) * nx), *yy = malloc(sizeof(double) * ny);
@@ -457,8 +462,19 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
   for (int i = 0; i < nx; ++i) { Ta[IDX(i, 0)] = 0; Ta[IDX(i, ny - 1)] = 0; }
   for (int j = 0; j < ny; ++j) { Ta[IDX(0, j)] = 0; Ta[IDX(nx - 1, j)] = 0; }
 
+  int it32 = 0, done32 = 0;
+  if (mixed) {
+    it32 = adj_phase32(nx, ny, dx, dy, a1m, a1p, a2m, a2p, b1m, b1p, b2m, b2p, delta, Ta,
+                       &done32);
+    if (it32 < 0) {
+      free(xx); free(yy); free(buf); free(bb);
+      return it32;
+    }
+  }
+  /* mixed mode: exactly one FP64 polish iteration after the FP32 phase */
+  const int maxit = mixed ? 1 : ADJ_MAXITER;
   int iter, converged = 0;
-  for (iter = 1; iter <= ADJ_MAXITER; ++iter) {
+  for (iter = 1; iter <= maxit; ++iter) {
     memcpy(old, Ta, n * sizeof(double));
     /* do xdirec=-1,1,2; do ydirec=-1,1,2 with nint bounds (:1385-1390):
      * xdirec=-1 runs x over 2..nx-1 ascending, +1 descending. */
@@ -494,8 +510,13 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
       break;
     }
   }
-  if (!converged) iter = ADJ_MAXITER;
-  if (iters) *iters = iter;
+  if (mixed) {
+    converged = done32;
+    if (iters) { iters[0] = it32; iters[1] = 1; }
+  } else {
+    if (!converged) iter = ADJ_MAXITER;
+    if (iters) *iters = iter;
+  }
   free(xx); free(yy); free(buf); free(bb);
   return converged ? ORC_OK : ORC_W_MAXITER;
 }
@@ -506,7 +527,7 @@ int orc_adj_lonlat(const double *xx_deg, const double *yy_deg, int nx, int ny,
                    const double *yrec_deg, const double *src_adj, int nr,
                    double *Ta, int *iters) {
   return adj_common(xx_deg, yy_deg, nx, ny, spha, sphb, sphc, T, xrec_deg,
-                    yrec_deg, src_adj, nr, NULL, Ta, iters);
+                    yrec_deg, src_adj, nr, NULL, 0, Ta, iters);
 }
 
 int orc_adj_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
@@ -521,7 +542,7 @@ int orc_adj_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
   memcpy(pre + 2 * n, c, n * sizeof(double));
   memcpy(pre + 3 * n, delta, n * sizeof(double));
   const int st = adj_common(xx_deg, yy_deg, nx, ny, NULL, NULL, NULL, T, NULL,
-                            NULL, NULL, 0, pre, Ta, iters);
+                            NULL, NULL, 0, pre, 0, Ta, iters);
   free(pre);
   return st;
 }
@@ -793,5 +814,320 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
     free(blk[r].tt_fwd); free(blk[r].adj_s); free(blk[r].adj_den);
   }
   free(blk); free(th);
+  return st;
+}
+
+/* ======================================================================
+ * Mixed-precision mode of the device (not a reference algorithm).
+ *
+ * Restates, in the device's exact operation order, the FP32 sweep phase and
+ * the FP64 polish of the mixed mode, so that device results can be checked
+ * bit for bit; accuracy against the FP64 restatement above is checked
+ * separately, within the stated tolerances.
+ *
+ * Forward.  FP32 phase: the reference's update (:910-1079) with every operand
+ * rounded to float, each sweep evaluating the candidates whose neighbours
+ * are upwind in its direction, rewritten in the increment u = tau - tau_nb so that no
+ * coefficient carries the (T0/dx)^2 scale: with tt = tX + u,
+ *   Tx = gx + px*u,  gx = T0x*tX
+ *   Ty = gy + py*u,  gy = T0y*tX + sy*(T0/dy)*(tX - tY)
+ * so a*Tx^2 + b*Ty^2 - 2c*Tx*Ty = fun^2 becomes ea*u^2 + eb*u + ec = 0 with
+ * coefficients of the size of the eikonal residual; roots by the
+ * cancellation-free form q = -(eb + sign(eb)*sqrt(disc))/2, u = q/ea, ec/q.
+ * One-sided candidates: u = (+-dis - tn*T0d)/p.  Causality tests, acceptance
+ * rules, sweep order, tau init, frozen set and the stop rule (with the
+ * residual |dtau|*T0 accumulated in double) are the reference's.  FP64
+ * polish: tau = (double)tau32, then the reference's iterations (at least
+ * one) with its stop rule; T = tau*T0.
+ *
+ * Adjoint.  The face coefficients, d and delta are computed in FP64 as the
+ * reference does.  FP32 phase: Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN
+ * with k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
+ * kN = -b2m/dy/d rounded to float (all zero where |d| < eps), iterated until
+ * the reference's absolute rule holds or the largest change is at most
+ * 2^-21 of the largest |Ta| (float resolution).  FP64 polish: the
+ * reference's update and stop rule from Ta = (double)Ta32, at least one
+ * iteration.
+ * ====================================================================== */
+
+static const float MIX_REL_STOP = 4.76837158203125e-07f; /* 2^-21 */
+
+/* One cell of the FP32 forward sweep (see the note above) in the sweep
+ * direction (xd, yd).  Only the candidates whose neighbours are upwind in
+ * this sweep are evaluated: the quadrant of neighbours (i-xd, j) and
+ * (i, j-yd), and the one-sided candidates from each of them.  Over the four
+ * sweeps of an iteration every candidate of the reference's update is thus
+ * evaluated once, with its neighbours' newest values; a fixed point is a
+ * fixed point of the full update.  The field holds dtau = tau - 1: tau stays
+ * near 1 wherever T0 is a good approximation, so the offset keeps 3-4 more
+ * bits of tau than tau itself would in float. */
+static void fwd_cell32(int xd, int yd, int nx, int ny, int i, int j, float rdx,
+                       float rdy, const float *a, const float *b, const float *c,
+                       const float *f, const float *disx, const float *disy,
+                       const float *T0, const float *T0x, const float *T0y,
+                       float *dtau) {
+  const size_t k = IDX(i, j);
+  const float ak = a[k], bk = b[k], ck = c[k], fk = f[k];
+  const float t0 = T0[k], t0x = T0x[k], t0y = T0y[k];
+  const float t0dx = t0 * rdx, t0dy = t0 * rdy;
+  const int hasX = xd > 0 ? i > 0 : i < nx - 1;
+  const int hasY = yd > 0 ? j > 0 : j < ny - 1;
+  const size_t kX = hasX ? IDX(i - xd, j) : k, kY = hasY ? IDX(i, j - yd) : k;
+  const float dX = dtau[kX], dY = dtau[kY], zX = T0[kX], zY = T0[kY];
+  const float px = xd > 0 ? t0x + t0dx : t0x - t0dx;
+  const float py = yd > 0 ? t0y + t0dy : t0y - t0dy;
+  float best = dtau[k];
+  if (hasX && hasY) {
+    const float gx = t0x + t0x * dX;
+    const float gy = yd > 0 ? t0y + t0y * dX + t0dy * (dX - dY)
+                            : t0y + t0y * dX - t0dy * (dX - dY);
+    const float ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
+    const float eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));
+    const float ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;
+    const float disc = eb * eb - 4 * ea * ec;
+    if (disc >= 0) {
+      const float sq = sqrtf(disc);
+      const float qq = -0.5f * (eb >= 0 ? eb + sq : eb - sq);
+      for (int isol = 1; isol <= 2; ++isol) {
+        const float u = isol == 1 ? qq / ea : ec / qq;
+        const float dt = dX + u;
+        const float Tx = gx + px * u, Ty = gy + py * u;
+        const float chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;
+        const int ok = (xd > 0 ? chx >= 0 : chx <= 0) && (yd > 0 ? chy >= 0 : chy <= 0);
+        if (ok && dt < best) best = dt;
+      }
+    }
+  }
+  /* one-sided candidates (:996-1075): tt*T0 >= tn*T0n and tt > tn*0.5 in
+   * offsets; p is the quadrant's px (x neighbour) or py (y neighbour) */
+  for (int side = 0; side < 2; ++side) {
+    if (side == 0 ? !hasX : !hasY) continue;
+    const float p = side == 0 ? px : py, dis = side == 0 ? disx[k] : disy[k];
+    const float t0d = side == 0 ? t0x : t0y;
+    const float dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
+    const float rp = 1.0f / p, base = t0d + t0d * dn;
+    const float rhs = zn + dn * zn, half = 0.5f * dn - 0.5f;
+    float dt = dn + (dis - base) * rp;
+    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
+    dt = dn + (-dis - base) * rp;
+    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
+  }
+  dtau[k] = best;
+}
+
+/* FP32 phase iterations; returns the iteration count (FWD_MAXITER when the
+ * stop rule never held). */
+static int fwd_iterate32(int nx, int ny, float rdx, float rdy, const float *f32,
+                         const unsigned char *ischange, float *tau, float *old) {
+  const size_t n = (size_t)nx * ny;
+  const float *a = f32, *b = f32 + n, *c = f32 + 2 * n, *f = f32 + 3 * n,
+              *disx = f32 + 4 * n, *disy = f32 + 5 * n, *T0 = f32 + 6 * n,
+              *T0x = f32 + 7 * n, *T0y = f32 + 8 * n;
+  for (int iter = 1; iter <= FWD_MAXITER; ++iter) {
+    memcpy(old, tau, n * sizeof(float));
+    for (int isw = 1; isw <= 4; ++isw) {
+      const int xd = isw <= 2 ? 1 : -1;
+      const int yd = (isw == 1 || isw == 3) ? 1 : -1;
+      for (int ii = 0; ii < nx; ++ii) {
+        const int i = xd > 0 ? ii : nx - 1 - ii;
+        for (int jj = 0; jj < ny; ++jj) {
+          const int j = yd > 0 ? jj : ny - 1 - jj;
+          if (ischange[IDX(i, j)])
+            fwd_cell32(xd, yd, nx, ny, i, j, rdx, rdy, a, b, c, f, disx, disy, T0, T0x, T0y,
+                       tau);
+        }
+      }
+    }
+    double l1 = 0, linf = 0;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        const size_t k = IDX(i, j);
+        const double d = fabs((double)tau[k] - (double)old[k]) * (double)T0[k];
+        l1 = l1 + d;
+        linf = linf > d ? linf : d;
+      }
+    l1 = l1 / (double)nx / (double)ny;
+    if (fabs(l1) < FWD_TOL && linf < FWD_TOL) return iter;
+  }
+  return FWD_MAXITER;
+}
+
+static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
+                            int ny, const double *spha, const double *sphb,
+                            const double *sphc, const double *fun, double x0_deg,
+                            double y0_deg, const double *abc_in,
+                            const double *T0_in, const double *T0x_in,
+                            const double *T0y_in, double *T, int *iters) {
+  if (nx < 2 || ny < 2) return ORC_E_ARG;
+  const size_t n = (size_t)nx * ny;
+  double *xx = malloc(sizeof(double) * nx), *yy = malloc(sizeof(double) * ny);
+  double *buf = malloc(sizeof(double) * n * 8);
+  float *f32 = malloc(sizeof(float) * n * 11);
+  unsigned char *isc = malloc(n);
+  if (!xx || !yy || !buf || !f32 || !isc) {
+    free(xx); free(yy); free(buf); free(f32); free(isc);
+    return ORC_E_NOMEM;
+  }
+  double *a = buf, *b = buf + n, *c = buf + 2 * n, *T0 = buf + 3 * n,
+         *T0x = buf + 4 * n, *T0y = buf + 5 * n, *tau = buf + 6 * n,
+         *tau_old = buf + 7 * n;
+  float *tau32 = f32 + 9 * n, *old32 = f32 + 10 * n;
+  for (int i = 0; i < nx; ++i) xx[i] = deg2rad_ref(xx_deg[i]);
+  for (int j = 0; j < ny; ++j) yy[j] = deg2rad_ref(yy_deg[j]);
+  const double x0 = deg2rad_ref(x0_deg), y0 = deg2rad_ref(y0_deg);
+  const double dx = xx[1] - xx[0], dy = yy[1] - yy[0];
+  if (abc_in) {
+    memcpy(a, abc_in, n * sizeof(double));
+    memcpy(b, abc_in + n, n * sizeof(double));
+    memcpy(c, abc_in + 2 * n, n * sizeof(double));
+  } else {
+    metric_abc(nx, ny, yy, spha, sphb, sphc, a, b, c);
+  }
+  int st = ORC_OK;
+  if (T0_in) {
+    memcpy(T0, T0_in, n * sizeof(double));
+    memcpy(T0x, T0x_in, n * sizeof(double));
+    memcpy(T0y, T0y_in, n * sizeof(double));
+  } else {
+    double fun0;
+    st = fwd_source(nx, ny, xx, yy, fun, x0, y0, dx, dy, &fun0);
+    if (st == ORC_OK) fwd_t0_fields(nx, ny, xx, yy, x0, y0, fun0, T0, T0x, T0y);
+  }
+  if (st == ORC_OK) st = fwd_tau_init(nx, ny, xx, yy, x0, y0, dx, dy, tau, isc);
+  int it32 = 0, it64 = 0;
+  if (st == ORC_OK) {
+    for (size_t k = 0; k < n; ++k) {
+      const double ak = a[k], bk = b[k], ck = c[k], fk = fun[k];
+      f32[k] = (float)ak;
+      f32[n + k] = (float)bk;
+      f32[2 * n + k] = (float)ck;
+      f32[3 * n + k] = (float)fk;
+      f32[4 * n + k] = (float)sqrt(fk * fk * (bk / (ak * bk - ck * ck)));
+      f32[5 * n + k] = (float)sqrt(fk * fk * (ak / (ak * bk - ck * ck)));
+      f32[6 * n + k] = (float)T0[k];
+      f32[7 * n + k] = (float)T0x[k];
+      f32[8 * n + k] = (float)T0y[k];
+      tau32[k] = (float)(tau[k] - 1.0);
+    }
+    it32 = fwd_iterate32(nx, ny, (float)(1.0 / dx), (float)(1.0 / dy), f32, isc, tau32, old32);
+    for (size_t k = 0; k < n; ++k) tau[k] = 1.0 + (double)tau32[k];
+    st = fwd_iterate(nx, ny, dx, dy, a, b, c, fun, T0, T0x, T0y, tau, isc, tau_old, T, &it64);
+  }
+  if (iters) { iters[0] = it32; iters[1] = it64; }
+  free(xx); free(yy); free(buf); free(f32); free(isc);
+  return st;
+}
+
+int orc_fwd_mixed(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                  const double *spha, const double *sphb, const double *sphc,
+                  const double *fun, double x0_deg, double y0_deg, double *T,
+                  int *iters) {
+  return fwd_mixed_common(xx_deg, yy_deg, nx, ny, spha, sphb, sphc, fun, x0_deg,
+                          y0_deg, NULL, NULL, NULL, NULL, T, iters);
+}
+
+int orc_fwd_mixed_pre(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                      const double *a, const double *b, const double *c,
+                      const double *fun, double x0_deg, double y0_deg,
+                      const double *T0, const double *T0x, const double *T0y,
+                      double *T, int *iters) {
+  const size_t n = (size_t)nx * ny;
+  double *abc = malloc(sizeof(double) * 3 * n);
+  if (!abc) return ORC_E_NOMEM;
+  memcpy(abc, a, n * sizeof(double));
+  memcpy(abc + n, b, n * sizeof(double));
+  memcpy(abc + 2 * n, c, n * sizeof(double));
+  const int st = fwd_mixed_common(xx_deg, yy_deg, nx, ny, NULL, NULL, NULL, fun, x0_deg,
+                                  y0_deg, abc, T0, T0x, T0y, T, iters);
+  free(abc);
+  return st;
+}
+
+/* FP32 phase of the mixed adjoint (see the note above).  Ta holds the
+ * reference's initial field on entry and the FP32 result, widened, on exit.
+ * Returns the iteration count or a negative status; *done tells whether the
+ * stop rule held. */
+static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
+                       const double *a1p, const double *a2m, const double *a2p,
+                       const double *b1m, const double *b1p, const double *b2m,
+                       const double *b2p, const double *delta, double *Ta, int *done_out) {
+  const size_t n = (size_t)nx * ny;
+  float *kk = malloc(sizeof(float) * n * 7);
+  if (!kk) return ORC_E_NOMEM;
+  float *k0 = kk, *kW = kk + n, *kE = kk + 2 * n, *kS = kk + 3 * n, *kN = kk + 4 * n,
+        *ta = kk + 5 * n, *old = kk + 6 * n;
+  for (size_t k = 0; k < n; ++k) {
+    k0[k] = kW[k] = kE[k] = kS[k] = kN[k] = 0;
+    ta[k] = (float)Ta[k];
+  }
+  for (int i = 1; i < nx - 1; ++i)
+    for (int j = 1; j < ny - 1; ++j) {
+      const size_t k = IDX(i, j);
+      const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
+      if (fabs(d) < ADJ_EPS) continue;
+      k0[k] = (float)(delta[k] / d);
+      kW[k] = (float)(a1p[k] / dx / d);
+      kE[k] = (float)(-a2m[k] / dx / d);
+      kS[k] = (float)(b1p[k] / dy / d);
+      kN[k] = (float)(-b2m[k] / dy / d);
+    }
+  int iter, done = 0;
+  for (iter = 1; iter <= ADJ_MAXITER; ++iter) {
+    memcpy(old, ta, n * sizeof(float));
+    for (int xdir = -1; xdir <= 1; xdir += 2)
+      for (int ydir = -1; ydir <= 1; ydir += 2)
+        for (int ii = 1; ii < nx - 1; ++ii) {
+          const int i = xdir < 0 ? ii : nx - 1 - ii;
+          for (int jj = 1; jj < ny - 1; ++jj) {
+            const int j = ydir < 0 ? jj : ny - 1 - jj;
+            const size_t k = IDX(i, j);
+            ta[k] = k0[k] + kW[k] * ta[IDX(i - 1, j)] + kE[k] * ta[IDX(i + 1, j)] +
+                    kS[k] * ta[IDX(i, j - 1)] + kN[k] * ta[IDX(i, j + 1)];
+          }
+        }
+    double l1 = 0, linf = 0, amax = 0;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        const size_t k = IDX(i, j);
+        const double dd = fabs((double)ta[k] - (double)old[k]);
+        const double av = fabs((double)ta[k]);
+        l1 = l1 + dd;
+        linf = linf > dd ? linf : dd;
+        amax = amax > av ? amax : av;
+      }
+    l1 = l1 / (double)(nx * ny);
+    if ((fabs(l1) < ADJ_TOL && linf < ADJ_TOL) || linf <= (double)MIX_REL_STOP * amax) {
+      done = 1;
+      break;
+    }
+  }
+  if (!done) iter = ADJ_MAXITER;
+  for (size_t k = 0; k < n; ++k) Ta[k] = (double)ta[k];
+  free(kk);
+  *done_out = done;
+  return iter;
+}
+
+int orc_adj_mixed(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                  const double *spha, const double *sphb, const double *sphc,
+                  const double *T, const double *xrec_deg, const double *yrec_deg,
+                  const double *src_adj, int nr, double *Ta, int *iters) {
+  return adj_common(xx_deg, yy_deg, nx, ny, spha, sphb, sphc, T, xrec_deg,
+                    yrec_deg, src_adj, nr, NULL, 1, Ta, iters);
+}
+
+int orc_adj_mixed_pre(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                      const double *a, const double *b, const double *c,
+                      const double *T, const double *delta, double *Ta, int *iters) {
+  const size_t n = (size_t)nx * ny;
+  double *pre = malloc(sizeof(double) * 4 * n);
+  if (!pre) return ORC_E_NOMEM;
+  memcpy(pre, a, n * sizeof(double));
+  memcpy(pre + n, b, n * sizeof(double));
+  memcpy(pre + 2 * n, c, n * sizeof(double));
+  memcpy(pre + 3 * n, delta, n * sizeof(double));
+  const int st = adj_common(xx_deg, yy_deg, nx, ny, NULL, NULL, NULL, T, NULL,
+                            NULL, NULL, 0, pre, 1, Ta, iters);
+  free(pre);
   return st;
 }

@@ -70,6 +70,27 @@ int orc_adj_lonlat_pre(const double *xx_deg, const double *yy_deg, int nx,
                        const double *c, const double *T, const double *delta,
                        double *Ta, int *iters);
 
+/* The device's mixed-precision mode, restated (not a reference algorithm):
+ * an FP32 sweep phase followed by FP64 polish iterations with the
+ * reference's update and stop rule (satt_oracle.c describes both phases).
+ * iters[0] = FP32 iterations, iters[1] = FP64 polish iterations. */
+int orc_fwd_mixed(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                  const double *spha, const double *sphb, const double *sphc,
+                  const double *fun, double x0_deg, double y0_deg, double *T,
+                  int *iters);
+int orc_fwd_mixed_pre(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                      const double *a, const double *b, const double *c,
+                      const double *fun, double x0_deg, double y0_deg,
+                      const double *T0, const double *T0x, const double *T0y,
+                      double *T, int *iters);
+int orc_adj_mixed(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                  const double *spha, const double *sphb, const double *sphc,
+                  const double *T, const double *xrec_deg, const double *yrec_deg,
+                  const double *src_adj, int nr, double *Ta, int *iters);
+int orc_adj_mixed_pre(const double *xx_deg, const double *yy_deg, int nx, int ny,
+                      const double *a, const double *b, const double *c,
+                      const double *T, const double *delta, double *Ta, int *iters);
+
 /* src/eikon_solver_2d.f90:1449-1467  mask */
 void orc_mask(const double *xx, const double *yy, int nx, int ny,
               double *table, double x0, double y0);

@@ -412,6 +412,14 @@ extern "C" int satt_adj_setup_fields(satt_ctx *ctx, const double *xx_deg,
 
 // ------------------------------------------------------------ batched plan
 
+// Iteration counts per item: forward, adjoint, density adjoint (FP64), then
+// the same three for the FP32 phase of the mixed mode.
+constexpr int kIterInts = 6;
+
+// Stages timed by the plan's events.
+enum : int { kStFwdSetup, kStFwd64, kStFwd32, kStAdjSetup, kStAdj64, kStAdj32, kStOther,
+             kStCount };
+
 struct satt_plan {
   satt_ctx *ctx = nullptr;
   int dim = 2, nx = 0, ny = 0, np = 0, nitems = 0, npath = 0, nrec = 0, precision = 0;
@@ -429,6 +437,7 @@ struct satt_plan {
   DBuf lists;   // per chunk: per_off (np+1), per_slot (chunk)
   DBuf fwdw, adjw;  // one-solve-per-launch scratch (one slot)
   DBuf pskew;   // per-period skewed fields (strip pipeline)
+  DBuf pskew32; // the same rounded to float (mixed precision)
   DBuf strip;   // per-slot strip scratch: skew fields, tau, snap, T, Ta, Td, delta
   SkewGeom sg{};
   PeriodFields pf{};
@@ -439,7 +448,9 @@ struct satt_plan {
   std::vector<int> h_per_off, h_per_slot;  // host copies of the chunk lists
   int *d_per_off = nullptr, *d_per_slot = nullptr;
   std::vector<hipEvent_t> ev;
+  std::vector<int> marks;  // stage of the interval ev[k]..ev[k+1]
   int last_istotable = 0, last_isadj = 0, ran = 0;
+  int last_item0 = 0, last_count = 0, last_batched = 0;  // resident slots
   satt_plan_stats stats{};
   ~satt_plan() {
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -469,7 +480,9 @@ int plan_period_fields(satt_plan *P) {
   P->pf.c = f + 2 * M;
   P->pf.fun = f + 3 * M;
   P->pf.vel = f + 4 * M;
-  if (P->pskew.p) launch_period_skew(ctx->stream, P->dg.g, P->sg, P->np, P->pf, P->pskew.as<double>());
+  if (P->pskew.p)
+    launch_period_skew(ctx->stream, P->dg.g, P->sg, P->np, P->pf, P->pskew.as<double>(),
+                       P->pskew32.p ? P->pskew32.as<float>() : nullptr);
   HIPCHK(ctx, hipGetLastError());
   return SATT_OK;
 }
@@ -515,8 +528,10 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
   int st;
   if ((st = make_grid(ctx, B->xgrids, B->ygrids, B->nx, B->ny, P->dg)) < 0) return bail(st);
   P->sg = make_skew_geom(B->nx, B->ny);
+  const size_t pskew_elems = (size_t)P->np * 2 * kPF * P->sg.skew_elems();
   if (P->model.alloc(4 * M * 8) != hipSuccess || P->pfields.alloc(5 * M * 8) != hipSuccess ||
-      P->pskew.alloc((size_t)P->np * 2 * kPF * P->sg.skew_elems() * 8) != hipSuccess)
+      P->pskew.alloc(pskew_elems * 8) != hipSuccess ||
+      (precision == SATT_PREC_MIXED && P->pskew32.alloc(pskew_elems * 4) != hipSuccess))
     return bail(fail(ctx, SATT_E_NOMEM, "model allocation failed"));
   if ((st = satt_plan_set_model(P, B->svel, B->m11, B->m12, B->m22)) < 0) return bail(st);
 
@@ -555,7 +570,7 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
     P->it.idx_path = reinterpret_cast<int *>(d + 4 * (size_t)nrec);
   }
   if (P->rec_out.alloc(4 * (size_t)(nrec + 1) * 8) != hipSuccess ||
-      P->item_out.alloc((size_t)(ni + 1) * (8 + 16)) != hipSuccess ||
+      P->item_out.alloc((size_t)(ni + 1) * (8 + 4 * kIterInts + 4)) != hipSuccess ||
       P->tt_fwd.alloc((size_t)(P->npath + 1) * 8) != hipSuccess ||
       P->acc.alloc((P->dim == 3 ? 4 : 2) * M * 8) != hipSuccess)
     return bail(fail(ctx, SATT_E_NOMEM, "output allocation failed"));
@@ -567,7 +582,7 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
     P->rhs_den = d + 3 * (size_t)(nrec + 1);
     P->item_chi = P->item_out.as<double>();
     P->iters = reinterpret_cast<int *>(P->item_chi + ni + 1);
-    P->status = P->iters + 3 * (size_t)(ni + 1);
+    P->status = P->iters + (size_t)kIterInts * (ni + 1);
   }
 
   // chunk size from free device memory
@@ -616,7 +631,7 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
       (long long)(P->dg.buf.bytes + P->model.bytes + P->pfields.bytes + P->items.bytes +
                   P->recs.bytes + P->rec_out.bytes + P->item_out.bytes + P->tt_fwd.bytes +
                   P->acc.bytes + P->lists.bytes + P->fwdw.bytes + P->adjw.bytes +
-                  P->pskew.bytes + P->strip.bytes);
+                  P->pskew.bytes + P->pskew32.bytes + P->strip.bytes);
   *out = P;
   return SATT_OK;
 }
@@ -641,22 +656,34 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   satt_ctx *ctx = P->ctx;
   if (launch_form != SATT_LAUNCH_BATCHED && launch_form != SATT_LAUNCH_PER_SOLVE)
     return fail(ctx, SATT_E_ARG, "satt_plan_run: unknown launch form");
+  const bool mixed = P->precision == SATT_PREC_MIXED;
+  if (mixed && launch_form != SATT_LAUNCH_BATCHED)
+    return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form is FP64 only");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const size_t N = P->N, M = (size_t)P->np * N;
   const int ni = P->nitems;
-  // events: start; per launch group: forward start/end, adjoint start/end;
-  // end.  They bracket the solver kernels only (not the receiver and
-  // stacking kernels), on the stream the kernels run on.
+  // Events on the kernels' stream: one at the start and one after every
+  // launch, each interval tagged with the stage it times.
   const int groups = launch_form == SATT_LAUNCH_BATCHED ? P->nchunks : ni;
-  const size_t nev = 2 + 4 * (size_t)groups;
-  while (P->ev.size() < nev) {
+  if (P->ev.empty()) {
     hipEvent_t e;
     HIPCHK(ctx, hipEventCreate(&e));
     P->ev.push_back(e);
   }
+  P->marks.clear();
+  auto mark = [&](int stage) -> hipError_t {
+    P->marks.push_back(stage);
+    while (P->ev.size() <= P->marks.size()) {  // the pool grows as launches are added
+      hipEvent_t e;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      P->ev.push_back(e);
+    }
+    return hipEventRecord(P->ev[P->marks.size()], s);
+  };
   HIPCHK(ctx, hipMemsetAsync(P->status, 0, (size_t)(ni + 1) * 4, s));
-  HIPCHK(ctx, hipMemsetAsync(P->iters, 0, (size_t)(ni + 1) * 12, s));
+  HIPCHK(ctx, hipMemsetAsync(P->iters, 0, (size_t)(ni + 1) * 4 * kIterInts, s));
   HIPCHK(ctx, hipMemsetAsync(P->item_chi, 0, (size_t)(ni + 1) * 8, s));
   HIPCHK(ctx, hipMemsetAsync(P->tt_fwd.p, 0, P->tt_fwd.bytes, s));
   if (isadj) HIPCHK(ctx, hipMemsetAsync(P->acc.p, 0, P->acc.bytes, s));
@@ -689,7 +716,6 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
     fwk.T = sw.T;
   }
   double *acc_s = P->acc.as<double>(), *acc_d = acc_s + M;
-  int ei = 1;
   for (int g0 = 0; g0 < groups; ++g0) {
     Items it = P->it;
     const int *per_off, *per_slot;
@@ -704,34 +730,52 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
       per_off = nullptr;
       per_slot = nullptr;
     }
-    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
-    if (batched)
-      launch_fwd_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), P->pf, it, sw, P->iters + 0,
-                       P->status, 3);
-    else
-      launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, 3);
-    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
+    const double *pk = P->pskew.as<double>();
+    if (batched) {
+      launch_fwd_setup(s, P->dg.g, P->sg, P->pf, it, sw, P->status, mixed);
+      HIPCHK(ctx, mark(kStFwdSetup));
+    }
+    if (batched && mixed) {
+      launch_fwd_mixed32(s, P->dg.g, P->sg, P->pskew32.as<float>(), P->pf, it, sw, P->iters + 3,
+                         P->status, kIterInts);
+      HIPCHK(ctx, mark(kStFwd32));
+      launch_fwd_polish(s, P->dg.g, P->sg, pk, P->pf, it, sw, P->iters + 0, P->status, kIterInts);
+    } else if (batched) {
+      launch_fwd_strip(s, P->dg.g, P->sg, pk, P->pf, it, sw, P->iters + 0, P->status, kIterInts);
+    } else {
+      launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, kIterInts);
+    }
+    HIPCHK(ctx, mark(kStFwd64));
     RecWork rw{fwk.T, P->tekio, P->weight_out, P->rhs_adj, P->rhs_den, P->item_chi,
                istotable ? P->tt_fwd.as<double>() : nullptr};
     launch_receivers(s, P->dg.g, it, rw, P->status);
-    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
+    HIPCHK(ctx, mark(kStOther));
     if (isadj) {
-      if (batched) {
-        launch_adj_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), it, sw, P->rhs_adj, kNatTa,
-                         P->iters + 1, P->status, 3, 1);
-        if (P->dim == 3)
-          launch_adj_strip(s, P->dg.g, P->sg, P->pskew.as<double>(), it, sw, P->rhs_den, kNatTd,
-                           P->iters + 2, P->status, 3, 1);
-      } else {
-        AdjWork awk{fwk.T, Ta, coef, delta, old, P->rhs_adj};
-        launch_adjoint(s, P->dg.g, P->pf, it, awk, P->iters + 1, P->status, 3, 1);
-        if (P->dim == 3) {
-          AdjWork dwk{fwk.T, Td, coef, delta, old, P->rhs_den};
-          launch_adjoint(s, P->dg.g, P->pf, it, dwk, P->iters + 2, P->status, 3, 1);
+      for (int q = 0; q < (P->dim == 3 ? 2 : 1); ++q) {  // adjoint, then density adjoint
+        const double *rhs = q == 0 ? P->rhs_adj : P->rhs_den;
+        const int nat = q == 0 ? kNatTa : kNatTd;
+        if (batched) {
+          launch_adj_setup(s, P->dg.g, P->sg, pk, it, sw, rhs, P->iters + 1 + q, P->status,
+                           kIterInts, mixed ? 2 : 0);
+          HIPCHK(ctx, mark(kStAdjSetup));
         }
+        if (batched && mixed) {
+          launch_adj_mixed32(s, P->dg.g, P->sg, it, sw, P->iters + 4 + q, P->status, kIterInts);
+          HIPCHK(ctx, mark(kStAdj32));
+          launch_adj_build(s, P->dg.g, P->sg, pk, it, sw, P->status, 1);
+          HIPCHK(ctx, mark(kStAdjSetup));
+          launch_adj_polish(s, P->dg.g, P->sg, it, sw, nat, P->iters + 1 + q, P->status,
+                            kIterInts, 1);
+        } else if (batched) {
+          launch_adj_strip(s, P->dg.g, P->sg, it, sw, nat, P->iters + 1 + q, P->status, kIterInts,
+                           1);
+        } else {
+          AdjWork awk{fwk.T, q == 0 ? Ta : Td, coef, delta, old, rhs};
+          launch_adjoint(s, P->dg.g, P->pf, it, awk, P->iters + 1 + q, P->status, kIterInts, 1);
+        }
+        HIPCHK(ctx, mark(kStAdj64));
       }
     }
-    HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
     if (isadj) {
       const int power = P->dim == 3 ? 3 : 2;
       if (per_off) {
@@ -742,17 +786,21 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
         if (P->dim == 3) launch_stack_item(s, P->dg.g, P->pf, P->it.pidx, g0, Td, acc_d, power);
       }
     }
+    HIPCHK(ctx, mark(kStOther));
   }
   if (isadj) {
     double *pm = P->acc.as<double>() + (P->dim == 3 ? 2 : 1) * M;
     launch_to_period_major(s, (int)N, P->np, acc_s, pm);
     if (P->dim == 3) launch_to_period_major(s, (int)N, P->np, acc_d, pm + M);
   }
-  HIPCHK(ctx, hipEventRecord(P->ev[ei++], s));
+  HIPCHK(ctx, mark(kStOther));
   HIPCHK(ctx, hipGetLastError());
   P->last_istotable = istotable;
   P->last_isadj = isadj;
   P->ran = groups;
+  P->last_batched = batched;
+  P->last_item0 = groups > 0 ? (batched ? (groups - 1) * P->chunk : ni - 1) : 0;
+  P->last_count = groups > 0 ? (batched ? ni - P->last_item0 : 1) : 0;
   P->stats.chunk = cap;
   P->stats.launches = groups;
   return SATT_OK;
@@ -771,39 +819,54 @@ extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
   TRY(sync_status(ctx));
   const int ni = P->nitems;
   const size_t M = (size_t)P->np * P->N;
-  // timing
+  // timing: the intervals between consecutive events, by stage
   {
     float ms = 0;
-    double f = 0, a = 0;
-    for (int g0 = 0; g0 < P->ran; ++g0) {
-      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[1 + 4 * g0], P->ev[2 + 4 * g0]));
-      f += ms;
-      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[3 + 4 * g0], P->ev[4 + 4 * g0]));
-      a += ms;
+    double st_ms[kStCount] = {};
+    for (size_t k = 0; k < P->marks.size(); ++k) {
+      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[k], P->ev[k + 1]));
+      st_ms[P->marks[k]] += ms;
     }
-    HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[0], P->ev[1 + 4 * P->ran]));
-    P->stats.ms_forward = f;
-    P->stats.ms_adjoint = a;
+    ms = 0;
+    if (!P->marks.empty())
+      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[0], P->ev[P->marks.size()]));
+    P->stats.ms_fwd_fp64 = st_ms[kStFwd64];
+    P->stats.ms_fwd_fp32 = st_ms[kStFwd32];
+    P->stats.ms_adj_fp64 = st_ms[kStAdj64];
+    P->stats.ms_adj_fp32 = st_ms[kStAdj32];
+    P->stats.ms_fwd_setup = st_ms[kStFwdSetup];
+    P->stats.ms_adj_setup = st_ms[kStAdjSetup];
+    P->stats.ms_forward = st_ms[kStFwdSetup] + st_ms[kStFwd64] + st_ms[kStFwd32];
+    P->stats.ms_adjoint = st_ms[kStAdjSetup] + st_ms[kStAdj64] + st_ms[kStAdj32];
     P->stats.ms_total = ms;
   }
-  std::vector<int> st((size_t)ni + 1), its(3 * (size_t)ni + 3);
+  std::vector<int> st((size_t)ni + 1), its((size_t)kIterInts * (ni + 1));
   std::vector<double> chi((size_t)ni + 1);
   HIPCHK(ctx, hipMemcpy(st.data(), P->status, (size_t)(ni + 1) * 4, hipMemcpyDeviceToHost));
-  HIPCHK(ctx, hipMemcpy(its.data(), P->iters, (size_t)(ni + 1) * 12, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(its.data(), P->iters, its.size() * 4, hipMemcpyDeviceToHost));
   HIPCHK(ctx, hipMemcpy(chi.data(), P->item_chi, (size_t)(ni + 1) * 8, hipMemcpyDeviceToHost));
-  long long fs = 0, as = 0;
+  long long fs = 0, as = 0, fs32 = 0, as32 = 0;
   int worst = SATT_OK, worst_item = -1;
   for (int i = 0; i < ni; ++i) {
-    fs += 4LL * its[3 * i];
-    as += 4LL * (its[3 * i + 1] + its[3 * i + 2]);
+    const int *q = &its[(size_t)kIterInts * i];
+    fs += 4LL * q[0];
+    as += 4LL * (q[1] + q[2]);
+    fs32 += 4LL * q[3];
+    as32 += 4LL * (q[4] + q[5]);
     if (st[i] < 0 && worst >= 0) { worst = st[i]; worst_item = i; }
     else if (st[i] > 0 && worst == SATT_OK) { worst = st[i]; worst_item = i; }
   }
   P->stats.fwd_sweeps = fs;
   P->stats.adj_sweeps = as;
+  P->stats.fwd_sweeps32 = fs32;
+  P->stats.adj_sweeps32 = as32;
   if (R) {
     if (R->item_status) std::memcpy(R->item_status, st.data(), (size_t)ni * 4);
-    if (R->iters) std::memcpy(R->iters, its.data(), (size_t)ni * 12);
+    for (int i = 0; i < ni; ++i)
+      for (int q = 0; q < 3; ++q) {
+        if (R->iters) R->iters[3 * i + q] = its[(size_t)kIterInts * i + q];
+        if (R->iters_fp32) R->iters_fp32[3 * i + q] = its[(size_t)kIterInts * i + 3 + q];
+      }
     if (R->item_chi) std::memcpy(R->item_chi, chi.data(), (size_t)ni * 8);
     if (R->chi) {
       // chi_local = chi_local + ma%chi over items in order (src/tomo2d.f90:193)
@@ -832,6 +895,32 @@ extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
     return fail(ctx, worst, "item %d: %s", worst_item, status_text(worst));
   if (worst > 0) fail(ctx, worst, "item %d: %s", worst_item, status_text(worst));
   return worst;
+}
+
+extern "C" int satt_plan_item_field(satt_plan *P, int item, int which, double *out) {
+  if (!P) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  if (!out || which < SATT_FIELD_T || which > SATT_FIELD_TD ||
+      (which == SATT_FIELD_TD && P->dim != 3))
+    return fail(ctx, SATT_E_ARG, "satt_plan_item_field: bad field");
+  if (!P->ran || item < P->last_item0 || item >= P->last_item0 + P->last_count)
+    return fail(ctx, SATT_E_ARG, "satt_plan_item_field: item %d is not resident", item);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  TRY(sync_status(ctx));
+  const size_t N = P->N;
+  const double *src;
+  if (P->last_batched) {
+    const int cap = P->chunk;
+    const size_t SE = P->sg.skew_elems();
+    const double *nat = P->strip.as<double>() + (size_t)kSlotSkews * SE * cap;
+    const int nat_field = which == SATT_FIELD_T ? kNatT : which == SATT_FIELD_TA ? kNatTa : kNatTd;
+    src = nat + ((size_t)nat_field * cap + (item - P->last_item0)) * N;
+  } else {
+    const double *fw = P->fwdw.as<double>(), *aw = P->adjw.as<double>();
+    src = which == SATT_FIELD_T ? fw : which == SATT_FIELD_TA ? aw : aw + N;
+  }
+  HIPCHK(ctx, hipMemcpy(out, src, N * 8, hipMemcpyDeviceToHost));
+  return SATT_OK;
 }
 
 extern "C" int satt_plan_get_stats(satt_plan *P, satt_plan_stats *S) {

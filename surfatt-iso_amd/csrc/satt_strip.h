@@ -31,6 +31,10 @@ inline SkewGeom make_skew_geom(int nx, int ny) {
 //   skew fields 12,13 : travel time T, plus and minus skew (forward output)
 //   skew fields 14..16: Ta plus, Ta minus, snapshot (adjoint)
 //   natural 0..3      : T, Ta, Td (density adjoint), delta source
+// The mixed mode keeps its float fields inside the same slot: the forward's
+// from double skew 9 (free until the output T skews are written), the
+// adjoint's from skew 0 (its FP64 coefficients are built after the FP32
+// phase has ended).
 constexpr int kSlotSkews = 17;
 constexpr int kSlotNat = 4;
 enum : int { kSkTauP = 6, kSkTauM = 7, kSkSnapF = 8, kSkTP = 12, kSkTM = 13,
@@ -42,14 +46,42 @@ struct StripWork {
   double *T, *Ta, *Td, *delta; // natural layout, [slot][N] each
 };
 
+// out32 (may be null): the same fields rounded to float, same layout.
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
-                        const PeriodFields &pf, double *out);
+                        const PeriodFields &pf, double *out, float *out32);
+
+// Forward: setup (T0 skews, tau init; one thread per element), then the
+// sweeps: k_fwd_strip (FP64), or k_fwd_mixed32 (FP32 phase, iteration count
+// to iters32) followed by k_fwd_polish (FP64 to the reference's stop rule).
+void launch_fwd_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
+                      const Items &it, const StripWork &w, int *status, bool mixed);
 void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
                       const PeriodFields &pf, const Items &it, const StripWork &w,
                       int *iters, int *status, int iter_stride);
-// out_nat: kNatTa (adjoint) or kNatTd (density adjoint); rhs per receiver.
-void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const Items &it, const StripWork &w, const double *rhs, int out_nat,
-                      int *iters, int *status, int iter_stride, int apply_mask);
+void launch_fwd_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const float *pskew32,
+                        const PeriodFields &pf, const Items &it, const StripWork &w,
+                        int *iters32, int *status, int iter_stride);
+void launch_fwd_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                       const PeriodFields &pf, const Items &it, const StripWork &w, int *iters,
+                       int *status, int iter_stride);
+
+// Adjoint: setup (delta source from rhs per receiver, then the coefficients:
+// kind 0 FP64 with the Ta init, 2 FP32 with its Ta init), then the sweeps:
+// k_adj_strip (FP64), or k_adj_mixed32 followed by launch_adj_build(kind 1)
+// and k_adj_polish (one FP64 iteration).  out_nat: kNatTa (adjoint) or
+// kNatTd (density adjoint).
+void launch_adj_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                      const Items &it, const StripWork &w, const double *rhs, int *iters,
+                      int *status, int iter_stride, int kind);
+void launch_adj_build(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                      const Items &it, const StripWork &w, const int *status, int kind);
+void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
+                      const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
+                      int apply_mask);
+void launch_adj_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
+                        const StripWork &w, int *iters32, int *status, int iter_stride);
+void launch_adj_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
+                       const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
+                       int apply_mask);
 
 }  // namespace satt

@@ -1,4 +1,4 @@
-// Batched FP64 sweeps for gfx950: one wavefront per (source, period) item.
+// Batched sweeps for gfx950: one wavefront per (source, period) item.
 //
 // Strip pipeline.  The grid is cut into strips of 64 columns; lane l of the
 // wave owns column i = 64c+l of strip c.  For a sweep in direction (xd, yd),
@@ -18,7 +18,7 @@
 // There is no barrier: one wave, in-order lanes.
 //
 // Layout.  Every per-cell field a sweep touches is stored skewed so that the
-// 64 cells of one step are one contiguous 512-byte row:
+// 64 cells of one step are one contiguous row (512 bytes in FP64):
 //   plus skew  (sweeps with xd == yd):  row r = j + l
 //   minus skew (sweeps with xd != yd):  row r = j - l + 63
 // each as [strip][row 0..ny+62][lane 0..63], zero padded.  Read-only
@@ -26,7 +26,31 @@
 // the same family as the previous one updates its skew in place; the first
 // sweep after a family change reads one skew and writes the other.  Natural
 // layout (i + nx*j) is used only for the solve's inputs and outputs.
+//
+// Precision.  The FP64 kernels (k_fwd_strip, k_adj_strip) run the
+// reference's arithmetic.  The mixed mode runs an FP32 phase
+// (k_fwd_mixed32, k_adj_mixed32) and then an FP64 polish (k_fwd_polish,
+// k_adj_polish) in separate launches, so that the FP32 phase runs at the
+// occupancy of its own (smaller) register footprint.
+//   Forward FP32 phase: the reference's update with float operands, each
+//   sweep evaluating only the candidates whose neighbours are upwind in its
+//   direction, written in the increment u = tau - tau_nb: with tt = tX + u, Tx = gx + px*u and
+//   Ty = gy + py*u where gx = T0x*tX, gy = T0y*tX + sy*(T0/dy)*(tX - tY), so
+//   the quadratic a*Tx^2 + b*Ty^2 - 2c*Tx*Ty = fun^2 has coefficients of the
+//   size of the eikonal residual instead of (T0/dx)^2 (whose cancellation
+//   would leave about two significant digits in float).  Roots by
+//   q = -(eb + sign(eb)*sqrt(disc))/2, u = q/ea and ec/q.  The field holds
+//   dtau = tau - 1, which keeps 3-4 more bits than tau near 1.  Causality
+//   tests, acceptance, sweep order and the stop rule (residual accumulated in
+//   double) are the reference's.  Polish: tau = 1 + dtau, then the
+//   reference's FP64 iterations to its stop rule.
+//   Adjoint FP32 phase: Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN with
+//   k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d
+//   formed in FP64 and rounded to float; stop when the reference's rule
+//   holds or the largest change is at most 2^-21 of the largest |Ta|.
+//   Polish: one FP64 iteration of the reference's update.
 #include <climits>
+#include <type_traits>
 
 #include "satt_internal.h"
 #include "satt_strip.h"
@@ -34,22 +58,30 @@
 namespace satt {
 namespace {
 
+constexpr float kMixRelStop = 4.76837158203125e-07f;  // 2^-21
+
 __device__ __forceinline__ double deg2rad_s(double d) { return d / 180.0 * kPiF; }
 __device__ __forceinline__ float weight_r4_s(double v) { return (float)(v < 1.0 ? v : 1.0); }
-
-__device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }
 
 // Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1 of the GFX9
 // family): lane l receives lane l-1 (SHR) or lane l+1 (SHL); the lane that
 // has no source receives 0.  One VALU op per 32-bit half instead of an LDS
 // permute, which keeps the step-to-step dependency off the LDS latency.
 template <int FROM>  // FROM = -1: take lane l-1; +1: take lane l+1
-__device__ __forceinline__ double lane_shift(double v) {
+__device__ __forceinline__ int dpp_shift(int v) {
   constexpr int ctrl = FROM < 0 ? 0x138 : 0x130;  // wave_shr:1 : wave_shl:1
+  return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xf, 0xf, false);
+}
+template <int FROM>
+__device__ __forceinline__ double lane_shift(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xf, 0xf, false);
+  const int lo = dpp_shift<FROM>((int)(b & 0xffffffffLL));
+  const int hi = dpp_shift<FROM>((int)(b >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int FROM>
+__device__ __forceinline__ float lane_shift(float v) {
+  return __int_as_float(dpp_shift<FROM>(__float_as_int(v)));
 }
 
 __device__ __forceinline__ double wave_sum(double s) {
@@ -166,12 +198,72 @@ __device__ __forceinline__ double fwd_cell_r(int i, int j, int nx, int ny, doubl
   return tnew;
 }
 
+// One cell of the FP32 phase in the sweep direction (XD, YD) (see the
+// precision note at the top).  Only the candidates whose neighbours are
+// upwind in this sweep are evaluated: the quadrant of the neighbours
+// (i-XD, j) and (i, j-YD), and the one-sided candidates from each.  Over the
+// four sweeps of an iteration every candidate of the reference's update is
+// evaluated once with its neighbours' newest values, and a fixed point is a
+// fixed point of the full update.  Operands are offsets dtau = tau - 1; zX, zY
+// are the upwind neighbours' T0.
+template <int XD, int YD>
+__device__ __forceinline__ float fwd_cell32_r(int i, int j, int nx, int ny, float rdx, float rdy,
+                                              float ak, float bk, float ck, float fk, float disx,
+                                              float disy, float t0, float t0x, float t0y,
+                                              float dc, float dX, float dY, float zX, float zY) {
+  const float t0dx = t0 * rdx, t0dy = t0 * rdy;
+  const bool hasX = XD > 0 ? i > 0 : i < nx - 1;
+  const bool hasY = YD > 0 ? j > 0 : j < ny - 1;
+  const float px = XD > 0 ? t0x + t0dx : t0x - t0dx;
+  const float py = YD > 0 ? t0y + t0dy : t0y - t0dy;
+  float best = dc;
+  if (hasX && hasY) {
+    const float gx = t0x + t0x * dX;
+    const float gy = YD > 0 ? t0y + t0y * dX + t0dy * (dX - dY)
+                            : t0y + t0y * dX - t0dy * (dX - dY);
+    const float ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
+    const float eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));
+    const float ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;
+    const float disc = eb * eb - 4 * ea * ec;
+    if (disc >= 0) {
+      const float sq = sqrtf(disc);
+      const float qq = -0.5f * (eb >= 0 ? eb + sq : eb - sq);
+#pragma unroll
+      for (int isol = 1; isol <= 2; ++isol) {
+        const float u = isol == 1 ? qq / ea : ec / qq;
+        const float dt = dX + u;
+        const float Tx = gx + px * u, Ty = gy + py * u;
+        const float chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;
+        const bool ok = (XD > 0 ? chx >= 0 : chx <= 0) && (YD > 0 ? chy >= 0 : chy <= 0);
+        if (ok && dt < best) best = dt;
+      }
+    }
+  }
+  // one-sided candidates (:996-1075): tt*T0 >= tn*T0n and tt > tn*0.5 in
+  // offsets; p is the quadrant's px (x neighbour) or py (y neighbour)
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    if (side == 0 ? !hasX : !hasY) continue;
+    const float p = side == 0 ? px : py, dis = side == 0 ? disx : disy;
+    const float t0d = side == 0 ? t0x : t0y;
+    const float dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
+    const float rp = 1.0f / p, base = t0d + t0d * dn;
+    const float rhs = zn + dn * zn, half = 0.5f * dn - 0.5f;
+    float dt = dn + (dis - base) * rp;
+    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
+    dt = dn + (-dis - base) * rp;
+    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
+  }
+  return best;
+}
+
 // ---------------------------------------------------------- period skews
 
 // Per-period skewed fields: a, b, c, fun, dis_x, dis_y (kPF fields), both
-// skews.  Thread = (strip, row, lane) of one skew of one period.
+// skews, in FP64 and (out32 != null) rounded to float.  Thread = (strip,
+// row, lane) of one skew of one period.
 __global__ void k_period_skew(Grid g, SkewGeom sg, int np, const double *a, const double *b,
-                              const double *c, const double *fun, double *out) {
+                              const double *c, const double *fun, double *out, float *out32) {
   const size_t per_skew = sg.skew_elems();
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const int sk = blockIdx.y, p = blockIdx.z;
@@ -192,6 +284,11 @@ __global__ void k_period_skew(Grid g, SkewGeom sg, int np, const double *a, cons
   double *base = out + ((size_t)p * 2 + sk) * kPF * per_skew;
 #pragma unroll
   for (int f = 0; f < kPF; ++f) base[f * per_skew + e] = v[f];
+  if (out32) {
+    float *b32 = out32 + ((size_t)p * 2 + sk) * kPF * per_skew;
+#pragma unroll
+    for (int f = 0; f < kPF; ++f) b32[f * per_skew + e] = (float)v[f];
+  }
 }
 
 // ------------------------------------------------------------- forward
@@ -211,6 +308,27 @@ __device__ int fwd_fun0_s(const Grid &g, const double *fun, double x0, double y0
   *fun0 = (double)w00 * fun[i0 + (size_t)nx * j0] + (double)w01 * fun[i0 + (size_t)nx * (j0 + 1)] +
           (double)w10 * fun[i0 + 1 + (size_t)nx * j0] +
           (double)w11 * fun[i0 + 1 + (size_t)nx * (j0 + 1)];
+  return kOk;
+}
+
+// Source slowness fun0 (src/eikon_solver_2d.f90:825-841) and the frozen
+// set (:859-872), a rectangle of columns rect[0..1] and rows rect[2..3].
+// Returns kOk or the item's error status.
+__device__ int fwd_source_rect(const Grid &g, const double *fun_p, double x0, double y0,
+                               int lane, double &fun0, int rect[4]) {
+  fun0 = 0;
+  const int st0 = fwd_fun0_s(g, fun_p, x0, y0, &fun0);
+  if (st0 != kOk) return st0;
+  int fi0 = INT_MAX, fi1 = -1, fj0 = INT_MAX, fj1 = -1;
+  for (int i = lane; i < g.nx; i += 64)
+    if (fabs((g.xx[i] - x0) / g.dx) <= 1) { fi0 = min(fi0, i); fi1 = max(fi1, i); }
+  for (int j = lane; j < g.ny; j += 64)
+    if (fabs((g.yy[j] - y0) / g.dy) <= 1) { fj0 = min(fj0, j); fj1 = max(fj1, j); }
+  fi0 = wave_min_i(fi0); fi1 = wave_max_i(fi1); fj0 = wave_min_i(fj0); fj1 = wave_max_i(fj1);
+  if (fi0 <= fi1 && fj0 <= fj1 &&
+      (fi0 == 0 || fi1 == g.nx - 1 || fj0 == 0 || fj1 == g.ny - 1))
+    return kESrcOnBoundary;
+  rect[0] = fi0; rect[1] = fi1; rect[2] = fj0; rect[3] = fj1;
   return kOk;
 }
 
@@ -234,7 +352,6 @@ __device__ __forceinline__ void t0_cell_s(double xxi, double yyj, double x0, dou
   }
 }
 
-
 // Position of cell (i, j) in skew sk (0 plus, 1 minus).  For 64 lanes of one
 // strip and a common offset (di, dj) the positions of (i+di, j+dj) are one
 // contiguous run, except for the lane that crosses into the next strip.
@@ -257,42 +374,47 @@ __device__ __forceinline__ int skew_row(int t, int ny, int L) {
 // convergence values; they are fetched two steps ahead.  "Near" operands are
 // the cell's own coefficients, fetched one step ahead.  Loads use clamped
 // (always valid) addresses so that no branch separates a load from its use.
+template <typename V>
 struct FFar {
-  double tau, t0;    // old tau and T0 of the lane's cell
-  double tup, zup;   // upstream neighbour column (edge lane): final tau, T0
-  double tdn, zdn;   // downstream neighbour column (edge lane): old tau, T0
-  double snap;       // tau at the start of the iteration (last sweep only)
+  V tau, t0;    // old tau and T0 of the lane's cell
+  V tup, zup;   // upstream neighbour column (edge lane): final tau, T0
+  V tdn, zdn;   // downstream neighbour column (edge lane): old tau, T0
+  V snap;       // tau at the start of the iteration (last sweep only)
   int j;
   bool act;
 };
+template <typename V>
 struct FNear {
-  double a, b, c, f, dx_, dy_, t0x, t0y;
+  V a, b, c, f, dx_, dy_, t0x, t0y;
 };
 
 // Everything one forward solve's strip passes need.
+template <typename V>
 struct FwdCtx {
   int nx, ny, R, lane;
   size_t SE;
-  double dx, dy;
+  double dx, dy;           // FP64 update
+  float rdx, rdy;          // FP32 update: 1/dx, 1/dy rounded to float
   int fi0, fi1, fj0, fj1;  // frozen rectangle
-  const double *ps[2];     // period skews (plus, minus), kPF fields each
-  const double *is[2];     // item T0 skews (plus, minus), kFF fields each
-  double *snap;            // plus skew
+  const V *ps[2];          // period skews (plus, minus), kPF fields each
+  const V *is[2];          // item T0 skews (plus, minus), kFF fields each
+  V *snap;                 // plus skew
 };
+
 
 // One sweep (XD, YD) over strip c, reading tau from `tin` (the sweep's own
 // skew) and writing it to `tout`: in place, or (CROSS) into the other skew,
 // which is where the next sweep family reads it.  MODE 0: plain; 1: record
-// the snapshot (first sweep of an iteration); 2: accumulate |tau-snap|*T0
-// (last sweep).
-template <int XD, int YD, int MODE, bool CROSS>
-__device__ void fwd_strip_pass(const FwdCtx &C, int c, const double *tin, double *tout,
-                               double &acc_s, double &acc_m) {
+// the snapshot (first sweep of an iteration); 2: accumulate |tau-snap|*T0 in
+// double (last sweep).
+template <typename V, int XD, int YD, int MODE, bool CROSS>
+__device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, const V *tin, V *tout, double &acc_s,
+                               double &acc_m) {
   constexpr int SK = XD == YD ? 0 : 1;
   constexpr int SO = CROSS ? 1 - SK : SK;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const size_t SE = C.SE;
-  const double *ps = C.ps[SK], *is = C.is[SK];
+  const V *ps = C.ps[SK], *is = C.is[SK];
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -305,9 +427,11 @@ __device__ void fwd_strip_pass(const FwdCtx &C, int c, const double *tin, double
   const bool dn_edge = XD > 0 ? lane == L - 1 : lane == 0;
   const int iu = i - XD, id = i + XD;
   const bool up_ok = lv && up_edge && iu >= 0 && iu < nx;
-  const bool dn_ok = lv && dn_edge && id >= 0 && id < nx;
+  // the FP32 update reads upwind neighbours only
+  constexpr bool kDown = std::is_same<V, double>::value;
+  const bool dn_ok = kDown && lv && dn_edge && id >= 0 && id < nx;
 
-  auto load_far = [&](int t, FFar &q) {
+  auto load_far = [&](int t, FFar<V> &q) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
     q.j = j;
@@ -321,13 +445,13 @@ __device__ void fwd_strip_pass(const FwdCtx &C, int c, const double *tin, double
       q.tup = tout[skpos(SO, iu, jc, R)];
       q.zup = is[skpos(SK, iu, jc, R)];
     }
-    if (dn_ok) {
+    if (kDown && dn_ok) {
       q.tdn = tin[skpos(SK, id, jc, R)];
       q.zdn = is[skpos(SK, id, jc, R)];
     }
     if (MODE == 2) q.snap = C.snap[e];
   };
-  auto load_near = [&](int t, FNear &q) {
+  auto load_near = [&](int t, FNear<V> &q) {
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
     const size_t e = skidx(c, r, lane, R);
     q.a = ps[e]; q.b = ps[SE + e]; q.c = ps[2 * SE + e]; q.f = ps[3 * SE + e];
@@ -335,35 +459,48 @@ __device__ void fwd_strip_pass(const FwdCtx &C, int c, const double *tin, double
     q.t0x = is[SE + e]; q.t0y = is[2 * SE + e];
   };
 
-  FFar fc, fn, f2;
-  FNear nc, nn;
+  FFar<V> fc, fn, f2;
+  FNear<V> nc, nn;
   load_far(0, fc);
   load_far(1, fn);
   load_near(0, nc);
-  double tprev = 0, zprev = 0;  // own result and T0 of the previous step
+  V tprev = 0, zprev = 0;  // own result and T0 of the previous step
   for (int t = 0; t < nsteps; ++t) {
     load_far(t + 2, f2);
     load_near(t + 1, nn);
-    double tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
-    double tDx = lane_shift<XD>(fn.tau), zDx = lane_shift<XD>(fn.t0);
+    // lane shifts read other lanes' registers: they run with every lane
+    // enabled, outside the per-cell branch
+    V tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
     if (up_ok) { tUx = fc.tup; zUx = fc.zup; }
-    if (dn_ok) { tDx = fc.tdn; zDx = fc.zdn; }
-    const double tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
-    const double zW = XD > 0 ? zUx : zDx, zE = XD > 0 ? zDx : zUx;
-    const double tS = YD > 0 ? tprev : fn.tau, tN = YD > 0 ? fn.tau : tprev;
-    const double zS = YD > 0 ? zprev : fn.t0, zN = YD > 0 ? fn.t0 : zprev;
+    V tDx = 0, zDx = 0;
+    if constexpr (kDown) {
+      tDx = lane_shift<XD>(fn.tau);
+      zDx = lane_shift<XD>(fn.t0);
+      if (dn_ok) { tDx = fc.tdn; zDx = fc.zdn; }
+    }
     const int jc = fc.j;
-    double tnew = fc.tau;
+    V tnew = fc.tau;
     if (fc.act) {
       const bool frozen = i >= C.fi0 && i <= C.fi1 && jc >= C.fj0 && jc <= C.fj1;
-      if (!frozen)
-        tnew = fwd_cell_r(i, jc, nx, ny, C.dx, C.dy, nc.a, nc.b, nc.c, nc.f, nc.dx_, nc.dy_,
-                          fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
+      if constexpr (kDown) {
+        const V tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
+        const V zW = XD > 0 ? zUx : zDx, zE = XD > 0 ? zDx : zUx;
+        const V tS = YD > 0 ? tprev : fn.tau, tN = YD > 0 ? fn.tau : tprev;
+        const V zS = YD > 0 ? zprev : fn.t0, zN = YD > 0 ? fn.t0 : zprev;
+        if (!frozen)
+          tnew = fwd_cell_r(i, jc, nx, ny, C.dx, C.dy, nc.a, nc.b, nc.c, nc.f, nc.dx_, nc.dy_,
+                            fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
+      } else {
+        if (!frozen)
+          tnew = fwd_cell32_r<XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
+                                      nc.dx_, nc.dy_, fc.t0, nc.t0x, nc.t0y, fc.tau, tUx, tprev,
+                                      zUx, zprev);
+      }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       tout[CROSS ? skpos(SO, i, jc, R) : e] = tnew;
       if (MODE == 1) C.snap[e] = fc.tau;
       if (MODE == 2) {
-        const double d = fabs(tnew - fc.snap) * fc.t0;
+        const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
         acc_s += d;
         acc_m = d > acc_m ? d : acc_m;
       }
@@ -377,89 +514,115 @@ __device__ void fwd_strip_pass(const FwdCtx &C, int c, const double *tin, double
   wave_fence();
 }
 
-template <int XD, int YD, int MODE, bool CROSS>
-__device__ void fwd_sweep(const FwdCtx &C, int ns, const double *tin, double *tout,
-                          double &acc_s, double &acc_m) {
+template <typename V, int XD, int YD, int MODE, bool CROSS>
+__device__ void fwd_sweep(const FwdCtx<V> &C, int ns, const V *tin, V *tout, double &acc_s,
+                          double &acc_m) {
   for (int cq = 0; cq < ns; ++cq)
-    fwd_strip_pass<XD, YD, MODE, CROSS>(C, XD > 0 ? cq : ns - 1 - cq, tin, tout, acc_s, acc_m);
+    fwd_strip_pass<V, XD, YD, MODE, CROSS>(C, XD > 0 ? cq : ns - 1 - cq, tin, tout, acc_s,
+                                           acc_m);
 }
 
-__global__ __launch_bounds__(64) void k_fwd_strip(Grid g, SkewGeom sg, const double *pskew,
-                                                  const double *fun_nat, Items it, StripWork w,
-                                                  int *iters, int *status, int iter_stride) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
-  const int lane = threadIdx.x;
-  const int nx = g.nx, ny = g.ny, R = sg.R, ns = sg.ns;
-  const size_t N = (size_t)nx * ny, SE = sg.skew_elems();
-  const int p = it.pidx[item];
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
-  double *tauP = sk0 + kSkTauP * SE, *tauM = sk0 + kSkTauM * SE;
-  double *TP = sk0 + kSkTP * SE, *TM = sk0 + kSkTM * SE;
-  double *Tnat = w.T + (size_t)slot * N;
-  FwdCtx C;
-  C.nx = nx; C.ny = ny; C.R = R; C.lane = lane; C.SE = SE; C.dx = g.dx; C.dy = g.dy;
-  C.ps[0] = pskew + ((size_t)p * 2 + 0) * kPF * SE;
-  C.ps[1] = pskew + ((size_t)p * 2 + 1) * kPF * SE;
-  C.is[0] = sk0;
-  C.is[1] = sk0 + kFF * SE;
-  C.snap = sk0 + kSkSnapF * SE;
-  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
+// One iteration of four sweeps, (x+,y+), (x+,y-), (x-,y+), (x-,y-)
+// (src/eikon_solver_2d.f90:884-898), starting and ending in the plus skew;
+// returns whether the stop rule (:1091-1115) holds.
+template <typename V>
+__device__ bool fwd_iteration(const FwdCtx<V> &C, int ns, V *tP, V *tM) {
+  double acc_s = 0, acc_m = 0;
+  fwd_sweep<V, 1, 1, 1, true>(C, ns, tP, tM, acc_s, acc_m);
+  fwd_sweep<V, 1, -1, 0, false>(C, ns, tM, tM, acc_s, acc_m);
+  fwd_sweep<V, -1, 1, 0, true>(C, ns, tM, tP, acc_s, acc_m);
+  fwd_sweep<V, -1, -1, 2, false>(C, ns, tP, tP, acc_s, acc_m);
+  const double s = wave_sum(acc_s), m = wave_max(acc_m);
+  const double l1 = s / (double)C.nx / (double)C.ny;
+  return fabs(l1) < kFwdTol && m < kFwdTol;
+}
 
+// Per-slot FP32 region of the mixed forward, in float skews from double
+// skew 9: T0, T0x, T0y plus (0..2) and minus (3..5), dtau plus (6) and
+// minus (7), snapshot (8).  Double skews 9..13 are unused by the forward
+// until the output T skews (12, 13) are written after the FP32 phase.
+constexpr int kSkF32 = 9;
+
+// Forward setup of every item of a chunk, one thread per skew element:
+// T0, T0x, T0y (src/eikon_solver_2d.f90:845-857) in both skews (and, MIXED,
+// rounded to float), and the tau init in the plus skew (:859-872: 1 on the
+// frozen cells, 10 elsewhere; MIXED: dtau = tau - 1 in float).  Padding
+// elements get zeros.  An item whose source bracket leaves the grid or whose
+// frozen set touches the boundary gets its error status (:863-867).
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_fwd_setup(Grid g, SkewGeom sg, const double *fun_nat,
+                                                   Items it, StripWork w, int *status) {
+  const size_t SE = sg.skew_elems();
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const int sk = blockIdx.y, slot = blockIdx.z;
+  const int item = it.item0 + slot;
+  if (e >= SE) return;
+  const int nx = g.nx, ny = g.ny;
+  const size_t N = (size_t)nx * ny;
+  const int p = it.pidx[item];
+  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
   double fun0 = 0;
   const int st0 = fwd_fun0_s(g, fun_nat + (size_t)p * N, x0, y0, &fun0);
   if (st0 != kOk) {
-    if (lane == 0) { status[item] = st0; iters[item * iter_stride] = 0; }
+    if (e == 0 && sk == 0) status[item] = st0;
     return;
   }
-  // frozen set (src/eikon_solver_2d.f90:861-872): a rectangle of columns and rows
-  int fi0 = INT_MAX, fi1 = -1, fj0 = INT_MAX, fj1 = -1;
-  for (int i = lane; i < nx; i += 64)
-    if (fabs((g.xx[i] - x0) / g.dx) <= 1) { fi0 = min(fi0, i); fi1 = max(fi1, i); }
-  for (int j = lane; j < ny; j += 64)
-    if (fabs((g.yy[j] - y0) / g.dy) <= 1) { fj0 = min(fj0, j); fj1 = max(fj1, j); }
-  fi0 = wave_min_i(fi0); fi1 = wave_max_i(fi1); fj0 = wave_min_i(fj0); fj1 = wave_max_i(fj1);
-  if (fi0 <= fi1 && fj0 <= fj1 && (fi0 == 0 || fi1 == nx - 1 || fj0 == 0 || fj1 == ny - 1)) {
-    if (lane == 0) { status[item] = kESrcOnBoundary; iters[item * iter_stride] = 0; }
-    return;
+  const int l = (int)(e & 63);
+  const int r = (int)((e >> 6) % sg.R);
+  const int cc = (int)((e >> 6) / sg.R);
+  const int i = 64 * cc + l;
+  const int j = sk == 0 ? r - l : r + l - 63;
+  const bool valid = i < nx && j >= 0 && j < ny;
+  double t0 = 0, t0x = 0, t0y = 0;
+  bool frz = false;
+  if (valid) {
+    t0_cell_s(g.xx[i], g.yy[j], x0, sin(y0), cos(y0), fun0, t0, t0x, t0y);
+    frz = fabs((g.xx[i] - x0) / g.dx) <= 1 && fabs((g.yy[j] - y0) / g.dy) <= 1;
+    if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) status[item] = kESrcOnBoundary;
   }
-  C.fi0 = fi0; C.fi1 = fi1; C.fj0 = fj0; C.fj1 = fj1;
-  // T0 fields in both skews; tau init in the plus skew
-  const double sy0 = sin(y0), cy0 = cos(y0);
-  for (int sk = 0; sk < 2; ++sk) {
-    double *b0 = sk0 + (size_t)sk * kFF * SE;
-    for (int cc = 0; cc < ns; ++cc) {
-      const int i = 64 * cc + lane;
-      for (int r = 0; r < R; ++r) {
-        const int j = sk == 0 ? r - lane : r + lane - 63;
-        const bool valid = i < nx && j >= 0 && j < ny;
-        double t0 = 0, t0x = 0, t0y = 0;
-        if (valid) t0_cell_s(g.xx[i], g.yy[j], x0, sy0, cy0, fun0, t0, t0x, t0y);
-        const size_t e = skidx(cc, r, lane, R);
-        b0[e] = t0; b0[SE + e] = t0x; b0[2 * SE + e] = t0y;
-        if (sk == 0)
-          tauP[e] = (valid && i >= fi0 && i <= fi1 && j >= fj0 && j <= fj1) ? 1.0 : 10.0;
-      }
-    }
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *b0 = sk0 + (size_t)sk * kFF * SE;
+  b0[e] = t0; b0[SE + e] = t0x; b0[2 * SE + e] = t0y;
+  if (MIXED) {
+    float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
+    float *b32 = f32 + (size_t)sk * kFF * SE;
+    b32[e] = (float)t0; b32[SE + e] = (float)t0x; b32[2 * SE + e] = (float)t0y;
+    if (sk == 0) f32[6 * SE + e] = frz ? 0.0f : 9.0f;
+  } else if (sk == 0) {
+    sk0[kSkTauP * SE + e] = frz ? 1.0 : 10.0;
   }
-  wave_fence();
+}
 
+template <typename V>
+__device__ void fwd_ctx_init(FwdCtx<V> &C, const Grid &g, const SkewGeom &sg, int lane,
+                             const int rect[4]) {
+  C.nx = g.nx; C.ny = g.ny; C.R = sg.R; C.lane = lane; C.SE = sg.skew_elems();
+  C.dx = g.dx; C.dy = g.dy;
+  C.rdx = (float)(1.0 / g.dx); C.rdy = (float)(1.0 / g.dy);
+  C.fi0 = rect[0]; C.fi1 = rect[1]; C.fj0 = rect[2]; C.fj1 = rect[3];
+}
+
+// FP64 iterations from the tau in the plus skew, then T = tau*T0
+// (src/eikon_solver_2d.f90:1124) in both skews and in natural layout.
+__device__ void fwd_finish64(const Grid &g, const SkewGeom &sg, int lane, const double *pskew_p,
+                             double *sk0, const int rect[4], double *Tnat, int item,
+                             int *iters, int *status, int iter_stride) {
+  const int nx = g.nx, ny = g.ny, R = sg.R, ns = sg.ns;
+  const size_t N = (size_t)nx * ny, SE = sg.skew_elems();
+  double *tauP = sk0 + kSkTauP * SE, *tauM = sk0 + kSkTauM * SE;
+  double *TP = sk0 + kSkTP * SE, *TM = sk0 + kSkTM * SE;
+  FwdCtx<double> C;
+  fwd_ctx_init(C, g, sg, lane, rect);
+  C.ps[0] = pskew_p;
+  C.ps[1] = pskew_p + (size_t)kPF * SE;
+  C.is[0] = sk0;
+  C.is[1] = sk0 + kFF * SE;
+  C.snap = sk0 + kSkSnapF * SE;
   int iter = 1;
   bool conv = false;
-  for (; iter <= kFwdMaxIter; ++iter) {
-    double acc_s = 0, acc_m = 0;
-    // sweeps (x+,y+), (x+,y-), (x-,y+), (x-,y-)  (src/eikon_solver_2d.f90:884-898)
-    fwd_sweep<1, 1, 1, true>(C, ns, tauP, tauM, acc_s, acc_m);
-    fwd_sweep<1, -1, 0, false>(C, ns, tauM, tauM, acc_s, acc_m);
-    fwd_sweep<-1, 1, 0, true>(C, ns, tauM, tauP, acc_s, acc_m);
-    fwd_sweep<-1, -1, 2, false>(C, ns, tauP, tauP, acc_s, acc_m);
-    // convergence (src/eikon_solver_2d.f90:1091-1115)
-    const double s = wave_sum(acc_s), m = wave_max(acc_m);
-    const double l1 = s / (double)nx / (double)ny;
-    if (fabs(l1) < kFwdTol && m < kFwdTol) { conv = true; break; }
-  }
+  for (; iter <= kFwdMaxIter; ++iter)
+    if (fwd_iteration(C, ns, tauP, tauM)) { conv = true; break; }
   if (!conv) iter = kFwdMaxIter;
-  // T = tau*T0 (src/eikon_solver_2d.f90:1124) in both skews and natural layout
   for (int cc = 0; cc < ns; ++cc) {
     const int i = 64 * cc + lane;
     for (int r = 0; r < R; ++r) {
@@ -480,6 +643,81 @@ __global__ __launch_bounds__(64) void k_fwd_strip(Grid g, SkewGeom sg, const dou
     status[item] = conv ? kOk : kWMaxIter;
     iters[item * iter_stride] = iter;
   }
+}
+
+// The frozen rectangle of an item whose setup succeeded.
+__device__ bool fwd_item_rect(const Grid &g, const Items &it, const double *fun_nat, int item,
+                              int lane, int rect[4]) {
+  const size_t N = (size_t)g.nx * g.ny;
+  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
+  double fun0;
+  return fwd_source_rect(g, fun_nat + (size_t)it.pidx[item] * N, x0, y0, lane, fun0, rect) == kOk;
+}
+
+__global__ __launch_bounds__(64) void k_fwd_strip(Grid g, SkewGeom sg, const double *pskew,
+                                                  const double *fun_nat, Items it, StripWork w,
+                                                  int *iters, int *status, int iter_stride) {
+  const int slot = blockIdx.x;
+  const int item = it.item0 + slot;
+  const int lane = threadIdx.x;
+  if (status[item] < 0) return;  // the setup failed
+  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
+  int rect[4];
+  if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  fwd_finish64(g, sg, lane, pskew + (size_t)it.pidx[item] * 2 * kPF * SE, sk0, rect,
+               w.T + (size_t)slot * N, item, iters, status, iter_stride);
+}
+
+// Mixed forward, FP32 phase: the FP32 iterations (count to iters32), then
+// tau = 1 + dtau into the FP64 plus skew for the polish.
+__global__ __launch_bounds__(64) void k_fwd_mixed32(Grid g, SkewGeom sg, const float *pskew32,
+                                                    const double *fun_nat, Items it,
+                                                    StripWork w, int *iters32, int *status,
+                                                    int iter_stride) {
+  const int slot = blockIdx.x;
+  const int item = it.item0 + slot;
+  const int lane = threadIdx.x;
+  if (status[item] < 0) return;
+  const size_t SE = sg.skew_elems();
+  int rect[4];
+  if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
+  FwdCtx<float> C;
+  fwd_ctx_init(C, g, sg, lane, rect);
+  const float *ps = pskew32 + (size_t)it.pidx[item] * 2 * kPF * SE;
+  C.ps[0] = ps;
+  C.ps[1] = ps + (size_t)kPF * SE;
+  C.is[0] = f32;
+  C.is[1] = f32 + (size_t)kFF * SE;
+  C.snap = f32 + 8 * SE;
+  float *dP = f32 + 6 * SE, *dM = f32 + 7 * SE;
+  int iter = 1;
+  for (; iter <= kFwdMaxIter; ++iter)
+    if (fwd_iteration(C, sg.ns, dP, dM)) break;
+  if (iter > kFwdMaxIter) iter = kFwdMaxIter;
+  double *tauP = sk0 + kSkTauP * SE;
+  for (size_t e = lane; e < SE; e += 64) tauP[e] = 1.0 + (double)dP[e];
+  if (lane == 0) iters32[item * iter_stride] = iter;
+}
+
+// Mixed forward, FP64 polish: the reference's iterations to its stop rule
+// from the FP32 result, and the outputs.
+__global__ __launch_bounds__(64) void k_fwd_polish(Grid g, SkewGeom sg, const double *pskew,
+                                                   const double *fun_nat, Items it,
+                                                   StripWork w, int *iters, int *status,
+                                                   int iter_stride) {
+  const int slot = blockIdx.x;
+  const int item = it.item0 + slot;
+  const int lane = threadIdx.x;
+  if (status[item] < 0) return;
+  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
+  int rect[4];
+  if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  fwd_finish64(g, sg, lane, pskew + (size_t)it.pidx[item] * 2 * kPF * SE, sk0, rect,
+               w.T + (size_t)slot * N, item, iters, status, iter_stride);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -527,31 +765,151 @@ __device__ int adj_splat_s(const Grid &g, const double *xr_deg, const double *yr
   return kOk;
 }
 
+// Delta sources of a chunk (src/eikon_solver_2d.f90:1320-1336): zero the
+// natural fields, one thread per cell ...
+__global__ __launch_bounds__(256) void k_adj_zero(size_t N, Items it, StripWork w) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k < N) w.delta[(size_t)blockIdx.y * N + k] = 0;
+}
+
+// ... then splat each item's receivers in order, one thread per item.
+__global__ __launch_bounds__(64) void k_adj_splat(Grid g, Items it, StripWork w, const double *rhs,
+                                                  int *iters, int *status, int iter_stride) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= it.count) return;
+  const int item = it.item0 + slot;
+  if (status[item] < 0) return;  // an earlier stage failed
+  const int r0 = it.rec_off[item], nr = it.rec_off[item + 1] - r0;
+  const int err = adj_splat_s(g, it.recx + r0, it.recy + r0, rhs + r0, nr,
+                              w.delta + (size_t)slot * g.nx * g.ny);
+  if (err != kOk) {
+    status[item] = err;
+    iters[item * iter_stride] = 0;
+  }
+}
+
+// Coefficients of every item of a chunk, one thread per skew element; the
+// neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
+//   KIND 0: FP64 a1, a2, b1, b2, d, delta and Ta = 100 inside, 0 on the
+//           boundary (:1368-1380), in the plus skew
+//   KIND 1: FP64 coefficients only (the mixed polish)
+//   KIND 2: FP32 k0, kW, kE, kS, kN and the FP32 Ta init
+// Padding elements get zeros.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_adj_build(Grid g, SkewGeom sg, const double *pskew,
+                                                   Items it, StripWork w, const int *status) {
+  const size_t SE = sg.skew_elems();
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const int sk = blockIdx.y, slot = blockIdx.z;
+  const int item = it.item0 + slot;
+  if (e >= SE || status[item] < 0) return;
+  const int nx = g.nx, ny = g.ny, R = sg.R;
+  const double dx = g.dx, dy = g.dy;
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  const double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
+  const double *pa = pskew + ((size_t)it.pidx[item] * 2 + sk) * kPF * SE;
+  const double *pb = pa + SE, *pc = pa + 2 * SE;
+  const double *delta = w.delta + (size_t)slot * nx * ny;
+  const int l = (int)(e & 63);
+  const int r = (int)((e >> 6) % R);
+  const int cc = (int)((e >> 6) / R);
+  const int i = 64 * cc + l;
+  const int j = sk == 0 ? r - l : r + l - 63;
+  double v[kAF] = {0, 0, 0, 0, 0, 0};
+  const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+  if (interior) {
+    auto at = [&](int di, int dj) { return skpos(sk, i + di, j + dj, R); };
+    const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
+    adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
+               Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
+               pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
+    v[5] = delta[i + (size_t)nx * j];
+  }
+  if (KIND != 2) {
+    double *b0 = sk0 + (size_t)sk * kAF * SE;
+#pragma unroll
+    for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
+    if (KIND == 0 && sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
+  } else {
+    float k[5] = {0, 0, 0, 0, 0};
+    if (interior && v[4] != 0.0) {
+      const double d = v[4];
+      const double a1p = (v[0] + fabs(v[0])) / 2, a2m = (v[1] - fabs(v[1])) / 2;
+      const double b1p = (v[2] + fabs(v[2])) / 2, b2m = (v[3] - fabs(v[3])) / 2;
+      k[0] = (float)(v[5] / d);
+      k[1] = (float)(a1p / dx / d);
+      k[2] = (float)(-a2m / dx / d);
+      k[3] = (float)(b1p / dy / d);
+      k[4] = (float)(-b2m / dy / d);
+    }
+    float *f32 = reinterpret_cast<float *>(sk0);
+    float *b0 = f32 + (size_t)sk * 5 * SE;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) b0[f * SE + e] = k[f];
+    if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
+  }
+}
+
+template <typename V>
 struct AFar {
-  double ta, tup, tdn, snap;
+  V ta, tup, tdn, snap;
   int j;
   bool act;
 };
-struct ANear {
+template <typename V>
+struct ANear;
+template <>
+struct ANear<double> {
   double a1, a2, b1, b2, d, dl;
 };
+template <>
+struct ANear<float> {
+  float k0, kW, kE, kS, kN;
+};
 
+__device__ __forceinline__ void adj_load_near(const double *is, size_t SE, size_t e,
+                                              ANear<double> &q) {
+  q.a1 = is[e]; q.a2 = is[SE + e]; q.b1 = is[2 * SE + e]; q.b2 = is[3 * SE + e];
+  q.d = is[4 * SE + e]; q.dl = is[5 * SE + e];
+}
+__device__ __forceinline__ void adj_load_near(const float *is, size_t SE, size_t e,
+                                              ANear<float> &q) {
+  q.k0 = is[e]; q.kW = is[SE + e]; q.kE = is[2 * SE + e]; q.kS = is[3 * SE + e];
+  q.kN = is[4 * SE + e];
+}
+
+// The interior update (src/eikon_solver_2d.f90:1392-1402), and its FP32 form.
+__device__ __forceinline__ double adj_update(const ANear<double> &n, double tW, double tE,
+                                             double tS, double tN, double dx, double dy) {
+  if (n.d == 0.0) return 0;
+  const double a1p = (n.a1 + fabs(n.a1)) / 2, a2m = (n.a2 - fabs(n.a2)) / 2;
+  const double b1p = (n.b1 + fabs(n.b1)) / 2, b2m = (n.b2 - fabs(n.b2)) / 2;
+  const double e = (tW * a1p - tE * a2m) / dx + (tS * b1p - tN * b2m) / dy;
+  return (n.dl + e) / n.d;
+}
+__device__ __forceinline__ float adj_update(const ANear<float> &n, float tW, float tE, float tS,
+                                            float tN, double, double) {
+  return n.k0 + n.kW * tW + n.kE * tE + n.kS * tS + n.kN * tN;
+}
+
+template <typename V>
 struct AdjCtx {
   int nx, ny, R, lane;
   size_t SE;
   double dx, dy;
-  const double *is[2];  // item coefficient skews, kAF fields each
-  double *snap;         // plus skew
+  const V *is[2];  // item coefficient skews
+  V *snap;         // plus skew
 };
 
-template <int XD, int YD, int MODE, bool CROSS>
-__device__ void adj_strip_pass(const AdjCtx &C, int c, const double *tin, double *tout,
-                               double &acc_s, double &acc_m) {
+// MODE 2 also accumulates max |Ta| (acc_a) for the FP32 stop rule.
+template <typename V, int XD, int YD, int MODE, bool CROSS>
+__device__ void adj_strip_pass(const AdjCtx<V> &C, int c, const V *tin, V *tout, double &acc_s,
+                               double &acc_m, double &acc_a) {
   constexpr int SK = XD == YD ? 0 : 1;
   constexpr int SO = CROSS ? 1 - SK : SK;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const size_t SE = C.SE;
-  const double *is = C.is[SK];
+  const V *is = C.is[SK];
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -564,7 +922,7 @@ __device__ void adj_strip_pass(const AdjCtx &C, int c, const double *tin, double
   const bool dn_ok = lv && dn_edge && id >= 0 && id < nx;
   const bool interior_col = i >= 1 && i <= nx - 2;
 
-  auto load_far = [&](int t, AFar &q) {
+  auto load_far = [&](int t, AFar<V> &q) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
     q.j = j;
@@ -577,48 +935,41 @@ __device__ void adj_strip_pass(const AdjCtx &C, int c, const double *tin, double
     if (dn_ok) q.tdn = tin[skpos(SK, id, jc, R)];
     if (MODE == 2) q.snap = C.snap[e];
   };
-  auto load_near = [&](int t, ANear &q) {
+  auto load_near = [&](int t, ANear<V> &q) {
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
-    const size_t e = skidx(c, r, lane, R);
-    q.a1 = is[e]; q.a2 = is[SE + e]; q.b1 = is[2 * SE + e]; q.b2 = is[3 * SE + e];
-    q.d = is[4 * SE + e]; q.dl = is[5 * SE + e];
+    adj_load_near(is, SE, skidx(c, r, lane, R), q);
   };
-  AFar fc, fn, f2;
-  ANear nc, nn;
+  AFar<V> fc, fn, f2;
+  ANear<V> nc, nn;
   load_far(0, fc);
   load_far(1, fn);
   load_near(0, nc);
-  double tprev = 0;
+  V tprev = 0;
   for (int t = 0; t < nsteps; ++t) {
     load_far(t + 2, f2);
     load_near(t + 1, nn);
-    double tUx = lane_shift<-XD>(tprev);
-    double tDx = lane_shift<XD>(fn.ta);
+    V tUx = lane_shift<-XD>(tprev);
+    V tDx = lane_shift<XD>(fn.ta);
     if (up_ok) tUx = fc.tup;
     if (dn_ok) tDx = fc.tdn;
-    const double tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
-    const double tS = YD > 0 ? tprev : fn.ta, tN = YD > 0 ? fn.ta : tprev;
+    const V tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
+    const V tS = YD > 0 ? tprev : fn.ta, tN = YD > 0 ? fn.ta : tprev;
     const int jc = fc.j;
-    double tnew = fc.ta;
+    V tnew = fc.ta;
     if (fc.act) {
-      if (interior_col && jc >= 1 && jc <= ny - 2) {
-        // src/eikon_solver_2d.f90:1392-1402
-        if (nc.d == 0.0) {
-          tnew = 0;
-        } else {
-          const double a1p = (nc.a1 + fabs(nc.a1)) / 2, a2m = (nc.a2 - fabs(nc.a2)) / 2;
-          const double b1p = (nc.b1 + fabs(nc.b1)) / 2, b2m = (nc.b2 - fabs(nc.b2)) / 2;
-          const double e = (tW * a1p - tE * a2m) / C.dx + (tS * b1p - tN * b2m) / C.dy;
-          tnew = (nc.dl + e) / nc.d;
-        }
-      }
+      if (interior_col && jc >= 1 && jc <= ny - 2)
+        tnew = adj_update(nc, tW, tE, tS, tN, C.dx, C.dy);
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       tout[CROSS ? skpos(SO, i, jc, R) : e] = tnew;
       if (MODE == 1) C.snap[e] = fc.ta;
       if (MODE == 2) {
-        const double dd = fabs(tnew - fc.snap);
+        const double dd = fabs((double)tnew - (double)fc.snap);
         acc_s += dd;
         acc_m = dd > acc_m ? dd : acc_m;
+        if (std::is_same<V, float>::value) {
+          const double av = fabs((double)tnew);
+          acc_a = av > acc_a ? av : acc_a;
+        }
       }
     }
     tprev = tnew;
@@ -629,126 +980,162 @@ __device__ void adj_strip_pass(const AdjCtx &C, int c, const double *tin, double
   wave_fence();
 }
 
-template <int XD, int YD, int MODE, bool CROSS>
-__device__ void adj_sweep(const AdjCtx &C, int ns, const double *tin, double *tout,
-                          double &acc_s, double &acc_m) {
+template <typename V, int XD, int YD, int MODE, bool CROSS>
+__device__ void adj_sweep(const AdjCtx<V> &C, int ns, const V *tin, V *tout, double &acc_s,
+                          double &acc_m, double &acc_a) {
   for (int cq = 0; cq < ns; ++cq)
-    adj_strip_pass<XD, YD, MODE, CROSS>(C, XD > 0 ? cq : ns - 1 - cq, tin, tout, acc_s, acc_m);
+    adj_strip_pass<V, XD, YD, MODE, CROSS>(C, XD > 0 ? cq : ns - 1 - cq, tin, tout, acc_s,
+                                           acc_m, acc_a);
 }
 
-__global__ __launch_bounds__(64) void k_adj_strip(Grid g, SkewGeom sg, const double *pskew,
-                                                  Items it, StripWork w, const double *rhs,
+// One iteration: (xdirec, ydirec) = (-1,-1), (-1,1), (1,-1), (1,1), x
+// ascending first (src/eikon_solver_2d.f90:1383-1390); with SNAP the
+// snapshot and the residuals (:1412-1434) are taken.
+template <typename V, bool SNAP>
+__device__ void adj_iteration(const AdjCtx<V> &C, int ns, V *tP, V *tM, double &s, double &m,
+                              double &a) {
+  double acc_s = 0, acc_m = 0, acc_a = 0;
+  adj_sweep<V, 1, 1, SNAP ? 1 : 0, true>(C, ns, tP, tM, acc_s, acc_m, acc_a);
+  adj_sweep<V, 1, -1, 0, false>(C, ns, tM, tM, acc_s, acc_m, acc_a);
+  adj_sweep<V, -1, 1, 0, true>(C, ns, tM, tP, acc_s, acc_m, acc_a);
+  adj_sweep<V, -1, -1, SNAP ? 2 : 0, false>(C, ns, tP, tP, acc_s, acc_m, acc_a);
+  s = wave_sum(acc_s);
+  m = wave_max(acc_m);
+  a = wave_max(acc_a);
+}
+
+template <typename V>
+__device__ void adj_ctx_init(AdjCtx<V> &C, const Grid &g, const SkewGeom &sg, int lane) {
+  C.nx = g.nx; C.ny = g.ny; C.R = sg.R; C.lane = lane; C.SE = sg.skew_elems();
+  C.dx = g.dx; C.dy = g.dy;
+}
+
+// Natural-layout output from the plus skew, with the mask
+// (src/eikon_solver_2d.f90:1449-1467, degrees).
+__device__ void adj_output(const Grid &g, const SkewGeom &sg, int lane, const double *TaP,
+                           double x0, double y0, int apply_mask, double *Tout) {
+  const int nx = g.nx;
+  const size_t N = (size_t)nx * g.ny;
+  const double dxd = g.xd[1] - g.xd[0], dyd = g.yd[1] - g.yd[0];
+  for (size_t k = lane; k < N; k += 64) {
+    const int i = (int)(k % nx), j = (int)(k / nx);
+    double v = TaP[skpos(0, i, j, sg.R)];
+    if (apply_mask && fabs(g.xd[i] - x0) < 1.5 * dxd && fabs(g.yd[j] - y0) < 1.5 * dyd) v = 0;
+    Tout[k] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_adj_strip(Grid g, SkewGeom sg, Items it, StripWork w,
                                                   int out_nat, int *iters, int *status,
                                                   int iter_stride, int apply_mask) {
   const int slot = blockIdx.x;
   const int item = it.item0 + slot;
   const int lane = threadIdx.x;
-  const int nx = g.nx, ny = g.ny, R = sg.R, ns = sg.ns;
-  const size_t N = (size_t)nx * ny, SE = sg.skew_elems();
   if (status[item] < 0) return;  // an earlier stage failed
-  const int p = it.pidx[item];
+  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
   double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
-  const double *Tsk[2] = {sk0 + kSkTP * SE, sk0 + kSkTM * SE};
-  double *delta = w.delta + (size_t)slot * N;
   double *Tout = (out_nat == kNatTd ? w.Td : w.Ta) + (size_t)slot * N;
-  AdjCtx C;
-  C.nx = nx; C.ny = ny; C.R = R; C.lane = lane; C.SE = SE; C.dx = g.dx; C.dy = g.dy;
+  AdjCtx<double> C;
+  adj_ctx_init(C, g, sg, lane);
   C.is[0] = sk0;
   C.is[1] = sk0 + kAF * SE;
   C.snap = sk0 + kSkSnapA * SE;
-  const double dx = g.dx, dy = g.dy;
-
-  for (size_t k = lane; k < N; k += 64) delta[k] = 0;
-  wave_fence();
-  int err = kOk;
-  if (lane == 0) {
-    const int r0 = it.rec_off[item], nr = it.rec_off[item + 1] - r0;
-    err = adj_splat_s(g, it.recx + r0, it.recy + r0, rhs + r0, nr, delta);
-  }
-  err = __shfl(err, 0, 64);
-  if (err != kOk) {
-    if (lane == 0) { status[item] = err; iters[item * iter_stride] = 0; }
-    return;
-  }
-  wave_fence();
-  // coefficients in both skews; neighbours read with shifted loads from the
-  // same skew (one contiguous run per (di, dj) offset)
-  for (int sk = 0; sk < 2; ++sk) {
-    double *b0 = sk0 + (size_t)sk * kAF * SE;
-    const double *Ts = Tsk[sk];
-    const double *pa = pskew + ((size_t)p * 2 + sk) * kPF * SE;
-    const double *pb = pa + SE, *pc = pa + 2 * SE;
-    for (int cc = 0; cc < ns; ++cc) {
-      const int i = 64 * cc + lane;
-      for (int r = 0; r < R; ++r) {
-        const int j = sk == 0 ? r - lane : r + lane - 63;
-        double v[kAF] = {0, 0, 0, 0, 0, 0};
-        if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) {
-          auto at = [&](int di, int dj) { return skpos(sk, i + di, j + dj, R); };
-          const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
-          adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
-                     Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
-                     pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
-          v[5] = delta[i + (size_t)nx * j];
-        }
-        const size_t e = skidx(cc, r, lane, R);
-#pragma unroll
-        for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
-        if (sk == 0) {
-          const bool in_grid = i < nx && j >= 0 && j < ny;
-          TaP[e] = (in_grid && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) ? 100.0 : 0.0;
-        }
-      }
-    }
-  }
-  wave_fence();
-
   int iter = 1;
   bool conv = false;
   for (; iter <= kAdjMaxIter; ++iter) {
-    double acc_s = 0, acc_m = 0;
-    // (xdirec, ydirec) = (-1,-1), (-1,1), (1,-1), (1,1): x ascending first
-    // (src/eikon_solver_2d.f90:1383-1390)
-    adj_sweep<1, 1, 1, true>(C, ns, TaP, TaM, acc_s, acc_m);
-    adj_sweep<1, -1, 0, false>(C, ns, TaM, TaM, acc_s, acc_m);
-    adj_sweep<-1, 1, 0, true>(C, ns, TaM, TaP, acc_s, acc_m);
-    adj_sweep<-1, -1, 2, false>(C, ns, TaP, TaP, acc_s, acc_m);
-    // convergence (src/eikon_solver_2d.f90:1412-1434)
-    const double s = wave_sum(acc_s), m = wave_max(acc_m);
-    const double l1 = s / (double)(nx * ny);
+    double s, m, a;
+    adj_iteration<double, true>(C, sg.ns, TaP, TaM, s, m, a);
+    const double l1 = s / (double)(g.nx * g.ny);
     if (fabs(l1) < kAdjTol && m < kAdjTol) { conv = true; break; }
   }
   if (!conv) iter = kAdjMaxIter;
-  // natural-layout output with the mask (src/eikon_solver_2d.f90:1449-1467, degrees)
-  const double x0 = it.srcx[item], y0 = it.srcy[item];
-  const double dxd = g.xd[1] - g.xd[0], dyd = g.yd[1] - g.yd[0];
-  for (size_t k = lane; k < N; k += 64) {
-    const int i = (int)(k % nx), j = (int)(k / nx);
-    double v = TaP[skpos(0, i, j, R)];
-    if (apply_mask && fabs(g.xd[i] - x0) < 1.5 * dxd && fabs(g.yd[j] - y0) < 1.5 * dyd) v = 0;
-    Tout[k] = v;
-  }
+  adj_output(g, sg, lane, TaP, it.srcx[item], it.srcy[item], apply_mask, Tout);
   if (lane == 0) {
     if (!conv && status[item] == kOk) status[item] = kWMaxIter;
     iters[item * iter_stride] = iter;
   }
 }
 
-}  // namespace
-
-void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const Items &it, const StripWork &w, const double *rhs, int out_nat,
-                      int *iters, int *status, int iter_stride, int apply_mask) {
-  hipLaunchKernelGGL(k_adj_strip, dim3(it.count), dim3(64), 0, s, g, sg, pskew, it, w, rhs, out_nat,
-                     iters, status, iter_stride, apply_mask);
+// Mixed adjoint, FP32 phase: iterations on the float coefficients (float
+// skews from the slot base: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
+// minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
+__global__ __launch_bounds__(64) void k_adj_mixed32(Grid g, SkewGeom sg, Items it, StripWork w,
+                                                    int *iters32, int *status,
+                                                    int iter_stride) {
+  const int slot = blockIdx.x;
+  const int item = it.item0 + slot;
+  const int lane = threadIdx.x;
+  if (status[item] < 0) return;
+  const size_t SE = sg.skew_elems();
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  float *f32 = reinterpret_cast<float *>(sk0);
+  float *taP = f32 + 10 * SE, *taM = f32 + 11 * SE;
+  AdjCtx<float> C;
+  adj_ctx_init(C, g, sg, lane);
+  C.is[0] = f32;
+  C.is[1] = f32 + 5 * SE;
+  C.snap = f32 + 12 * SE;
+  int iter = 1;
+  bool done = false;
+  for (; iter <= kAdjMaxIter; ++iter) {
+    double s, m, a;
+    adj_iteration<float, true>(C, sg.ns, taP, taM, s, m, a);
+    const double l1 = s / (double)(g.nx * g.ny);
+    if ((fabs(l1) < kAdjTol && m < kAdjTol) || m <= (double)kMixRelStop * a) {
+      done = true;
+      break;
+    }
+  }
+  if (!done) iter = kAdjMaxIter;
+  double *TaP = sk0 + kSkTaP * SE;
+  for (size_t e = lane; e < SE; e += 64) TaP[e] = (double)taP[e];
+  if (lane == 0) {
+    if (!done && status[item] == kOk) status[item] = kWMaxIter;
+    iters32[item * iter_stride] = iter;
+  }
 }
 
+// Mixed adjoint, FP64 polish: one iteration of the reference's update on the
+// FP64 coefficients (built after the FP32 phase), and the masked output.
+__global__ __launch_bounds__(64) void k_adj_polish(Grid g, SkewGeom sg, Items it, StripWork w,
+                                                   int out_nat, int *iters, int *status,
+                                                   int iter_stride, int apply_mask) {
+  const int slot = blockIdx.x;
+  const int item = it.item0 + slot;
+  const int lane = threadIdx.x;
+  if (status[item] < 0) return;
+  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
+  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
+  AdjCtx<double> C;
+  adj_ctx_init(C, g, sg, lane);
+  C.is[0] = sk0;
+  C.is[1] = sk0 + kAF * SE;
+  C.snap = nullptr;
+  double s, m, a;
+  adj_iteration<double, false>(C, sg.ns, TaP, TaM, s, m, a);
+  double *Tout = (out_nat == kNatTd ? w.Td : w.Ta) + (size_t)slot * N;
+  adj_output(g, sg, lane, TaP, it.srcx[item], it.srcy[item], apply_mask, Tout);
+  if (lane == 0) iters[item * iter_stride] = 1;
+}
+
+}  // namespace
+
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
-                        const PeriodFields &pf, double *out) {
+                        const PeriodFields &pf, double *out, float *out32) {
   const size_t SE = sg.skew_elems();
   hipLaunchKernelGGL(k_period_skew, dim3((unsigned)((SE + 255) / 256), 2, np), dim3(256), 0, s,
-                     g, sg, np, pf.a, pf.b, pf.c, pf.fun, out);
+                     g, sg, np, pf.a, pf.b, pf.c, pf.fun, out, out32);
+}
+
+void launch_fwd_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
+                      const Items &it, const StripWork &w, int *status, bool mixed) {
+  const dim3 grid((unsigned)((sg.skew_elems() + 255) / 256), 2, it.count);
+  if (mixed)
+    hipLaunchKernelGGL(k_fwd_setup<true>, grid, dim3(256), 0, s, g, sg, pf.fun, it, w, status);
+  else
+    hipLaunchKernelGGL(k_fwd_setup<false>, grid, dim3(256), 0, s, g, sg, pf.fun, it, w, status);
 }
 
 void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
@@ -756,6 +1143,62 @@ void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const do
                       int *status, int iter_stride) {
   hipLaunchKernelGGL(k_fwd_strip, dim3(it.count), dim3(64), 0, s, g, sg, pskew, pf.fun, it, w,
                      iters, status, iter_stride);
+}
+
+void launch_fwd_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const float *pskew32,
+                        const PeriodFields &pf, const Items &it, const StripWork &w,
+                        int *iters32, int *status, int iter_stride) {
+  hipLaunchKernelGGL(k_fwd_mixed32, dim3(it.count), dim3(64), 0, s, g, sg, pskew32, pf.fun, it,
+                     w, iters32, status, iter_stride);
+}
+
+void launch_fwd_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                       const PeriodFields &pf, const Items &it, const StripWork &w, int *iters,
+                       int *status, int iter_stride) {
+  hipLaunchKernelGGL(k_fwd_polish, dim3(it.count), dim3(64), 0, s, g, sg, pskew, pf.fun, it, w,
+                     iters, status, iter_stride);
+}
+
+void launch_adj_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                      const Items &it, const StripWork &w, const double *rhs, int *iters,
+                      int *status, int iter_stride, int kind) {
+  const size_t N = (size_t)g.nx * g.ny;
+  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((N + 255) / 256), it.count), dim3(256), 0, s, N,
+                     it, w);
+  hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((it.count + 63) / 64)), dim3(64), 0, s, g, it,
+                     w, rhs, iters, status, iter_stride);
+  launch_adj_build(s, g, sg, pskew, it, w, status, kind);
+}
+
+void launch_adj_build(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
+                      const Items &it, const StripWork &w, const int *status, int kind) {
+  const dim3 grid((unsigned)((sg.skew_elems() + 255) / 256), 2, it.count);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_adj_build<0>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
+  else if (kind == 1)
+    hipLaunchKernelGGL(k_adj_build<1>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
+  else
+    hipLaunchKernelGGL(k_adj_build<2>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
+}
+
+void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
+                      const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
+                      int apply_mask) {
+  hipLaunchKernelGGL(k_adj_strip, dim3(it.count), dim3(64), 0, s, g, sg, it, w, out_nat, iters,
+                     status, iter_stride, apply_mask);
+}
+
+void launch_adj_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
+                        const StripWork &w, int *iters32, int *status, int iter_stride) {
+  hipLaunchKernelGGL(k_adj_mixed32, dim3(it.count), dim3(64), 0, s, g, sg, it, w, iters32,
+                     status, iter_stride);
+}
+
+void launch_adj_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
+                       const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
+                       int apply_mask) {
+  hipLaunchKernelGGL(k_adj_polish, dim3(it.count), dim3(64), 0, s, g, sg, it, w, out_nat, iters,
+                     status, iter_stride, apply_mask);
 }
 
 }  // namespace satt

@@ -29,6 +29,7 @@ module satt_cabi
     type(c_ptr) :: tt_fwd = c_null_ptr, chi = c_null_ptr, adj_s = c_null_ptr
     type(c_ptr) :: adj_density = c_null_ptr, item_chi = c_null_ptr, iters = c_null_ptr
     type(c_ptr) :: item_status = c_null_ptr, weight_out = c_null_ptr
+    type(c_ptr) :: iters_fp32 = c_null_ptr
   end type satt_result
 
   interface

@@ -23,6 +23,7 @@ OK, W_MAXITER = 0, 1
 E_ARG, E_SRC_ON_BOUNDARY, E_REC_OUT_OF_GRID, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
 PREC_FP64, PREC_MIXED = 0, 1
 LAUNCH_BATCHED, LAUNCH_PER_SOLVE = 0, 1
+FIELD_T, FIELD_TA, FIELD_TD = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -50,6 +51,7 @@ class Result(ctypes.Structure):
     _fields_ = [
         ("tt_fwd", _dp), ("chi", _dp), ("adj_s", _dp), ("adj_density", _dp),
         ("item_chi", _dp), ("iters", _ip), ("item_status", _ip), ("weight_out", _dp),
+        ("iters_fp32", _ip),
     ]
 
 
@@ -58,7 +60,11 @@ class PlanStats(ctypes.Structure):
         ("chunk", ctypes.c_int), ("launches", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
         ("ms_forward", ctypes.c_double), ("ms_adjoint", ctypes.c_double),
         ("ms_total", ctypes.c_double), ("fwd_sweeps", ctypes.c_longlong),
-        ("adj_sweeps", ctypes.c_longlong),
+        ("adj_sweeps", ctypes.c_longlong), ("fwd_sweeps32", ctypes.c_longlong),
+        ("adj_sweeps32", ctypes.c_longlong), ("ms_fwd_fp64", ctypes.c_double),
+        ("ms_fwd_fp32", ctypes.c_double), ("ms_adj_fp64", ctypes.c_double),
+        ("ms_adj_fp32", ctypes.c_double), ("ms_fwd_setup", ctypes.c_double),
+        ("ms_adj_setup", ctypes.c_double),
     ]
 
 
@@ -273,6 +279,7 @@ class Plan:
             "adj_density": np.zeros((npd, nx, ny), order="F") if b["dim"] == 3 else None,
             "item_chi": np.zeros(nit),
             "iters": np.zeros((nit, 3), dtype=np.int32),
+            "iters_fp32": np.zeros((nit, 3), dtype=np.int32),
             "item_status": np.zeros(nit, dtype=np.int32),
             "weight_out": np.zeros(len(b["recx"])),
         }
@@ -281,11 +288,18 @@ class Plan:
         r.adj_density = _d(out["adj_density"])
         r.item_chi, r.iters, r.item_status = _d(out["item_chi"]), _i(out["iters"]), _i(out["item_status"])
         r.weight_out = _d(out["weight_out"])
+        r.iters_fp32 = _i(out["iters_fp32"])
         st = self.lib.satt_plan_results(self.ptr, ctypes.byref(r))
         out["status"] = st
         out["chi"] = float(out["chi"][0])
         if st < 0:
             raise SattError(st, self.ctx.error())
+        return out
+
+    def item_field(self, item, which=FIELD_T):
+        """Natural-layout field of one item of the last run's last launch group."""
+        out = np.zeros((self.b["nx"], self.b["ny"]), order="F")
+        self.ctx._check(self.lib.satt_plan_item_field(self.ptr, int(item), int(which), _d(out)))
         return out
 
     def stats(self):

@@ -29,4 +29,25 @@ def test_version_string_needs_no_device():
 def test_struct_layouts_match_header_sizes():
     # satt_batch: 4 ints, 6 pointers, int, 4 pointers... -> padded like C
     assert ctypes.sizeof(satt.Batch) == 4 * 4 + 6 * 8 + 8 + 4 * 8 + 4 * 8 + 8 + 8
-    assert ctypes.sizeof(satt.Result) == 8 * 8
+    assert ctypes.sizeof(satt.Result) == 9 * 8
+    # satt_plan_stats: 2 ints, long long, 3 doubles, 4 long longs, 6 doubles
+    assert ctypes.sizeof(satt.PlanStats) == 8 + 8 + 3 * 8 + 4 * 8 + 6 * 8
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof of the ABI structs as gcc lays them out equals the ctypes mirror."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        import pytest
+        pytest.skip("no C compiler")
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include "satt.h"\nint main(void){printf("%zu %zu %zu",'
+                   ' sizeof(satt_batch), sizeof(satt_result), sizeof(satt_plan_stats));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run([cc, "-I", os.path.dirname(satt.HEADER), str(src), "-o", str(exe)], check=True)
+    sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                            check=True).stdout.split()]
+    assert sizes == [ctypes.sizeof(satt.Batch), ctypes.sizeof(satt.Result),
+                     ctypes.sizeof(satt.PlanStats)]
