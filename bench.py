@@ -275,6 +275,9 @@ def main():
     kernels = {k: {"ms": round(st[STAGE_OF[k]] / launches, 3),
                    "alg_GBs": round(abytes[k] / (st[STAGE_OF[k]] * 1e-3) / 1e9, 1)}
                for k in abytes if st[STAGE_OF[k]] > 0}
+    kernels["fwd_setup"] = {"ms": round(st["ms_fwd_setup"] / launches, 3)}
+    kernels["adj_setup"] = {"ms": round(st["ms_adj_setup"] / launches, 3)}
+    kernels["step_device"] = {"ms": round(st["ms_total"], 3)}
 
     # mixed mode: the same step in FP64 (throughput and the error of the mixed
     # results against it), and a sample of items against the FP64 oracle

@@ -121,79 +121,83 @@ __device__ __forceinline__ size_t skidx(int c, int r, int l, int R) {
 }
 
 // One cell of the forward sweep body (src/eikon_solver_2d.f90:910-1079) with
-// every operand in registers.  The arithmetic and its order are those of the
-// reference; dis_x = sqrt(fun**2*(b/(a*b-c**2))) and dis_y = sqrt(fun**2*
-// (a/(a*b-c**2))) are per-cell constants precomputed by the same expression.
+// every operand in registers.  Every floating-point expression is the
+// reference's, in its order; what the four quadrants have in common is
+// evaluated once: px, qx and the a-terms depend only on the x neighbour (W or
+// E), py, qy and the b-terms only on the y neighbour (S or N), so e.g.
+// ak*(px*px) + bk*(py*py) - 2*ck*px*py is (A1x + B1y) - C1x*py with the same
+// operands and roundings.  Quadrants and one-sided candidates at the grid
+// boundary, which the reference skips, are evaluated and discarded.
+// dis_x = sqrt(fun**2*(b/(a*b-c**2))) and dis_y = sqrt(fun**2*(a/(a*b-c**2)))
+// are per-cell constants precomputed by the same expression.
 __device__ __forceinline__ double fwd_cell_r(int i, int j, int nx, int ny, double dx, double dy,
                                              double ak, double bk, double ck, double fk,
                                              double disx, double disy, double t0, double t0x,
                                              double t0y, double tc, double tW, double tE,
                                              double tS, double tN, double zW, double zE,
                                              double zS, double zN) {
-  double tnew = tc;
   const double t0dx = t0 / dx, t0dy = t0 / dy;
+  // x neighbour: 0 = W (i-1), 1 = E (i+1); y neighbour: 0 = S (j-1), 1 = N (j+1)
+  const double px[2] = {t0x + t0dx, t0x - t0dx};
+  const double qx[2] = {-(t0dx * tW), t0dx * tE};
+  const double py[2] = {t0y + t0dy, t0y - t0dy};
+  const double qy[2] = {-(t0dy * tS), t0dy * tN};
+  const bool hx[2] = {i > 0, i < nx - 1};
+  const bool hy[2] = {j > 0, j < ny - 1};
+  const double ak2 = 2 * ak, bk2 = 2 * bk, ck2 = 2 * ck, ff = fk * fk;
+  double A1[2], A2[2], A3[2], C1[2], C2[2], B1[2], B2[2], B3[2];
 #pragma unroll
-  for (int icase = 1; icase <= 4; ++icase) {
-    double px, qx, py, qy;
-    if (icase == 1) {
-      if (i == 0 || j == 0) continue;
-      px = (t0x + t0dx); qx = -(t0dx * tW);
-      py = (t0y + t0dy); qy = -(t0dy * tS);
-    } else if (icase == 2) {
-      if (i == 0 || j == ny - 1) continue;
-      px = (t0x + t0dx); qx = -(t0dx * tW);
-      py = (t0y - t0dy); qy = t0dy * tN;
-    } else if (icase == 3) {
-      if (i == nx - 1 || j == 0) continue;
-      px = (t0x - t0dx); qx = +(t0dx * tE);
-      py = (t0y + t0dy); qy = -(t0dy * tS);
-    } else {
-      if (i == nx - 1 || j == ny - 1) continue;
-      px = (t0x - t0dx); qx = t0dx * tE;
-      py = (t0y - t0dy); qy = t0dy * tN;
-    }
-    const double ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
-    const double eb = 2 * ak * px * qx + 2 * bk * py * qy - 2 * ck * (px * qy + py * qx);
-    const double ec = ak * (qx * qx) + bk * (qy * qy) - 2 * ck * qx * qy - fk * fk;
-    const double delta = eb * eb - 4 * ea * ec;
-    if (delta >= 0) {
-      const double sq = sqrt(delta);
-#pragma unroll
-      for (int isol = 1; isol <= 2; ++isol) {
-        const double tt = isol == 1 ? (-eb + sq) / (2 * ea) : (-eb - sq) / (2 * ea);
-        const double Tx = px * tt + qx;
-        const double Ty = py * tt + qy;
-        const double chx = ak * Tx - ck * Ty;
-        const double chy = bk * Ty - ck * Tx;
-        bool ok;
-        if (icase == 1) ok = (chx >= 0 && chy >= 0);
-        else if (icase == 2) ok = (chx >= 0 && chy <= 0);
-        else if (icase == 3) ok = (chx <= 0 && chy >= 0);
-        else ok = (chx <= 0 && chy <= 0);
-        if (ok && tt < tnew) tnew = tt;
-      }
-    }
+  for (int x = 0; x < 2; ++x) {
+    A1[x] = ak * (px[x] * px[x]);
+    A2[x] = ak2 * px[x] * qx[x];
+    A3[x] = ak * (qx[x] * qx[x]);
+    C1[x] = ck2 * px[x];
+    C2[x] = ck2 * qx[x];
   }
 #pragma unroll
-  for (int icase = 1; icase <= 4; ++icase) {
-    double p, q, dis, tn, t0n;
-    if (icase == 1) {
-      if (i == 0) continue;
-      p = (t0x + t0dx); q = -(t0dx * tW); dis = disx; tn = tW; t0n = zW;
-    } else if (icase == 2) {
-      if (i == nx - 1) continue;
-      p = (t0x - t0dx); q = t0dx * tE; dis = disx; tn = tE; t0n = zE;
-    } else if (icase == 3) {
-      if (j == 0) continue;
-      p = (t0y + t0dy); q = -(t0dy * tS); dis = disy; tn = tS; t0n = zS;
-    } else {
-      if (j == ny - 1) continue;
-      p = (t0y - t0dy); q = t0dy * tN; dis = disy; tn = tN; t0n = zN;
+  for (int y = 0; y < 2; ++y) {
+    B1[y] = bk * (py[y] * py[y]);
+    B2[y] = bk2 * py[y] * qy[y];
+    B3[y] = bk * (qy[y] * qy[y]);
+  }
+  double tnew = tc;
+  // quadrants in the reference's order: (W,S), (W,N), (E,S), (E,N); the
+  // causality test requires chx >= 0 with W, <= 0 with E, chy >= 0 with S,
+  // <= 0 with N
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int x = q >> 1, y = q & 1;
+    const double ea = A1[x] + B1[y] - C1[x] * py[y];
+    const double eb = A2[x] + B2[y] - ck2 * (px[x] * qy[y] + py[y] * qx[x]);
+    const double ec = A3[x] + B3[y] - C2[x] * qy[y] - ff;
+    const double delta = eb * eb - 4 * ea * ec;
+    const bool use = hx[x] && hy[y] && delta >= 0;
+    const double sq = sqrt(delta);
+#pragma unroll
+    for (int isol = 1; isol <= 2; ++isol) {
+      const double tt = isol == 1 ? (-eb + sq) / (2 * ea) : (-eb - sq) / (2 * ea);
+      const double Tx = px[x] * tt + qx[x];
+      const double Ty = py[y] * tt + qy[y];
+      const double chx = ak * Tx - ck * Ty;
+      const double chy = bk * Ty - ck * Tx;
+      const bool ok = (x == 0 ? chx >= 0 : chx <= 0) && (y == 0 ? chy >= 0 : chy <= 0);
+      if (use && ok && tt < tnew) tnew = tt;
     }
+  }
+  // one-sided candidates (:996-1075): W, E, S, N
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const bool isx = n < 2;
+    const int s = n & 1;
+    const double p = isx ? px[s] : py[s], q = isx ? qx[s] : qy[s];
+    const double dis = isx ? disx : disy;
+    const double tn = n == 0 ? tW : n == 1 ? tE : n == 2 ? tS : tN;
+    const double t0n = n == 0 ? zW : n == 1 ? zE : n == 2 ? zS : zN;
+    const bool use = isx ? hx[s] : hy[s];
     double tt = (dis - q) / p;
-    if (tt * t0 >= tn * t0n && tt > tn * 0.5 && tt < tnew) tnew = tt;
+    if (use && tt * t0 >= tn * t0n && tt > tn * 0.5 && tt < tnew) tnew = tt;
     tt = (-dis - q) / p;
-    if (tt * t0 >= tn * t0n && tt > tn * 0.5 && tt < tnew) tnew = tt;
+    if (use && tt * t0 >= tn * t0n && tt > tn * 0.5 && tt < tnew) tnew = tt;
   }
   return tnew;
 }
