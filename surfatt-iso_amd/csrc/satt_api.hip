@@ -84,7 +84,7 @@ struct DevGrid {
 int make_grid(satt_ctx *ctx, const double *xd, const double *yd, int nx, int ny,
               DevGrid &dg) {
   if (!xd || !yd || nx < 2 || ny < 2) return fail(ctx, SATT_E_ARG, "bad grid axes");
-  std::vector<double> h(2 * (size_t)(nx + ny));
+  std::vector<double> h(2 * (size_t)(nx + ny) + 2 * (size_t)ny);
   double *hx = h.data(), *hy = hx + nx, *hxd = hy + ny, *hyd = hxd + nx;
   for (int i = 0; i < nx; ++i) hx[i] = xd[i] / 180.0 * kPiF;
   for (int j = 0; j < ny; ++j) hy[j] = yd[j] / 180.0 * kPiF;
@@ -102,6 +102,11 @@ int make_grid(satt_ctx *ctx, const double *xd, const double *yd, int nx, int ny,
   dg.g.yy = d + nx;
   dg.g.xd = d + nx + ny;
   dg.g.yd = d + 2 * nx + ny;
+  double *sy = d + 2 * (nx + ny), *cy = sy + ny;
+  dg.g.sy = sy;
+  dg.g.cy = cy;
+  launch_grid_trig(ctx->stream, dg.g, sy, cy);
+  HIPCHK(ctx, hipGetLastError());
   return SATT_OK;
 }
 
@@ -438,7 +443,8 @@ struct satt_plan {
   DBuf fwdw, adjw;  // one-solve-per-launch scratch (one slot)
   DBuf pskew;   // per-period skewed fields (strip pipeline)
   DBuf pskew32; // the same rounded to float (mixed precision)
-  DBuf strip;   // per-slot strip scratch: skew fields, tau, snap, T, Ta, Td, delta
+  DBuf strip;   // per-slot strip scratch: skew fields, natural fields, tables
+  int slot_skews = kSlotSkews, aux_len = 0;
   SkewGeom sg{};
   PeriodFields pf{};
   Items it{};
@@ -461,10 +467,11 @@ namespace {
 
 size_t fwd_item_bytes(size_t N) { return 6 * N * sizeof(double); }
 size_t adj_item_bytes(size_t N, int dim) { return (dim == 3 ? 10 : 9) * N * sizeof(double); }
-// strip scratch of one slot: kSlotSkews skewed fields and kSlotNat natural
+// strip scratch of one slot: its skewed fields, kSlotNat natural fields and
+// the forward tables
 // fields (see satt_strip.h)
-size_t strip_slot_doubles(size_t N, const SkewGeom &sg) {
-  return (size_t)kSlotSkews * sg.skew_elems() + (size_t)kSlotNat * N;
+size_t strip_slot_doubles(size_t N, const SkewGeom &sg, int slot_skews, int aux) {
+  return (size_t)slot_skews * sg.skew_elems() + (size_t)kSlotNat * N + (size_t)aux;
 }
 
 int plan_period_fields(satt_plan *P) {
@@ -592,11 +599,14 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
   if (P->fwdw.alloc(fwd_item_bytes(N)) != hipSuccess ||
       P->adjw.alloc(adj_item_bytes(N, P->dim)) != hipSuccess)
     return bail(fail(ctx, SATT_E_NOMEM, "scratch allocation failed"));
-  // batched: as many slots as 60% of the free memory holds (every item of a
-  // 4000-solve 401x401 batch in one launch on a 288 GB device)
+  // batched: as many slots as 3/4 of the free memory holds, at most 220 GiB
+  // (every item of a 4000-solve 401x401 batch in one launch on a 288 GB
+  // device, in either precision)
   HIPCHK(ctx, hipMemGetInfo(&freeb, &totalb));
-  const size_t per_item = strip_slot_doubles(N, P->sg) * 8;
-  size_t budget = std::min(freeb / 10 * 6, (size_t)180 << 30);
+  P->slot_skews = precision == SATT_PREC_MIXED ? kSlotSkewsMixed : kSlotSkews;
+  P->aux_len = aux_len(B->nx, B->ny);
+  const size_t per_item = strip_slot_doubles(N, P->sg, P->slot_skews, P->aux_len) * 8;
+  size_t budget = std::min(freeb / 4 * 3, (size_t)220 << 30);
   size_t ch = std::max<size_t>(1, budget / per_item);
   ch = std::min<size_t>(ch, 8192);
   ch = std::min<size_t>(ch, (size_t)std::max(ni, 1));
@@ -705,12 +715,15 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   if (batched) {
     const size_t SE = P->sg.skew_elems();
     double *b0 = P->strip.as<double>();
-    double *nat = b0 + (size_t)kSlotSkews * SE * cap;
+    double *nat = b0 + (size_t)P->slot_skews * SE * cap;
     sw.skew = b0;
     sw.T = nat + (size_t)kNatT * N * cap;
     sw.Ta = nat + (size_t)kNatTa * N * cap;
     sw.Td = nat + (size_t)kNatTd * N * cap;
     sw.delta = nat + (size_t)kNatDelta * N * cap;
+    sw.aux = nat + (size_t)kSlotNat * N * cap;
+    sw.aux_len = P->aux_len;
+    sw.slot_skews = P->slot_skews;
     Ta = sw.Ta;
     Td = sw.Td;
     fwk.T = sw.T;
@@ -756,14 +769,12 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
         const int nat = q == 0 ? kNatTa : kNatTd;
         if (batched) {
           launch_adj_setup(s, P->dg.g, P->sg, pk, it, sw, rhs, P->iters + 1 + q, P->status,
-                           kIterInts, mixed ? 2 : 0);
+                           kIterInts, mixed);
           HIPCHK(ctx, mark(kStAdjSetup));
         }
         if (batched && mixed) {
           launch_adj_mixed32(s, P->dg.g, P->sg, it, sw, P->iters + 4 + q, P->status, kIterInts);
           HIPCHK(ctx, mark(kStAdj32));
-          launch_adj_build(s, P->dg.g, P->sg, pk, it, sw, P->status, 1);
-          HIPCHK(ctx, mark(kStAdjSetup));
           launch_adj_polish(s, P->dg.g, P->sg, it, sw, nat, P->iters + 1 + q, P->status,
                             kIterInts, 1);
         } else if (batched) {
@@ -912,7 +923,7 @@ extern "C" int satt_plan_item_field(satt_plan *P, int item, int which, double *o
   if (P->last_batched) {
     const int cap = P->chunk;
     const size_t SE = P->sg.skew_elems();
-    const double *nat = P->strip.as<double>() + (size_t)kSlotSkews * SE * cap;
+    const double *nat = P->strip.as<double>() + (size_t)P->slot_skews * SE * cap;
     const int nat_field = which == SATT_FIELD_T ? kNatT : which == SATT_FIELD_TA ? kNatTa : kNatTd;
     src = nat + ((size_t)nat_field * cap + (item - P->last_item0)) * N;
   } else {

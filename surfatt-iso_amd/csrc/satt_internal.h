@@ -27,6 +27,7 @@ struct Grid {
   double dx, dy;          // radians, xx(2)-xx(1), yy(2)-yy(1)
   const double *xx, *yy;  // radians
   const double *xd, *yd;  // degrees (interp2 and mask work in degrees)
+  const double *sy, *cy;  // sin(yy), cos(yy), computed on the device
 };
 
 // Per-period fields in natural layout, period p at offset p*nx*ny.
@@ -69,6 +70,8 @@ struct RecWork {
 };
 
 // ---- launches (satt_kernels.hip); all asynchronous on `s`
+// sin(yy), cos(yy) of a grid into sy, cy (satt_strip.hip)
+void launch_grid_trig(hipStream_t s, const Grid &g, double *sy, double *cy);
 void launch_period_fields(hipStream_t s, const Grid &g, int nperiod,
                           const double *svel_pm, const double *m11_pm,
                           const double *m12_pm, const double *m22_pm,

@@ -40,17 +40,29 @@ constexpr int kSlotNat = 4;
 enum : int { kSkTauP = 6, kSkTauM = 7, kSkSnapF = 8, kSkTP = 12, kSkTM = 13,
              kSkTaP = 14, kSkTaM = 15, kSkSnapA = 16 };
 enum : int { kNatT = 0, kNatTa = 1, kNatTd = 2, kNatDelta = 3 };
+// The mixed mode adds 6 skewed fields per slot: the adjoint's float fields
+// (coefficients k0..kN in both skews, Ta plus and minus, snapshot) from
+// double skew 16, beside the FP64 coefficients of its polish (0..11).
+constexpr int kSlotSkewsMixed = 23;
+constexpr int kSkA32 = 16;
+
+// Per-slot forward tables (aux): sin and cos of (xx - x0) per column, the
+// frozen flags of the columns and the rows, then fun0, sin(y0), cos(y0).
+inline int aux_len(int nx, int ny) { return (3 * nx + ny + 4 + 1) & ~1; }
 
 struct StripWork {
-  double *skew;                // [slot][kSlotSkews][SE]
+  double *skew;                // [slot][slot_skews][SE]
   double *T, *Ta, *Td, *delta; // natural layout, [slot][N] each
+  double *aux;                 // [slot][aux_len]
+  int aux_len, slot_skews;
 };
 
 // out32 (may be null): the same fields rounded to float, same layout.
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
                         const PeriodFields &pf, double *out, float *out32);
 
-// Forward: setup (T0 skews, tau init; one thread per element), then the
+// Forward: setup (per-item tables, then T0 skews and the tau init, one
+// thread per element), then the
 // sweeps: k_fwd_strip (FP64), or k_fwd_mixed32 (FP32 phase, iteration count
 // to iters32) followed by k_fwd_polish (FP64 to the reference's stop rule).
 void launch_fwd_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
@@ -65,16 +77,15 @@ void launch_fwd_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const d
                        const PeriodFields &pf, const Items &it, const StripWork &w, int *iters,
                        int *status, int iter_stride);
 
-// Adjoint: setup (delta source from rhs per receiver, then the coefficients:
-// kind 0 FP64 with the Ta init, 2 FP32 with its Ta init), then the sweeps:
-// k_adj_strip (FP64), or k_adj_mixed32 followed by launch_adj_build(kind 1)
-// and k_adj_polish (one FP64 iteration).  out_nat: kNatTa (adjoint) or
-// kNatTd (density adjoint).
+// Adjoint: setup (the coefficients without the source -- FP64 with the Ta
+// init, and in the mixed mode the float set too -- then the delta source
+// from rhs per receiver, accumulated in natural layout and scattered into
+// the coefficient fields), then the sweeps: k_adj_strip (FP64), or
+// k_adj_mixed32 followed by k_adj_polish (one FP64 iteration).  out_nat:
+// kNatTa (adjoint) or kNatTd (density adjoint).
 void launch_adj_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
                       const Items &it, const StripWork &w, const double *rhs, int *iters,
-                      int *status, int iter_stride, int kind);
-void launch_adj_build(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const Items &it, const StripWork &w, const int *status, int kind);
+                      int *status, int iter_stride, bool mixed);
 void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
                       const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
                       int apply_mask);

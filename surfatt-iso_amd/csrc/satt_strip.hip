@@ -547,30 +547,78 @@ __device__ bool fwd_iteration(const FwdCtx<V> &C, int ns, V *tP, V *tM) {
 // until the output T skews (12, 13) are written after the FP32 phase.
 constexpr int kSkF32 = 9;
 
+// sin and cos of the grid's latitudes, once per grid.
+__global__ void k_grid_trig(Grid g, double *sy, double *cy) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < g.ny) {
+    sy[j] = sin(g.yy[j]);
+    cy[j] = cos(g.yy[j]);
+  }
+}
+
+// Per-item forward tables, one thread per column / row: sin(xx - x0),
+// cos(xx - x0), the frozen flags abs((xx-x0)/dx) <= 1 and abs((yy-y0)/dy) <= 1
+// (src/eikon_solver_2d.f90:859-872), and fun0 (:825-841), sin(y0), cos(y0).
+// An item whose source bracket leaves the grid gets its error status.
+__global__ __launch_bounds__(64) void k_fwd_prep(Grid g, const double *fun_nat, Items it,
+                                                 StripWork w, int *status) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slot = blockIdx.y, item = it.item0 + slot;
+  const int nx = g.nx, ny = g.ny;
+  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
+  double *aux = w.aux + (size_t)slot * w.aux_len;
+  if (k < nx) {
+    aux[k] = sin(g.xx[k] - x0);
+    aux[nx + k] = cos(g.xx[k] - x0);
+    aux[2 * nx + k] = fabs((g.xx[k] - x0) / g.dx) <= 1 ? 1.0 : 0.0;
+  }
+  if (k < ny) aux[3 * nx + k] = fabs((g.yy[k] - y0) / g.dy) <= 1 ? 1.0 : 0.0;
+  if (k == 0) {
+    double fun0 = 0;
+    const int st0 = fwd_fun0_s(g, fun_nat + (size_t)it.pidx[item] * nx * ny, x0, y0, &fun0);
+    aux[3 * nx + ny] = fun0;
+    aux[3 * nx + ny + 1] = sin(y0);
+    aux[3 * nx + ny + 2] = cos(y0);
+    if (st0 != kOk) status[item] = st0;
+  }
+}
+
+// T0, T0x, T0y of one cell (src/eikon_solver_2d.f90:845-857) from the
+// row's sin/cos of the latitude and the column's sin/cos of the longitude
+// difference: the expressions of t0_cell_s, with the same operand values.
+__device__ __forceinline__ void t0_cell_tab(double sy, double cy, double sdx, double cdx,
+                                            double sy0, double cy0, double fun0, double &T0,
+                                            double &T0x, double &T0y) {
+  const double t = sy * sy0 + cy * cy0 * cdx;
+  double tc = t < 1.0 ? t : 1.0;
+  tc = tc > -1.0 ? tc : -1.0;
+  T0 = acos(tc) * kREarth * fun0;
+  if (T0 == 0) {
+    T0x = 0;
+    T0y = 0;
+  } else {
+    const double s = sqrt(1 - t * t);
+    T0x = -(1.0 / s * cy * cy0 * (-1.0) * sdx * kREarth * fun0);
+    T0y = -(1.0 / s * (cy * sy0 - sy * cy0 * cdx) * kREarth * fun0);
+  }
+}
+
 // Forward setup of every item of a chunk, one thread per skew element:
-// T0, T0x, T0y (src/eikon_solver_2d.f90:845-857) in both skews (and, MIXED,
-// rounded to float), and the tau init in the plus skew (:859-872: 1 on the
-// frozen cells, 10 elsewhere; MIXED: dtau = tau - 1 in float).  Padding
-// elements get zeros.  An item whose source bracket leaves the grid or whose
-// frozen set touches the boundary gets its error status (:863-867).
+// T0, T0x, T0y in both skews (and, MIXED, rounded to float), and the tau
+// init in the plus skew (:859-872: 1 on the frozen cells, 10 elsewhere;
+// MIXED: dtau = tau - 1 in float).  Padding elements get zeros.  A frozen
+// cell on the boundary is the reference's 'source on the boundary' error
+// (:863-867).
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_fwd_setup(Grid g, SkewGeom sg, const double *fun_nat,
-                                                   Items it, StripWork w, int *status) {
+__global__ __launch_bounds__(256) void k_fwd_setup(Grid g, SkewGeom sg, Items it, StripWork w,
+                                                   int *status) {
   const size_t SE = sg.skew_elems();
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const int sk = blockIdx.y, slot = blockIdx.z;
   const int item = it.item0 + slot;
-  if (e >= SE) return;
+  if (e >= SE || status[item] < 0) return;
   const int nx = g.nx, ny = g.ny;
-  const size_t N = (size_t)nx * ny;
-  const int p = it.pidx[item];
-  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
-  double fun0 = 0;
-  const int st0 = fwd_fun0_s(g, fun_nat + (size_t)p * N, x0, y0, &fun0);
-  if (st0 != kOk) {
-    if (e == 0 && sk == 0) status[item] = st0;
-    return;
-  }
+  const double *aux = w.aux + (size_t)slot * w.aux_len;
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % sg.R);
   const int cc = (int)((e >> 6) / sg.R);
@@ -580,11 +628,12 @@ __global__ __launch_bounds__(256) void k_fwd_setup(Grid g, SkewGeom sg, const do
   double t0 = 0, t0x = 0, t0y = 0;
   bool frz = false;
   if (valid) {
-    t0_cell_s(g.xx[i], g.yy[j], x0, sin(y0), cos(y0), fun0, t0, t0x, t0y);
-    frz = fabs((g.xx[i] - x0) / g.dx) <= 1 && fabs((g.yy[j] - y0) / g.dy) <= 1;
+    const double *tail = aux + 3 * nx + ny;
+    t0_cell_tab(g.sy[j], g.cy[j], aux[i], aux[nx + i], tail[1], tail[2], tail[0], t0, t0x, t0y);
+    frz = aux[2 * nx + i] != 0.0 && aux[3 * nx + j] != 0.0;
     if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) status[item] = kESrcOnBoundary;
   }
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   double *b0 = sk0 + (size_t)sk * kFF * SE;
   b0[e] = t0; b0[SE + e] = t0x; b0[2 * SE + e] = t0y;
   if (MIXED) {
@@ -668,7 +717,7 @@ __global__ __launch_bounds__(64) void k_fwd_strip(Grid g, SkewGeom sg, const dou
   const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
   int rect[4];
   if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   fwd_finish64(g, sg, lane, pskew + (size_t)it.pidx[item] * 2 * kPF * SE, sk0, rect,
                w.T + (size_t)slot * N, item, iters, status, iter_stride);
 }
@@ -686,7 +735,7 @@ __global__ __launch_bounds__(64) void k_fwd_mixed32(Grid g, SkewGeom sg, const f
   const size_t SE = sg.skew_elems();
   int rect[4];
   if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
   FwdCtx<float> C;
   fwd_ctx_init(C, g, sg, lane, rect);
@@ -719,7 +768,7 @@ __global__ __launch_bounds__(64) void k_fwd_polish(Grid g, SkewGeom sg, const do
   const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
   int rect[4];
   if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   fwd_finish64(g, sg, lane, pskew + (size_t)it.pidx[item] * 2 * kPF * SE, sk0, rect,
                w.T + (size_t)slot * N, item, iters, status, iter_stride);
 }
@@ -776,28 +825,14 @@ __global__ __launch_bounds__(256) void k_adj_zero(size_t N, Items it, StripWork 
   if (k < N) w.delta[(size_t)blockIdx.y * N + k] = 0;
 }
 
-// ... then splat each item's receivers in order, one thread per item.
-__global__ __launch_bounds__(64) void k_adj_splat(Grid g, Items it, StripWork w, const double *rhs,
-                                                  int *iters, int *status, int iter_stride) {
-  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= it.count) return;
-  const int item = it.item0 + slot;
-  if (status[item] < 0) return;  // an earlier stage failed
-  const int r0 = it.rec_off[item], nr = it.rec_off[item + 1] - r0;
-  const int err = adj_splat_s(g, it.recx + r0, it.recy + r0, rhs + r0, nr,
-                              w.delta + (size_t)slot * g.nx * g.ny);
-  if (err != kOk) {
-    status[item] = err;
-    iters[item * iter_stride] = 0;
-  }
-}
-
 // Coefficients of every item of a chunk, one thread per skew element; the
 // neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
-//   KIND 0: FP64 a1, a2, b1, b2, d, delta and Ta = 100 inside, 0 on the
-//           boundary (:1368-1380), in the plus skew
-//   KIND 1: FP64 coefficients only (the mixed polish)
-//   KIND 2: FP32 k0, kW, kE, kS, kN and the FP32 Ta init
+// The source field is left 0 here and scattered in by k_adj_splat.
+//   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
+//           the boundary (:1368-1380), in the plus skew
+//   KIND 1: the same without the Ta init, plus (mixed mode) the float set
+//           k0 (0 here), kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
+//           kN = -b2m/dy/d and the float Ta init, from double skew 16
 // Padding elements get zeros.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_adj_build(Grid g, SkewGeom sg, const double *pskew,
@@ -809,11 +844,10 @@ __global__ __launch_bounds__(256) void k_adj_build(Grid g, SkewGeom sg, const do
   if (e >= SE || status[item] < 0) return;
   const int nx = g.nx, ny = g.ny, R = sg.R;
   const double dx = g.dx, dy = g.dy;
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   const double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
   const double *pa = pskew + ((size_t)it.pidx[item] * 2 + sk) * kPF * SE;
   const double *pb = pa + SE, *pc = pa + 2 * SE;
-  const double *delta = w.delta + (size_t)slot * nx * ny;
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % R);
   const int cc = (int)((e >> 6) / R);
@@ -827,30 +861,72 @@ __global__ __launch_bounds__(256) void k_adj_build(Grid g, SkewGeom sg, const do
     adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
                Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
                pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
-    v[5] = delta[i + (size_t)nx * j];
   }
-  if (KIND != 2) {
-    double *b0 = sk0 + (size_t)sk * kAF * SE;
+  double *b0 = sk0 + (size_t)sk * kAF * SE;
 #pragma unroll
-    for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
-    if (KIND == 0 && sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
+  for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
+  if (KIND == 0) {
+    if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
   } else {
     float k[5] = {0, 0, 0, 0, 0};
     if (interior && v[4] != 0.0) {
       const double d = v[4];
       const double a1p = (v[0] + fabs(v[0])) / 2, a2m = (v[1] - fabs(v[1])) / 2;
       const double b1p = (v[2] + fabs(v[2])) / 2, b2m = (v[3] - fabs(v[3])) / 2;
-      k[0] = (float)(v[5] / d);
       k[1] = (float)(a1p / dx / d);
       k[2] = (float)(-a2m / dx / d);
       k[3] = (float)(b1p / dy / d);
       k[4] = (float)(-b2m / dy / d);
     }
-    float *f32 = reinterpret_cast<float *>(sk0);
-    float *b0 = f32 + (size_t)sk * 5 * SE;
+    float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
+    float *k32 = f32 + (size_t)sk * 5 * SE;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) b0[f * SE + e] = k[f];
+    for (int f = 0; f < 5; ++f) k32[f * SE + e] = k[f];
     if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
+  }
+}
+
+// ... then, one thread per item: splat the receivers in order into the
+// natural field (the reference's sums, in its order), and scatter the
+// source of every touched interior cell into the coefficient fields of both
+// skews (MIXED: also k0 = delta/d rounded to float, where d != 0).
+__global__ __launch_bounds__(64) void k_adj_splat(Grid g, SkewGeom sg, Items it, StripWork w,
+                                                  const double *rhs, int *iters, int *status,
+                                                  int iter_stride, int mixed) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= it.count) return;
+  const int item = it.item0 + slot;
+  if (status[item] < 0) return;  // an earlier stage failed
+  const int nx = g.nx, ny = g.ny, R = sg.R;
+  const size_t SE = sg.skew_elems();
+  double *delta = w.delta + (size_t)slot * nx * ny;
+  const int r0 = it.rec_off[item], nr = it.rec_off[item + 1] - r0;
+  const int err = adj_splat_s(g, it.recx + r0, it.recy + r0, rhs + r0, nr, delta);
+  if (err != kOk) {
+    status[item] = err;
+    iters[item * iter_stride] = 0;
+    return;
+  }
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  float *k32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
+  for (int q = 0; q < nr; ++q) {
+    const double xr = deg2rad_s(it.recx[r0 + q]), yr = deg2rad_s(it.recy[r0 + q]);
+    const int i0 = (int)floor((xr - g.xx[0]) / g.dx + 1) - 1;
+    const int j0 = (int)floor((yr - g.yy[0]) / g.dy + 1) - 1;
+    for (int c = 0; c < 4; ++c) {
+      const int i = i0 + (c >> 1), j = j0 + (c & 1);
+      if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
+      const double dl = delta[i + (size_t)nx * j];
+      for (int sk = 0; sk < 2; ++sk) {
+        const size_t e = skpos(sk, i, j, R);
+        double *b0 = sk0 + (size_t)sk * kAF * SE;
+        b0[5 * SE + e] = dl;
+        if (mixed) {
+          const double d = b0[4 * SE + e];
+          k32[(size_t)sk * 5 * SE + e] = d != 0.0 ? (float)(dl / d) : 0.0f;
+        }
+      }
+    }
   }
 }
 
@@ -1037,7 +1113,7 @@ __global__ __launch_bounds__(64) void k_adj_strip(Grid g, SkewGeom sg, Items it,
   const int lane = threadIdx.x;
   if (status[item] < 0) return;  // an earlier stage failed
   const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   double *Tout = (out_nat == kNatTd ? w.Td : w.Ta) + (size_t)slot * N;
   AdjCtx<double> C;
@@ -1062,7 +1138,7 @@ __global__ __launch_bounds__(64) void k_adj_strip(Grid g, SkewGeom sg, Items it,
 }
 
 // Mixed adjoint, FP32 phase: iterations on the float coefficients (float
-// skews from the slot base: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
+// skews from double skew 16: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
 // minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
 __global__ __launch_bounds__(64) void k_adj_mixed32(Grid g, SkewGeom sg, Items it, StripWork w,
                                                     int *iters32, int *status,
@@ -1072,8 +1148,8 @@ __global__ __launch_bounds__(64) void k_adj_mixed32(Grid g, SkewGeom sg, Items i
   const int lane = threadIdx.x;
   if (status[item] < 0) return;
   const size_t SE = sg.skew_elems();
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
-  float *f32 = reinterpret_cast<float *>(sk0);
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
   float *taP = f32 + 10 * SE, *taM = f32 + 11 * SE;
   AdjCtx<float> C;
   adj_ctx_init(C, g, sg, lane);
@@ -1101,7 +1177,7 @@ __global__ __launch_bounds__(64) void k_adj_mixed32(Grid g, SkewGeom sg, Items i
 }
 
 // Mixed adjoint, FP64 polish: one iteration of the reference's update on the
-// FP64 coefficients (built after the FP32 phase), and the masked output.
+// FP64 coefficients, and the masked output.
 __global__ __launch_bounds__(64) void k_adj_polish(Grid g, SkewGeom sg, Items it, StripWork w,
                                                    int out_nat, int *iters, int *status,
                                                    int iter_stride, int apply_mask) {
@@ -1110,7 +1186,7 @@ __global__ __launch_bounds__(64) void k_adj_polish(Grid g, SkewGeom sg, Items it
   const int lane = threadIdx.x;
   if (status[item] < 0) return;
   const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
-  double *sk0 = w.skew + (size_t)slot * kSlotSkews * SE;
+  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
   adj_ctx_init(C, g, sg, lane);
@@ -1133,13 +1209,20 @@ void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np
                      g, sg, np, pf.a, pf.b, pf.c, pf.fun, out, out32);
 }
 
+void launch_grid_trig(hipStream_t s, const Grid &g, double *sy, double *cy) {
+  hipLaunchKernelGGL(k_grid_trig, dim3((unsigned)((g.ny + 63) / 64)), dim3(64), 0, s, g, sy, cy);
+}
+
 void launch_fwd_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
                       const Items &it, const StripWork &w, int *status, bool mixed) {
+  const int nk = g.nx > g.ny ? g.nx : g.ny;
+  hipLaunchKernelGGL(k_fwd_prep, dim3((unsigned)((nk + 63) / 64), it.count), dim3(64), 0, s, g,
+                     pf.fun, it, w, status);
   const dim3 grid((unsigned)((sg.skew_elems() + 255) / 256), 2, it.count);
   if (mixed)
-    hipLaunchKernelGGL(k_fwd_setup<true>, grid, dim3(256), 0, s, g, sg, pf.fun, it, w, status);
+    hipLaunchKernelGGL(k_fwd_setup<true>, grid, dim3(256), 0, s, g, sg, it, w, status);
   else
-    hipLaunchKernelGGL(k_fwd_setup<false>, grid, dim3(256), 0, s, g, sg, pf.fun, it, w, status);
+    hipLaunchKernelGGL(k_fwd_setup<false>, grid, dim3(256), 0, s, g, sg, it, w, status);
 }
 
 void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
@@ -1165,24 +1248,17 @@ void launch_fwd_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const d
 
 void launch_adj_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
                       const Items &it, const StripWork &w, const double *rhs, int *iters,
-                      int *status, int iter_stride, int kind) {
+                      int *status, int iter_stride, bool mixed) {
   const size_t N = (size_t)g.nx * g.ny;
   hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((N + 255) / 256), it.count), dim3(256), 0, s, N,
                      it, w);
-  hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((it.count + 63) / 64)), dim3(64), 0, s, g, it,
-                     w, rhs, iters, status, iter_stride);
-  launch_adj_build(s, g, sg, pskew, it, w, status, kind);
-}
-
-void launch_adj_build(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const Items &it, const StripWork &w, const int *status, int kind) {
   const dim3 grid((unsigned)((sg.skew_elems() + 255) / 256), 2, it.count);
-  if (kind == 0)
-    hipLaunchKernelGGL(k_adj_build<0>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
-  else if (kind == 1)
+  if (mixed)
     hipLaunchKernelGGL(k_adj_build<1>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
   else
-    hipLaunchKernelGGL(k_adj_build<2>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
+    hipLaunchKernelGGL(k_adj_build<0>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
+  hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((it.count + 63) / 64)), dim3(64), 0, s, g, sg,
+                     it, w, rhs, iters, status, iter_stride, (int)mixed);
 }
 
 void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
