@@ -151,14 +151,25 @@ int satt_forward_simulate(satt_ctx *ctx, const satt_batch *batch,
  * stream; satt_plan_results waits for it and downloads. */
 int satt_plan_create(satt_ctx *ctx, const satt_batch *batch, int precision,
                      satt_plan **plan);
+/* Several grids in one plan (uneven batching): batch k is part k, with its
+ * own grid, model, items and outputs; the parts' items share each launch
+ * (the most expensive grid first).  All parts have the same dim.  A plan of
+ * one part is satt_plan_create's.  No counterpart in the reference, whose
+ * runs hold one grid. */
+int satt_plan_create_multi(satt_ctx *ctx, int nbatch, const satt_batch *batches,
+                           int precision, satt_plan **plan);
 /* Replace the model (same shapes) for the next run, e.g. after a model
  * update or inside a line search (src/tomo2d.f90:272). */
 int satt_plan_set_model(satt_plan *plan, const double *svel, const double *m11,
                         const double *m12, const double *m22);
+int satt_plan_set_model_part(satt_plan *plan, int part, const double *svel,
+                             const double *m11, const double *m12, const double *m22);
 int satt_plan_run(satt_plan *plan, int istotable, int isadj, int launch_form);
 /* Wait for the plan's last run (no download). */
 int satt_plan_wait(satt_plan *plan);
 int satt_plan_results(satt_plan *plan, satt_result *result);
+/* The results of part `part` (satt_plan_results: part 0). */
+int satt_plan_results_part(satt_plan *plan, int part, satt_result *result);
 void satt_plan_destroy(satt_plan *plan);
 
 /* Sizing and timing hooks of a plan (for measurement harnesses). */
@@ -182,7 +193,7 @@ typedef struct {
 } satt_plan_stats;
 int satt_plan_get_stats(satt_plan *plan, satt_plan_stats *stats);
 
-/* A field of one item of the plan's last batched run, in natural layout
+/* A field of one item of part 0 in the plan's last run, in natural layout
  * (nx,ny): SATT_FIELD_T travel times, SATT_FIELD_TA adjoint field (masked),
  * SATT_FIELD_TD density adjoint field (dim 3).  Only items of the run's last
  * launch group are resident; others give SATT_E_ARG. */

@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <memory>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -425,38 +428,60 @@ constexpr int kIterInts = 6;
 enum : int { kStFwdSetup, kStFwd64, kStFwd32, kStAdjSetup, kStAdj64, kStAdj32, kStOther,
              kStCount };
 
-struct satt_plan {
-  satt_ctx *ctx = nullptr;
-  int dim = 2, nx = 0, ny = 0, np = 0, nitems = 0, npath = 0, nrec = 0, precision = 0;
+namespace {
+
+// One grid of a plan (one satt_batch): its model, items, receivers and
+// outputs on the device, and the host copies that descriptors and lists are
+// built from.
+struct Part {
+  int dim = 2, nx = 0, ny = 0, np = 0, nitems = 0, npath = 0, nrec = 0;
   size_t N = 0;
-  int chunk = 1, nchunks = 0;
   DevGrid dg;
-  DBuf model;   // period-major svel, m11, m12, m22
-  DBuf pfields; // per-period a, b, c, fun, vel (natural layout)
-  DBuf items;   // pidx, rec_off, srcx, srcy
-  DBuf recs;    // recx, recy, tt, weight, idx_path
-  DBuf rec_out; // tekio, weight_out, rhs_adj, rhs_den
-  DBuf item_out;// item_chi, iters(3), status
-  DBuf tt_fwd;
-  DBuf acc;     // stacked kernels, natural then period-major staging
-  DBuf lists;   // per chunk: per_off (np+1), per_slot (chunk)
-  DBuf fwdw, adjw;  // one-solve-per-launch scratch (one slot)
-  DBuf pskew;   // per-period skewed fields (strip pipeline)
-  DBuf pskew32; // the same rounded to float (mixed precision)
-  DBuf strip;   // per-slot strip scratch: skew fields, natural fields, tables
-  int slot_skews = kSlotSkews, aux_len = 0;
   SkewGeom sg{};
+  DBuf model;    // period-major svel, m11, m12, m22
+  DBuf pfields;  // per-period a, b, c, fun, vel (natural layout)
+  DBuf pskew;    // per-period skewed fields
+  DBuf pskew32;  // the same in float (mixed precision)
+  DBuf items;    // pidx, rec_off, srcx, srcy
+  DBuf recs;     // recx, recy, tt, weight, idx_path
+  DBuf rec_out;  // tekio, weight_out, rhs_adj, rhs_den
+  DBuf item_out; // item_chi, iters (kIterInts), status
+  DBuf tt_fwd;
+  DBuf acc;      // stacked kernels: natural, then period-major
   PeriodFields pf{};
   Items it{};
   double *tekio = nullptr, *weight_out = nullptr, *rhs_adj = nullptr, *rhs_den = nullptr;
   double *item_chi = nullptr;
   int *iters = nullptr, *status = nullptr;
-  std::vector<int> h_per_off, h_per_slot;  // host copies of the chunk lists
-  int *d_per_off = nullptr, *d_per_slot = nullptr;
+  std::vector<int> h_pidx, h_rec_off;
+  std::vector<double> h_srcx, h_srcy;
+};
+
+// One launch group of the batched form: the descriptors of its slots and,
+// per part, the stacking lists (the part's slots of each period, in item
+// order).
+struct Group {
+  GroupDims gd{};
+  SlotDesc *desc = nullptr;
+  std::vector<std::pair<int, int>> slots;  // (part, item)
+  std::vector<const int *> per_off, per_slot;  // per part; null: no slot
+};
+
+}  // namespace
+
+struct satt_plan {
+  satt_ctx *ctx = nullptr;
+  int precision = 0, dim = 2, slot_skews = kSlotSkews;
+  std::vector<std::unique_ptr<Part>> parts;
+  std::vector<Group> groups;
+  DBuf strip;    // slot scratch of the largest launch group
+  DBuf desc;     // descriptors of every group, then the per-solve ones
+  DBuf lists;    // stacking lists of every group
+  DBuf fwdw, adjw;            // one-solve-per-launch scratch (one-grid plans)
+  SlotDesc *d_solo = nullptr; // one-solve-per-launch descriptors, one per item
   std::vector<hipEvent_t> ev;
   std::vector<int> marks;  // stage of the interval ev[k]..ev[k+1]
-  int last_istotable = 0, last_isadj = 0, ran = 0;
-  int last_item0 = 0, last_count = 0, last_batched = 0;  // resident slots
+  int last_istotable = 0, last_isadj = 0, ran = 0, last_batched = 0, last_group = -1;
   satt_plan_stats stats{};
   ~satt_plan() {
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -467,97 +492,91 @@ namespace {
 
 size_t fwd_item_bytes(size_t N) { return 6 * N * sizeof(double); }
 size_t adj_item_bytes(size_t N, int dim) { return (dim == 3 ? 10 : 9) * N * sizeof(double); }
-// strip scratch of one slot: its skewed fields, kSlotNat natural fields and
-// the forward tables
-// fields (see satt_strip.h)
-size_t strip_slot_doubles(size_t N, const SkewGeom &sg, int slot_skews, int aux) {
-  return (size_t)slot_skews * sg.skew_elems() + (size_t)kSlotNat * N + (size_t)aux;
+
+int check_batch(satt_ctx *ctx, const satt_batch *B, int k) {
+  if (!B || (B->dim != 2 && B->dim != 3) || B->nx < 3 || B->ny < 3 || B->nperiod < 1 ||
+      B->nitems < 0 || !B->xgrids || !B->ygrids || !B->svel || !B->m11 || !B->m12 || !B->m22 ||
+      (B->nitems > 0 && (!B->item_pidx || !B->item_srcx || !B->item_srcy || !B->rec_off)))
+    return fail(ctx, SATT_E_ARG, "batch %d: bad arguments", k);
+  const int nrec = B->nitems > 0 ? B->rec_off[B->nitems] : 0;
+  if (nrec < 0 || (nrec > 0 && (!B->recx || !B->recy || !B->tt || !B->weight || !B->idx_path)))
+    return fail(ctx, SATT_E_ARG, "batch %d: bad receiver gather", k);
+  if (B->nitems > 0 && B->rec_off[0] != 0)
+    return fail(ctx, SATT_E_ARG, "batch %d: receiver offsets must start at 0", k);
+  for (int i = 0; i < B->nitems; ++i) {
+    if (B->item_pidx[i] < 0 || B->item_pidx[i] >= B->nperiod)
+      return fail(ctx, SATT_E_ARG, "batch %d item %d: period index %d out of range", k, i,
+                  B->item_pidx[i]);
+    if (B->rec_off[i + 1] < B->rec_off[i])
+      return fail(ctx, SATT_E_ARG, "batch %d item %d: receiver offsets decrease", k, i);
+  }
+  for (int r = 0; r < nrec; ++r)
+    if (B->idx_path[r] < 0 || B->idx_path[r] >= B->npath)
+      return fail(ctx, SATT_E_ARG, "batch %d receiver %d: path index out of range", k, r);
+  return SATT_OK;
 }
 
-int plan_period_fields(satt_plan *P) {
-  satt_ctx *ctx = P->ctx;
-  const size_t N = P->N;
-  const double *m = P->model.as<double>();
-  const size_t M = (size_t)P->np * N;
-  double *f = P->pfields.as<double>();
-  launch_period_fields(ctx->stream, P->dg.g, P->np, m, m + M, m + 2 * M, m + 3 * M, f, f + M,
+int part_period_fields(satt_ctx *ctx, Part &Q) {
+  const size_t N = Q.N, M = (size_t)Q.np * N;
+  const double *m = Q.model.as<double>();
+  double *f = Q.pfields.as<double>();
+  launch_period_fields(ctx->stream, Q.dg.g, Q.np, m, m + M, m + 2 * M, m + 3 * M, f, f + M,
                        f + 2 * M, f + 3 * M, f + 4 * M);
-  P->pf.a = f;
-  P->pf.b = f + M;
-  P->pf.c = f + 2 * M;
-  P->pf.fun = f + 3 * M;
-  P->pf.vel = f + 4 * M;
-  if (P->pskew.p)
-    launch_period_skew(ctx->stream, P->dg.g, P->sg, P->np, P->pf, P->pskew.as<double>(),
-                       P->pskew32.p ? P->pskew32.as<float>() : nullptr);
+  Q.pf.a = f;
+  Q.pf.b = f + M;
+  Q.pf.c = f + 2 * M;
+  Q.pf.fun = f + 3 * M;
+  Q.pf.vel = f + 4 * M;
+  launch_period_skew(ctx->stream, Q.dg.g, Q.sg, Q.np, Q.pf, Q.pskew.as<double>(),
+                     Q.pskew32.p ? Q.pskew32.as<float>() : nullptr);
   HIPCHK(ctx, hipGetLastError());
   return SATT_OK;
 }
 
-}  // namespace
+int part_set_model(satt_ctx *ctx, Part &Q, const double *svel, const double *m11,
+                   const double *m12, const double *m22) {
+  if (!svel || !m11 || !m12 || !m22) return fail(ctx, SATT_E_ARG, "set_model: null model");
+  const size_t M = (size_t)Q.np * Q.N;
+  double *m = Q.model.as<double>();
+  const double *src[4] = {svel, m11, m12, m22};
+  for (int q = 0; q < 4; ++q)
+    HIPCHK(ctx, hipMemcpyAsync(m + q * M, src[q], M * 8, hipMemcpyHostToDevice, ctx->stream));
+  return part_period_fields(ctx, Q);
+}
 
-extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precision,
-                                satt_plan **out) {
-  if (!ctx || !out) return SATT_E_ARG;
-  *out = nullptr;
-  if (!B || (B->dim != 2 && B->dim != 3) || B->nx < 3 || B->ny < 3 || B->nperiod < 1 ||
-      B->nitems < 0 || !B->svel || !B->m11 || !B->m12 || !B->m22 ||
-      (B->nitems > 0 && (!B->item_pidx || !B->item_srcx || !B->item_srcy || !B->rec_off)))
-    return fail(ctx, SATT_E_ARG, "satt_plan_create: bad batch");
-  if (precision != SATT_PREC_FP64 && precision != SATT_PREC_MIXED)
-    return fail(ctx, SATT_E_ARG, "satt_plan_create: unknown precision %d", precision);
-  const int nrec = B->nitems > 0 ? B->rec_off[B->nitems] : 0;
-  if (nrec < 0 || (nrec > 0 && (!B->recx || !B->recy || !B->tt || !B->weight || !B->idx_path)))
-    return fail(ctx, SATT_E_ARG, "satt_plan_create: bad receiver gather");
-  for (int i = 0; i < B->nitems; ++i) {
-    if (B->item_pidx[i] < 0 || B->item_pidx[i] >= B->nperiod)
-      return fail(ctx, SATT_E_ARG, "item %d: period index %d out of range", i, B->item_pidx[i]);
-    if (B->rec_off[i + 1] < B->rec_off[i])
-      return fail(ctx, SATT_E_ARG, "item %d: receiver offsets decrease", i);
-  }
-  for (int r = 0; r < nrec; ++r)
-    if (B->idx_path[r] < 0 || B->idx_path[r] >= B->npath)
-      return fail(ctx, SATT_E_ARG, "receiver %d: path index out of range", r);
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  auto *P = new satt_plan();
-  auto bail = [&](int st) { delete P; return st; };
-  P->ctx = ctx;
-  P->dim = B->dim;
-  P->nx = B->nx;
-  P->ny = B->ny;
-  P->np = B->nperiod;
-  P->nitems = B->nitems;
-  P->npath = B->npath;
-  P->nrec = nrec;
-  P->precision = precision;
-  P->N = (size_t)B->nx * B->ny;
-  const size_t N = P->N, M = (size_t)P->np * N;
-  int st;
-  if ((st = make_grid(ctx, B->xgrids, B->ygrids, B->nx, B->ny, P->dg)) < 0) return bail(st);
-  P->sg = make_skew_geom(B->nx, B->ny);
-  const size_t pskew_elems = (size_t)P->np * 2 * kPF * P->sg.skew_elems();
-  if (P->model.alloc(4 * M * 8) != hipSuccess || P->pfields.alloc(5 * M * 8) != hipSuccess ||
-      P->pskew.alloc(pskew_elems * 8) != hipSuccess ||
-      (precision == SATT_PREC_MIXED && P->pskew32.alloc(pskew_elems * 4) != hipSuccess))
-    return bail(fail(ctx, SATT_E_NOMEM, "model allocation failed"));
-  if ((st = satt_plan_set_model(P, B->svel, B->m11, B->m12, B->m22)) < 0) return bail(st);
-
-  // items and receivers
-  const int ni = P->nitems;
+int part_create(satt_ctx *ctx, Part &Q, const satt_batch *B, int precision) {
+  Q.dim = B->dim;
+  Q.nx = B->nx;
+  Q.ny = B->ny;
+  Q.np = B->nperiod;
+  Q.nitems = B->nitems;
+  Q.npath = B->npath;
+  Q.nrec = B->nitems > 0 ? B->rec_off[B->nitems] : 0;
+  Q.N = (size_t)B->nx * B->ny;
+  const size_t N = Q.N, M = (size_t)Q.np * N;
+  const int ni = Q.nitems, nrec = Q.nrec;
+  TRY(make_grid(ctx, B->xgrids, B->ygrids, B->nx, B->ny, Q.dg));
+  Q.sg = make_skew_geom(B->nx, B->ny);
+  const size_t pse = (size_t)Q.np * 2 * kPF * Q.sg.skew_elems();
+  if (Q.model.alloc(4 * M * 8) != hipSuccess || Q.pfields.alloc(5 * M * 8) != hipSuccess ||
+      Q.pskew.alloc(pse * 8) != hipSuccess ||
+      (precision == SATT_PREC_MIXED && Q.pskew32.alloc(pse * 4) != hipSuccess))
+    return fail(ctx, SATT_E_NOMEM, "model allocation failed");
+  TRY(part_set_model(ctx, Q, B->svel, B->m11, B->m12, B->m22));
   {
     std::vector<char> h((size_t)ni * 4 + (size_t)(ni + 1) * 4 + (size_t)ni * 16 + 16);
     char *q = h.data();
     std::memcpy(q, B->item_pidx, (size_t)ni * 4);
     std::memcpy(q + (size_t)ni * 4, B->rec_off, (size_t)(ni + 1) * 4);
-    size_t off_d = ((size_t)ni * 8 + 4 + 7) & ~size_t(7);
+    const size_t off_d = ((size_t)ni * 8 + 4 + 7) & ~size_t(7);
     std::memcpy(q + off_d, B->item_srcx, (size_t)ni * 8);
     std::memcpy(q + off_d + (size_t)ni * 8, B->item_srcy, (size_t)ni * 8);
-    if ((st = upload(ctx, P->items, h.data(), h.size())) < 0) return bail(st);
-    char *d = P->items.as<char>();
-    P->it.pidx = reinterpret_cast<int *>(d);
-    P->it.rec_off = reinterpret_cast<int *>(d + (size_t)ni * 4);
-    P->it.srcx = reinterpret_cast<double *>(d + off_d);
-    P->it.srcy = reinterpret_cast<double *>(d + off_d + (size_t)ni * 8);
+    TRY(upload(ctx, Q.items, h.data(), h.size()));
+    char *d = Q.items.as<char>();
+    Q.it.pidx = reinterpret_cast<int *>(d);
+    Q.it.rec_off = reinterpret_cast<int *>(d + (size_t)ni * 4);
+    Q.it.srcx = reinterpret_cast<double *>(d + off_d);
+    Q.it.srcy = reinterpret_cast<double *>(d + off_d + (size_t)ni * 8);
   }
   {
     std::vector<double> h(4 * (size_t)nrec + (nrec + 1) / 2 + 1);
@@ -568,97 +587,243 @@ extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precisio
       std::memcpy(h.data() + 3 * (size_t)nrec, B->weight, nrec * 8);
       std::memcpy(h.data() + 4 * (size_t)nrec, B->idx_path, nrec * 4);
     }
-    if ((st = upload(ctx, P->recs, h.data(), h.size() * 8)) < 0) return bail(st);
-    double *d = P->recs.as<double>();
-    P->it.recx = d;
-    P->it.recy = d + nrec;
-    P->it.tt = d + 2 * (size_t)nrec;
-    P->it.weight = d + 3 * (size_t)nrec;
-    P->it.idx_path = reinterpret_cast<int *>(d + 4 * (size_t)nrec);
+    TRY(upload(ctx, Q.recs, h.data(), h.size() * 8));
+    double *d = Q.recs.as<double>();
+    Q.it.recx = d;
+    Q.it.recy = d + nrec;
+    Q.it.tt = d + 2 * (size_t)nrec;
+    Q.it.weight = d + 3 * (size_t)nrec;
+    Q.it.idx_path = reinterpret_cast<int *>(d + 4 * (size_t)nrec);
   }
-  if (P->rec_out.alloc(4 * (size_t)(nrec + 1) * 8) != hipSuccess ||
-      P->item_out.alloc((size_t)(ni + 1) * (8 + 4 * kIterInts + 4)) != hipSuccess ||
-      P->tt_fwd.alloc((size_t)(P->npath + 1) * 8) != hipSuccess ||
-      P->acc.alloc((P->dim == 3 ? 4 : 2) * M * 8) != hipSuccess)
-    return bail(fail(ctx, SATT_E_NOMEM, "output allocation failed"));
-  {
-    double *d = P->rec_out.as<double>();
-    P->tekio = d;
-    P->weight_out = d + (nrec + 1);
-    P->rhs_adj = d + 2 * (size_t)(nrec + 1);
-    P->rhs_den = d + 3 * (size_t)(nrec + 1);
-    P->item_chi = P->item_out.as<double>();
-    P->iters = reinterpret_cast<int *>(P->item_chi + ni + 1);
-    P->status = P->iters + (size_t)kIterInts * (ni + 1);
-  }
+  if (Q.rec_out.alloc(4 * (size_t)(nrec + 1) * 8) != hipSuccess ||
+      Q.item_out.alloc((size_t)(ni + 1) * (8 + 4 * kIterInts + 4)) != hipSuccess ||
+      Q.tt_fwd.alloc((size_t)(Q.npath + 1) * 8) != hipSuccess ||
+      Q.acc.alloc((Q.dim == 3 ? 4 : 2) * M * 8) != hipSuccess)
+    return fail(ctx, SATT_E_NOMEM, "output allocation failed");
+  double *d = Q.rec_out.as<double>();
+  Q.tekio = d;
+  Q.weight_out = d + (nrec + 1);
+  Q.rhs_adj = d + 2 * (size_t)(nrec + 1);
+  Q.rhs_den = d + 3 * (size_t)(nrec + 1);
+  Q.item_chi = Q.item_out.as<double>();
+  Q.iters = reinterpret_cast<int *>(Q.item_chi + ni + 1);
+  Q.status = Q.iters + (size_t)kIterInts * (ni + 1);
+  Q.h_pidx.assign(B->item_pidx, B->item_pidx + ni);
+  Q.h_rec_off.assign(B->rec_off, B->rec_off + ni + 1);
+  Q.h_srcx.assign(B->item_srcx, B->item_srcx + ni);
+  Q.h_srcy.assign(B->item_srcy, B->item_srcy + ni);
+  return SATT_OK;
+}
 
-  // chunk size from free device memory
+// The descriptor of item k of part Q, with its scratch at `base` (natural
+// fields T, Ta, Td, delta after the skewed fields, then the tables).
+SlotDesc make_desc(const Part &Q, int k, double *base, int slot_skews, int mixed) {
+  SlotDesc D{};
+  const size_t N = Q.N, SE = Q.sg.skew_elems();
+  const int p = Q.h_pidx[k], r0 = Q.h_rec_off[k];
+  D.g = Q.dg.g;
+  D.sg = Q.sg;
+  D.pskew = Q.pskew.as<double>() + (size_t)p * 2 * kPF * SE;
+  D.pskew32 = mixed ? Q.pskew32.as<float>() + (size_t)p * 2 * kPF * SE : nullptr;
+  D.fun = Q.pf.fun + (size_t)p * N;
+  D.skew = base;
+  double *nat = base + (size_t)slot_skews * SE;
+  D.T = nat + kNatT * N;
+  D.Ta = nat + kNatTa * N;
+  D.Td = nat + kNatTd * N;
+  D.delta = nat + kNatDelta * N;
+  D.aux = nat + kSlotNat * N;
+  D.slot_skews = slot_skews;
+  D.aux_len = aux_len(Q.nx, Q.ny);
+  D.x0 = Q.h_srcx[k];
+  D.y0 = Q.h_srcy[k];
+  D.nr = Q.h_rec_off[k + 1] - r0;
+  D.recx = Q.it.recx + r0;
+  D.recy = Q.it.recy + r0;
+  D.tt = Q.it.tt + r0;
+  D.weight = Q.it.weight + r0;
+  D.idx_path = Q.it.idx_path + r0;
+  D.tekio = Q.tekio + r0;
+  D.weight_out = Q.weight_out + r0;
+  D.rhs_adj = Q.rhs_adj + r0;
+  D.rhs_den = Q.rhs_den + r0;
+  D.item_chi = Q.item_chi + k;
+  D.tt_fwd = Q.tt_fwd.as<double>();
+  D.status = Q.status + k;
+  D.iters = Q.iters + (size_t)kIterInts * k;
+  return D;
+}
+
+// Launch groups: the slots in order (one grid: item order; several grids:
+// the most expensive grid first, each grid's items in item order), as many
+// per group as the scratch budget holds.
+int plan_groups(satt_plan *P) {
+  satt_ctx *ctx = P->ctx;
+  const bool mixed = P->precision == SATT_PREC_MIXED;
+  std::vector<std::pair<int, int>> order;
+  for (int q = 0; q < (int)P->parts.size(); ++q)
+    for (int k = 0; k < P->parts[q]->nitems; ++k) order.push_back({q, k});
+  auto cost = [&](int q) {
+    const Part &Q = *P->parts[q];
+    return (double)Q.sg.ns * (Q.ny + 63);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](const std::pair<int, int> &a,
+                                                   const std::pair<int, int> &b) {
+    return cost(a.first) > cost(b.first);
+  });
   size_t freeb = 0, totalb = 0;
   HIPCHK(ctx, hipMemGetInfo(&freeb, &totalb));
-  // one-solve-per-launch scratch: one slot
-  if (P->fwdw.alloc(fwd_item_bytes(N)) != hipSuccess ||
-      P->adjw.alloc(adj_item_bytes(N, P->dim)) != hipSuccess)
-    return bail(fail(ctx, SATT_E_NOMEM, "scratch allocation failed"));
-  // batched: as many slots as 3/4 of the free memory holds, at most 220 GiB
-  // (every item of a 4000-solve 401x401 batch in one launch on a 288 GB
-  // device, in either precision)
-  HIPCHK(ctx, hipMemGetInfo(&freeb, &totalb));
-  P->slot_skews = precision == SATT_PREC_MIXED ? kSlotSkewsMixed : kSlotSkews;
-  P->aux_len = aux_len(B->nx, B->ny);
-  const size_t per_item = strip_slot_doubles(N, P->sg, P->slot_skews, P->aux_len) * 8;
-  size_t budget = std::min(freeb / 4 * 3, (size_t)220 << 30);
-  size_t ch = std::max<size_t>(1, budget / per_item);
-  ch = std::min<size_t>(ch, 8192);
-  ch = std::min<size_t>(ch, (size_t)std::max(ni, 1));
-  P->chunk = (int)ch;
-  P->nchunks = ni > 0 ? (ni + P->chunk - 1) / P->chunk : 0;
-  if (P->strip.alloc(per_item * ch) != hipSuccess)
-    return bail(fail(ctx, SATT_E_NOMEM, "scratch allocation failed (chunk %zu)", ch));
+  // 3/4 of the free memory, at most 220 GiB: every item of a 4000-solve
+  // 401x401 batch in one group on a 288 GB device, in either precision
+  const size_t budget = std::min(freeb / 4 * 3, (size_t)220 << 30) / 8;
+  std::vector<size_t> sd(P->parts.size());
+  for (size_t q = 0; q < P->parts.size(); ++q)
+    sd[q] = slot_doubles(P->parts[q]->nx, P->parts[q]->ny, P->slot_skews);
+  size_t biggest = 0;
+  for (size_t a = 0; a < order.size();) {
+    Group G;
+    size_t used = 0;
+    while (a < order.size() && G.slots.size() < 8192 &&
+           (G.slots.empty() || used + sd[order[a].first] <= budget)) {
+      used += sd[order[a].first];
+      G.slots.push_back(order[a++]);
+    }
+    biggest = std::max(biggest, used);
+    P->groups.push_back(std::move(G));
+  }
+  if (P->strip.alloc(std::max<size_t>(biggest, 1) * 8) != hipSuccess)
+    return fail(ctx, SATT_E_NOMEM, "scratch allocation of %zu bytes failed", biggest * 8);
+  // descriptors and stacking lists
+  std::vector<SlotDesc> hd;
+  std::vector<int> hl;
+  std::vector<size_t> desc_at, list_at;
+  for (Group &G : P->groups) {
+    desc_at.push_back(hd.size());
+    double *base = P->strip.as<double>();
+    GroupDims gd{(int)G.slots.size(), 0, 0, 0};
+    for (auto &sl : G.slots) {
+      const Part &Q = *P->parts[sl.first];
+      hd.push_back(make_desc(Q, sl.second, base, P->slot_skews, mixed));
+      base += sd[sl.first];
+      gd.max_se = std::max(gd.max_se, Q.sg.skew_elems());
+      gd.max_n = std::max(gd.max_n, Q.N);
+      gd.max_nxy = std::max(gd.max_nxy, std::max(Q.nx, Q.ny));
+    }
+    G.gd = gd;
+    for (size_t q = 0; q < P->parts.size(); ++q) {
+      const Part &Q = *P->parts[q];
+      std::vector<int> off(Q.np + 1, 0), slots;
+      for (size_t s = 0; s < G.slots.size(); ++s)
+        if (G.slots[s].first == (int)q) off[Q.h_pidx[G.slots[s].second] + 1]++;
+      for (int p = 0; p < Q.np; ++p) off[p + 1] += off[p];
+      if (off[Q.np] == 0) {
+        list_at.push_back(SIZE_MAX);
+        continue;
+      }
+      slots.assign(off[Q.np], 0);
+      std::vector<int> fill(off.begin(), off.end() - 1);
+      for (size_t s = 0; s < G.slots.size(); ++s)  // item order within each period
+        if (G.slots[s].first == (int)q) slots[fill[Q.h_pidx[G.slots[s].second]]++] = (int)s;
+      list_at.push_back(hl.size());
+      hl.insert(hl.end(), off.begin(), off.end());
+      hl.insert(hl.end(), slots.begin(), slots.end());
+    }
+  }
+  // the one-solve-per-launch descriptors (one-grid plans): scratch fwdw/adjw
+  const size_t nsolo = P->parts.size() == 1 ? (size_t)P->parts[0]->nitems : 0;
+  if (nsolo) {
+    const Part &Q = *P->parts[0];
+    double *fw = P->fwdw.as<double>(), *aw = P->adjw.as<double>();
+    for (size_t k = 0; k < nsolo; ++k) {
+      SlotDesc D = make_desc(Q, (int)k, nullptr, 0, 0);
+      D.skew = nullptr;
+      D.T = fw;
+      D.Ta = aw;
+      D.Td = Q.dim == 3 ? aw + Q.N : nullptr;
+      hd.push_back(D);
+    }
+  }
+  TRY(upload(ctx, P->desc, hd.data(), hd.size() * sizeof(SlotDesc)));
+  TRY(upload(ctx, P->lists, hl.data(), std::max<size_t>(hl.size(), 1) * 4));
+  size_t li = 0;
+  for (size_t g = 0; g < P->groups.size(); ++g) {
+    Group &G = P->groups[g];
+    G.desc = P->desc.as<SlotDesc>() + desc_at[g];
+    for (size_t q = 0; q < P->parts.size(); ++q, ++li) {
+      const size_t at = list_at[li];
+      const int np = P->parts[q]->np;
+      G.per_off.push_back(at == SIZE_MAX ? nullptr : P->lists.as<int>() + at);
+      G.per_slot.push_back(at == SIZE_MAX ? nullptr : P->lists.as<int>() + at + np + 1);
+    }
+  }
+  P->d_solo = nsolo ? P->desc.as<SlotDesc>() + (hd.size() - nsolo) : nullptr;
+  return sync_status(ctx);
+}
 
-  // per chunk: the chunk's slots of each period, in item order
-  P->h_per_off.assign((size_t)P->nchunks * (P->np + 1), 0);
-  P->h_per_slot.assign((size_t)std::max(ni, 1), 0);
-  for (int cix = 0; cix < P->nchunks; ++cix) {
-    const int i0 = cix * P->chunk, i1 = std::min(ni, i0 + P->chunk);
-    int *off = &P->h_per_off[(size_t)cix * (P->np + 1)];
-    for (int i = i0; i < i1; ++i) off[B->item_pidx[i] + 1]++;
-    for (int p = 0; p < P->np; ++p) off[p + 1] += off[p];
-    std::vector<int> fill(off, off + P->np);
-    for (int i = i0; i < i1; ++i) P->h_per_slot[i0 + fill[B->item_pidx[i]]++] = i - i0;
+}  // namespace
+
+extern "C" int satt_plan_create_multi(satt_ctx *ctx, int nbatch, const satt_batch *batches,
+                                      int precision, satt_plan **out) {
+  if (!ctx || !out) return SATT_E_ARG;
+  *out = nullptr;
+  if (nbatch < 1 || !batches) return fail(ctx, SATT_E_ARG, "satt_plan_create: no batch");
+  if (precision != SATT_PREC_FP64 && precision != SATT_PREC_MIXED)
+    return fail(ctx, SATT_E_ARG, "satt_plan_create: unknown precision %d", precision);
+  for (int k = 0; k < nbatch; ++k) {
+    TRY(check_batch(ctx, &batches[k], k));
+    if (batches[k].dim != batches[0].dim)
+      return fail(ctx, SATT_E_ARG, "satt_plan_create: batches of one plan share dim");
   }
-  {
-    const size_t a = P->h_per_off.size() * 4, b = P->h_per_slot.size() * 4;
-    if (P->lists.alloc(a + b + 8) != hipSuccess)
-      return bail(fail(ctx, SATT_E_NOMEM, "list allocation failed"));
-    P->d_per_off = P->lists.as<int>();
-    P->d_per_slot = P->d_per_off + P->h_per_off.size();
-    if (a) HIPCHK(ctx, hipMemcpyAsync(P->d_per_off, P->h_per_off.data(), a, hipMemcpyHostToDevice, ctx->stream));
-    if (b) HIPCHK(ctx, hipMemcpyAsync(P->d_per_slot, P->h_per_slot.data(), b, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto *P = new satt_plan();
+  auto bail = [&](int st) { delete P; return st; };
+  P->ctx = ctx;
+  P->precision = precision;
+  P->dim = batches[0].dim;
+  P->slot_skews = precision == SATT_PREC_MIXED ? kSlotSkewsMixed : kSlotSkews;
+  int st;
+  for (int k = 0; k < nbatch; ++k) {
+    P->parts.emplace_back(new Part());
+    if ((st = part_create(ctx, *P->parts.back(), &batches[k], precision)) < 0) return bail(st);
   }
-  if ((st = sync_status(ctx)) < 0) return bail(st);
-  P->stats.chunk = P->chunk;
-  P->stats.device_bytes =
-      (long long)(P->dg.buf.bytes + P->model.bytes + P->pfields.bytes + P->items.bytes +
-                  P->recs.bytes + P->rec_out.bytes + P->item_out.bytes + P->tt_fwd.bytes +
-                  P->acc.bytes + P->lists.bytes + P->fwdw.bytes + P->adjw.bytes +
-                  P->pskew.bytes + P->pskew32.bytes + P->strip.bytes);
+  if (nbatch == 1) {  // one-solve-per-launch scratch: one slot
+    const Part &Q = *P->parts[0];
+    if (P->fwdw.alloc(fwd_item_bytes(Q.N)) != hipSuccess ||
+        P->adjw.alloc(adj_item_bytes(Q.N, Q.dim)) != hipSuccess)
+      return bail(fail(ctx, SATT_E_NOMEM, "scratch allocation failed"));
+  }
+  if ((st = plan_groups(P)) < 0) return bail(st);
+  size_t bytes = P->strip.bytes + P->desc.bytes + P->lists.bytes + P->fwdw.bytes + P->adjw.bytes;
+  for (auto &Q : P->parts)
+    bytes += Q->dg.buf.bytes + Q->model.bytes + Q->pfields.bytes + Q->pskew.bytes +
+             Q->pskew32.bytes + Q->items.bytes + Q->recs.bytes + Q->rec_out.bytes +
+             Q->item_out.bytes + Q->tt_fwd.bytes + Q->acc.bytes;
+  P->stats.device_bytes = (long long)bytes;
+  P->stats.chunk = P->groups.empty() ? 0 : (int)P->groups[0].slots.size();
   *out = P;
   return SATT_OK;
 }
 
-extern "C" int satt_plan_set_model(satt_plan *P, const double *svel, const double *m11,
-                                   const double *m12, const double *m22) {
+extern "C" int satt_plan_create(satt_ctx *ctx, const satt_batch *B, int precision,
+                                satt_plan **out) {
+  return satt_plan_create_multi(ctx, 1, B, precision, out);
+}
+
+extern "C" int satt_plan_set_model_part(satt_plan *P, int part, const double *svel,
+                                        const double *m11, const double *m12,
+                                        const double *m22) {
   if (!P) return SATT_E_ARG;
   satt_ctx *ctx = P->ctx;
-  if (!svel || !m11 || !m12 || !m22) return fail(ctx, SATT_E_ARG, "satt_plan_set_model: null");
+  if (part < 0 || part >= (int)P->parts.size())
+    return fail(ctx, SATT_E_ARG, "satt_plan_set_model: no part %d", part);
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  const size_t M = (size_t)P->np * P->N;
-  double *m = P->model.as<double>();
-  const double *src[4] = {svel, m11, m12, m22};
-  for (int q = 0; q < 4; ++q)
-    HIPCHK(ctx, hipMemcpyAsync(m + q * M, src[q], M * 8, hipMemcpyHostToDevice, ctx->stream));
-  TRY(plan_period_fields(P));
+  TRY(part_set_model(ctx, *P->parts[part], svel, m11, m12, m22));
   return sync_status(ctx);
+}
+
+extern "C" int satt_plan_set_model(satt_plan *P, const double *svel, const double *m11,
+                                   const double *m12, const double *m22) {
+  return satt_plan_set_model_part(P, 0, svel, m11, m12, m22);
 }
 
 extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_form) {
@@ -667,15 +832,15 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   if (launch_form != SATT_LAUNCH_BATCHED && launch_form != SATT_LAUNCH_PER_SOLVE)
     return fail(ctx, SATT_E_ARG, "satt_plan_run: unknown launch form");
   const bool mixed = P->precision == SATT_PREC_MIXED;
-  if (mixed && launch_form != SATT_LAUNCH_BATCHED)
+  const bool batched = launch_form == SATT_LAUNCH_BATCHED;
+  if (mixed && !batched)
     return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form is FP64 only");
+  if (!batched && P->parts.size() != 1)
+    return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form takes one grid");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  const size_t N = P->N, M = (size_t)P->np * N;
-  const int ni = P->nitems;
   // Events on the kernels' stream: one at the start and one after every
   // launch, each interval tagged with the stage it times.
-  const int groups = launch_form == SATT_LAUNCH_BATCHED ? P->nchunks : ni;
   if (P->ev.empty()) {
     hipEvent_t e;
     HIPCHK(ctx, hipEventCreate(&e));
@@ -692,128 +857,105 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
     }
     return hipEventRecord(P->ev[P->marks.size()], s);
   };
-  HIPCHK(ctx, hipMemsetAsync(P->status, 0, (size_t)(ni + 1) * 4, s));
-  HIPCHK(ctx, hipMemsetAsync(P->iters, 0, (size_t)(ni + 1) * 4 * kIterInts, s));
-  HIPCHK(ctx, hipMemsetAsync(P->item_chi, 0, (size_t)(ni + 1) * 8, s));
-  HIPCHK(ctx, hipMemsetAsync(P->tt_fwd.p, 0, P->tt_fwd.bytes, s));
-  if (isadj) HIPCHK(ctx, hipMemsetAsync(P->acc.p, 0, P->acc.bytes, s));
+  for (auto &Qp : P->parts) {
+    Part &Q = *Qp;
+    HIPCHK(ctx, hipMemsetAsync(Q.status, 0, (size_t)(Q.nitems + 1) * 4, s));
+    HIPCHK(ctx, hipMemsetAsync(Q.iters, 0, (size_t)(Q.nitems + 1) * 4 * kIterInts, s));
+    HIPCHK(ctx, hipMemsetAsync(Q.item_chi, 0, (size_t)(Q.nitems + 1) * 8, s));
+    HIPCHK(ctx, hipMemsetAsync(Q.tt_fwd.p, 0, Q.tt_fwd.bytes, s));
+    if (isadj) HIPCHK(ctx, hipMemsetAsync(Q.acc.p, 0, Q.acc.bytes, s));
+  }
   HIPCHK(ctx, hipEventRecord(P->ev[0], s));
-
-  const bool batched = launch_form == SATT_LAUNCH_BATCHED;
-  const int cap = batched ? P->chunk : 1;
-  // one-solve-per-launch buffers (workgroup wavefront kernels)
-  double *fw = P->fwdw.as<double>(), *aw = P->adjw.as<double>();
-  FwdWork fwk{fw, fw + N, fw + 2 * N, fw + 3 * N, fw + 4 * N, fw + 5 * N};
-  double *Ta = aw, *Td = aw + N, *coef = aw + 2 * N, *delta = aw + 7 * N, *old = aw + 8 * N;
-  if (P->dim == 2) {  // no density field: shift scratch down by one slot
-    coef = aw + N;
-    delta = aw + 6 * N;
-    old = aw + 7 * N;
-  }
-  // batched buffers (strip-pipeline kernels)
-  StripWork sw{};
+  const int nadj = P->dim == 3 ? 2 : 1;  // adjoint, then density adjoint
+  const int power = P->dim == 3 ? 3 : 2;
   if (batched) {
-    const size_t SE = P->sg.skew_elems();
-    double *b0 = P->strip.as<double>();
-    double *nat = b0 + (size_t)P->slot_skews * SE * cap;
-    sw.skew = b0;
-    sw.T = nat + (size_t)kNatT * N * cap;
-    sw.Ta = nat + (size_t)kNatTa * N * cap;
-    sw.Td = nat + (size_t)kNatTd * N * cap;
-    sw.delta = nat + (size_t)kNatDelta * N * cap;
-    sw.aux = nat + (size_t)kSlotNat * N * cap;
-    sw.aux_len = P->aux_len;
-    sw.slot_skews = P->slot_skews;
-    Ta = sw.Ta;
-    Td = sw.Td;
-    fwk.T = sw.T;
-  }
-  double *acc_s = P->acc.as<double>(), *acc_d = acc_s + M;
-  for (int g0 = 0; g0 < groups; ++g0) {
-    Items it = P->it;
-    const int *per_off, *per_slot;
-    if (launch_form == SATT_LAUNCH_BATCHED) {
-      it.item0 = g0 * P->chunk;
-      it.count = std::min(P->chunk, ni - it.item0);
-      per_off = P->d_per_off + (size_t)g0 * (P->np + 1);
-      per_slot = P->d_per_slot + it.item0;
-    } else {
-      it.item0 = g0;
-      it.count = 1;
-      per_off = nullptr;
-      per_slot = nullptr;
-    }
-    const double *pk = P->pskew.as<double>();
-    if (batched) {
-      launch_fwd_setup(s, P->dg.g, P->sg, P->pf, it, sw, P->status, mixed);
+    for (const Group &G : P->groups) {
+      const GroupDims &gd = G.gd;
+      launch_fwd_setup(s, G.desc, gd, mixed);
       HIPCHK(ctx, mark(kStFwdSetup));
-    }
-    if (batched && mixed) {
-      launch_fwd_mixed32(s, P->dg.g, P->sg, P->pskew32.as<float>(), P->pf, it, sw, P->iters + 3,
-                         P->status, kIterInts);
-      HIPCHK(ctx, mark(kStFwd32));
-      launch_fwd_polish(s, P->dg.g, P->sg, pk, P->pf, it, sw, P->iters + 0, P->status, kIterInts);
-    } else if (batched) {
-      launch_fwd_strip(s, P->dg.g, P->sg, pk, P->pf, it, sw, P->iters + 0, P->status, kIterInts);
-    } else {
-      launch_forward(s, P->dg.g, P->pf, it, fwk, P->iters + 0, P->status, kIterInts);
-    }
-    HIPCHK(ctx, mark(kStFwd64));
-    RecWork rw{fwk.T, P->tekio, P->weight_out, P->rhs_adj, P->rhs_den, P->item_chi,
-               istotable ? P->tt_fwd.as<double>() : nullptr};
-    launch_receivers(s, P->dg.g, it, rw, P->status);
-    HIPCHK(ctx, mark(kStOther));
-    if (isadj) {
-      for (int q = 0; q < (P->dim == 3 ? 2 : 1); ++q) {  // adjoint, then density adjoint
-        const double *rhs = q == 0 ? P->rhs_adj : P->rhs_den;
-        const int nat = q == 0 ? kNatTa : kNatTd;
-        if (batched) {
-          launch_adj_setup(s, P->dg.g, P->sg, pk, it, sw, rhs, P->iters + 1 + q, P->status,
-                           kIterInts, mixed);
-          HIPCHK(ctx, mark(kStAdjSetup));
-        }
-        if (batched && mixed) {
-          launch_adj_mixed32(s, P->dg.g, P->sg, it, sw, P->iters + 4 + q, P->status, kIterInts);
+      if (mixed) {
+        launch_fwd_mixed32(s, G.desc, gd);
+        HIPCHK(ctx, mark(kStFwd32));
+        launch_fwd_polish(s, G.desc, gd);
+      } else {
+        launch_fwd_strip(s, G.desc, gd);
+      }
+      HIPCHK(ctx, mark(kStFwd64));
+      launch_receivers(s, G.desc, gd.count, istotable);
+      HIPCHK(ctx, mark(kStOther));
+      if (!isadj) continue;
+      for (int q = 0; q < nadj; ++q) {
+        launch_adj_setup(s, G.desc, gd, q, mixed);
+        HIPCHK(ctx, mark(kStAdjSetup));
+        if (mixed) {
+          launch_adj_mixed32(s, G.desc, gd, q);
           HIPCHK(ctx, mark(kStAdj32));
-          launch_adj_polish(s, P->dg.g, P->sg, it, sw, nat, P->iters + 1 + q, P->status,
-                            kIterInts, 1);
-        } else if (batched) {
-          launch_adj_strip(s, P->dg.g, P->sg, it, sw, nat, P->iters + 1 + q, P->status, kIterInts,
-                           1);
+          launch_adj_polish(s, G.desc, gd, q);
         } else {
-          AdjWork awk{fwk.T, q == 0 ? Ta : Td, coef, delta, old, rhs};
-          launch_adjoint(s, P->dg.g, P->pf, it, awk, P->iters + 1 + q, P->status, kIterInts, 1);
+          launch_adj_strip(s, G.desc, gd, q);
         }
         HIPCHK(ctx, mark(kStAdj64));
       }
-    }
-    if (isadj) {
-      const int power = P->dim == 3 ? 3 : 2;
-      if (per_off) {
-        launch_stack(s, P->dg.g, P->pf, P->np, per_off, per_slot, Ta, acc_s, power);
-        if (P->dim == 3) launch_stack(s, P->dg.g, P->pf, P->np, per_off, per_slot, Td, acc_d, power);
-      } else {
-        launch_stack_item(s, P->dg.g, P->pf, P->it.pidx, g0, Ta, acc_s, power);
-        if (P->dim == 3) launch_stack_item(s, P->dg.g, P->pf, P->it.pidx, g0, Td, acc_d, power);
+      for (size_t pq = 0; pq < P->parts.size(); ++pq) {
+        if (!G.per_off[pq]) continue;
+        Part &Q = *P->parts[pq];
+        double *acc = Q.acc.as<double>();
+        for (int q = 0; q < nadj; ++q)
+          launch_stack(s, Q.dg.g, Q.pf, Q.np, G.per_off[pq], G.per_slot[pq], G.desc, q,
+                       acc + (size_t)q * Q.np * Q.N, power);
       }
+      HIPCHK(ctx, mark(kStOther));
     }
-    HIPCHK(ctx, mark(kStOther));
+  } else {
+    Part &Q = *P->parts[0];
+    const size_t N = Q.N;
+    double *fw = P->fwdw.as<double>(), *aw = P->adjw.as<double>();
+    FwdWork fwk{fw, fw + N, fw + 2 * N, fw + 3 * N, fw + 4 * N, fw + 5 * N};
+    double *Ta = aw, *Td = aw + N, *coef = aw + 2 * N, *delta = aw + 7 * N, *old = aw + 8 * N;
+    if (Q.dim == 2) {  // no density field: shift the scratch down by one field
+      coef = aw + N;
+      delta = aw + 6 * N;
+      old = aw + 7 * N;
+    }
+    for (int k = 0; k < Q.nitems; ++k) {
+      Items it = Q.it;
+      it.item0 = k;
+      it.count = 1;
+      launch_forward(s, Q.dg.g, Q.pf, it, fwk, Q.iters + 0, Q.status, kIterInts);
+      HIPCHK(ctx, mark(kStFwd64));
+      launch_receivers(s, P->d_solo + k, 1, istotable);
+      HIPCHK(ctx, mark(kStOther));
+      if (!isadj) continue;
+      for (int q = 0; q < nadj; ++q) {
+        AdjWork awk{fwk.T, q == 0 ? Ta : Td, coef, delta, old, q == 0 ? Q.rhs_adj : Q.rhs_den};
+        launch_adjoint(s, Q.dg.g, Q.pf, it, awk, Q.iters + 1 + q, Q.status, kIterInts, 1);
+        HIPCHK(ctx, mark(kStAdj64));
+      }
+      double *acc = Q.acc.as<double>();
+      for (int q = 0; q < nadj; ++q)
+        launch_stack_item(s, Q.dg.g, Q.pf, Q.it.pidx, k, q == 0 ? Ta : Td,
+                          acc + (size_t)q * Q.np * N, power);
+      HIPCHK(ctx, mark(kStOther));
+    }
   }
   if (isadj) {
-    double *pm = P->acc.as<double>() + (P->dim == 3 ? 2 : 1) * M;
-    launch_to_period_major(s, (int)N, P->np, acc_s, pm);
-    if (P->dim == 3) launch_to_period_major(s, (int)N, P->np, acc_d, pm + M);
+    for (auto &Qp : P->parts) {
+      Part &Q = *Qp;
+      const size_t M = (size_t)Q.np * Q.N;
+      double *acc = Q.acc.as<double>(), *pm = acc + (size_t)nadj * M;
+      for (int q = 0; q < nadj; ++q)
+        launch_to_period_major(s, (int)Q.N, Q.np, acc + (size_t)q * M, pm + (size_t)q * M);
+    }
   }
   HIPCHK(ctx, mark(kStOther));
   HIPCHK(ctx, hipGetLastError());
   P->last_istotable = istotable;
   P->last_isadj = isadj;
-  P->ran = groups;
   P->last_batched = batched;
-  P->last_item0 = groups > 0 ? (batched ? (groups - 1) * P->chunk : ni - 1) : 0;
-  P->last_count = groups > 0 ? (batched ? ni - P->last_item0 : 1) : 0;
-  P->stats.chunk = cap;
-  P->stats.launches = groups;
+  P->ran = 1;
+  P->last_group = batched ? (int)P->groups.size() - 1 : -1;
+  P->stats.launches = batched ? (int)P->groups.size() : P->parts[0]->nitems;
+  P->stats.chunk = batched ? (P->groups.empty() ? 0 : (int)P->groups[0].slots.size()) : 1;
   return SATT_OK;
 }
 
@@ -823,54 +965,72 @@ extern "C" int satt_plan_wait(satt_plan *P) {
   return sync_status(P->ctx);
 }
 
-extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
-  if (!P) return SATT_E_ARG;
+namespace {
+
+// Timing and iteration statistics of the last run, over every part.
+int plan_stats(satt_plan *P) {
   satt_ctx *ctx = P->ctx;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  TRY(sync_status(ctx));
-  const int ni = P->nitems;
-  const size_t M = (size_t)P->np * P->N;
-  // timing: the intervals between consecutive events, by stage
-  {
-    float ms = 0;
-    double st_ms[kStCount] = {};
-    for (size_t k = 0; k < P->marks.size(); ++k) {
-      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[k], P->ev[k + 1]));
-      st_ms[P->marks[k]] += ms;
-    }
-    ms = 0;
-    if (!P->marks.empty())
-      HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[0], P->ev[P->marks.size()]));
-    P->stats.ms_fwd_fp64 = st_ms[kStFwd64];
-    P->stats.ms_fwd_fp32 = st_ms[kStFwd32];
-    P->stats.ms_adj_fp64 = st_ms[kStAdj64];
-    P->stats.ms_adj_fp32 = st_ms[kStAdj32];
-    P->stats.ms_fwd_setup = st_ms[kStFwdSetup];
-    P->stats.ms_adj_setup = st_ms[kStAdjSetup];
-    P->stats.ms_forward = st_ms[kStFwdSetup] + st_ms[kStFwd64] + st_ms[kStFwd32];
-    P->stats.ms_adjoint = st_ms[kStAdjSetup] + st_ms[kStAdj64] + st_ms[kStAdj32];
-    P->stats.ms_total = ms;
+  float ms = 0;
+  double st_ms[kStCount] = {};
+  for (size_t k = 0; k < P->marks.size(); ++k) {
+    HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[k], P->ev[k + 1]));
+    st_ms[P->marks[k]] += ms;
   }
-  std::vector<int> st((size_t)ni + 1), its((size_t)kIterInts * (ni + 1));
-  std::vector<double> chi((size_t)ni + 1);
-  HIPCHK(ctx, hipMemcpy(st.data(), P->status, (size_t)(ni + 1) * 4, hipMemcpyDeviceToHost));
-  HIPCHK(ctx, hipMemcpy(its.data(), P->iters, its.size() * 4, hipMemcpyDeviceToHost));
-  HIPCHK(ctx, hipMemcpy(chi.data(), P->item_chi, (size_t)(ni + 1) * 8, hipMemcpyDeviceToHost));
+  ms = 0;
+  if (!P->marks.empty())
+    HIPCHK(ctx, hipEventElapsedTime(&ms, P->ev[0], P->ev[P->marks.size()]));
+  P->stats.ms_fwd_fp64 = st_ms[kStFwd64];
+  P->stats.ms_fwd_fp32 = st_ms[kStFwd32];
+  P->stats.ms_adj_fp64 = st_ms[kStAdj64];
+  P->stats.ms_adj_fp32 = st_ms[kStAdj32];
+  P->stats.ms_fwd_setup = st_ms[kStFwdSetup];
+  P->stats.ms_adj_setup = st_ms[kStAdjSetup];
+  P->stats.ms_forward = st_ms[kStFwdSetup] + st_ms[kStFwd64] + st_ms[kStFwd32];
+  P->stats.ms_adjoint = st_ms[kStAdjSetup] + st_ms[kStAdj64] + st_ms[kStAdj32];
+  P->stats.ms_total = ms;
   long long fs = 0, as = 0, fs32 = 0, as32 = 0;
-  int worst = SATT_OK, worst_item = -1;
-  for (int i = 0; i < ni; ++i) {
-    const int *q = &its[(size_t)kIterInts * i];
-    fs += 4LL * q[0];
-    as += 4LL * (q[1] + q[2]);
-    fs32 += 4LL * q[3];
-    as32 += 4LL * (q[4] + q[5]);
-    if (st[i] < 0 && worst >= 0) { worst = st[i]; worst_item = i; }
-    else if (st[i] > 0 && worst == SATT_OK) { worst = st[i]; worst_item = i; }
+  for (auto &Qp : P->parts) {
+    const Part &Q = *Qp;
+    std::vector<int> its((size_t)kIterInts * (Q.nitems + 1));
+    HIPCHK(ctx, hipMemcpy(its.data(), Q.iters, its.size() * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < Q.nitems; ++i) {
+      const int *q = &its[(size_t)kIterInts * i];
+      fs += 4LL * q[0];
+      as += 4LL * (q[1] + q[2]);
+      fs32 += 4LL * q[3];
+      as32 += 4LL * (q[4] + q[5]);
+    }
   }
   P->stats.fwd_sweeps = fs;
   P->stats.adj_sweeps = as;
   P->stats.fwd_sweeps32 = fs32;
   P->stats.adj_sweeps32 = as32;
+  return SATT_OK;
+}
+
+}  // namespace
+
+extern "C" int satt_plan_results_part(satt_plan *P, int part, satt_result *R) {
+  if (!P) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  if (part < 0 || part >= (int)P->parts.size())
+    return fail(ctx, SATT_E_ARG, "satt_plan_results: no part %d", part);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  TRY(sync_status(ctx));
+  if (P->ran) TRY(plan_stats(P));
+  const Part &Q = *P->parts[part];
+  const int ni = Q.nitems;
+  const size_t M = (size_t)Q.np * Q.N;
+  std::vector<int> st((size_t)ni + 1), its((size_t)kIterInts * (ni + 1));
+  std::vector<double> chi((size_t)ni + 1);
+  HIPCHK(ctx, hipMemcpy(st.data(), Q.status, (size_t)(ni + 1) * 4, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(its.data(), Q.iters, its.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(chi.data(), Q.item_chi, (size_t)(ni + 1) * 8, hipMemcpyDeviceToHost));
+  int worst = SATT_OK, worst_item = -1;
+  for (int i = 0; i < ni; ++i) {
+    if (st[i] < 0 && worst >= 0) { worst = st[i]; worst_item = i; }
+    else if (st[i] > 0 && worst == SATT_OK) { worst = st[i]; worst_item = i; }
+  }
   if (R) {
     if (R->item_status) std::memcpy(R->item_status, st.data(), (size_t)ni * 4);
     for (int i = 0; i < ni; ++i)
@@ -885,20 +1045,21 @@ extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
       for (int i = 0; i < ni; ++i) c = c + chi[i];
       *R->chi = c;
     }
-    if (R->tt_fwd && P->npath > 0) {
+    if (R->tt_fwd && Q.npath > 0) {
       if (P->last_istotable)
-        HIPCHK(ctx, hipMemcpy(R->tt_fwd, P->tt_fwd.p, (size_t)P->npath * 8, hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(R->tt_fwd, Q.tt_fwd.p, (size_t)Q.npath * 8, hipMemcpyDeviceToHost));
       else
-        std::memset(R->tt_fwd, 0, (size_t)P->npath * 8);
+        std::memset(R->tt_fwd, 0, (size_t)Q.npath * 8);
     }
-    if (R->weight_out && P->nrec > 0) {
-      const double *src = P->last_isadj ? P->weight_out : P->it.weight;
-      HIPCHK(ctx, hipMemcpy(R->weight_out, src, (size_t)P->nrec * 8, hipMemcpyDeviceToHost));
+    if (R->weight_out && Q.nrec > 0) {
+      const double *src = P->last_isadj ? Q.weight_out : Q.it.weight;
+      HIPCHK(ctx, hipMemcpy(R->weight_out, src, (size_t)Q.nrec * 8, hipMemcpyDeviceToHost));
     }
     if (P->last_isadj) {
-      double *pm = P->acc.as<double>() + (P->dim == 3 ? 2 : 1) * M;
+      const int nadj = Q.dim == 3 ? 2 : 1;
+      const double *pm = Q.acc.as<double>() + (size_t)nadj * M;
       if (R->adj_s) HIPCHK(ctx, hipMemcpy(R->adj_s, pm, M * 8, hipMemcpyDeviceToHost));
-      if (R->adj_density && P->dim == 3)
+      if (R->adj_density && Q.dim == 3)
         HIPCHK(ctx, hipMemcpy(R->adj_density, pm + M, M * 8, hipMemcpyDeviceToHost));
     }
   }
@@ -908,34 +1069,44 @@ extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
   return worst;
 }
 
+extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
+  return satt_plan_results_part(P, 0, R);
+}
+
 extern "C" int satt_plan_item_field(satt_plan *P, int item, int which, double *out) {
   if (!P) return SATT_E_ARG;
   satt_ctx *ctx = P->ctx;
+  const Part &Q = *P->parts[0];
   if (!out || which < SATT_FIELD_T || which > SATT_FIELD_TD ||
-      (which == SATT_FIELD_TD && P->dim != 3))
+      (which == SATT_FIELD_TD && Q.dim != 3))
     return fail(ctx, SATT_E_ARG, "satt_plan_item_field: bad field");
-  if (!P->ran || item < P->last_item0 || item >= P->last_item0 + P->last_count)
-    return fail(ctx, SATT_E_ARG, "satt_plan_item_field: item %d is not resident", item);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   TRY(sync_status(ctx));
-  const size_t N = P->N;
-  const double *src;
-  if (P->last_batched) {
-    const int cap = P->chunk;
-    const size_t SE = P->sg.skew_elems();
-    const double *nat = P->strip.as<double>() + (size_t)P->slot_skews * SE * cap;
-    const int nat_field = which == SATT_FIELD_T ? kNatT : which == SATT_FIELD_TA ? kNatTa : kNatTd;
-    src = nat + ((size_t)nat_field * cap + (item - P->last_item0)) * N;
-  } else {
+  const double *src = nullptr;
+  if (P->ran && P->last_batched && P->last_group >= 0) {
+    const Group &G = P->groups[P->last_group];
+    for (size_t s = 0; s < G.slots.size(); ++s)
+      if (G.slots[s].first == 0 && G.slots[s].second == item) {
+        SlotDesc D;
+        HIPCHK(ctx, hipMemcpy(&D, G.desc + s, sizeof D, hipMemcpyDeviceToHost));
+        src = which == SATT_FIELD_T ? D.T : which == SATT_FIELD_TA ? D.Ta : D.Td;
+      }
+  } else if (P->ran && !P->last_batched && item == Q.nitems - 1) {
     const double *fw = P->fwdw.as<double>(), *aw = P->adjw.as<double>();
-    src = which == SATT_FIELD_T ? fw : which == SATT_FIELD_TA ? aw : aw + N;
+    src = which == SATT_FIELD_T ? fw : which == SATT_FIELD_TA ? aw : aw + Q.N;
   }
-  HIPCHK(ctx, hipMemcpy(out, src, N * 8, hipMemcpyDeviceToHost));
+  if (!src) return fail(ctx, SATT_E_ARG, "satt_plan_item_field: item %d is not resident", item);
+  HIPCHK(ctx, hipMemcpy(out, src, Q.N * 8, hipMemcpyDeviceToHost));
   return SATT_OK;
 }
 
 extern "C" int satt_plan_get_stats(satt_plan *P, satt_plan_stats *S) {
   if (!P || !S) return SATT_E_ARG;
+  if (P->ran) {
+    HIPCHK(P->ctx, hipSetDevice(P->ctx->device));
+    TRY(sync_status(P->ctx));
+    TRY(plan_stats(P));
+  }
   *S = P->stats;
   return SATT_OK;
 }

@@ -30,6 +30,43 @@ struct Grid {
   const double *sy, *cy;  // sin(yy), cos(yy), computed on the device
 };
 
+// Skewed layout of one field: ns strips of 64 columns, R = ny+63 rows of 64
+// lanes.  Plus skew: cell (64c+l, j) at row j+l; minus skew: row j-l+63.
+struct SkewGeom {
+  int ns, R;
+  __host__ __device__ size_t skew_elems() const { return (size_t)ns * R * 64; }
+};
+
+inline SkewGeom make_skew_geom(int nx, int ny) {
+  SkewGeom s;
+  s.ns = (nx + 63) / 64;
+  s.R = ny + 63;
+  return s;
+}
+
+// Everything the batched kernels need about one item, one descriptor per
+// slot of a launch group: the item's grid (items of different grids share a
+// launch), its period's fields, its scratch and its inputs and outputs.
+struct SlotDesc {
+  Grid g;
+  SkewGeom sg;
+  const double *pskew;    // the period's skewed fields, [2 skews][kPF][SE]
+  const float *pskew32;   // the same in float (mixed mode), or null
+  const double *fun;      // the period's 1/svel, natural layout
+  double *skew;           // scratch: slot_skews skewed fields
+  double *T, *Ta, *Td, *delta, *aux;  // natural fields and forward tables
+  int slot_skews, aux_len;
+  double x0, y0;          // source (degrees)
+  int nr;                 // receivers; the pointers below start at the item's first
+  const double *recx, *recy, *tt, *weight;
+  const int *idx_path;
+  double *tekio, *weight_out, *rhs_adj, *rhs_den;
+  double *item_chi;       // &item_chi[item]
+  double *tt_fwd;         // the part's tt_fwd (indexed by idx_path)
+  int *status;            // &status[item]
+  int *iters;             // &iters[item * 6]: fwd, adj, density (FP64), then FP32
+};
+
 // Per-period fields in natural layout, period p at offset p*nx*ny.
 struct PeriodFields {
   const double *a, *b, *c;  // converted metric (src/eikon_solver_2d.f90:817-821)
@@ -62,12 +99,6 @@ struct AdjWork {
   const double *rhs;                // per receiver (global index)
 };
 
-struct RecWork {
-  const double *T;                  // [slot][N]
-  double *tekio, *weight_out, *rhs_adj, *rhs_den;  // per receiver
-  double *item_chi;                 // per global item
-  double *tt_fwd;                   // per path (scatter), may be null
-};
 
 // ---- launches (satt_kernels.hip); all asynchronous on `s`
 // sin(yy), cos(yy) of a grid into sy, cy (satt_strip.hip)
@@ -83,14 +114,17 @@ void launch_metric_fields(hipStream_t s, const Grid &g, const double *spha,
 void launch_forward(hipStream_t s, const Grid &g, const PeriodFields &pf,
                     const Items &it, const FwdWork &w, int *iters, int *status,
                     int iter_stride);
-void launch_receivers(hipStream_t s, const Grid &g, const Items &it,
-                      const RecWork &w, int *status);
+// Receivers of every slot of a launch group (descriptor-driven); tt_fwd
+// is written when istotable is set.
+void launch_receivers(hipStream_t s, const SlotDesc *sd, int count, int istotable);
 void launch_adjoint(hipStream_t s, const Grid &g, const PeriodFields &pf,
                     const Items &it, const AdjWork &w, int *iters, int *status,
                     int iter_stride, int apply_mask);
+// acc(p) += field/vel(p)**power over the slots per_slot[per_off[p]..) of a
+// launch group, in list order; field = Ta (which 0) or Td (which 1).
 void launch_stack(hipStream_t s, const Grid &g, const PeriodFields &pf,
                   int nperiod, const int *per_off, const int *per_slot,
-                  const double *fields, double *acc, int power);
+                  const SlotDesc *sd, int which, double *acc, int power);
 void launch_stack_item(hipStream_t s, const Grid &g, const PeriodFields &pf,
                        const int *pidx, int item, const double *field,
                        double *acc, int power);

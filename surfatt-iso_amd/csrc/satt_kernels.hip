@@ -328,46 +328,42 @@ __device__ bool interp2_point(const double *x, const double *y, int nx, int ny,
 }
 
 // Receiver interpolation, misfit, outlier weights and adjoint sources of one
-// item (src/measadj.f90:71-72, :82-88, :102, :111-113).
-__global__ __launch_bounds__(kBS) void k_receivers(Grid g, Items it, RecWork w,
-                                                   int *status) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+// item (src/measadj.f90:71-72, :82-88, :102, :111-113); one block per slot.
+__global__ __launch_bounds__(kBS) void k_receivers(const SlotDesc *sd, int istotable) {
+  const SlotDesc D = sd[blockIdx.x];
+  const Grid &g = D.g;
   const int nx = g.nx, ny = g.ny, tid = threadIdx.x;
-  const size_t N = (size_t)nx * ny;
-  const double *T = w.T + slot * N;
-  const int r0 = it.rec_off[item], nr = it.rec_off[item + 1] - r0;
+  const int nr = D.nr;
   __shared__ int s_bad;
   if (tid == 0) s_bad = 0;
   __syncthreads();
-  if (status[item] < 0) return;  // forward failed: nothing to measure
+  if (*D.status < 0) return;  // forward failed: nothing to measure
   for (int r = tid; r < nr; r += kBS) {
     double v = 0;
-    if (!interp2_point(g.xd, g.yd, nx, ny, T, it.recx[r0 + r], it.recy[r0 + r], &v))
-      s_bad = 1;
-    w.tekio[r0 + r] = v;
+    if (!interp2_point(g.xd, g.yd, nx, ny, D.T, D.recx[r], D.recy[r], &v)) s_bad = 1;
+    D.tekio[r] = v;
   }
   __syncthreads();
   if (s_bad) {
-    if (tid == 0) status[item] = kERecOutOfGrid;
+    if (tid == 0) *D.status = kERecOutOfGrid;
     return;
   }
   if (tid == 0) {
     double s = 0;
     for (int r = 0; r < nr; ++r) {
-      const double d = w.tekio[r0 + r] - it.tt[r0 + r];
-      s = s + it.weight[r0 + r] * (d * d);
+      const double d = D.tekio[r] - D.tt[r];
+      s = s + D.weight[r] * (d * d);
     }
-    w.item_chi[item] = 0.5 * s;
+    *D.item_chi = 0.5 * s;
   }
   for (int r = tid; r < nr; r += kBS) {
-    const double tek = w.tekio[r0 + r], tobs = it.tt[r0 + r];
-    double wt = it.weight[r0 + r];
+    const double tek = D.tekio[r], tobs = D.tt[r];
+    double wt = D.weight[r];
     if (fabs(tek - tobs) / tobs > 0.5) wt = 0.;
-    w.weight_out[r0 + r] = wt;
-    w.rhs_adj[r0 + r] = wt * (tek - tobs);
-    w.rhs_den[r0 + r] = -1.0 * wt;
-    if (w.tt_fwd) w.tt_fwd[it.idx_path[r0 + r]] = tek;
+    D.weight_out[r] = wt;
+    D.rhs_adj[r] = wt * (tek - tobs);
+    D.rhs_den[r] = -1.0 * wt;
+    if (istotable) D.tt_fwd[D.idx_path[r]] = tek;
   }
 }
 
@@ -557,7 +553,7 @@ __global__ void k_metric_fields(Grid g, const double *spha, const double *sphb,
 // adj_s(pidx,:,:) += adj / svel**power, items of each period in item order
 // (src/tomo2d.f90:155, src/acqui.f90:314-315).
 __global__ void k_stack(Grid g, PeriodFields pf, const int *per_off, const int *per_slot,
-                        const double *fields, double *acc, int power) {
+                        const SlotDesc *sd, int which, double *acc, int power) {
   const size_t N = (size_t)g.nx * g.ny;
   const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const int p = blockIdx.y;
@@ -565,8 +561,10 @@ __global__ void k_stack(Grid g, PeriodFields pf, const int *per_off, const int *
   const double v = pf.vel[p * N + k];
   const double den = power == 3 ? v * v * v : v * v;
   double s = acc[p * N + k];
-  for (int q = per_off[p]; q < per_off[p + 1]; ++q)
-    s = s + fields[(size_t)per_slot[q] * N + k] / den;
+  for (int q = per_off[p]; q < per_off[p + 1]; ++q) {
+    const SlotDesc D = sd[per_slot[q]];
+    s = s + (which ? D.Td : D.Ta)[k] / den;
+  }
   acc[p * N + k] = s;
 }
 
@@ -659,9 +657,8 @@ void launch_forward(hipStream_t s, const Grid &g, const PeriodFields &pf, const 
                      iter_stride);
 }
 
-void launch_receivers(hipStream_t s, const Grid &g, const Items &it, const RecWork &w,
-                      int *status) {
-  hipLaunchKernelGGL(k_receivers, dim3(it.count), dim3(kBS), 0, s, g, it, w, status);
+void launch_receivers(hipStream_t s, const SlotDesc *sd, int count, int istotable) {
+  hipLaunchKernelGGL(k_receivers, dim3(count), dim3(kBS), 0, s, sd, istotable);
 }
 
 void launch_adjoint(hipStream_t s, const Grid &g, const PeriodFields &pf, const Items &it,
@@ -672,10 +669,11 @@ void launch_adjoint(hipStream_t s, const Grid &g, const PeriodFields &pf, const 
 }
 
 void launch_stack(hipStream_t s, const Grid &g, const PeriodFields &pf, int np,
-                  const int *per_off, const int *per_slot, const double *fields, double *acc,
-                  int power) {
-  hipLaunchKernelGGL(k_stack, cells_grid((size_t)g.nx * g.ny, np), dim3(256), 0, s, g, pf,
-                     per_off, per_slot, fields, acc, power);
+                  const int *per_off, const int *per_slot, const SlotDesc *sd, int which,
+                  double *acc, int power) {
+  const size_t N = (size_t)g.nx * g.ny;
+  hipLaunchKernelGGL(k_stack, dim3((unsigned)((N + 255) / 256), np), dim3(256), 0, s, g, pf,
+                     per_off, per_slot, sd, which, acc, power);
 }
 
 void launch_stack_item(hipStream_t s, const Grid &g, const PeriodFields &pf, const int *pidx,

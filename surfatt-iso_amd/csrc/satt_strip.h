@@ -9,20 +9,6 @@ constexpr int kPF = 6;  // per-period skew fields: a, b, c, fun, dis_x, dis_y
 constexpr int kFF = 3;  // per-item forward skew fields: T0, T0x, T0y
 constexpr int kAF = 6;  // per-item adjoint skew fields: a1, a2, b1, b2, d, delta
 
-// Skewed layout of one field: ns strips of 64 columns, R = ny+63 rows of 64
-// lanes.  Plus skew: cell (64c+l, j) at row j+l; minus skew: row j-l+63.
-struct SkewGeom {
-  int ns, R;
-  __host__ __device__ size_t skew_elems() const { return (size_t)ns * R * 64; }
-};
-
-inline SkewGeom make_skew_geom(int nx, int ny) {
-  SkewGeom s;
-  s.ns = (nx + 63) / 64;
-  s.R = ny + 63;
-  return s;
-}
-
 // Per-slot scratch of the batched path, in units of one skewed field (SE
 // doubles) and one natural field (N doubles).
 //   skew fields 0..11 : adjoint coefficients (a1,a2,b1,b2,d,delta) x 2 skews;
@@ -31,10 +17,8 @@ inline SkewGeom make_skew_geom(int nx, int ny) {
 //   skew fields 12,13 : travel time T, plus and minus skew (forward output)
 //   skew fields 14..16: Ta plus, Ta minus, snapshot (adjoint)
 //   natural 0..3      : T, Ta, Td (density adjoint), delta source
-// The mixed mode keeps its float fields inside the same slot: the forward's
-// from double skew 9 (free until the output T skews are written), the
-// adjoint's from skew 0 (its FP64 coefficients are built after the FP32
-// phase has ended).
+// The mixed mode keeps the forward's float fields from double skew 9 (free
+// until the output T skews are written).
 constexpr int kSlotSkews = 17;
 constexpr int kSlotNat = 4;
 enum : int { kSkTauP = 6, kSkTauM = 7, kSkSnapF = 8, kSkTP = 12, kSkTM = 13,
@@ -50,49 +34,44 @@ constexpr int kSkA32 = 16;
 // frozen flags of the columns and the rows, then fun0, sin(y0), cos(y0).
 inline int aux_len(int nx, int ny) { return (3 * nx + ny + 4 + 1) & ~1; }
 
-struct StripWork {
-  double *skew;                // [slot][slot_skews][SE]
-  double *T, *Ta, *Td, *delta; // natural layout, [slot][N] each
-  double *aux;                 // [slot][aux_len]
-  int aux_len, slot_skews;
+// Doubles of one slot's scratch: its skewed fields, kSlotNat natural fields
+// and the forward tables, rounded up to 256 bytes.
+inline size_t slot_doubles(int nx, int ny, int slot_skews) {
+  const size_t n = (size_t)slot_skews * make_skew_geom(nx, ny).skew_elems() +
+                   (size_t)kSlotNat * nx * ny + (size_t)aux_len(nx, ny);
+  return (n + 31) & ~(size_t)31;
+}
+
+// Sizes of a launch group, for the grids of the element-wise kernels.
+struct GroupDims {
+  int count;        // slots
+  size_t max_se;    // largest skewed field
+  size_t max_n;     // largest grid
+  int max_nxy;      // largest nx or ny
 };
 
-// out32 (may be null): the same fields rounded to float, same layout.
+// Batched kernels over the slots of a launch group (satt_strip.hip); all
+// asynchronous on `s`.  Forward: setup (per-item tables, then T0 skews and
+// the tau init, one thread per element), then the sweeps: k_fwd_strip
+// (FP64), or k_fwd_mixed32 (FP32 phase) followed by k_fwd_polish (FP64 to
+// the reference's stop rule).
+void launch_fwd_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, bool mixed);
+void launch_fwd_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd);
+void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd);
+void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd);
+// Adjoint (which 0) or density adjoint (which 1): setup (the coefficients
+// without the source -- FP64 with the Ta init, and in the mixed mode the
+// float set too -- then the delta source from the rhs, accumulated in
+// natural layout and scattered into the coefficient fields), then the
+// sweeps: k_adj_strip (FP64), or k_adj_mixed32 followed by k_adj_polish
+// (one FP64 iteration).  Output Ta (which 0) or Td (which 1), masked.
+void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
+                      bool mixed);
+void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which);
+void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which);
+void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which);
+// Per-period skewed fields of one grid (out32: the same in float, or null).
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
                         const PeriodFields &pf, double *out, float *out32);
-
-// Forward: setup (per-item tables, then T0 skews and the tau init, one
-// thread per element), then the
-// sweeps: k_fwd_strip (FP64), or k_fwd_mixed32 (FP32 phase, iteration count
-// to iters32) followed by k_fwd_polish (FP64 to the reference's stop rule).
-void launch_fwd_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
-                      const Items &it, const StripWork &w, int *status, bool mixed);
-void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const PeriodFields &pf, const Items &it, const StripWork &w,
-                      int *iters, int *status, int iter_stride);
-void launch_fwd_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const float *pskew32,
-                        const PeriodFields &pf, const Items &it, const StripWork &w,
-                        int *iters32, int *status, int iter_stride);
-void launch_fwd_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                       const PeriodFields &pf, const Items &it, const StripWork &w, int *iters,
-                       int *status, int iter_stride);
-
-// Adjoint: setup (the coefficients without the source -- FP64 with the Ta
-// init, and in the mixed mode the float set too -- then the delta source
-// from rhs per receiver, accumulated in natural layout and scattered into
-// the coefficient fields), then the sweeps: k_adj_strip (FP64), or
-// k_adj_mixed32 followed by k_adj_polish (one FP64 iteration).  out_nat:
-// kNatTa (adjoint) or kNatTd (density adjoint).
-void launch_adj_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const Items &it, const StripWork &w, const double *rhs, int *iters,
-                      int *status, int iter_stride, bool mixed);
-void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
-                      const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
-                      int apply_mask);
-void launch_adj_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
-                        const StripWork &w, int *iters32, int *status, int iter_stride);
-void launch_adj_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
-                       const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
-                       int apply_mask);
 
 }  // namespace satt

@@ -560,13 +560,13 @@ __global__ void k_grid_trig(Grid g, double *sy, double *cy) {
 // cos(xx - x0), the frozen flags abs((xx-x0)/dx) <= 1 and abs((yy-y0)/dy) <= 1
 // (src/eikon_solver_2d.f90:859-872), and fun0 (:825-841), sin(y0), cos(y0).
 // An item whose source bracket leaves the grid gets its error status.
-__global__ __launch_bounds__(64) void k_fwd_prep(Grid g, const double *fun_nat, Items it,
-                                                 StripWork w, int *status) {
+__global__ __launch_bounds__(64) void k_fwd_prep(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.y];
+  const Grid &g = D.g;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = blockIdx.y, item = it.item0 + slot;
   const int nx = g.nx, ny = g.ny;
-  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
-  double *aux = w.aux + (size_t)slot * w.aux_len;
+  const double x0 = deg2rad_s(D.x0), y0 = deg2rad_s(D.y0);
+  double *aux = D.aux;
   if (k < nx) {
     aux[k] = sin(g.xx[k] - x0);
     aux[nx + k] = cos(g.xx[k] - x0);
@@ -575,11 +575,11 @@ __global__ __launch_bounds__(64) void k_fwd_prep(Grid g, const double *fun_nat, 
   if (k < ny) aux[3 * nx + k] = fabs((g.yy[k] - y0) / g.dy) <= 1 ? 1.0 : 0.0;
   if (k == 0) {
     double fun0 = 0;
-    const int st0 = fwd_fun0_s(g, fun_nat + (size_t)it.pidx[item] * nx * ny, x0, y0, &fun0);
+    const int st0 = fwd_fun0_s(g, D.fun, x0, y0, &fun0);
     aux[3 * nx + ny] = fun0;
     aux[3 * nx + ny + 1] = sin(y0);
     aux[3 * nx + ny + 2] = cos(y0);
-    if (st0 != kOk) status[item] = st0;
+    if (st0 != kOk) *D.status = st0;
   }
 }
 
@@ -610,15 +610,16 @@ __device__ __forceinline__ void t0_cell_tab(double sy, double cy, double sdx, do
 // cell on the boundary is the reference's 'source on the boundary' error
 // (:863-867).
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_fwd_setup(Grid g, SkewGeom sg, Items it, StripWork w,
-                                                   int *status) {
+__global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.z];
+  const Grid &g = D.g;
+  const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const int sk = blockIdx.y, slot = blockIdx.z;
-  const int item = it.item0 + slot;
-  if (e >= SE || status[item] < 0) return;
+  const int sk = blockIdx.y;
+  if (e >= SE || *D.status < 0) return;
   const int nx = g.nx, ny = g.ny;
-  const double *aux = w.aux + (size_t)slot * w.aux_len;
+  const double *aux = D.aux;
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % sg.R);
   const int cc = (int)((e >> 6) / sg.R);
@@ -631,9 +632,9 @@ __global__ __launch_bounds__(256) void k_fwd_setup(Grid g, SkewGeom sg, Items it
     const double *tail = aux + 3 * nx + ny;
     t0_cell_tab(g.sy[j], g.cy[j], aux[i], aux[nx + i], tail[1], tail[2], tail[0], t0, t0x, t0y);
     frz = aux[2 * nx + i] != 0.0 && aux[3 * nx + j] != 0.0;
-    if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) status[item] = kESrcOnBoundary;
+    if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) *D.status = kESrcOnBoundary;
   }
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  double *sk0 = D.skew;
   double *b0 = sk0 + (size_t)sk * kFF * SE;
   b0[e] = t0; b0[SE + e] = t0x; b0[2 * SE + e] = t0y;
   if (MIXED) {
@@ -657,17 +658,18 @@ __device__ void fwd_ctx_init(FwdCtx<V> &C, const Grid &g, const SkewGeom &sg, in
 
 // FP64 iterations from the tau in the plus skew, then T = tau*T0
 // (src/eikon_solver_2d.f90:1124) in both skews and in natural layout.
-__device__ void fwd_finish64(const Grid &g, const SkewGeom &sg, int lane, const double *pskew_p,
-                             double *sk0, const int rect[4], double *Tnat, int item,
-                             int *iters, int *status, int iter_stride) {
+__device__ void fwd_finish64(const SlotDesc &D, int lane, const int rect[4]) {
+  const Grid &g = D.g;
+  const SkewGeom &sg = D.sg;
   const int nx = g.nx, ny = g.ny, R = sg.R, ns = sg.ns;
   const size_t N = (size_t)nx * ny, SE = sg.skew_elems();
+  double *sk0 = D.skew, *Tnat = D.T;
   double *tauP = sk0 + kSkTauP * SE, *tauM = sk0 + kSkTauM * SE;
   double *TP = sk0 + kSkTP * SE, *TM = sk0 + kSkTM * SE;
   FwdCtx<double> C;
   fwd_ctx_init(C, g, sg, lane, rect);
-  C.ps[0] = pskew_p;
-  C.ps[1] = pskew_p + (size_t)kPF * SE;
+  C.ps[0] = D.pskew;
+  C.ps[1] = D.pskew + (size_t)kPF * SE;
   C.is[0] = sk0;
   C.is[1] = sk0 + kFF * SE;
   C.snap = sk0 + kSkSnapF * SE;
@@ -693,55 +695,43 @@ __device__ void fwd_finish64(const Grid &g, const SkewGeom &sg, int lane, const 
     Tnat[k] = TP[skpos(0, i, j, R)];
   }
   if (lane == 0) {
-    status[item] = conv ? kOk : kWMaxIter;
-    iters[item * iter_stride] = iter;
+    *D.status = conv ? kOk : kWMaxIter;
+    D.iters[0] = iter;
   }
 }
 
 // The frozen rectangle of an item whose setup succeeded.
-__device__ bool fwd_item_rect(const Grid &g, const Items &it, const double *fun_nat, int item,
-                              int lane, int rect[4]) {
-  const size_t N = (size_t)g.nx * g.ny;
-  const double x0 = deg2rad_s(it.srcx[item]), y0 = deg2rad_s(it.srcy[item]);
+__device__ bool fwd_item_rect(const SlotDesc &D, int lane, int rect[4]) {
+  const double x0 = deg2rad_s(D.x0), y0 = deg2rad_s(D.y0);
   double fun0;
-  return fwd_source_rect(g, fun_nat + (size_t)it.pidx[item] * N, x0, y0, lane, fun0, rect) == kOk;
+  return fwd_source_rect(D.g, D.fun, x0, y0, lane, fun0, rect) == kOk;
 }
 
-__global__ __launch_bounds__(64) void k_fwd_strip(Grid g, SkewGeom sg, const double *pskew,
-                                                  const double *fun_nat, Items it, StripWork w,
-                                                  int *iters, int *status, int iter_stride) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+__global__ __launch_bounds__(64) void k_fwd_strip(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x;
-  if (status[item] < 0) return;  // the setup failed
-  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
+  if (*D.status < 0) return;  // the setup failed
   int rect[4];
-  if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
-  fwd_finish64(g, sg, lane, pskew + (size_t)it.pidx[item] * 2 * kPF * SE, sk0, rect,
-               w.T + (size_t)slot * N, item, iters, status, iter_stride);
+  if (!fwd_item_rect(D, lane, rect)) return;
+  fwd_finish64(D, lane, rect);
 }
 
-// Mixed forward, FP32 phase: the FP32 iterations (count to iters32), then
+// Mixed forward, FP32 phase: the FP32 iterations (count to iters[3]), then
 // tau = 1 + dtau into the FP64 plus skew for the polish.
-__global__ __launch_bounds__(64) void k_fwd_mixed32(Grid g, SkewGeom sg, const float *pskew32,
-                                                    const double *fun_nat, Items it,
-                                                    StripWork w, int *iters32, int *status,
-                                                    int iter_stride) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+__global__ __launch_bounds__(64) void k_fwd_mixed32(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x;
-  if (status[item] < 0) return;
+  if (*D.status < 0) return;
+  const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
   int rect[4];
-  if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  if (!fwd_item_rect(D, lane, rect)) return;
+  double *sk0 = D.skew;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
   FwdCtx<float> C;
-  fwd_ctx_init(C, g, sg, lane, rect);
-  const float *ps = pskew32 + (size_t)it.pidx[item] * 2 * kPF * SE;
-  C.ps[0] = ps;
-  C.ps[1] = ps + (size_t)kPF * SE;
+  fwd_ctx_init(C, D.g, sg, lane, rect);
+  C.ps[0] = D.pskew32;
+  C.ps[1] = D.pskew32 + (size_t)kPF * SE;
   C.is[0] = f32;
   C.is[1] = f32 + (size_t)kFF * SE;
   C.snap = f32 + 8 * SE;
@@ -752,25 +742,18 @@ __global__ __launch_bounds__(64) void k_fwd_mixed32(Grid g, SkewGeom sg, const f
   if (iter > kFwdMaxIter) iter = kFwdMaxIter;
   double *tauP = sk0 + kSkTauP * SE;
   for (size_t e = lane; e < SE; e += 64) tauP[e] = 1.0 + (double)dP[e];
-  if (lane == 0) iters32[item * iter_stride] = iter;
+  if (lane == 0) D.iters[3] = iter;
 }
 
 // Mixed forward, FP64 polish: the reference's iterations to its stop rule
 // from the FP32 result, and the outputs.
-__global__ __launch_bounds__(64) void k_fwd_polish(Grid g, SkewGeom sg, const double *pskew,
-                                                   const double *fun_nat, Items it,
-                                                   StripWork w, int *iters, int *status,
-                                                   int iter_stride) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+__global__ __launch_bounds__(64) void k_fwd_polish(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x;
-  if (status[item] < 0) return;
-  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
+  if (*D.status < 0) return;
   int rect[4];
-  if (!fwd_item_rect(g, it, fun_nat, item, lane, rect)) return;
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
-  fwd_finish64(g, sg, lane, pskew + (size_t)it.pidx[item] * 2 * kPF * SE, sk0, rect,
-               w.T + (size_t)slot * N, item, iters, status, iter_stride);
+  if (!fwd_item_rect(D, lane, rect)) return;
+  fwd_finish64(D, lane, rect);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -818,35 +801,37 @@ __device__ int adj_splat_s(const Grid &g, const double *xr_deg, const double *yr
   return kOk;
 }
 
-// Delta sources of a chunk (src/eikon_solver_2d.f90:1320-1336): zero the
-// natural fields, one thread per cell ...
-__global__ __launch_bounds__(256) void k_adj_zero(size_t N, Items it, StripWork w) {
+// Delta sources of a launch group (src/eikon_solver_2d.f90:1320-1336): zero
+// the natural fields, one thread per cell ...
+__global__ __launch_bounds__(256) void k_adj_zero(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.y];
   const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (k < N) w.delta[(size_t)blockIdx.y * N + k] = 0;
+  if (k < (size_t)D.g.nx * D.g.ny) D.delta[k] = 0;
 }
 
-// Coefficients of every item of a chunk, one thread per skew element; the
-// neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
+// Coefficients of every item of a launch group, one thread per skew element;
+// the neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
 // The source field is left 0 here and scattered in by k_adj_splat.
 //   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
 //           the boundary (:1368-1380), in the plus skew
-//   KIND 1: the same without the Ta init, plus (mixed mode) the float set
+//   KIND 1: the FP64 set without the Ta init, plus (mixed mode) the float set
 //           k0 (0 here), kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
 //           kN = -b2m/dy/d and the float Ta init, from double skew 16
 // Padding elements get zeros.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_adj_build(Grid g, SkewGeom sg, const double *pskew,
-                                                   Items it, StripWork w, const int *status) {
+__global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.z];
+  const Grid &g = D.g;
+  const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const int sk = blockIdx.y, slot = blockIdx.z;
-  const int item = it.item0 + slot;
-  if (e >= SE || status[item] < 0) return;
+  const int sk = blockIdx.y;
+  if (e >= SE || *D.status < 0) return;
   const int nx = g.nx, ny = g.ny, R = sg.R;
   const double dx = g.dx, dy = g.dy;
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  double *sk0 = D.skew;
   const double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
-  const double *pa = pskew + ((size_t)it.pidx[item] * 2 + sk) * kPF * SE;
+  const double *pa = D.pskew + (size_t)sk * kPF * SE;
   const double *pb = pa + SE, *pc = pa + 2 * SE;
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % R);
@@ -889,34 +874,33 @@ __global__ __launch_bounds__(256) void k_adj_build(Grid g, SkewGeom sg, const do
 // ... then, one thread per item: splat the receivers in order into the
 // natural field (the reference's sums, in its order), and scatter the
 // source of every touched interior cell into the coefficient fields of both
-// skews (MIXED: also k0 = delta/d rounded to float, where d != 0).
-__global__ __launch_bounds__(64) void k_adj_splat(Grid g, SkewGeom sg, Items it, StripWork w,
-                                                  const double *rhs, int *iters, int *status,
-                                                  int iter_stride, int mixed) {
+// skews (mixed: also k0 = delta/d rounded to float, where d != 0).  The
+// right-hand side is w*(t - tobs) (which 0) or -w (which 1).
+__global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count, int which,
+                                                  int mixed) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= it.count) return;
-  const int item = it.item0 + slot;
-  if (status[item] < 0) return;  // an earlier stage failed
-  const int nx = g.nx, ny = g.ny, R = sg.R;
-  const size_t SE = sg.skew_elems();
-  double *delta = w.delta + (size_t)slot * nx * ny;
-  const int r0 = it.rec_off[item], nr = it.rec_off[item + 1] - r0;
-  const int err = adj_splat_s(g, it.recx + r0, it.recy + r0, rhs + r0, nr, delta);
+  if (slot >= count) return;
+  const SlotDesc D = sd[slot];
+  if (*D.status < 0) return;  // an earlier stage failed
+  const Grid &g = D.g;
+  const int nx = g.nx, ny = g.ny, R = D.sg.R;
+  const size_t SE = D.sg.skew_elems();
+  const int err = adj_splat_s(g, D.recx, D.recy, which ? D.rhs_den : D.rhs_adj, D.nr, D.delta);
   if (err != kOk) {
-    status[item] = err;
-    iters[item * iter_stride] = 0;
+    *D.status = err;
+    D.iters[1 + which] = 0;
     return;
   }
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  double *sk0 = D.skew;
   float *k32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
-  for (int q = 0; q < nr; ++q) {
-    const double xr = deg2rad_s(it.recx[r0 + q]), yr = deg2rad_s(it.recy[r0 + q]);
+  for (int q = 0; q < D.nr; ++q) {
+    const double xr = deg2rad_s(D.recx[q]), yr = deg2rad_s(D.recy[q]);
     const int i0 = (int)floor((xr - g.xx[0]) / g.dx + 1) - 1;
     const int j0 = (int)floor((yr - g.yy[0]) / g.dy + 1) - 1;
     for (int c = 0; c < 4; ++c) {
       const int i = i0 + (c >> 1), j = j0 + (c & 1);
       if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
-      const double dl = delta[i + (size_t)nx * j];
+      const double dl = D.delta[i + (size_t)nx * j];
       for (int sk = 0; sk < 2; ++sk) {
         const size_t e = skpos(sk, i, j, R);
         double *b0 = sk0 + (size_t)sk * kAF * SE;
@@ -1105,17 +1089,15 @@ __device__ void adj_output(const Grid &g, const SkewGeom &sg, int lane, const do
   }
 }
 
-__global__ __launch_bounds__(64) void k_adj_strip(Grid g, SkewGeom sg, Items it, StripWork w,
-                                                  int out_nat, int *iters, int *status,
-                                                  int iter_stride, int apply_mask) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+__global__ __launch_bounds__(64) void k_adj_strip(const SlotDesc *sd, int which) {
+  const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x;
-  if (status[item] < 0) return;  // an earlier stage failed
-  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  if (*D.status < 0) return;  // an earlier stage failed
+  const Grid &g = D.g;
+  const SkewGeom &sg = D.sg;
+  const size_t SE = sg.skew_elems();
+  double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
-  double *Tout = (out_nat == kNatTd ? w.Td : w.Ta) + (size_t)slot * N;
   AdjCtx<double> C;
   adj_ctx_init(C, g, sg, lane);
   C.is[0] = sk0;
@@ -1130,25 +1112,24 @@ __global__ __launch_bounds__(64) void k_adj_strip(Grid g, SkewGeom sg, Items it,
     if (fabs(l1) < kAdjTol && m < kAdjTol) { conv = true; break; }
   }
   if (!conv) iter = kAdjMaxIter;
-  adj_output(g, sg, lane, TaP, it.srcx[item], it.srcy[item], apply_mask, Tout);
+  adj_output(g, sg, lane, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
   if (lane == 0) {
-    if (!conv && status[item] == kOk) status[item] = kWMaxIter;
-    iters[item * iter_stride] = iter;
+    if (!conv && *D.status == kOk) *D.status = kWMaxIter;
+    D.iters[1 + which] = iter;
   }
 }
 
 // Mixed adjoint, FP32 phase: iterations on the float coefficients (float
 // skews from double skew 16: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
 // minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
-__global__ __launch_bounds__(64) void k_adj_mixed32(Grid g, SkewGeom sg, Items it, StripWork w,
-                                                    int *iters32, int *status,
-                                                    int iter_stride) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+__global__ __launch_bounds__(64) void k_adj_mixed32(const SlotDesc *sd, int which) {
+  const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x;
-  if (status[item] < 0) return;
+  if (*D.status < 0) return;
+  const Grid &g = D.g;
+  const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  double *sk0 = D.skew;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
   float *taP = f32 + 10 * SE, *taM = f32 + 11 * SE;
   AdjCtx<float> C;
@@ -1171,22 +1152,21 @@ __global__ __launch_bounds__(64) void k_adj_mixed32(Grid g, SkewGeom sg, Items i
   double *TaP = sk0 + kSkTaP * SE;
   for (size_t e = lane; e < SE; e += 64) TaP[e] = (double)taP[e];
   if (lane == 0) {
-    if (!done && status[item] == kOk) status[item] = kWMaxIter;
-    iters32[item * iter_stride] = iter;
+    if (!done && *D.status == kOk) *D.status = kWMaxIter;
+    D.iters[4 + which] = iter;
   }
 }
 
 // Mixed adjoint, FP64 polish: one iteration of the reference's update on the
 // FP64 coefficients, and the masked output.
-__global__ __launch_bounds__(64) void k_adj_polish(Grid g, SkewGeom sg, Items it, StripWork w,
-                                                   int out_nat, int *iters, int *status,
-                                                   int iter_stride, int apply_mask) {
-  const int slot = blockIdx.x;
-  const int item = it.item0 + slot;
+__global__ __launch_bounds__(64) void k_adj_polish(const SlotDesc *sd, int which) {
+  const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x;
-  if (status[item] < 0) return;
-  const size_t N = (size_t)g.nx * g.ny, SE = sg.skew_elems();
-  double *sk0 = w.skew + (size_t)slot * w.slot_skews * SE;
+  if (*D.status < 0) return;
+  const Grid &g = D.g;
+  const SkewGeom &sg = D.sg;
+  const size_t SE = sg.skew_elems();
+  double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
   adj_ctx_init(C, g, sg, lane);
@@ -1195,9 +1175,8 @@ __global__ __launch_bounds__(64) void k_adj_polish(Grid g, SkewGeom sg, Items it
   C.snap = nullptr;
   double s, m, a;
   adj_iteration<double, false>(C, sg.ns, TaP, TaM, s, m, a);
-  double *Tout = (out_nat == kNatTd ? w.Td : w.Ta) + (size_t)slot * N;
-  adj_output(g, sg, lane, TaP, it.srcx[item], it.srcy[item], apply_mask, Tout);
-  if (lane == 0) iters[item * iter_stride] = 1;
+  adj_output(g, sg, lane, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
+  if (lane == 0) D.iters[1 + which] = 1;
 }
 
 }  // namespace
@@ -1213,72 +1192,55 @@ void launch_grid_trig(hipStream_t s, const Grid &g, double *sy, double *cy) {
   hipLaunchKernelGGL(k_grid_trig, dim3((unsigned)((g.ny + 63) / 64)), dim3(64), 0, s, g, sy, cy);
 }
 
-void launch_fwd_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const PeriodFields &pf,
-                      const Items &it, const StripWork &w, int *status, bool mixed) {
-  const int nk = g.nx > g.ny ? g.nx : g.ny;
-  hipLaunchKernelGGL(k_fwd_prep, dim3((unsigned)((nk + 63) / 64), it.count), dim3(64), 0, s, g,
-                     pf.fun, it, w, status);
-  const dim3 grid((unsigned)((sg.skew_elems() + 255) / 256), 2, it.count);
+namespace {
+dim3 skew_grid(const GroupDims &gd) {
+  return dim3((unsigned)((gd.max_se + 255) / 256), 2, (unsigned)gd.count);
+}
+}  // namespace
+
+void launch_fwd_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, bool mixed) {
+  hipLaunchKernelGGL(k_fwd_prep, dim3((unsigned)((gd.max_nxy + 63) / 64), gd.count), dim3(64), 0,
+                     s, sd);
   if (mixed)
-    hipLaunchKernelGGL(k_fwd_setup<true>, grid, dim3(256), 0, s, g, sg, it, w, status);
+    hipLaunchKernelGGL(k_fwd_setup<true>, skew_grid(gd), dim3(256), 0, s, sd);
   else
-    hipLaunchKernelGGL(k_fwd_setup<false>, grid, dim3(256), 0, s, g, sg, it, w, status);
+    hipLaunchKernelGGL(k_fwd_setup<false>, skew_grid(gd), dim3(256), 0, s, sd);
 }
 
-void launch_fwd_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const PeriodFields &pf, const Items &it, const StripWork &w, int *iters,
-                      int *status, int iter_stride) {
-  hipLaunchKernelGGL(k_fwd_strip, dim3(it.count), dim3(64), 0, s, g, sg, pskew, pf.fun, it, w,
-                     iters, status, iter_stride);
+void launch_fwd_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
+  hipLaunchKernelGGL(k_fwd_strip, dim3(gd.count), dim3(64), 0, s, sd);
 }
 
-void launch_fwd_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const float *pskew32,
-                        const PeriodFields &pf, const Items &it, const StripWork &w,
-                        int *iters32, int *status, int iter_stride) {
-  hipLaunchKernelGGL(k_fwd_mixed32, dim3(it.count), dim3(64), 0, s, g, sg, pskew32, pf.fun, it,
-                     w, iters32, status, iter_stride);
+void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
+  hipLaunchKernelGGL(k_fwd_mixed32, dim3(gd.count), dim3(64), 0, s, sd);
 }
 
-void launch_fwd_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                       const PeriodFields &pf, const Items &it, const StripWork &w, int *iters,
-                       int *status, int iter_stride) {
-  hipLaunchKernelGGL(k_fwd_polish, dim3(it.count), dim3(64), 0, s, g, sg, pskew, pf.fun, it, w,
-                     iters, status, iter_stride);
+void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
+  hipLaunchKernelGGL(k_fwd_polish, dim3(gd.count), dim3(64), 0, s, sd);
 }
 
-void launch_adj_setup(hipStream_t s, const Grid &g, const SkewGeom &sg, const double *pskew,
-                      const Items &it, const StripWork &w, const double *rhs, int *iters,
-                      int *status, int iter_stride, bool mixed) {
-  const size_t N = (size_t)g.nx * g.ny;
-  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((N + 255) / 256), it.count), dim3(256), 0, s, N,
-                     it, w);
-  const dim3 grid((unsigned)((sg.skew_elems() + 255) / 256), 2, it.count);
+void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
+                      bool mixed) {
+  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((gd.max_n + 255) / 256), gd.count), dim3(256),
+                     0, s, sd);
   if (mixed)
-    hipLaunchKernelGGL(k_adj_build<1>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
+    hipLaunchKernelGGL(k_adj_build<1>, skew_grid(gd), dim3(256), 0, s, sd);
   else
-    hipLaunchKernelGGL(k_adj_build<0>, grid, dim3(256), 0, s, g, sg, pskew, it, w, status);
-  hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((it.count + 63) / 64)), dim3(64), 0, s, g, sg,
-                     it, w, rhs, iters, status, iter_stride, (int)mixed);
+    hipLaunchKernelGGL(k_adj_build<0>, skew_grid(gd), dim3(256), 0, s, sd);
+  hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((gd.count + 63) / 64)), dim3(64), 0, s, sd,
+                     gd.count, which, (int)mixed);
 }
 
-void launch_adj_strip(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
-                      const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
-                      int apply_mask) {
-  hipLaunchKernelGGL(k_adj_strip, dim3(it.count), dim3(64), 0, s, g, sg, it, w, out_nat, iters,
-                     status, iter_stride, apply_mask);
+void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
+  hipLaunchKernelGGL(k_adj_strip, dim3(gd.count), dim3(64), 0, s, sd, which);
 }
 
-void launch_adj_mixed32(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
-                        const StripWork &w, int *iters32, int *status, int iter_stride) {
-  hipLaunchKernelGGL(k_adj_mixed32, dim3(it.count), dim3(64), 0, s, g, sg, it, w, iters32,
-                     status, iter_stride);
+void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
+  hipLaunchKernelGGL(k_adj_mixed32, dim3(gd.count), dim3(64), 0, s, sd, which);
 }
 
-void launch_adj_polish(hipStream_t s, const Grid &g, const SkewGeom &sg, const Items &it,
-                       const StripWork &w, int out_nat, int *iters, int *status, int iter_stride,
-                       int apply_mask) {
-  hipLaunchKernelGGL(k_adj_polish, dim3(it.count), dim3(64), 0, s, g, sg, it, w, out_nat, iters,
-                     status, iter_stride, apply_mask);
+void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
+  hipLaunchKernelGGL(k_adj_polish, dim3(gd.count), dim3(64), 0, s, sd, which);
 }
 
 }  // namespace satt

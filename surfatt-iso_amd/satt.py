@@ -217,6 +217,10 @@ class Context:
     def plan(self, batch, precision=PREC_FP64):
         return Plan(self, batch, precision)
 
+    def plan_multi(self, batches, precision=PREC_FP64):
+        """One plan over several grids (one batch dictionary per grid)."""
+        return Plan(self, list(batches), precision)
+
     def forward_simulate(self, batch, istotable=True, isadj=True, precision=PREC_FP64):
         with Plan(self, batch, precision) as p:
             p.run(istotable, isadj)
@@ -236,14 +240,18 @@ def _batch_struct(b):
 
 
 class Plan:
-    """satt_plan: model, gathers and fields resident on the device."""
+    """satt_plan: model, gathers and fields resident on the device.  `batch`
+    is one batch dictionary, or a list of them (one grid each)."""
 
     def __init__(self, ctx, batch, precision=PREC_FP64):
-        self.ctx, self.lib, self.b = ctx, ctx.lib, batch
-        self._bs = _batch_struct(batch)
+        self.ctx, self.lib = ctx, ctx.lib
+        self.parts = batch if isinstance(batch, list) else [batch]
+        self.b = self.parts[0]
+        arr = (Batch * len(self.parts))(*[_batch_struct(x) for x in self.parts])
+        self._bs = arr
         p = ctypes.c_void_p()
-        st = self.lib.satt_plan_create(ctx.ptr, ctypes.byref(self._bs), int(precision),
-                                       ctypes.byref(p))
+        st = self.lib.satt_plan_create_multi(ctx.ptr, len(self.parts), arr, int(precision),
+                                             ctypes.byref(p))
         ctx._check(st)
         self.ptr = p
 
@@ -258,9 +266,10 @@ class Plan:
     def __exit__(self, *a):
         self.close()
 
-    def set_model(self, svel, m11, m12, m22):
+    def set_model(self, svel, m11, m12, m22, part=0):
         arrs = [_f(v) for v in (svel, m11, m12, m22)]
-        self.ctx._check(self.lib.satt_plan_set_model(self.ptr, *(_d(a) for a in arrs)))
+        self.ctx._check(self.lib.satt_plan_set_model_part(self.ptr, int(part),
+                                                          *(_d(a) for a in arrs)))
 
     def run(self, istotable=True, isadj=True, launch=LAUNCH_BATCHED):
         self.flags = (istotable, isadj)
@@ -269,8 +278,8 @@ class Plan:
     def wait(self):
         self.ctx._check(self.lib.satt_plan_wait(self.ptr))
 
-    def results(self):
-        b = self.b
+    def results(self, part=0):
+        b = self.parts[part]
         nit, npd, nx, ny = len(b["item_pidx"]), b["nperiod"], b["nx"], b["ny"]
         out = {
             "tt_fwd": np.zeros(b["npath"]),
@@ -289,7 +298,7 @@ class Plan:
         r.item_chi, r.iters, r.item_status = _d(out["item_chi"]), _i(out["iters"]), _i(out["item_status"])
         r.weight_out = _d(out["weight_out"])
         r.iters_fp32 = _i(out["iters_fp32"])
-        st = self.lib.satt_plan_results(self.ptr, ctypes.byref(r))
+        st = self.lib.satt_plan_results_part(self.ptr, int(part), ctypes.byref(r))
         out["status"] = st
         out["chi"] = float(out["chi"][0])
         if st < 0:
