@@ -197,6 +197,55 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
+def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
+    """The serving mix (grid sizes 101/201/401/801 squared with probabilities
+    0.4/0.3/0.2/0.1, 100 receivers) as a Poisson stream of `nreq` requests at
+    `qps`: latency percentiles from submit to results, throughput, and the
+    plain FP64 one-solve-per-launch time of a sample of the same requests."""
+    reqs = W.serving_requests(nreq, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    gaps = rng.exponential(1.0 / qps, nreq)
+    with satt.Server(ctx, prec) as srv:
+        for r in reqs[:4]:                      # warm the kernels and pools
+            srv.wait(srv.submit(r))
+        tickets, t0 = [], time.perf_counter()
+        due = t0
+        for r, gap in zip(reqs, gaps):
+            due += gap
+            wait = due - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            tickets.append(srv.submit(r))
+        outs = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([o["latency_ms"] for o in outs])
+    sizes = np.array([r["nx"] for r in reqs])
+    by_size = {int(m): {"n": int(np.sum(sizes == m)),
+                        "p50_ms": round(float(np.percentile(lat[sizes == m], 50)), 2)}
+               for m in sorted(set(sizes.tolist()))}
+    plain = None
+    if plain_sample > 0:
+        tot, tplain = 0.0, 0.0
+        for r in reqs[:plain_sample]:
+            with ctx.plan(W.request_batch(r), satt.PREC_FP64) as pp:
+                t1 = time.perf_counter()
+                pp.run(True, True, satt.LAUNCH_PER_SOLVE)
+                pp.wait()
+                tplain += time.perf_counter() - t1
+            tot += 1
+        plain = tot / tplain
+    return {"qps": qps, "requests": nreq,
+            "mix": "squared grids 101/201/401/801 (p .4/.3/.2/.1) at 0.005 deg, 100 receivers",
+            "p50_ms": round(float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "mean_ms": round(float(lat.mean()), 2), "by_size": by_size,
+            "solves_per_s": round(nreq / wall, 2), "wall_s": round(wall, 2),
+            "launch_groups": int(st["launches"]), "max_in_flight": int(st["max_in_flight"]),
+            "all_ok": bool(all(o["status"] == 0 for o in outs)),
+            "plain_fp64_per_solve_solves_per_s": None if plain is None else round(plain, 3)}
+
+
 def host_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -214,6 +263,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--oracle-items", type=int, default=4,
                     help="items of the workload checked against the FP64 oracle (mixed)")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the serving stream (0: skip the serving measurement)")
+    ap.add_argument("--serve-qps", type=float, default=40.0)
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="mixed mode: skip the FP64 batched run and the error report")
     ap.add_argument("--plain-items", type=int, default=48,
@@ -327,6 +379,11 @@ def main():
         plain = args.plain_items / pdt
         log(f"[bench] plain per-solve FP64: {plain:.1f} solves/s")
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve(ctx, prec, args.serve_requests, args.serve_qps)
+        log(f"[bench] serving: {serving}")
+
     cpu = None
     if not args.no_cpu:
         th = host_threads()
@@ -364,6 +421,7 @@ def main():
         "mixed_error": mixed_err,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),
         "batched_vs_plain": None if plain is None else round(value / plain, 2),
+        "serving": serving,
         "cpu_baseline": cpu,
         "vs_cpu": None if cpu is None else round(value / cpu["value"], 1),
     }

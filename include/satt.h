@@ -202,6 +202,67 @@ int satt_plan_get_stats(satt_plan *plan, satt_plan_stats *stats);
 #define SATT_FIELD_TD 2
 int satt_plan_item_field(satt_plan *plan, int item, int which, double *out);
 
+/* ------------------------------------------------------------ serving
+ * A request loop for independent fwd+adj solves (no counterpart in the
+ * reference, whose loop runs a fixed list).  Each request carries its own
+ * grid, phase-velocity map (one period) and receivers; the server copies the
+ * inputs at submit.  A scheduler thread admits waiting requests while device
+ * memory (a pool reserved at creation) and launch streams are free, runs each
+ * admitted group as one batched launch sequence on its own stream -- groups
+ * overlap on the device -- and fills the responses as groups complete.  A
+ * request's results do not depend on what it was batched with. */
+typedef struct satt_server satt_server;
+
+typedef struct {
+  int precision;          /* SATT_PREC_FP64 or SATT_PREC_MIXED                 */
+  int max_batch;          /* requests per launch group (0: 256)                */
+  int streams;            /* launch groups in flight (0: 4)                    */
+  long long device_bytes; /* pool for requests in flight (0: half the free memory) */
+} satt_server_config;
+
+typedef struct {
+  int nx, ny;
+  const double *xgrids, *ygrids;  /* degrees */
+  const double *svel;             /* (nx,ny) phase velocity of the request's period */
+  const double *m11, *m12, *m22;  /* (nx,ny) metric, or all NULL: 1, 0, 1       */
+  double srcx, srcy;              /* source, degrees                             */
+  int nrec;
+  const double *recx, *recy, *tt, *weight;  /* receivers (degrees), tobs, weight */
+} satt_request;
+
+typedef struct {
+  double *tt_rec;        /* nrec: travel times at the receivers (may be NULL)   */
+  double *kernel;        /* (nx,ny): masked adjoint field / svel**2 (may be NULL) */
+  double *T;             /* (nx,ny): travel-time field (may be NULL)           */
+  double chi;            /* 0.5*sum(w*(t-tobs)**2)                              */
+  int status;            /* the request's status, once done                     */
+  int iters[2];          /* forward, adjoint iterations (FP64; polish if mixed)  */
+  int iters_fp32[2];     /* FP32-phase iterations (mixed)                        */
+  double queue_ms;       /* submit to launch                                    */
+  double latency_ms;     /* submit to results in this struct                    */
+} satt_response;
+
+int satt_server_create(satt_ctx *ctx, const satt_server_config *cfg,
+                       satt_server **server);
+/* Queue one request; the response must stay valid until the request is done.
+ * Returns SATT_E_ARG for a malformed request or one larger than the pool. */
+int satt_server_submit(satt_server *server, const satt_request *req,
+                       satt_response *resp, long long *ticket);
+/* 1 if the request is done, 0 if not. */
+int satt_server_test(satt_server *server, long long ticket);
+/* Block until the request is done; returns its status. */
+int satt_server_wait(satt_server *server, long long ticket);
+/* Block until every submitted request is done. */
+int satt_server_drain(satt_server *server);
+typedef struct {
+  long long submitted, completed, launches;
+  long long max_in_flight;  /* most requests on the device at once */
+  double device_busy_ms;    /* sum of launch-group device times   */
+} satt_server_stats;
+int satt_server_get_stats(satt_server *server, satt_server_stats *stats);
+/* Drains, stops the scheduler and frees the pool. */
+void satt_server_destroy(satt_server *server);
+
 /* ------------------------------------------------------------ inspection
  * The per-solve setup fields as the device computes them, for bit-level
  * comparison against a CPU restatement that takes them as inputs (the

@@ -144,6 +144,8 @@ const char *status_text(int st) {
 
 // ------------------------------------------------------------------ context
 
+int satt::ctx_device(const satt_ctx *ctx) { return ctx->device; }
+
 extern "C" const char *satt_version(void) { return "satt 0.1 (gfx950)"; }
 
 extern "C" int satt_create(satt_ctx **out, int device) {
@@ -626,6 +628,8 @@ SlotDesc make_desc(const Part &Q, int k, double *base, int slot_skews, int mixed
   D.pskew = Q.pskew.as<double>() + (size_t)p * 2 * kPF * SE;
   D.pskew32 = mixed ? Q.pskew32.as<float>() + (size_t)p * 2 * kPF * SE : nullptr;
   D.fun = Q.pf.fun + (size_t)p * N;
+  D.vel = Q.pf.vel + (size_t)p * N;
+  D.kern = nullptr;
   D.skew = base;
   double *nat = base + (size_t)slot_skews * SE;
   D.T = nat + kNatT * N;

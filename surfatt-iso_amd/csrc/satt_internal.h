@@ -4,7 +4,12 @@
 
 #include <cstddef>
 
+struct satt_ctx;
+
 namespace satt {
+
+// The device of a context (satt_api.hip).
+int ctx_device(const satt_ctx *ctx);
 
 // Constants of the reference solvers, with their REAL(4) rounding kept:
 // src/eikon_solver_2d.f90:774-776,785 (forward) and :1293-1297 (adjoint).
@@ -53,6 +58,8 @@ struct SlotDesc {
   const double *pskew;    // the period's skewed fields, [2 skews][kPF][SE]
   const float *pskew32;   // the same in float (mixed mode), or null
   const double *fun;      // the period's 1/svel, natural layout
+  const double *vel;      // the period's svel, natural layout
+  double *kern;           // the item's own kernel Ta/svel**2 (serving), or null
   double *skew;           // scratch: slot_skews skewed fields
   double *T, *Ta, *Td, *delta, *aux;  // natural fields and forward tables
   int slot_skews, aux_len;

@@ -70,6 +70,8 @@ void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
 void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which);
 void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which);
 void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which);
+// Each slot's own kernel Ta/svel**2 into its `kern` field (when set).
+void launch_item_kernel(hipStream_t s, const SlotDesc *sd, const GroupDims &gd);
 // Per-period skewed fields of one grid (out32: the same in float, or null).
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
                         const PeriodFields &pf, double *out, float *out32);

@@ -1179,7 +1179,22 @@ __global__ __launch_bounds__(64) void k_adj_polish(const SlotDesc *sd, int which
   if (lane == 0) D.iters[1 + which] = 1;
 }
 
+// The item's own kernel, adj/svel**2 accumulated from zero
+// (src/tomo2d.f90:155), one thread per cell.
+__global__ __launch_bounds__(256) void k_item_kernel(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.y];
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= (size_t)D.g.nx * D.g.ny || !D.kern) return;
+  const double v = D.vel[k];
+  D.kern[k] = 0.0 + D.Ta[k] / (v * v);
+}
+
 }  // namespace
+
+void launch_item_kernel(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
+  hipLaunchKernelGGL(k_item_kernel, dim3((unsigned)((gd.max_n + 255) / 256), gd.count), dim3(256),
+                     0, s, sd);
+}
 
 void launch_period_skew(hipStream_t s, const Grid &g, const SkewGeom &sg, int np,
                         const PeriodFields &pf, double *out, float *out32) {

@@ -87,6 +87,7 @@ def load(path=LIB_PATH):
         fn.restype = ctypes.c_int
     L.satt_destroy.restype = None
     L.satt_plan_destroy.restype = None
+    L.satt_server_destroy.restype = None
     L.satt_last_error.restype = ctypes.c_char_p
     L.satt_version.restype = ctypes.c_char_p
     _lib = L
@@ -315,3 +316,97 @@ class Plan:
         s = PlanStats()
         self.ctx._check(self.lib.satt_plan_get_stats(self.ptr, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in PlanStats._fields_}
+
+
+# ------------------------------------------------------------------ serving
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("streams", ctypes.c_int), ("device_bytes", ctypes.c_longlong)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("xgrids", _dp), ("ygrids", _dp),
+                ("svel", _dp), ("m11", _dp), ("m12", _dp), ("m22", _dp),
+                ("srcx", ctypes.c_double), ("srcy", ctypes.c_double), ("nrec", ctypes.c_int),
+                ("recx", _dp), ("recy", _dp), ("tt", _dp), ("weight", _dp)]
+
+
+class Response(ctypes.Structure):
+    _fields_ = [("tt_rec", _dp), ("kernel", _dp), ("T", _dp), ("chi", ctypes.c_double),
+                ("status", ctypes.c_int), ("iters", ctypes.c_int * 2),
+                ("iters_fp32", ctypes.c_int * 2), ("queue_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+                ("launches", ctypes.c_longlong), ("max_in_flight", ctypes.c_longlong),
+                ("device_busy_ms", ctypes.c_double)]
+
+
+class Server:
+    """satt_server: a request loop of independent fwd+adj solves.  A request
+    is a dict with nx, ny, xgrids, ygrids, svel (nx, ny), optional m11, m12,
+    m22, srcx, srcy and receiver arrays recx, recy, tt, weight."""
+
+    def __init__(self, ctx, precision=PREC_FP64, max_batch=0, streams=0, device_bytes=0):
+        self.ctx, self.lib = ctx, ctx.lib
+        cfg = ServerConfig(int(precision), int(max_batch), int(streams), int(device_bytes))
+        p = ctypes.c_void_p()
+        ctx._check(self.lib.satt_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(p)))
+        self.ptr = p
+        self._live = {}
+
+    def submit(self, req, want_T=False):
+        nx, ny = int(req["nx"]), int(req["ny"])
+        keep = {k: (_f(req[k]) if k in ("svel", "m11", "m12", "m22")
+                    else np.ascontiguousarray(req[k], dtype=np.float64))
+                for k in ("xgrids", "ygrids", "svel", "m11", "m12", "m22", "recx", "recy", "tt",
+                          "weight") if req.get(k) is not None}
+        nr = len(keep["recx"])
+        q = Request()
+        q.nx, q.ny, q.srcx, q.srcy, q.nrec = nx, ny, float(req["srcx"]), float(req["srcy"]), nr
+        for k, v in keep.items():
+            setattr(q, k, _d(v))
+        out = {"tt_rec": np.zeros(nr), "kernel": np.zeros((nx, ny), order="F"),
+               "T": np.zeros((nx, ny), order="F") if want_T else None}
+        r = Response()
+        r.tt_rec, r.kernel = _d(out["tt_rec"]), _d(out["kernel"])
+        r.T = _d(out["T"]) if want_T else None
+        t = ctypes.c_longlong(0)
+        self.ctx._check(self.lib.satt_server_submit(self.ptr, ctypes.byref(q), ctypes.byref(r),
+                                                    ctypes.byref(t)))
+        self._live[t.value] = (r, out, keep)
+        return t.value
+
+    def test(self, ticket):
+        return self.lib.satt_server_test(self.ptr, ctypes.c_longlong(ticket)) == 1
+
+    def wait(self, ticket):
+        st = self.lib.satt_server_wait(self.ptr, ctypes.c_longlong(ticket))
+        r, out, _ = self._live.pop(ticket)
+        out.update(status=r.status, chi=r.chi, iters=tuple(r.iters),
+                   iters_fp32=tuple(r.iters_fp32), queue_ms=r.queue_ms, latency_ms=r.latency_ms)
+        if st < 0:
+            raise SattError(st, "request failed")
+        return out
+
+    def drain(self):
+        self.ctx._check(self.lib.satt_server_drain(self.ptr))
+
+    def stats(self):
+        s = ServerStats()
+        self.ctx._check(self.lib.satt_server_get_stats(self.ptr, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.ptr:
+            self.lib.satt_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -267,3 +267,110 @@ def topo_checkerboard(n=401, d=0.005, nsrc=200, nper=20, seed=3, pert=0.08,
     bt = make_batch(dim, xg, yg, periods, svel, rep(m11), rep(m12), rep(m22), paths)
     bi = with_model(bt, vref[:, None, None] * np.ones((nper, n, n)))
     return bt, bi
+
+
+def random_receiver_batch(xg, yg, svel, nitems, rng, nrec=(50, 200), period=10.0,
+                          m11=None, m12=None, m22=None):
+    """One grid and one period, `nitems` solves with a random source and
+    nrec[0]..nrec[1] random receivers each (no source-receiver table: every
+    receiver is its own path).  svel is (nx, ny).  Returns the batch with unit
+    observed times; set them with set_observed."""
+    nx, ny = len(xg), len(yg)
+    ones = np.ones((1, nx, ny)); zeros = np.zeros((1, nx, ny))
+    sx, sy = stations_uniform(rng, nitems, xg, yg)
+    counts = rng.integers(nrec[0], nrec[1] + 1, nitems)
+    rec_off = np.zeros(nitems + 1, dtype=np.int32)
+    rec_off[1:] = np.cumsum(counts)
+    nr = int(rec_off[-1])
+    rx, ry = stations_uniform(rng, nr, xg, yg)
+    f = lambda a, dflt: np.asfortranarray(dflt if a is None else a[None], dtype=np.float64)
+    return {
+        "dim": 2, "nx": nx, "ny": ny, "nperiod": 1, "periods": np.array([period]),
+        "xgrids": np.ascontiguousarray(xg, dtype=np.float64),
+        "ygrids": np.ascontiguousarray(yg, dtype=np.float64),
+        "svel": np.asfortranarray(svel[None], dtype=np.float64),
+        "m11": f(m11, ones), "m12": f(m12, zeros), "m22": f(m22, ones),
+        "item_pidx": np.zeros(nitems, dtype=np.int32),
+        "item_srcx": np.ascontiguousarray(sx), "item_srcy": np.ascontiguousarray(sy),
+        "rec_off": rec_off, "recx": np.ascontiguousarray(rx), "recy": np.ascontiguousarray(ry),
+        "tt": np.ones(nr), "weight": np.ones(nr),
+        "idx_path": np.arange(nr, dtype=np.int32), "npath": nr,
+    }
+
+
+UNEVEN_SIZES = ((101, 101), (201, 161), (321, 321), (81, 241))
+
+
+def uneven_mix(nsolve=2000, seed=2, d=0.01, sizes=UNEVEN_SIZES, pert=0.05):
+    """Uneven batching: nsolve fwd+adj solves over grids of the given sizes
+    (uniform choice), each grid with a smooth Gaussian random phase-velocity
+    field (correlation length 10 cells) scaled to 2.5-4.2 km/s as the current
+    model, the same field times (1 + a 3x3 checkerboard of +-pert) as the
+    true model, and 50-200 random receivers per solve.  Returns
+    [(batch_true, batch_current)] per size."""
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(sizes), nsolve)
+    out = []
+    for k, (nx, ny) in enumerate(sizes):
+        xg = 100.0 + d * np.arange(nx); yg = 20.0 + d * np.arange(ny)
+        v = smooth_random_field(nx, ny, 10.0, rng, 2.5, 4.2)
+        vt = v * (1 + checkerboard_2d(nx, ny, 3, 3, pert))
+        bt = random_receiver_batch(xg, yg, vt, int(np.sum(pick == k)), rng)
+        out.append((bt, with_model(bt, v[None])))
+    return out
+
+
+SERVE_SIZES = (101, 201, 401, 801)
+SERVE_PROBS = (0.4, 0.3, 0.2, 0.1)
+
+
+def serving_requests(n, seed=4, sizes=SERVE_SIZES, probs=SERVE_PROBS, d=0.005, nrec=100):
+    """The serving mix: n fwd+adj solve requests, grid size drawn from
+    `sizes` (square, d degrees) with `probs`, period uniform on 5-50 s
+    (setting the mean velocity 2.9 + 0.02*period km/s), a smooth random
+    velocity field (+-10%, correlation 10 cells) per request, a random source
+    and `nrec` random receivers whose observed times are the homogeneous
+    great-circle times with 3% noise.  Returns a list of request dicts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = int(rng.choice(sizes, p=probs))
+        xg = 100.0 + d * np.arange(m); yg = 20.0 + d * np.arange(m)
+        period = rng.uniform(5.0, 50.0)
+        v0 = 2.9 + 0.02 * period
+        v = smooth_random_field(m, m, 10.0, rng, 0.9 * v0, 1.1 * v0)
+        sx, sy = stations_uniform(rng, 1, xg, yg)
+        rx, ry = stations_uniform(rng, nrec, xg, yg)
+        lon0, lat0 = np.radians(sx[0]), np.radians(sy[0])
+        lon, lat = np.radians(rx), np.radians(ry)
+        cosd = np.sin(lat) * np.sin(lat0) + np.cos(lat) * np.cos(lat0) * np.cos(lon - lon0)
+        dist = RADIUS * np.arccos(np.clip(cosd, -1, 1))
+        tt = dist / v0 * (1 + 0.03 * rng.standard_normal(nrec))
+        out.append({"nx": m, "ny": m, "xgrids": xg, "ygrids": yg, "svel": v,
+                    "srcx": sx[0], "srcy": sy[0], "recx": rx, "recy": ry,
+                    "tt": np.maximum(tt, 0.1), "weight": np.ones(nrec)})
+    return out
+
+
+def request_batch(req):
+    """The one-item batch of a serving request (for the batched plan)."""
+    nx, ny = req["nx"], req["ny"]
+    nr = len(req["recx"])
+    ones = np.ones((1, nx, ny)); zeros = np.zeros((1, nx, ny))
+    g = lambda k, dflt: np.asfortranarray(dflt if req.get(k) is None else req[k][None])
+    return {
+        "dim": 2, "nx": nx, "ny": ny, "nperiod": 1, "periods": np.array([10.0]),
+        "xgrids": np.ascontiguousarray(req["xgrids"], dtype=np.float64),
+        "ygrids": np.ascontiguousarray(req["ygrids"], dtype=np.float64),
+        "svel": np.asfortranarray(req["svel"][None], dtype=np.float64),
+        "m11": g("m11", ones), "m12": g("m12", zeros), "m22": g("m22", ones),
+        "item_pidx": np.zeros(1, dtype=np.int32),
+        "item_srcx": np.array([req["srcx"]], dtype=np.float64),
+        "item_srcy": np.array([req["srcy"]], dtype=np.float64),
+        "rec_off": np.array([0, nr], dtype=np.int32),
+        "recx": np.ascontiguousarray(req["recx"], dtype=np.float64),
+        "recy": np.ascontiguousarray(req["recy"], dtype=np.float64),
+        "tt": np.ascontiguousarray(req["tt"], dtype=np.float64),
+        "weight": np.ascontiguousarray(req["weight"], dtype=np.float64),
+        "idx_path": np.arange(nr, dtype=np.int32), "npath": nr,
+    }
