@@ -30,12 +30,19 @@ import satt_workloads as W  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    # name: (builder, kwargs, description)
+    # name: (builder, kwargs, description); a builder returns (batch_true,
+    # batch_current) for one grid or a list of them for several
     "topo401": (W.topo_checkerboard, dict(n=401, d=0.005, nsrc=200, nper=20, seed=3),
                 "401x401 @0.005deg, 8-hill topographic metric, +-8% checkerboard, "
                 "200 sources x 20 periods (3-30 s), 199 receivers each"),
     "tomo81": (W.tomo2d_checkerboard, dict(n=81, d=0.025, nsta=25, seed=0),
                "81x81 @0.025deg, 2x2 checkerboard +-5%, 25 stations, 1 period"),
+    "acqui41": (W.acqui3d_checkerboard, dict(),
+                "3-D stage (density adjoint): 41x41 @0.05deg, Vs 3.0-4.5 km/s with a 2x3x2 "
+                "checkerboard +-6%, 16 periods 4-40 s, 100 stations -> 1600 solves"),
+    "uneven": (W.uneven_mix, dict(nsolve=2000, seed=2),
+               "2000 solves over 101x101, 201x161, 321x321, 81x241 @0.01deg in one plan, "
+               "random fields 2.5-4.2 km/s, 50-200 receivers each"),
 }
 
 
@@ -72,44 +79,54 @@ STAGE_OF = {"k_fwd_strip": "ms_fwd_fp64", "k_adj_strip": "ms_adj_fp64",
             "k_adj_mixed32": "ms_adj_fp32", "k_adj_polish": "ms_adj_fp64"}
 
 
-def cpu_baseline(batch, nsample, threads):
+def cpu_baseline(parts, nits, nsample, threads):
     """The reference's own solver file (oracle/_ref, built from
-    src/eikon_solver_2d.f90) driving the loop body over a sample of items on
-    `threads` host threads; interpolation, misfit and stacking by the
-    oracle.  Falls back to the oracle alone when the reference build is
-    absent."""
+    src/eikon_solver_2d.f90) driving the loop body over a sample of items
+    (spread over the parts in proportion) on `threads` host threads;
+    interpolation, misfit and stacking by the oracle.  Falls back to the
+    oracle alone when the reference build is absent."""
     import oracle as O
-    nit = len(batch["item_pidx"])
-    pick = np.linspace(0, nit - 1, nsample).round().astype(int)
-    use_ref = O.RefSolver.available()
-    if not use_ref:
-        sub = _subset(batch, pick)
+    nit = sum(nits)
+    picks = []
+    for q, n in enumerate(nits):
+        k = max(1, round(nsample * n / nit))
+        picks += [(q, int(i)) for i in np.linspace(0, n - 1, k).round().astype(int)]
+    if not O.RefSolver.available():
         t0 = time.perf_counter()
-        O.forward_simulate(sub, nthreads=threads)
+        for q in range(len(parts)):
+            sel = [i for qq, i in picks if qq == q]
+            O.forward_simulate(_subset(parts[q], np.array(sel)), nthreads=threads)
         dt = time.perf_counter() - t0
-        return dict(value=nsample / dt, unit="solves/s", cores=threads, kind="port",
-                    sample=f"{nsample} items of the workload, oracle loop, {dt:.1f} s wall")
+        return dict(value=len(picks) / dt, unit="solves/s", cores=threads, kind="port",
+                    sample=f"{len(picks)} items of the workload, oracle loop, {dt:.1f} s wall")
     ref = O.RefSolver()
-    b = batch
-    xg, yg = b["xgrids"], b["ygrids"]
-    blocks = np.array_split(pick, threads)
+    blocks = [picks[k::threads] for k in range(threads)]
     err = []
 
     def work(items):
         try:
             acc = {}
-            for it in items:
+            for q, it in items:
+                b = parts[q]
+                xg, yg = b["xgrids"], b["ygrids"]
                 p = b["item_pidx"][it]
                 r0, r1 = b["rec_off"][it], b["rec_off"][it + 1]
                 v = b["svel"][p]
                 a, c, bb = b["m11"][p], -b["m12"][p], b["m22"][p]
-                T = _ref_fwd(ref, xg, yg, a, bb, c, 1.0 / v, b["item_srcx"][it], b["item_srcy"][it])
-                tek = O.interp2(xg, yg, T, b["recx"][r0:r1], b["recy"][r0:r1])
+                x0, y0 = b["item_srcx"][it], b["item_srcy"][it]
+                rx, ry = b["recx"][r0:r1], b["recy"][r0:r1]
+                T = _ref_fwd(ref, xg, yg, a, bb, c, 1.0 / v, x0, y0)
+                tek = O.interp2(xg, yg, T, rx, ry)
                 tt, w = b["tt"][r0:r1], b["weight"][r0:r1].copy()
                 w[np.abs(tek - tt) / tt > 0.5] = 0
-                Ta = _ref_adj(ref, xg, yg, a, bb, c, T, b["recx"][r0:r1], b["recy"][r0:r1], w * (tek - tt))
-                Ta = ref.mask(xg, yg, Ta, b["item_srcx"][it], b["item_srcy"][it])
-                acc[p] = acc.get(p, 0) + Ta / (v * v)
+                Ta = ref.mask(xg, yg, _ref_adj(ref, xg, yg, a, bb, c, T, rx, ry, w * (tek - tt)),
+                              x0, y0)
+                if b["dim"] == 3:   # density adjoint, RHS -w (src/measadj.f90:92-104)
+                    Td = ref.mask(xg, yg, _ref_adj(ref, xg, yg, a, bb, c, T, rx, ry, -1.0 * w),
+                                  x0, y0)
+                    acc[(q, p)] = acc.get((q, p), 0) + Ta / (v * v * v) + Td / (v * v * v)
+                else:
+                    acc[(q, p)] = acc.get((q, p), 0) + Ta / (v * v)
         except BaseException as e:  # reported below
             err.append(e)
 
@@ -127,8 +144,8 @@ def cpu_baseline(batch, nsample, threads):
         threading.stack_size(old)
     if err:
         raise err[0]
-    return dict(value=nsample / dt, unit="solves/s", cores=threads, kind="reference",
-                sample=f"{nsample} of the workload's items (evenly spaced), reference "
+    return dict(value=len(picks) / dt, unit="solves/s", cores=threads, kind="reference",
+                sample=f"{len(picks)} of the workload's items (evenly spaced), reference "
                        f"src/eikon_solver_2d.f90 solvers built -O2, {threads} threads, "
                        f"{dt:.2f} s wall, {dt * threads:.1f} thread-s")
 
@@ -220,6 +237,7 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
         wall = time.perf_counter() - t0
         st = srv.stats()
     lat = np.array([o["latency_ms"] for o in outs])
+    queue = np.array([o["queue_ms"] for o in outs])
     sizes = np.array([r["nx"] for r in reqs])
     by_size = {int(m): {"n": int(np.sum(sizes == m)),
                         "p50_ms": round(float(np.percentile(lat[sizes == m], 50)), 2)}
@@ -240,6 +258,8 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
             "p50_ms": round(float(np.percentile(lat, 50)), 2),
             "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "mean_ms": round(float(lat.mean()), 2), "by_size": by_size,
+            "queue_p50_ms": round(float(np.percentile(queue, 50)), 2),
+            "queue_p99_ms": round(float(np.percentile(queue, 99)), 2),
             "solves_per_s": round(nreq / wall, 2), "wall_s": round(wall, 2),
             "launch_groups": int(st["launches"]), "max_in_flight": int(st["max_in_flight"]),
             "all_ok": bool(all(o["status"] == 0 for o in outs)),
@@ -287,19 +307,28 @@ def main():
 
     build, kw, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()
-    bt, bi = build(**kw)
-    nit, N = len(bi["item_pidx"]), bi["nx"] * bi["ny"]
-    log(f"[bench] workload {args.workload}: {nit} items, grid {bi['nx']}x{bi['ny']}, "
-        f"{bi['npath']} paths (built in {time.perf_counter() - t0:.1f}s)")
+    built = build(**kw)
+    pairs = built if isinstance(built, list) else [built]
+    nits = [len(bi["item_pidx"]) for _, bi in pairs]
+    nit = sum(nits)
+    grids = [[bi["nx"], bi["ny"]] for _, bi in pairs]
+    log(f"[bench] workload {args.workload}: {nit} items, grids {grids} "
+        f"(built in {time.perf_counter() - t0:.1f}s)")
     prec = satt.PREC_MIXED if args.precision == "mixed" else satt.PREC_FP64
+    mixed = prec == satt.PREC_MIXED
     ctx = satt.Context(0)
-    with ctx.plan(bt) as p:                     # observed times: FP64 forward, true model
+    # serving first, on an otherwise idle device
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve(ctx, prec, args.serve_requests, args.serve_qps)
+        log(f"[bench] serving: {serving}")
+    with ctx.plan_multi([bt for bt, _ in pairs]) as p:   # observed: FP64 forward, true model
         p.run(True, False)
-        obs = p.results()["tt_fwd"]
-    b = W.set_observed(bi, obs)
+        obs = [p.results(k)["tt_fwd"] for k in range(len(pairs))]
+    parts = [W.set_observed(bi, o) for (_, bi), o in zip(pairs, obs)]
     log("[bench] observed times computed")
 
-    with ctx.plan(b, prec) as plan:
+    with ctx.plan_multi(parts, prec) as plan:
         for _ in range(args.warmup):
             plan.run(True, True)
             plan.wait()
@@ -309,12 +338,14 @@ def main():
             plan.run(True, True)
             plan.wait()
         dt = time.perf_counter() - t0
-        out = plan.results()
+        outs = [plan.results(k) for k in range(len(parts))]
         st = plan.stats()
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
     value = args.steps * nit / dt
-    mixed = prec == satt.PREC_MIXED
-    abytes = alg_bytes(N, out, nit, mixed)
+    abytes = {}
+    for b, o, n in zip(parts, outs, nits):
+        for k, v in alg_bytes(b["nx"] * b["ny"], o, n, mixed).items():
+            abytes[k] = abytes.get(k, 0.0) + v
     launches = max(1, st["launches"])
     kname = max(abytes, key=lambda k: st[STAGE_OF[k]])      # the dominant kernel
     kbytes, kms = abytes[kname], st[STAGE_OF[kname]]
@@ -335,61 +366,70 @@ def main():
     # results against it), and a sample of items against the FP64 oracle
     fp64_value, mixed_err = None, None
     if mixed and not args.no_fp64_compare:
-        with ctx.plan(b, satt.PREC_FP64) as p64:
+        with ctx.plan_multi(parts, satt.PREC_FP64) as p64:
             p64.run(True, True)
             p64.wait()
             t1 = time.perf_counter()
             p64.run(True, True)
             p64.wait()
             fp64_value = nit / (time.perf_counter() - t1)
-            o64 = p64.results()
+            o64 = [p64.results(k) for k in range(len(parts))]
         log(f"[bench] batched FP64: {fp64_value:.1f} solves/s")
-        d = np.abs(out["tt_fwd"] - o64["tt_fwd"])
+        cat = lambda key, ol: np.concatenate([o[key].ravel() for o in ol])
+        d = np.abs(cat("tt_fwd", outs) - cat("tt_fwd", o64))
         mixed_err = {
             "vs": "batched FP64 on the device (parity-checked against the oracle)",
             "tt_max_ms": float(d.max() * 1e3), "tt_mean_ms": float(d.mean() * 1e3),
-            "kernel_rel_l2": float(np.linalg.norm((out["adj_s"] - o64["adj_s"]).ravel())
-                                   / np.linalg.norm(o64["adj_s"].ravel())),
-            "chi_rel": float(abs(out["chi"] - o64["chi"]) / abs(o64["chi"])),
+            "kernel_rel_l2": float(np.linalg.norm(cat("adj_s", outs) - cat("adj_s", o64))
+                                   / np.linalg.norm(cat("adj_s", o64))),
+            "chi_rel": float(abs(sum(o["chi"] for o in outs) - sum(o["chi"] for o in o64))
+                             / abs(sum(o["chi"] for o in o64))),
         }
+        if parts[0]["dim"] == 3:
+            mixed_err["density_rel_l2"] = float(
+                np.linalg.norm(cat("adj_density", outs) - cat("adj_density", o64))
+                / np.linalg.norm(cat("adj_density", o64)))
         if args.oracle_items > 0:
             import oracle as O
-            pick = np.linspace(0, nit - 1, args.oracle_items).round().astype(int)
-            sub = _subset(b, pick)
-            ref = O.forward_simulate(sub, istotable=True, isadj=False, nthreads=len(pick))
-            paths = sub["idx_path"]
-            mixed_err["oracle_sample"] = {
-                "items": int(len(pick)),
-                "tt_max_ms": float(np.abs(out["tt_fwd"][paths] - ref["tt_fwd"][paths]).max() * 1e3),
-                "fp64_tt_max_rel": float(np.abs(o64["tt_fwd"][paths] - ref["tt_fwd"][paths]).max()
-                                         / np.abs(ref["tt_fwd"][paths]).max())}
+            worst, worst64 = 0.0, 0.0
+            for b, o, o6, n in zip(parts, outs, o64, nits):
+                k = max(1, args.oracle_items // len(parts))
+                sub = _subset(b, np.linspace(0, n - 1, k).round().astype(int))
+                ref = O.forward_simulate(sub, istotable=True, isadj=False, nthreads=k)
+                ps = sub["idx_path"]
+                worst = max(worst, float(np.abs(o["tt_fwd"][ps] - ref["tt_fwd"][ps]).max()))
+                worst64 = max(worst64, float(np.abs(o6["tt_fwd"][ps] - ref["tt_fwd"][ps]).max()
+                                             / np.abs(ref["tt_fwd"][ps]).max()))
+            mixed_err["oracle_sample"] = {"items": max(1, args.oracle_items // len(parts)) * len(parts),
+                                          "tt_max_ms": worst * 1e3, "fp64_tt_max_rel": worst64}
         log(f"[bench] mixed error: {mixed_err}")
 
-    # plain FP64 form: one solve per launch, resident inputs
+    # plain FP64 form: one solve per launch, resident inputs, a sample of
+    # items spread over the parts in proportion
     plain = None
     if args.plain_items > 0:
-        sub = _subset(b, np.linspace(0, nit - 1, args.plain_items).round().astype(int))
-        with ctx.plan(sub, satt.PREC_FP64) as pp:
-            pp.run(True, True, satt.LAUNCH_PER_SOLVE)
-            pp.wait()
-            t1 = time.perf_counter()
-            pp.run(True, True, satt.LAUNCH_PER_SOLVE)
-            pp.wait()
-            pdt = time.perf_counter() - t1
-        plain = args.plain_items / pdt
+        tot, pdt = 0, 0.0
+        for b, n in zip(parts, nits):
+            k = max(1, round(args.plain_items * n / nit))
+            sub = _subset(b, np.linspace(0, n - 1, k).round().astype(int))
+            with ctx.plan(sub, satt.PREC_FP64) as pp:
+                pp.run(True, True, satt.LAUNCH_PER_SOLVE)
+                pp.wait()
+                t1 = time.perf_counter()
+                pp.run(True, True, satt.LAUNCH_PER_SOLVE)
+                pp.wait()
+                pdt += time.perf_counter() - t1
+            tot += k
+        plain = tot / pdt
         log(f"[bench] plain per-solve FP64: {plain:.1f} solves/s")
-
-    serving = None
-    if args.serve_requests > 0:
-        serving = serve(ctx, prec, args.serve_requests, args.serve_qps)
-        log(f"[bench] serving: {serving}")
 
     cpu = None
     if not args.no_cpu:
         th = host_threads()
-        cpu = cpu_baseline(b, max(args.cpu_sample, th), th)
+        cpu = cpu_baseline(parts, nits, max(args.cpu_sample, th), th)
         log(f"[bench] cpu baseline: {cpu['value']:.2f} solves/s on {th} threads")
 
+    out = outs[0]
     iters = out["iters"]
     line = {
         "metric": "fwd+adj solves/s at 401^2 grid, batched mixed vs plain FP64 one-solve-per-launch",
@@ -405,12 +445,12 @@ def main():
         "dtype": "f64" if prec == satt.PREC_FP64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "items": nit,
-                   "grid": [bi["nx"], bi["ny"]], "precision": args.precision,
+                   "grid": grids[0] if len(grids) == 1 else grids, "precision": args.precision,
                    "launch": "batched", "chunk": st["chunk"],
-                   "mean_iters_fwd": float(iters[:, 0].mean()),
-                   "mean_iters_adj": float(iters[:, 1].mean()),
-                   "mean_iters_fwd_fp32": float(out["iters_fp32"][:, 0].mean()),
-                   "mean_iters_adj_fp32": float(out["iters_fp32"][:, 1].mean())},
+                   "mean_iters_fwd": float(np.concatenate([o["iters"][:, 0] for o in outs]).mean()),
+                   "mean_iters_adj": float(np.concatenate([o["iters"][:, 1] for o in outs]).mean()),
+                   "mean_iters_fwd_fp32": float(np.concatenate([o["iters_fp32"][:, 0] for o in outs]).mean()),
+                   "mean_iters_adj_fp32": float(np.concatenate([o["iters_fp32"][:, 1] for o in outs]).mean())},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -112,8 +112,15 @@ struct Req {
   Clock::time_point t_submit, t_launch;
   Layout L{};
   size_t dev_off = 0, pin_in = 0, pin_out = 0, out_bytes = 0;
+  int cls = 0;  // 1: a large grid (see kLargeSteps)
   bool done = false;
 };
+
+// Requests whose sweep takes at least this many wave steps (strips x rows:
+// 401x401 takes 3248) run in launch groups of their own, on their own
+// streams: a group ends with its slowest item, and the device has few
+// hardware queues, so small requests must not wait behind large ones.
+constexpr long kLargeSteps = 2000;
 
 struct Batch {
   int stream = 0;
@@ -336,17 +343,20 @@ void satt_server::finish(Batch &b) {
 
 void satt_server::run() {
   (void)hipSetDevice(device);
-  std::deque<Req *> waiting;
+  std::deque<Req *> waiting[2];  // small, large
   std::vector<Batch> flying;
+  // streams [0, nlarge) take large requests, the rest small ones
+  const int nlarge = nstreams >= 2 ? nstreams / 2 : 0;
   for (;;) {
     bool progressed = false;
     {
       std::unique_lock<std::mutex> lk(mu);
       while (!incoming.empty()) {
-        waiting.push_back(incoming.front());
+        Req *r = incoming.front();
+        waiting[nlarge > 0 ? r->cls : 0].push_back(r);
         incoming.pop_front();
       }
-      if (stop && waiting.empty() && flying.empty()) break;
+      if (stop && waiting[0].empty() && waiting[1].empty() && flying.empty()) break;
     }
     // completions
     for (size_t k = 0; k < flying.size();) {
@@ -360,25 +370,30 @@ void satt_server::run() {
       flying.erase(flying.begin() + k);
       progressed = true;
     }
-    // admissions: arrival order, while the pools and a stream have room
-    while (!waiting.empty()) {
+    // admissions: arrival order within a class, while the pools and a stream
+    // of the class have room
+    for (int c = 0; c < 2; ++c)
+    while (!waiting[c].empty()) {
+      std::deque<Req *> &wq = waiting[c];
+      const int s0 = nlarge == 0 ? 0 : (c == 1 ? 0 : nlarge);
+      const int s1 = nlarge == 0 ? nstreams : (c == 1 ? nlarge : nstreams);
       int si = -1;
-      for (int k = 0; k < nstreams; ++k)
+      for (int k = s0; k < s1; ++k)
         if (!stream_busy[k]) { si = k; break; }
       if (si < 0) break;
       Batch b;
       b.stream = si;
       {
         std::lock_guard<std::mutex> lk(mu);
-        const size_t most = std::min<size_t>(waiting.size(), (size_t)max_batch);
+        const size_t most = std::min<size_t>(wq.size(), (size_t)max_batch);
         b.desc_bytes = most * sizeof(SlotDesc);
         if (!dev_pool.alloc(b.desc_bytes, &b.desc_dev)) break;
         if (!pin_pool.alloc(b.desc_bytes, &b.desc_pin)) {
           dev_pool.release(b.desc_dev, b.desc_bytes);
           break;
         }
-        while (!waiting.empty() && b.reqs.size() < most) {
-          Req *r = waiting.front();
+        while (!wq.empty() && b.reqs.size() < most) {
+          Req *r = wq.front();
           size_t d, pi, po;
           if (!dev_pool.alloc(r->L.total * 8, &d)) break;
           if (!pin_pool.alloc(r->L.in_end * 8, &pi)) {
@@ -395,7 +410,7 @@ void satt_server::run() {
           r->pin_out = po;
           r->t_launch = Clock::now();
           b.reqs.push_back(r);
-          waiting.pop_front();
+          wq.pop_front();
         }
         if (b.reqs.empty()) {
           dev_pool.release(b.desc_dev, b.desc_bytes);
@@ -450,8 +465,9 @@ extern "C" int satt_server_create(satt_ctx *ctx, const satt_server_config *cfg,
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return SATT_E_HIP;
   S->dev_cap = c.device_bytes > 0 ? (size_t)c.device_bytes : freeb / 2;
   S->dev_cap &= ~(size_t)255;
-  // inputs and outputs of requests in flight: a fixed share of the pool
-  S->pin_cap = std::max<size_t>(S->dev_cap / 8, (size_t)64 << 20) & ~(size_t)255;
+  // inputs and outputs of requests in flight: 1/8 of the pool, 64 MiB to 4 GiB
+  S->pin_cap = std::min(std::max<size_t>(S->dev_cap / 8, (size_t)64 << 20), (size_t)4 << 30) &
+               ~(size_t)255;
   if (hipMalloc(&S->dev, S->dev_cap) != hipSuccess) return SATT_E_NOMEM;
   if (hipHostMalloc(&S->pin, S->pin_cap, hipHostMallocDefault) != hipSuccess) {
     (void)hipFree(S->dev);
@@ -506,6 +522,7 @@ extern "C" int satt_server_submit(satt_server *S, const satt_request *q, satt_re
     for (int k = 0; k < nr; ++k) idx[k] = k;
   }
   r->L = layout(nx, ny, nr, S->slot_skews);
+  r->cls = (long)make_skew_geom(nx, ny).ns * (ny + 63) >= kLargeSteps ? 1 : 0;
   r->out_bytes = ((size_t)nr + 32 + 2 * N) * 8;
   resp->status = 0;
   std::lock_guard<std::mutex> lk(S->mu);

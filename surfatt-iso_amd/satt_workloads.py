@@ -374,3 +374,51 @@ def request_batch(req):
         "weight": np.ascontiguousarray(req["weight"], dtype=np.float64),
         "idx_path": np.arange(nr, dtype=np.int32), "npath": nr,
     }
+
+
+def phase_velocity_maps(vs3d, depths, periods):
+    """Synthetic phase-velocity maps from a Vs model, per column: 0.92 times
+    the Vs average weighted by exp(-z / (c*T/3)) (a Rayleigh-like depth
+    sensitivity, iterated for c), shape (nperiod, nx, ny).  A stand-in for the
+    reference's surface-wave dispersion code (src/surfker), which is outside
+    the path this build accelerates."""
+    nz = len(depths)
+    dz = np.gradient(depths)
+    out = np.empty((len(periods),) + vs3d.shape[:2])
+    for p, T in enumerate(periods):
+        c = 0.92 * vs3d.mean(axis=2)
+        for _ in range(4):
+            w = np.exp(-depths[None, None, :] / np.maximum(c[..., None] * T / 3.0, 1e-3))
+            w = w * dz[None, None, :]
+            c = 0.92 * (vs3d * w).sum(axis=2) / w.sum(axis=2)
+        out[p] = c
+    return out
+
+
+def acqui3d_checkerboard(n=41, d=0.05, nz=31, zmax=60.0, nsta=100, nper=16, seed=1,
+                         pert=0.06, ncb=(2, 3, 2)):
+    """The 3-D inversion's 2-D stage (att_acqui%forward_simulate, dim 3):
+    n x n grid at d degrees, Vs on nz depths 0..zmax km rising linearly from
+    3.0 to 4.5 km/s, times (1 + the reference's 3-D checkerboard of +-pert
+    with ncb = (x, y, z) half-periods, src/model.f90:276-300); nper periods
+    log-spaced 4-40 s with phase-velocity maps from phase_velocity_maps; nsta
+    random stations (seed), each a source with the others as receivers
+    (nsta*nper items).  The current model is the unperturbed 1-D Vs.
+    Returns (batch_true, batch_current)."""
+    xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
+    depths = np.linspace(0.0, zmax, nz)
+    vs1 = 3.0 + 1.5 * depths / zmax
+    xp = np.sin(ncb[0] * PI_F * np.arange(1, n + 1) / n)
+    yp = np.sin(ncb[1] * PI_F * np.arange(1, n + 1) / n)
+    zp = np.sin(ncb[2] * PI_F * np.arange(1, nz + 1) / nz)
+    vs_true = vs1[None, None, :] * (1 + pert * xp[:, None, None] * yp[None, :, None] * zp[None, None, :])
+    vs_cur = np.broadcast_to(vs1[None, None, :], (n, n, nz))
+    periods = np.geomspace(4.0, 40.0, nper)
+    ct = phase_velocity_maps(vs_true, depths, periods)
+    cc = phase_velocity_maps(np.ascontiguousarray(vs_cur), depths, periods)
+    rng = np.random.default_rng(seed)
+    sx, sy = stations_uniform(rng, nsta, xg, yg)
+    paths = all_pairs_paths(sx, sy, periods)
+    ones = np.ones((nper, n, n)); zeros = np.zeros((nper, n, n))
+    bt = make_batch(3, xg, yg, periods, ct, ones, zeros, ones, paths)
+    return bt, with_model(bt, cc)
