@@ -365,6 +365,40 @@ __device__ __forceinline__ size_t skpos(int sk, int i, int j, int R) {
   return skidx(c, r, l, R);
 }
 
+// Copies the grid cells of a field from skew FROM into the other skew, one
+// whole destination row per step (coalesced stores).  The loads of one row
+// gather from 64 source rows; each source line is read again by the next
+// lanes on the following rows, so it stays in cache while it is used.
+// Padding elements of the destination are left as they are.
+template <typename V, int FROM>
+__device__ void skew_copy(const V *__restrict__ src, V *__restrict__ dst, int nx, int ny, int R,
+                          int ns, int lane) {
+  constexpr int TO = 1 - FROM;
+  constexpr int kU = 8;
+  for (int c = 0; c < ns; ++c) {
+    const bool col = 64 * c + lane < nx;
+    // destination row r holds the lane's cell j = r - jof, which sits in
+    // source row j + sof
+    const int jof = TO == 0 ? lane : 63 - lane;
+    const int sof = FROM == 0 ? lane : 63 - lane;
+    const size_t base = (size_t)c * R * 64 + lane;
+    for (int r0 = 0; r0 < R; r0 += kU) {
+      V v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int j = min(max(r0 + u - jof, 0), ny - 1);
+        v[u] = src[base + (size_t)(j + sof) * 64];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int r = r0 + u, j = r - jof;
+        if (col && r < R && j >= 0 && j < ny) dst[base + (size_t)r * 64] = v[u];
+      }
+    }
+  }
+  wave_fence();
+}
+
 // Skew row of step t (uniform across lanes), see the layout note above.
 template <int XD, int YD>
 __device__ __forceinline__ int skew_row(int t, int ny, int L) {
@@ -406,16 +440,12 @@ struct FwdCtx {
 };
 
 
-// One sweep (XD, YD) over strip c, reading tau from `tin` (the sweep's own
-// skew) and writing it to `tout`: in place, or (CROSS) into the other skew,
-// which is where the next sweep family reads it.  MODE 0: plain; 1: record
-// the snapshot (first sweep of an iteration); 2: accumulate |tau-snap|*T0 in
-// double (last sweep).
-template <typename V, int XD, int YD, int MODE, bool CROSS>
-__device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, const V *tin, V *tout, double &acc_s,
-                               double &acc_m) {
+// One sweep (XD, YD) over strip c, updating tau in place in the sweep's own
+// skew.  MODE 0: plain; 1: record the snapshot (first sweep of an
+// iteration); 2: accumulate |tau-snap|*T0 in double (last sweep).
+template <typename V, int XD, int YD, int MODE>
+__device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, V *tau, double &acc_s, double &acc_m) {
   constexpr int SK = XD == YD ? 0 : 1;
-  constexpr int SO = CROSS ? 1 - SK : SK;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const size_t SE = C.SE;
   const V *ps = C.ps[SK], *is = C.is[SK];
@@ -425,8 +455,8 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, const V *tin, V *tout,
   const int lag = XD > 0 ? lane : L - 1 - lane;
   const int nsteps = ny + L - 1;
   // strip edges: the upstream neighbour column is final (its strip ran
-  // before this one, writing `tout`), the downstream one still holds the
-  // previous sweep's values (`tin`)
+  // before this one), the downstream one still holds the previous sweep's
+  // values
   const bool up_edge = XD > 0 ? lane == 0 : lane == L - 1;
   const bool dn_edge = XD > 0 ? lane == L - 1 : lane == 0;
   const int iu = i - XD, id = i + XD;
@@ -444,13 +474,13 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, const V *tin, V *tout,
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
     const size_t e = skidx(c, r, lane, R);
     q.t0 = is[e];
-    q.tau = tin[e];
+    q.tau = tau[e];
     if (up_ok) {
-      q.tup = tout[skpos(SO, iu, jc, R)];
+      q.tup = tau[skpos(SK, iu, jc, R)];
       q.zup = is[skpos(SK, iu, jc, R)];
     }
     if (kDown && dn_ok) {
-      q.tdn = tin[skpos(SK, id, jc, R)];
+      q.tdn = tau[skpos(SK, id, jc, R)];
       q.zdn = is[skpos(SK, id, jc, R)];
     }
     if (MODE == 2) q.snap = C.snap[e];
@@ -501,7 +531,7 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, const V *tin, V *tout,
                                       zUx, zprev);
       }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      tout[CROSS ? skpos(SO, i, jc, R) : e] = tnew;
+      tau[e] = tnew;
       if (MODE == 1) C.snap[e] = fc.tau;
       if (MODE == 2) {
         const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
@@ -518,24 +548,26 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, const V *tin, V *tout,
   wave_fence();
 }
 
-template <typename V, int XD, int YD, int MODE, bool CROSS>
-__device__ void fwd_sweep(const FwdCtx<V> &C, int ns, const V *tin, V *tout, double &acc_s,
-                          double &acc_m) {
+template <typename V, int XD, int YD, int MODE>
+__device__ void fwd_sweep(const FwdCtx<V> &C, int ns, V *tau, double &acc_s, double &acc_m) {
   for (int cq = 0; cq < ns; ++cq)
-    fwd_strip_pass<V, XD, YD, MODE, CROSS>(C, XD > 0 ? cq : ns - 1 - cq, tin, tout, acc_s,
-                                           acc_m);
+    fwd_strip_pass<V, XD, YD, MODE>(C, XD > 0 ? cq : ns - 1 - cq, tau, acc_s, acc_m);
 }
 
 // One iteration of four sweeps, (x+,y+), (x+,y-), (x-,y+), (x-,y-)
-// (src/eikon_solver_2d.f90:884-898), starting and ending in the plus skew;
-// returns whether the stop rule (:1091-1115) holds.
+// (src/eikon_solver_2d.f90:884-898), starting and ending in the plus skew:
+// the first sweep in the plus skew, the middle two in the minus skew, the
+// last in the plus skew, with the field copied between the skews at the
+// two family changes.  Returns whether the stop rule (:1091-1115) holds.
 template <typename V>
 __device__ bool fwd_iteration(const FwdCtx<V> &C, int ns, V *tP, V *tM) {
   double acc_s = 0, acc_m = 0;
-  fwd_sweep<V, 1, 1, 1, true>(C, ns, tP, tM, acc_s, acc_m);
-  fwd_sweep<V, 1, -1, 0, false>(C, ns, tM, tM, acc_s, acc_m);
-  fwd_sweep<V, -1, 1, 0, true>(C, ns, tM, tP, acc_s, acc_m);
-  fwd_sweep<V, -1, -1, 2, false>(C, ns, tP, tP, acc_s, acc_m);
+  fwd_sweep<V, 1, 1, 1>(C, ns, tP, acc_s, acc_m);
+  skew_copy<V, 0>(tP, tM, C.nx, C.ny, C.R, ns, C.lane);
+  fwd_sweep<V, 1, -1, 0>(C, ns, tM, acc_s, acc_m);
+  fwd_sweep<V, -1, 1, 0>(C, ns, tM, acc_s, acc_m);
+  skew_copy<V, 1>(tM, tP, C.nx, C.ny, C.R, ns, C.lane);
+  fwd_sweep<V, -1, -1, 2>(C, ns, tP, acc_s, acc_m);
   const double s = wave_sum(acc_s), m = wave_max(acc_m);
   const double l1 = s / (double)C.nx / (double)C.ny;
   return fabs(l1) < kFwdTol && m < kFwdTol;
@@ -965,12 +997,13 @@ struct AdjCtx {
   V *snap;         // plus skew
 };
 
-// MODE 2 also accumulates max |Ta| (acc_a) for the FP32 stop rule.
-template <typename V, int XD, int YD, int MODE, bool CROSS>
-__device__ void adj_strip_pass(const AdjCtx<V> &C, int c, const V *tin, V *tout, double &acc_s,
-                               double &acc_m, double &acc_a) {
+// One adjoint sweep over strip c, in place in the sweep's own skew; MODE as
+// for the forward, and MODE 2 also accumulates max |Ta| (acc_a) for the
+// FP32 stop rule.
+template <typename V, int XD, int YD, int MODE>
+__device__ void adj_strip_pass(const AdjCtx<V> &C, int c, V *ta, double &acc_s, double &acc_m,
+                               double &acc_a) {
   constexpr int SK = XD == YD ? 0 : 1;
-  constexpr int SO = CROSS ? 1 - SK : SK;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const size_t SE = C.SE;
   const V *is = C.is[SK];
@@ -994,9 +1027,9 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, const V *tin, V *tout,
     const int jc = min(max(j, 0), ny - 1);
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
     const size_t e = skidx(c, r, lane, R);
-    q.ta = tin[e];
-    if (up_ok) q.tup = tout[skpos(SO, iu, jc, R)];
-    if (dn_ok) q.tdn = tin[skpos(SK, id, jc, R)];
+    q.ta = ta[e];
+    if (up_ok) q.tup = ta[skpos(SK, iu, jc, R)];
+    if (dn_ok) q.tdn = ta[skpos(SK, id, jc, R)];
     if (MODE == 2) q.snap = C.snap[e];
   };
   auto load_near = [&](int t, ANear<V> &q) {
@@ -1024,7 +1057,7 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, const V *tin, V *tout,
       if (interior_col && jc >= 1 && jc <= ny - 2)
         tnew = adj_update(nc, tW, tE, tS, tN, C.dx, C.dy);
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      tout[CROSS ? skpos(SO, i, jc, R) : e] = tnew;
+      ta[e] = tnew;
       if (MODE == 1) C.snap[e] = fc.ta;
       if (MODE == 2) {
         const double dd = fabs((double)tnew - (double)fc.snap);
@@ -1044,12 +1077,11 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, const V *tin, V *tout,
   wave_fence();
 }
 
-template <typename V, int XD, int YD, int MODE, bool CROSS>
-__device__ void adj_sweep(const AdjCtx<V> &C, int ns, const V *tin, V *tout, double &acc_s,
-                          double &acc_m, double &acc_a) {
+template <typename V, int XD, int YD, int MODE>
+__device__ void adj_sweep(const AdjCtx<V> &C, int ns, V *ta, double &acc_s, double &acc_m,
+                          double &acc_a) {
   for (int cq = 0; cq < ns; ++cq)
-    adj_strip_pass<V, XD, YD, MODE, CROSS>(C, XD > 0 ? cq : ns - 1 - cq, tin, tout, acc_s,
-                                           acc_m, acc_a);
+    adj_strip_pass<V, XD, YD, MODE>(C, XD > 0 ? cq : ns - 1 - cq, ta, acc_s, acc_m, acc_a);
 }
 
 // One iteration: (xdirec, ydirec) = (-1,-1), (-1,1), (1,-1), (1,1), x
@@ -1059,10 +1091,12 @@ template <typename V, bool SNAP>
 __device__ void adj_iteration(const AdjCtx<V> &C, int ns, V *tP, V *tM, double &s, double &m,
                               double &a) {
   double acc_s = 0, acc_m = 0, acc_a = 0;
-  adj_sweep<V, 1, 1, SNAP ? 1 : 0, true>(C, ns, tP, tM, acc_s, acc_m, acc_a);
-  adj_sweep<V, 1, -1, 0, false>(C, ns, tM, tM, acc_s, acc_m, acc_a);
-  adj_sweep<V, -1, 1, 0, true>(C, ns, tM, tP, acc_s, acc_m, acc_a);
-  adj_sweep<V, -1, -1, SNAP ? 2 : 0, false>(C, ns, tP, tP, acc_s, acc_m, acc_a);
+  adj_sweep<V, 1, 1, SNAP ? 1 : 0>(C, ns, tP, acc_s, acc_m, acc_a);
+  skew_copy<V, 0>(tP, tM, C.nx, C.ny, C.R, ns, C.lane);
+  adj_sweep<V, 1, -1, 0>(C, ns, tM, acc_s, acc_m, acc_a);
+  adj_sweep<V, -1, 1, 0>(C, ns, tM, acc_s, acc_m, acc_a);
+  skew_copy<V, 1>(tM, tP, C.nx, C.ny, C.R, ns, C.lane);
+  adj_sweep<V, -1, -1, SNAP ? 2 : 0>(C, ns, tP, acc_s, acc_m, acc_a);
   s = wave_sum(acc_s);
   m = wave_max(acc_m);
   a = wave_max(acc_a);
