@@ -222,6 +222,12 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
     reqs = W.serving_requests(nreq, seed=seed)
     rng = np.random.default_rng(seed + 100)
     gaps = rng.exponential(1.0 / qps, nreq)
+    # team size per grid: the autotuner's choice for one request of each size
+    tuned = {}
+    for m in sorted(set(int(r["nx"]) for r in reqs)):
+        t = ctx.tune_waves(m, m, 1, prec)
+        tuned[m] = {"waves": t["waves"], "alone_ms_by_waves": [round(v, 2) for v in t["ms"]]}
+    log(f"[bench] tuned team sizes (one request): {tuned}")
     with satt.Server(ctx, prec) as srv:
         for r in reqs[:4]:                      # warm the kernels and pools
             srv.wait(srv.submit(r))
@@ -262,6 +268,7 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
             "queue_p99_ms": round(float(np.percentile(queue, 99)), 2),
             "solves_per_s": round(nreq / wall, 2), "wall_s": round(wall, 2),
             "launch_groups": int(st["launches"]), "max_in_flight": int(st["max_in_flight"]),
+            "team_waves": tuned,
             "all_ok": bool(all(o["status"] == 0 for o in outs)),
             "plain_fp64_per_solve_solves_per_s": None if plain is None else round(plain, 3)}
 
@@ -279,6 +286,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the HIP-graph replay timing of the same step")
     ap.add_argument("--workload", default="topo401", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--oracle-items", type=int, default=4,
@@ -340,6 +349,24 @@ def main():
         dt = time.perf_counter() - t0
         outs = [plan.results(k) for k in range(len(parts))]
         st = plan.stats()
+        # replay form: the same step captured once into a HIP graph
+        graph = None
+        if not args.no_graph:
+            plan.set_graph(True)
+            plan.run(True, True)
+            plan.wait()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                plan.run(True, True)
+                plan.wait()
+            gdt = time.perf_counter() - t1
+            same = all(np.array_equal(plan.results(k)[key], outs[k][key])
+                       for k in range(len(parts)) for key in ("tt_fwd", "adj_s"))
+            plan.set_graph(False)
+            graph = {"ms_per_step": round(gdt / args.steps * 1e3, 3),
+                     "solves_per_s": round(args.steps * sum(len(b["item_pidx"]) for b in parts) / gdt, 3),
+                     "results_same_as_eager": bool(same)}
+            log(f"[bench] graph replay: {graph}")
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
     value = args.steps * nit / dt
     abytes = {}
@@ -457,6 +484,7 @@ def main():
                      "alg_bytes_per_launch": kbytes / launches,
                      "avg_launch_ms": kms / launches},
         "kernels": kernels,
+        "graph_replay": graph,
         "batched_fp64": None if fp64_value is None else round(fp64_value, 3),
         "mixed_error": mixed_err,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),

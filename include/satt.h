@@ -49,6 +49,9 @@ extern "C" {
 #define SATT_E_NOMEM (-4)         /* device or host allocation failed        */
 #define SATT_E_HIP (-5)           /* HIP runtime error                       */
 #define SATT_E_NODEV (-6)         /* no usable GPU                           */
+#define SATT_E_TEAM_WAIT (-7)     /* a multi-wave solve's wait for another wave
+                                     ran past its bound; the item's fields are
+                                     not valid                               */
 
 /* precision modes for the batched path */
 #define SATT_PREC_FP64 0  /* the reference's FP64 arithmetic, sweep for sweep */
@@ -165,6 +168,38 @@ int satt_plan_set_model(satt_plan *plan, const double *svel, const double *m11,
 int satt_plan_set_model_part(satt_plan *plan, int part, const double *svel,
                              const double *m11, const double *m12, const double *m22);
 int satt_plan_run(satt_plan *plan, int istotable, int isadj, int launch_form);
+/* Replay form of the batched run: with `on` set, the first batched run of
+ * each (istotable, isadj) pair captures the whole launch chain -- for every
+ * launch group the forward, the receivers (misfit and adjoint sources), the
+ * adjoint(s) and the kernel stacking -- into one HIP graph, and later runs
+ * with that pair replay it as one launch.  The graph holds the plan's device
+ * buffers, which satt_plan_set_model refills in place, so model updates need
+ * no new capture.  Stage times are not split in this form (ms_total only).
+ * No counterpart in the reference. */
+int satt_plan_set_graph(satt_plan *plan, int on);
+/* Waves per item of the batched sweep kernels (default 1).  With W > 1 an
+ * item's strips are shared by W waves of one workgroup, each strip of a
+ * sweep following the previous one at least a strip width behind, which
+ * cuts an item's latency on wide grids; one wave per item is the throughput
+ * form when the items fill the device.  W is capped per launch group (at
+ * most 4, at most the strip count, and the LDS of the edge channels).  The
+ * results do not depend on W.  No counterpart in the reference. */
+int satt_plan_set_waves(satt_plan *plan, int waves);
+/* waves = 0: per launch group, the team size the autotuner measured for its
+ * grid shape and size (tuned on the spot when the context has none yet). */
+
+/* Autotuner of the team size: times a synthetic batch of `nitems` fwd+adj
+ * solves on an nx x ny grid for each W in 1..4 (as far as the grid's strips
+ * allow) and keeps the fastest in the context's table, keyed by strips, rows,
+ * item count and precision, for plans (satt_plan_set_waves(plan, 0)) and
+ * servers (waves 0).  ms[W-1] holds each measured run (0: not tried).  No
+ * counterpart in the reference. */
+typedef struct {
+  int waves;
+  double ms[4];
+} satt_tune_result;
+int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int precision,
+                    satt_tune_result *result);
 /* Wait for the plan's last run (no download). */
 int satt_plan_wait(satt_plan *plan);
 int satt_plan_results(satt_plan *plan, satt_result *result);
@@ -218,6 +253,11 @@ typedef struct {
   int max_batch;          /* requests per launch group (0: 256)                */
   int streams;            /* launch groups in flight (0: 4)                    */
   long long device_bytes; /* pool for requests in flight (0: half the free memory) */
+  int waves;              /* waves per request in the sweep kernels, capped by
+                             the grid's strips (0: the context's size tuned
+                             for one request of the group's widest grid, see
+                             satt_tune_waves, else 4); a team of waves cuts a
+                             request's latency, see satt_plan_set_waves      */
 } satt_server_config;
 
 typedef struct {

@@ -9,7 +9,9 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -26,7 +28,20 @@ struct satt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  std::mutex tune_mu;
+  std::map<TuneKey, int> tuned;  // team sizes measured by satt_tune_waves
 };
+
+int satt::ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k) {
+  std::lock_guard<std::mutex> lk(ctx->tune_mu);
+  auto it = ctx->tuned.find(k);
+  return it == ctx->tuned.end() ? 0 : it->second;
+}
+
+void satt::ctx_tune_store(satt_ctx *ctx, const TuneKey &k, int waves) {
+  std::lock_guard<std::mutex> lk(ctx->tune_mu);
+  ctx->tuned[k] = waves;
+}
 
 namespace {
 
@@ -484,9 +499,21 @@ struct satt_plan {
   std::vector<hipEvent_t> ev;
   std::vector<int> marks;  // stage of the interval ev[k]..ev[k+1]
   int last_istotable = 0, last_isadj = 0, ran = 0, last_batched = 0, last_group = -1;
+  int graph_on = 0;
+  hipGraphExec_t gexec[4] = {};  // replay form, per (istotable, isadj)
+  int waves = 1;                 // waves per item asked for (each group caps it)
+  void drop_graphs() {
+    for (auto &x : gexec)
+      if (x) {
+        (void)hipGraphExecDestroy(x);
+        x = nullptr;
+      }
+  }
   satt_plan_stats stats{};
   ~satt_plan() {
     for (auto e : ev) (void)hipEventDestroy(e);
+    for (auto x : gexec)
+      if (x) (void)hipGraphExecDestroy(x);
   }
 };
 
@@ -712,7 +739,10 @@ int plan_groups(satt_plan *P) {
       gd.max_se = std::max(gd.max_se, Q.sg.skew_elems());
       gd.max_n = std::max(gd.max_n, Q.N);
       gd.max_nxy = std::max(gd.max_nxy, std::max(Q.nx, Q.ny));
+      gd.max_ny = std::max(gd.max_ny, Q.ny);
+      gd.max_ns = std::max(gd.max_ns, Q.sg.ns);
     }
+    gd.waves = team_waves(P->waves, gd.max_ny, gd.max_ns);
     G.gd = gd;
     for (size_t q = 0; q < P->parts.size(); ++q) {
       const Part &Q = *P->parts[q];
@@ -830,46 +860,26 @@ extern "C" int satt_plan_set_model(satt_plan *P, const double *svel, const doubl
   return satt_plan_set_model_part(P, 0, svel, m11, m12, m22);
 }
 
-extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_form) {
-  if (!P) return SATT_E_ARG;
+namespace {
+
+// The launch chain of one run on stream s: the per-part output clears, then
+// per launch group the forward (setup, sweeps), the receivers (misfit,
+// adjoint sources, tt_fwd), the adjoint(s) and the stacking, then the
+// period-major kernels.  mark(stage) follows every launch (the stage
+// timing; it records nothing while the chain is being captured).
+template <typename Mark>
+int enqueue_run(satt_plan *P, hipStream_t s, int istotable, int isadj, bool batched, Mark mark) {
   satt_ctx *ctx = P->ctx;
-  if (launch_form != SATT_LAUNCH_BATCHED && launch_form != SATT_LAUNCH_PER_SOLVE)
-    return fail(ctx, SATT_E_ARG, "satt_plan_run: unknown launch form");
   const bool mixed = P->precision == SATT_PREC_MIXED;
-  const bool batched = launch_form == SATT_LAUNCH_BATCHED;
-  if (mixed && !batched)
-    return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form is FP64 only");
-  if (!batched && P->parts.size() != 1)
-    return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form takes one grid");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  // Events on the kernels' stream: one at the start and one after every
-  // launch, each interval tagged with the stage it times.
-  if (P->ev.empty()) {
-    hipEvent_t e;
-    HIPCHK(ctx, hipEventCreate(&e));
-    P->ev.push_back(e);
-  }
-  P->marks.clear();
-  auto mark = [&](int stage) -> hipError_t {
-    P->marks.push_back(stage);
-    while (P->ev.size() <= P->marks.size()) {  // the pool grows as launches are added
-      hipEvent_t e;
-      const hipError_t r = hipEventCreate(&e);
-      if (r != hipSuccess) return r;
-      P->ev.push_back(e);
-    }
-    return hipEventRecord(P->ev[P->marks.size()], s);
-  };
   for (auto &Qp : P->parts) {
     Part &Q = *Qp;
-    HIPCHK(ctx, hipMemsetAsync(Q.status, 0, (size_t)(Q.nitems + 1) * 4, s));
-    HIPCHK(ctx, hipMemsetAsync(Q.iters, 0, (size_t)(Q.nitems + 1) * 4 * kIterInts, s));
-    HIPCHK(ctx, hipMemsetAsync(Q.item_chi, 0, (size_t)(Q.nitems + 1) * 8, s));
-    HIPCHK(ctx, hipMemsetAsync(Q.tt_fwd.p, 0, Q.tt_fwd.bytes, s));
-    if (isadj) HIPCHK(ctx, hipMemsetAsync(Q.acc.p, 0, Q.acc.bytes, s));
+    launch_zero(s, Q.status, (size_t)(Q.nitems + 1) * 4);
+    launch_zero(s, Q.iters, (size_t)(Q.nitems + 1) * 4 * kIterInts);
+    launch_zero(s, Q.item_chi, (size_t)(Q.nitems + 1) * 8);
+    launch_zero(s, Q.tt_fwd.p, Q.tt_fwd.bytes);
+    if (isadj) launch_zero(s, Q.acc.p, Q.acc.bytes);
   }
-  HIPCHK(ctx, hipEventRecord(P->ev[0], s));
+  HIPCHK(ctx, mark(-1));
   const int nadj = P->dim == 3 ? 2 : 1;  // adjoint, then density adjoint
   const int power = P->dim == 3 ? 3 : 2;
   if (batched) {
@@ -952,6 +962,71 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
     }
   }
   HIPCHK(ctx, mark(kStOther));
+  return SATT_OK;
+}
+
+}  // namespace
+
+extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_form) {
+  if (!P) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  if (launch_form != SATT_LAUNCH_BATCHED && launch_form != SATT_LAUNCH_PER_SOLVE)
+    return fail(ctx, SATT_E_ARG, "satt_plan_run: unknown launch form");
+  const bool mixed = P->precision == SATT_PREC_MIXED;
+  const bool batched = launch_form == SATT_LAUNCH_BATCHED;
+  if (mixed && !batched)
+    return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form is FP64 only");
+  if (!batched && P->parts.size() != 1)
+    return fail(ctx, SATT_E_ARG, "satt_plan_run: the one-solve-per-launch form takes one grid");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  // Events on the kernels' stream: one at the start and one after every
+  // launch, each interval tagged with the stage it times.
+  if (P->ev.empty()) {
+    hipEvent_t e;
+    HIPCHK(ctx, hipEventCreate(&e));
+    P->ev.push_back(e);
+  }
+  P->marks.clear();
+  auto mark = [&](int stage) -> hipError_t {
+    if (stage < 0) return hipEventRecord(P->ev[0], s);  // start of the timed chain
+    P->marks.push_back(stage);
+    while (P->ev.size() <= P->marks.size()) {  // the pool grows as launches are added
+      hipEvent_t e;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      P->ev.push_back(e);
+    }
+    return hipEventRecord(P->ev[P->marks.size()], s);
+  };
+  if (batched && P->graph_on) {
+    // replay form: the chain of this (istotable, isadj) pair as one graph,
+    // captured on its first run; one timed interval around the launch
+    hipGraphExec_t &X = P->gexec[(istotable ? 1 : 0) + (isadj ? 2 : 0)];
+    if (!X) {
+      HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int st = enqueue_run(P, s, istotable, isadj, true,
+                                 [](int) -> hipError_t { return hipSuccess; });
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      if (st < 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      HIPCHK(ctx, e);
+      const hipError_t ei = hipGraphInstantiate(&X, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        X = nullptr;
+        HIPCHK(ctx, ei);
+      }
+    }
+    HIPCHK(ctx, mark(-1));
+    HIPCHK(ctx, hipGraphLaunch(X, s));
+    HIPCHK(ctx, mark(kStOther));
+  } else {
+    TRY(enqueue_run(P, s, istotable, isadj, batched, mark));
+  }
   HIPCHK(ctx, hipGetLastError());
   P->last_istotable = istotable;
   P->last_isadj = isadj;
@@ -960,6 +1035,37 @@ extern "C" int satt_plan_run(satt_plan *P, int istotable, int isadj, int launch_
   P->last_group = batched ? (int)P->groups.size() - 1 : -1;
   P->stats.launches = batched ? (int)P->groups.size() : P->parts[0]->nitems;
   P->stats.chunk = batched ? (P->groups.empty() ? 0 : (int)P->groups[0].slots.size()) : 1;
+  return SATT_OK;
+}
+
+extern "C" int satt_plan_set_waves(satt_plan *P, int waves) {
+  if (!P || waves < 0) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  P->waves = waves;
+  for (Group &G : P->groups) {
+    int w = waves;
+    if (w == 0) {  // tuned per launch group: its widest grid and its size
+      const int nx = 64 * G.gd.max_ns;
+      const TuneKey key = TuneKey::of(nx, G.gd.max_ny, G.gd.count, P->precision);
+      w = ctx_tune_lookup(ctx, key);
+      if (w == 0) {
+        satt_tune_result r{};
+        const int st = satt_tune_waves(ctx, nx, G.gd.max_ny, G.gd.count, P->precision, &r);
+        if (st < 0) return st;
+        w = r.waves;
+      }
+    }
+    G.gd.waves = team_waves(w, G.gd.max_ny, G.gd.max_ns);
+  }
+  P->drop_graphs();  // captured with the old launch shapes
+  return SATT_OK;
+}
+
+extern "C" int satt_plan_set_graph(satt_plan *P, int on) {
+  if (!P) return SATT_E_ARG;
+  P->graph_on = on ? 1 : 0;
   return SATT_OK;
 }
 

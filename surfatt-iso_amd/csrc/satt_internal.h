@@ -11,6 +11,28 @@ namespace satt {
 // The device of a context (satt_api.hip).
 int ctx_device(const satt_ctx *ctx);
 
+// Key of the context's tuning table (satt_tune.hip): strips, rows rounded
+// up to 32, the launch group's item count rounded up to a power of two,
+// precision.
+struct TuneKey {
+  int ns, ny32, count2, precision;
+  static TuneKey of(int nx, int ny, int count, int precision) {
+    int c2 = 1;
+    while (c2 < count) c2 <<= 1;
+    return TuneKey{(nx + 63) / 64, (ny + 31) / 32 * 32, c2, precision};
+  }
+  bool operator<(const TuneKey &o) const {
+    if (ns != o.ns) return ns < o.ns;
+    if (ny32 != o.ny32) return ny32 < o.ny32;
+    if (count2 != o.count2) return count2 < o.count2;
+    return precision < o.precision;
+  }
+};
+// The tuned team size of a key (0: not tuned), and storing one; both
+// thread-safe.
+int ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k);
+void ctx_tune_store(satt_ctx *ctx, const TuneKey &k, int waves);
+
 // Constants of the reference solvers, with their REAL(4) rounding kept:
 // src/eikon_solver_2d.f90:774-776,785 (forward) and :1293-1297 (adjoint).
 constexpr double kPiF = (double)3.14159265358979323846264338327950288f;
@@ -23,7 +45,7 @@ constexpr int kAdjMaxIter = 1000;
 
 // status codes (include/satt.h)
 constexpr int kOk = 0, kWMaxIter = 1, kEArg = -1, kESrcOnBoundary = -2,
-              kERecOutOfGrid = -3, kENoMem = -4, kEHip = -5;
+              kERecOutOfGrid = -3, kENoMem = -4, kEHip = -5, kETeamWait = -7;
 
 // The lon/lat grid shared by every item of a batch.  Radian axes are made on
 // the host with the reference's expression xx_deg/180.0*pi.
@@ -137,6 +159,8 @@ void launch_stack_item(hipStream_t s, const Grid &g, const PeriodFields &pf,
                        double *acc, int power);
 void launch_to_period_major(hipStream_t s, int N, int nperiod,
                             const double *nat, double *pm);
+// Zero `bytes` (a multiple of 4) of device memory with a kernel.
+void launch_zero(hipStream_t s, void *p, size_t bytes);
 void launch_mask(hipStream_t s, const Grid &g, double *table, double x0_deg,
                  double y0_deg);
 void launch_interp2(hipStream_t s, const Grid &g, const double *v,

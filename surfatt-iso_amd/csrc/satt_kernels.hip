@@ -13,6 +13,8 @@
 // into an add.
 #include <climits>
 
+#include <algorithm>
+
 #include "satt_internal.h"
 
 namespace satt {
@@ -580,6 +582,14 @@ __global__ void k_stack_item(Grid g, PeriodFields pf, const int *pidx, int item,
   acc[p * N + k] = acc[p * N + k] + field[k] / den;
 }
 
+// Zero n 32-bit words (a kernel rather than a memset, so that a captured
+// launch chain holds kernels only).
+__global__ void k_zero_words(unsigned int *p, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = 0u;
+}
+
 __global__ void k_to_period_major(int N, int np, const double *nat, double *pm) {
   const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const int p = blockIdx.y;
@@ -680,6 +690,14 @@ void launch_stack_item(hipStream_t s, const Grid &g, const PeriodFields &pf, con
                        int item, const double *field, double *acc, int power) {
   hipLaunchKernelGGL(k_stack_item, cells_grid((size_t)g.nx * g.ny), dim3(256), 0, s, g, pf, pidx,
                      item, field, acc, power);
+}
+
+void launch_zero(hipStream_t s, void *p, size_t bytes) {
+  const size_t n = bytes / 4;
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, s,
+                     static_cast<unsigned int *>(p), n);
 }
 
 void launch_to_period_major(hipStream_t s, int N, int np, const double *nat, double *pm) {

@@ -135,6 +135,7 @@ struct Batch {
 struct satt_server {
   satt_ctx *ctx = nullptr;
   int device = 0, precision = 0, max_batch = 256, nstreams = 4, slot_skews = kSlotSkews;
+  int waves = 0;           // waves per request in the sweep kernels (0: tuned, else 4)
   char *dev = nullptr;     // device pool
   char *pin = nullptr;     // pinned staging pool
   size_t dev_cap = 0, pin_cap = 0;
@@ -252,7 +253,16 @@ bool satt_server::launch(Batch &b) {
     gd.max_se = std::max(gd.max_se, sg.skew_elems());
     gd.max_n = std::max(gd.max_n, N);
     gd.max_nxy = std::max(gd.max_nxy, std::max(nx, ny));
+    gd.max_ny = std::max(gd.max_ny, ny);
+    gd.max_ns = std::max(gd.max_ns, sg.ns);
   }
+  int w = waves;
+  if (w == 0) {  // the team size tuned for one request of this shape, if any
+    const TuneKey key = TuneKey::of(64 * gd.max_ns, gd.max_ny, 1, precision);
+    w = ctx_tune_lookup(ctx, key);
+    if (w == 0) w = 4;
+  }
+  gd.waves = team_waves(w, gd.max_ny, gd.max_ns);
   std::memcpy(pin + b.desc_pin, desc.data(), desc.size() * sizeof(SlotDesc));
   SlotDesc *sd = reinterpret_cast<SlotDesc *>(dev + b.desc_dev);
   if (!hip_ok(hipMemcpyAsync(sd, pin + b.desc_pin, desc.size() * sizeof(SlotDesc),
@@ -460,6 +470,7 @@ extern "C" int satt_server_create(satt_ctx *ctx, const satt_server_config *cfg,
   S->slot_skews = c.precision == SATT_PREC_MIXED ? kSlotSkewsMixed : kSlotSkews;
   S->max_batch = c.max_batch > 0 ? c.max_batch : 256;
   S->nstreams = c.streams > 0 ? std::min(c.streams, 16) : 4;
+  S->waves = c.waves > 0 ? c.waves : 0;
   if (hipSetDevice(S->device) != hipSuccess) return SATT_E_HIP;
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return SATT_E_HIP;

@@ -42,13 +42,25 @@ inline size_t slot_doubles(int nx, int ny, int slot_skews) {
   return (n + 31) & ~(size_t)31;
 }
 
-// Sizes of a launch group, for the grids of the element-wise kernels.
+// Sizes of a launch group, for the grids of the element-wise kernels, and
+// the waves per item of its strip kernels.
 struct GroupDims {
   int count;        // slots
   size_t max_se;    // largest skewed field
   size_t max_n;     // largest grid
   int max_nxy;      // largest nx or ny
+  int max_ny = 0, max_ns = 0;  // largest ny and strip count
+  int waves = 1;    // waves per item (a team; at most kMaxTeam)
 };
+
+// Waves per item of the strip kernels: each strip of a sweep goes to one
+// wave of the item's team, the next strip's wave following at least one
+// strip width behind.  More waves cut an item's latency on wide grids; one
+// wave per item is the throughput form when items fill the device.
+constexpr int kMaxTeam = 4;
+constexpr size_t kTeamLdsMax = 64 * 1024;
+// The largest team (<= want) whose LDS fits for grids up to ny rows, ns strips.
+int team_waves(int want, int ny, int ns);
 
 // Batched kernels over the slots of a launch group (satt_strip.hip); all
 // asynchronous on `s`.  Forward: setup (per-item tables, then T0 skews and

@@ -22,10 +22,16 @@
 //   plus skew  (sweeps with xd == yd):  row r = j + l
 //   minus skew (sweeps with xd != yd):  row r = j - l + 63
 // each as [strip][row 0..ny+62][lane 0..63], zero padded.  Read-only
-// coefficients exist in both skews.  tau / Ta also exist in both: a sweep of
-// the same family as the previous one updates its skew in place; the first
-// sweep after a family change reads one skew and writes the other.  Natural
-// layout (i + nx*j) is used only for the solve's inputs and outputs.
+// coefficients exist in both skews.  tau / Ta also exist in both: every sweep
+// updates its own family's skew in place, and at a family change the field
+// is copied into the other skew row by row (coalesced stores, gathered
+// loads that stay in cache).  Natural layout (i + nx*j) is used only for
+// the solve's inputs and outputs.
+//
+// Teams.  A strip kernel runs one item per workgroup of W waves: the k-th
+// strip of a sweep goes to wave k mod W, which follows the previous strip's
+// wave at least a strip width behind, taking that strip's edge column from
+// LDS (see Team).  W = 1 is one wave doing every strip in order.
 //
 // Precision.  The FP64 kernels (k_fwd_strip, k_adj_strip) run the
 // reference's arithmetic.  The mixed mode runs an FP32 phase
@@ -49,6 +55,7 @@
 //   formed in FP64 and rounded to float; stop when the reference's rule
 //   holds or the largest change is at most 2^-21 of the largest |Ta|.
 //   Polish: one FP64 iteration of the reference's update.
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 
@@ -365,17 +372,143 @@ __device__ __forceinline__ size_t skpos(int sk, int i, int j, int R) {
   return skidx(c, r, l, R);
 }
 
+// The waves of one workgroup working on one item (a team of W waves).  The
+// k-th strip of a sweep goes to wave k mod W; the edge column a strip hands
+// to the next strip passes through LDS, with the count of rows published so
+// far, so that the next wave runs its strip as soon as the rows it needs are
+// out (at least L steps behind).  Each wave publishes only into its own
+// channel, and a channel row is overwritten (by the wave's next strip) only
+// after the consumer has read it, because that next strip runs behind the
+// consumer's strip.  W = 1: one wave does every strip in order; no LDS.
+// A wait is bounded: past the bound the item gets kETeamWait and every later
+// wait of the team returns at once, so the kernel always ends.
+struct Team {
+  int W, w;           // waves, this wave (wave-uniform)
+  int ny, ns;
+  int sweep;          // sweeps started (the count set in use is sweep & 1)
+  void *chan;         // [W][ny] edge values of the current strips (V)
+  volatile int *cnt;  // [2][W] rows published by each wave in the sweep
+  volatile int *stop; // set when a wait ran out
+  double *part;       // [ns][3] per-strip residual terms
+  int *status;        // the item's status
+  __device__ bool multi() const { return W > 1; }
+};
+
+constexpr int kTeamSpinLimit = 1 << 22;  // ~0.5 s of s_sleep per wait
+
+// LDS bytes of a team of W waves on grids up to ny rows and ns strips.
+__host__ __device__ inline size_t team_lds_bytes(int W, int ny, int ns, int vbytes) {
+  if (W <= 1) return 0;
+  return (size_t)ns * 3 * 8 + (size_t)W * ny * vbytes + (size_t)(2 * W + 2) * 4;
+}
+
+// TEAM false: the one-wave form, with W = 1 known at compile time so that
+// no team code is left in the kernel.
+template <bool TEAM>
+__device__ inline Team team_init(void *lds, int ny, int ns, int vbytes, int *status) {
+  Team T;
+  T.W = TEAM ? (int)(blockDim.x >> 6) : 1;
+  T.w = TEAM ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+  T.ny = ny;
+  T.ns = ns;
+  T.sweep = 0;
+  T.status = status;
+  T.part = static_cast<double *>(lds);
+  char *b = static_cast<char *>(lds) + (size_t)ns * 3 * 8;
+  T.chan = b;
+  T.cnt = reinterpret_cast<volatile int *>(b + (size_t)T.W * ny * vbytes);
+  T.stop = T.cnt + 2 * T.W;
+  if (TEAM) {
+    if (threadIdx.x < 2 * (unsigned)T.W) T.cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *T.stop = 0;
+    __syncthreads();
+  }
+  return T;
+}
+
+// Start of a sweep: clear this wave's count of the other set (used by the
+// previous sweep, which every wave has finished, and by the next one, which
+// starts after the barrier that ends this one).
+__device__ inline void team_sweep_begin(Team &T) {
+  if (T.multi() && (threadIdx.x & 63) == 0) T.cnt[((T.sweep + 1) & 1) * T.W + T.w] = 0;
+}
+__device__ inline void team_sweep_end(Team &T) {
+  ++T.sweep;
+  __syncthreads();
+}
+
+// Whether a wait of the team ran out (never for W = 1, which has no LDS).
+__device__ inline bool team_failed(const Team &T) { return T.multi() && *T.stop; }
+
+// Wait (one lane) until wave pw has published `need` rows in this sweep.
+__device__ inline void team_wait(const Team &T, int pw, int need) {
+  volatile int *c = T.cnt + (T.sweep & 1) * T.W + pw;
+  int spins = 0;
+  while (*c < need && !*T.stop) {
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > kTeamSpinLimit) {
+      *T.stop = 1;
+      *T.status = kETeamWait;
+    }
+  }
+}
+
+template <typename V>
+__device__ inline V team_read(const Team &T, int pw, int u) {
+  return static_cast<volatile V *>(T.chan)[(size_t)pw * T.ny + u];
+}
+
+// Publish (one lane) the edge value of row u, the `count`-th row of this
+// wave's strips in the sweep: the value first, then the count (LDS
+// operations of a wave complete in order).
+template <typename V>
+__device__ inline void team_publish(const Team &T, int u, V v, int count) {
+  static_cast<volatile V *>(T.chan)[(size_t)T.w * T.ny + u] = v;
+  T.cnt[(T.sweep & 1) * T.W + T.w] = count;
+}
+
+// Reduction of a sweep's per-strip residual terms (sum, max, max) in the
+// sweep's strip order: the same bits for any W.
+struct StripAcc {
+  double s = 0, m = 0, a = 0;
+};
+__device__ inline void team_strip_term(const Team &T, int k, double s, double m, double a,
+                                       StripAcc &acc) {
+  if (T.multi()) {
+    if ((threadIdx.x & 63) == 0) {
+      T.part[3 * k] = s;
+      T.part[3 * k + 1] = m;
+      T.part[3 * k + 2] = a;
+    }
+  } else {
+    acc.s = acc.s + s;
+    acc.m = m > acc.m ? m : acc.m;
+    acc.a = a > acc.a ? a : acc.a;
+  }
+}
+// After the barrier that ends the sweep.
+__device__ inline StripAcc team_strip_total(const Team &T, const StripAcc &acc) {
+  if (!T.multi()) return acc;
+  StripAcc r;
+  for (int k = 0; k < T.ns; ++k) {
+    r.s = r.s + T.part[3 * k];
+    r.m = T.part[3 * k + 1] > r.m ? T.part[3 * k + 1] : r.m;
+    r.a = T.part[3 * k + 2] > r.a ? T.part[3 * k + 2] : r.a;
+  }
+  return r;
+}
+
 // Copies the grid cells of a field from skew FROM into the other skew, one
 // whole destination row per step (coalesced stores).  The loads of one row
 // gather from 64 source rows; each source line is read again by the next
 // lanes on the following rows, so it stays in cache while it is used.
 // Padding elements of the destination are left as they are.
 template <typename V, int FROM>
-__device__ void skew_copy(const V *__restrict__ src, V *__restrict__ dst, int nx, int ny, int R,
-                          int ns, int lane) {
+__device__ void skew_copy(const Team &T, const V *__restrict__ src, V *__restrict__ dst, int nx,
+                          int ny, int R, int ns, int lane) {
   constexpr int TO = 1 - FROM;
   constexpr int kU = 8;
-  for (int c = 0; c < ns; ++c) {
+  for (int c = T.w; c < ns; c += T.W) {
     const bool col = 64 * c + lane < nx;
     // destination row r holds the lane's cell j = r - jof, which sits in
     // source row j + sof
@@ -386,17 +519,17 @@ __device__ void skew_copy(const V *__restrict__ src, V *__restrict__ dst, int nx
       V v[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int j = min(max(r0 + u - jof, 0), ny - 1);
-        v[u] = src[base + (size_t)(j + sof) * 64];
+        const int j = r0 + u - jof;
+        if (col && j >= 0 && j < ny) v[u] = src[base + (size_t)(j + sof) * 64];
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int r = r0 + u, j = r - jof;
-        if (col && r < R && j >= 0 && j < ny) dst[base + (size_t)r * 64] = v[u];
+        const int j = r0 + u - jof;
+        if (col && j >= 0 && j < ny) dst[base + (size_t)(r0 + u) * 64] = v[u];
       }
     }
   }
-  wave_fence();
+  __syncthreads();
 }
 
 // Skew row of step t (uniform across lanes), see the layout note above.
@@ -410,8 +543,11 @@ __device__ __forceinline__ int skew_row(int t, int ny, int L) {
 // The operands a lane needs for its cell of step t.  "Far" operands are the
 // ones other lanes read one step early (tau, T0) plus the strip-edge and
 // convergence values; they are fetched two steps ahead.  "Near" operands are
-// the cell's own coefficients, fetched one step ahead.  Loads use clamped
-// (always valid) addresses so that no branch separates a load from its use.
+// the cell's own coefficients, fetched one step ahead.  A lane loads only
+// while its cell of that step is in the grid, so the pipeline's fill and
+// drain steps and the padding lanes of a partial strip fetch nothing (whole
+// cache lines of a row are skipped); the values other lanes read from it
+// (lane shifts) are then never used for a cell in the grid.
 template <typename V>
 struct FFar {
   V tau, t0;    // old tau and T0 of the lane's cell
@@ -440,13 +576,16 @@ struct FwdCtx {
 };
 
 
-// One sweep (XD, YD) over strip c, updating tau in place in the sweep's own
-// skew.  MODE 0: plain; 1: record the snapshot (first sweep of an
-// iteration); 2: accumulate |tau-snap|*T0 in double (last sweep).
+// One sweep (XD, YD) over the k-th strip of the sweep, updating tau in place
+// in the sweep's own skew.  MODE 0: plain; 1: record the snapshot (first
+// sweep of an iteration); 2: the strip's sum and max of |tau-snap|*T0 in
+// double (last sweep) into ss, sm.
 template <typename V, int XD, int YD, int MODE>
-__device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, V *tau, double &acc_s, double &acc_m) {
+__device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, V *tau, double &ss,
+                               double &sm) {
   constexpr int SK = XD == YD ? 0 : 1;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
+  const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
   const V *ps = C.ps[SK], *is = C.is[SK];
   const int L = min(64, nx - 64 * c);
@@ -464,37 +603,52 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, V *tau, double &acc_s,
   // the FP32 update reads upwind neighbours only
   constexpr bool kDown = std::is_same<V, double>::value;
   const bool dn_ok = kDown && lv && dn_edge && id >= 0 && id < nx;
+  // team: the upstream strip's edge column comes from the previous wave's
+  // channel (rows counted over that wave's strips of the sweep); the lane
+  // next to the downstream strip publishes this strip's
+  const bool from_chan = T.multi() && k > 0;
+  const int pw = (T.w + T.W - 1) % T.W, base_in = ((k - 1) / T.W) * ny;
+  const bool pub = T.multi() && k < T.ns - 1 && lag == L - 1;
+  const int base_out = (k / T.W) * ny;
+  double acc_s = 0, acc_m = 0;
 
   auto load_far = [&](int t, FFar<V> &q) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
     q.j = j;
     q.act = lv && u >= 0 && u < ny;
-    const int jc = min(max(j, 0), ny - 1);
-    const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
-    const size_t e = skidx(c, r, lane, R);
-    q.t0 = is[e];
-    q.tau = tau[e];
-    if (up_ok) {
-      q.tup = tau[skpos(SK, iu, jc, R)];
-      q.zup = is[skpos(SK, iu, jc, R)];
+    if (q.act) {
+      const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
+      q.t0 = is[e];
+      q.tau = tau[e];
+      if (up_ok) {
+        if (from_chan) {
+          team_wait(T, pw, base_in + u + 1);
+          q.tup = team_read<V>(T, pw, u);
+        } else {
+          q.tup = tau[skpos(SK, iu, j, R)];
+        }
+        q.zup = is[skpos(SK, iu, j, R)];
+      }
+      if (kDown && dn_ok) {
+        q.tdn = tau[skpos(SK, id, j, R)];
+        q.zdn = is[skpos(SK, id, j, R)];
+      }
+      if (MODE == 2) q.snap = C.snap[e];
     }
-    if (kDown && dn_ok) {
-      q.tdn = tau[skpos(SK, id, jc, R)];
-      q.zdn = is[skpos(SK, id, jc, R)];
-    }
-    if (MODE == 2) q.snap = C.snap[e];
   };
   auto load_near = [&](int t, FNear<V> &q) {
-    const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
-    const size_t e = skidx(c, r, lane, R);
-    q.a = ps[e]; q.b = ps[SE + e]; q.c = ps[2 * SE + e]; q.f = ps[3 * SE + e];
-    q.dx_ = ps[4 * SE + e]; q.dy_ = ps[5 * SE + e];
-    q.t0x = is[SE + e]; q.t0y = is[2 * SE + e];
+    const int u = t - lag;
+    if (lv && u >= 0 && u < ny) {
+      const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
+      q.a = ps[e]; q.b = ps[SE + e]; q.c = ps[2 * SE + e]; q.f = ps[3 * SE + e];
+      q.dx_ = ps[4 * SE + e]; q.dy_ = ps[5 * SE + e];
+      q.t0x = is[SE + e]; q.t0y = is[2 * SE + e];
+    }
   };
 
-  FFar<V> fc, fn, f2;
-  FNear<V> nc, nn;
+  FFar<V> fc{}, fn{}, f2{};
+  FNear<V> nc{}, nn{};
   load_far(0, fc);
   load_far(1, fn);
   load_near(0, nc);
@@ -532,6 +686,7 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, V *tau, double &acc_s,
       }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       tau[e] = tnew;
+      if (pub) team_publish<V>(T, t - lag, tnew, base_out + t - lag + 1);
       if (MODE == 1) C.snap[e] = fc.tau;
       if (MODE == 2) {
         const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
@@ -545,13 +700,22 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, int c, V *tau, double &acc_s,
     fn = f2;
     nc = nn;
   }
+  if (MODE == 2) {
+    ss = wave_sum(acc_s);
+    sm = wave_max(acc_m);
+  }
   wave_fence();
 }
 
 template <typename V, int XD, int YD, int MODE>
-__device__ void fwd_sweep(const FwdCtx<V> &C, int ns, V *tau, double &acc_s, double &acc_m) {
-  for (int cq = 0; cq < ns; ++cq)
-    fwd_strip_pass<V, XD, YD, MODE>(C, XD > 0 ? cq : ns - 1 - cq, tau, acc_s, acc_m);
+__device__ void fwd_sweep(const FwdCtx<V> &C, Team &T, V *tau, StripAcc &acc) {
+  team_sweep_begin(T);
+  for (int k = T.w; k < T.ns; k += T.W) {
+    double ss = 0, sm = 0;
+    fwd_strip_pass<V, XD, YD, MODE>(C, T, k, tau, ss, sm);
+    if (MODE == 2) team_strip_term(T, k, ss, sm, 0.0, acc);
+  }
+  team_sweep_end(T);
 }
 
 // One iteration of four sweeps, (x+,y+), (x+,y-), (x-,y+), (x-,y-)
@@ -560,17 +724,17 @@ __device__ void fwd_sweep(const FwdCtx<V> &C, int ns, V *tau, double &acc_s, dou
 // last in the plus skew, with the field copied between the skews at the
 // two family changes.  Returns whether the stop rule (:1091-1115) holds.
 template <typename V>
-__device__ bool fwd_iteration(const FwdCtx<V> &C, int ns, V *tP, V *tM) {
-  double acc_s = 0, acc_m = 0;
-  fwd_sweep<V, 1, 1, 1>(C, ns, tP, acc_s, acc_m);
-  skew_copy<V, 0>(tP, tM, C.nx, C.ny, C.R, ns, C.lane);
-  fwd_sweep<V, 1, -1, 0>(C, ns, tM, acc_s, acc_m);
-  fwd_sweep<V, -1, 1, 0>(C, ns, tM, acc_s, acc_m);
-  skew_copy<V, 1>(tM, tP, C.nx, C.ny, C.R, ns, C.lane);
-  fwd_sweep<V, -1, -1, 2>(C, ns, tP, acc_s, acc_m);
-  const double s = wave_sum(acc_s), m = wave_max(acc_m);
-  const double l1 = s / (double)C.nx / (double)C.ny;
-  return fabs(l1) < kFwdTol && m < kFwdTol;
+__device__ bool fwd_iteration(const FwdCtx<V> &C, Team &T, V *tP, V *tM) {
+  StripAcc acc;
+  fwd_sweep<V, 1, 1, 1>(C, T, tP, acc);
+  skew_copy<V, 0>(T, tP, tM, C.nx, C.ny, C.R, T.ns, C.lane);
+  fwd_sweep<V, 1, -1, 0>(C, T, tM, acc);
+  fwd_sweep<V, -1, 1, 0>(C, T, tM, acc);
+  skew_copy<V, 1>(T, tM, tP, C.nx, C.ny, C.R, T.ns, C.lane);
+  fwd_sweep<V, -1, -1, 2>(C, T, tP, acc);
+  const StripAcc r = team_strip_total(T, acc);
+  const double l1 = r.s / (double)C.nx / (double)C.ny;
+  return fabs(l1) < kFwdTol && r.m < kFwdTol;
 }
 
 // Per-slot FP32 region of the mixed forward, in float skews from double
@@ -690,7 +854,7 @@ __device__ void fwd_ctx_init(FwdCtx<V> &C, const Grid &g, const SkewGeom &sg, in
 
 // FP64 iterations from the tau in the plus skew, then T = tau*T0
 // (src/eikon_solver_2d.f90:1124) in both skews and in natural layout.
-__device__ void fwd_finish64(const SlotDesc &D, int lane, const int rect[4]) {
+__device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rect[4]) {
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const int nx = g.nx, ny = g.ny, R = sg.R, ns = sg.ns;
@@ -708,9 +872,9 @@ __device__ void fwd_finish64(const SlotDesc &D, int lane, const int rect[4]) {
   int iter = 1;
   bool conv = false;
   for (; iter <= kFwdMaxIter; ++iter)
-    if (fwd_iteration(C, ns, tauP, tauM)) { conv = true; break; }
+    if (fwd_iteration(C, T, tauP, tauM)) { conv = true; break; }
   if (!conv) iter = kFwdMaxIter;
-  for (int cc = 0; cc < ns; ++cc) {
+  for (int cc = T.w; cc < ns; cc += T.W) {
     const int i = 64 * cc + lane;
     for (int r = 0; r < R; ++r) {
       const size_t e = skidx(cc, r, lane, R);
@@ -721,13 +885,13 @@ __device__ void fwd_finish64(const SlotDesc &D, int lane, const int rect[4]) {
       TM[e] = tm;
     }
   }
-  wave_fence();
-  for (size_t k = lane; k < N; k += 64) {
+  __syncthreads();
+  for (size_t k = threadIdx.x; k < N; k += blockDim.x) {
     const int i = (int)(k % nx), j = (int)(k / nx);
     Tnat[k] = TP[skpos(0, i, j, R)];
   }
-  if (lane == 0) {
-    *D.status = conv ? kOk : kWMaxIter;
+  if (threadIdx.x == 0) {
+    *D.status = team_failed(T) ? kETeamWait : conv ? kOk : kWMaxIter;
     D.iters[0] = iter;
   }
 }
@@ -739,25 +903,33 @@ __device__ bool fwd_item_rect(const SlotDesc &D, int lane, int rect[4]) {
   return fwd_source_rect(D.g, D.fun, x0, y0, lane, fun0, rect) == kOk;
 }
 
-__global__ __launch_bounds__(64) void k_fwd_strip(const SlotDesc *sd) {
+// The strip kernels run one item per workgroup of W = blockDim.x/64 waves
+// (a team), with team_lds_bytes of dynamic LDS when W > 1.
+template <bool TEAM>
+__global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_strip(const SlotDesc *sd) {
+  extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (*D.status < 0) return;  // the setup failed
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
-  fwd_finish64(D, lane, rect);
+  Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8, D.status);
+  fwd_finish64(D, T, lane, rect);
 }
 
 // Mixed forward, FP32 phase: the FP32 iterations (count to iters[3]), then
 // tau = 1 + dtau into the FP64 plus skew for the polish.
-__global__ __launch_bounds__(64) void k_fwd_mixed32(const SlotDesc *sd) {
+template <bool TEAM>
+__global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_mixed32(const SlotDesc *sd) {
+  extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (*D.status < 0) return;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
+  Team T = team_init<TEAM>(lds, D.g.ny, sg.ns, 4, D.status);
   double *sk0 = D.skew;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
   FwdCtx<float> C;
@@ -770,22 +942,28 @@ __global__ __launch_bounds__(64) void k_fwd_mixed32(const SlotDesc *sd) {
   float *dP = f32 + 6 * SE, *dM = f32 + 7 * SE;
   int iter = 1;
   for (; iter <= kFwdMaxIter; ++iter)
-    if (fwd_iteration(C, sg.ns, dP, dM)) break;
+    if (fwd_iteration(C, T, dP, dM)) break;
   if (iter > kFwdMaxIter) iter = kFwdMaxIter;
   double *tauP = sk0 + kSkTauP * SE;
-  for (size_t e = lane; e < SE; e += 64) tauP[e] = 1.0 + (double)dP[e];
-  if (lane == 0) D.iters[3] = iter;
+  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) tauP[e] = 1.0 + (double)dP[e];
+  if (threadIdx.x == 0) {
+    D.iters[3] = iter;
+    if (team_failed(T)) *D.status = kETeamWait;
+  }
 }
 
 // Mixed forward, FP64 polish: the reference's iterations to its stop rule
 // from the FP32 result, and the outputs.
-__global__ __launch_bounds__(64) void k_fwd_polish(const SlotDesc *sd) {
+template <bool TEAM>
+__global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_polish(const SlotDesc *sd) {
+  extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (*D.status < 0) return;
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
-  fwd_finish64(D, lane, rect);
+  Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8, D.status);
+  fwd_finish64(D, T, lane, rect);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -1001,10 +1179,11 @@ struct AdjCtx {
 // for the forward, and MODE 2 also accumulates max |Ta| (acc_a) for the
 // FP32 stop rule.
 template <typename V, int XD, int YD, int MODE>
-__device__ void adj_strip_pass(const AdjCtx<V> &C, int c, V *ta, double &acc_s, double &acc_m,
-                               double &acc_a) {
+__device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, V *ta, double &ss,
+                               double &sm, double &sa) {
   constexpr int SK = XD == YD ? 0 : 1;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
+  const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
   const V *is = C.is[SK];
   const int L = min(64, nx - 64 * c);
@@ -1018,7 +1197,15 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, V *ta, double &acc_s, 
   const bool up_ok = lv && up_edge && iu >= 0 && iu < nx;
   const bool dn_ok = lv && dn_edge && id >= 0 && id < nx;
   const bool interior_col = i >= 1 && i <= nx - 2;
+  // team edges as in the forward pass
+  const bool from_chan = T.multi() && k > 0;
+  const int pw = (T.w + T.W - 1) % T.W, base_in = ((k - 1) / T.W) * ny;
+  const bool pub = T.multi() && k < T.ns - 1 && lag == L - 1;
+  const int base_out = (k / T.W) * ny;
+  double acc_s = 0, acc_m = 0, acc_a = 0;
 
+  // The adjoint's step is a few operations, so its loads must not wait on
+  // a branch: every lane loads, at a clamped (always valid) row.
   auto load_far = [&](int t, AFar<V> &q) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
@@ -1028,7 +1215,16 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, V *ta, double &acc_s, 
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
     const size_t e = skidx(c, r, lane, R);
     q.ta = ta[e];
-    if (up_ok) q.tup = ta[skpos(SK, iu, jc, R)];
+    if (up_ok) {
+      if (from_chan) {
+        if (q.act) {
+          team_wait(T, pw, base_in + u + 1);
+          q.tup = team_read<V>(T, pw, u);
+        }
+      } else {
+        q.tup = ta[skpos(SK, iu, jc, R)];
+      }
+    }
     if (dn_ok) q.tdn = ta[skpos(SK, id, jc, R)];
     if (MODE == 2) q.snap = C.snap[e];
   };
@@ -1058,6 +1254,7 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, V *ta, double &acc_s, 
         tnew = adj_update(nc, tW, tE, tS, tN, C.dx, C.dy);
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       ta[e] = tnew;
+      if (pub) team_publish<V>(T, t - lag, tnew, base_out + t - lag + 1);
       if (MODE == 1) C.snap[e] = fc.ta;
       if (MODE == 2) {
         const double dd = fabs((double)tnew - (double)fc.snap);
@@ -1074,32 +1271,42 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, int c, V *ta, double &acc_s, 
     fn = f2;
     nc = nn;
   }
+  if (MODE == 2) {
+    ss = wave_sum(acc_s);
+    sm = wave_max(acc_m);
+    sa = wave_max(acc_a);
+  }
   wave_fence();
 }
 
 template <typename V, int XD, int YD, int MODE>
-__device__ void adj_sweep(const AdjCtx<V> &C, int ns, V *ta, double &acc_s, double &acc_m,
-                          double &acc_a) {
-  for (int cq = 0; cq < ns; ++cq)
-    adj_strip_pass<V, XD, YD, MODE>(C, XD > 0 ? cq : ns - 1 - cq, ta, acc_s, acc_m, acc_a);
+__device__ void adj_sweep(const AdjCtx<V> &C, Team &T, V *ta, StripAcc &acc) {
+  team_sweep_begin(T);
+  for (int k = T.w; k < T.ns; k += T.W) {
+    double ss = 0, sm = 0, sa = 0;
+    adj_strip_pass<V, XD, YD, MODE>(C, T, k, ta, ss, sm, sa);
+    if (MODE == 2) team_strip_term(T, k, ss, sm, sa, acc);
+  }
+  team_sweep_end(T);
 }
 
 // One iteration: (xdirec, ydirec) = (-1,-1), (-1,1), (1,-1), (1,1), x
 // ascending first (src/eikon_solver_2d.f90:1383-1390); with SNAP the
 // snapshot and the residuals (:1412-1434) are taken.
 template <typename V, bool SNAP>
-__device__ void adj_iteration(const AdjCtx<V> &C, int ns, V *tP, V *tM, double &s, double &m,
+__device__ void adj_iteration(const AdjCtx<V> &C, Team &T, V *tP, V *tM, double &s, double &m,
                               double &a) {
-  double acc_s = 0, acc_m = 0, acc_a = 0;
-  adj_sweep<V, 1, 1, SNAP ? 1 : 0>(C, ns, tP, acc_s, acc_m, acc_a);
-  skew_copy<V, 0>(tP, tM, C.nx, C.ny, C.R, ns, C.lane);
-  adj_sweep<V, 1, -1, 0>(C, ns, tM, acc_s, acc_m, acc_a);
-  adj_sweep<V, -1, 1, 0>(C, ns, tM, acc_s, acc_m, acc_a);
-  skew_copy<V, 1>(tM, tP, C.nx, C.ny, C.R, ns, C.lane);
-  adj_sweep<V, -1, -1, SNAP ? 2 : 0>(C, ns, tP, acc_s, acc_m, acc_a);
-  s = wave_sum(acc_s);
-  m = wave_max(acc_m);
-  a = wave_max(acc_a);
+  StripAcc acc;
+  adj_sweep<V, 1, 1, SNAP ? 1 : 0>(C, T, tP, acc);
+  skew_copy<V, 0>(T, tP, tM, C.nx, C.ny, C.R, T.ns, C.lane);
+  adj_sweep<V, 1, -1, 0>(C, T, tM, acc);
+  adj_sweep<V, -1, 1, 0>(C, T, tM, acc);
+  skew_copy<V, 1>(T, tM, tP, C.nx, C.ny, C.R, T.ns, C.lane);
+  adj_sweep<V, -1, -1, SNAP ? 2 : 0>(C, T, tP, acc);
+  const StripAcc r = team_strip_total(T, acc);
+  s = r.s;
+  m = r.m;
+  a = r.a;
 }
 
 template <typename V>
@@ -1110,12 +1317,12 @@ __device__ void adj_ctx_init(AdjCtx<V> &C, const Grid &g, const SkewGeom &sg, in
 
 // Natural-layout output from the plus skew, with the mask
 // (src/eikon_solver_2d.f90:1449-1467, degrees).
-__device__ void adj_output(const Grid &g, const SkewGeom &sg, int lane, const double *TaP,
-                           double x0, double y0, int apply_mask, double *Tout) {
+__device__ void adj_output(const Grid &g, const SkewGeom &sg, const double *TaP, double x0,
+                           double y0, int apply_mask, double *Tout) {
   const int nx = g.nx;
   const size_t N = (size_t)nx * g.ny;
   const double dxd = g.xd[1] - g.xd[0], dyd = g.yd[1] - g.yd[0];
-  for (size_t k = lane; k < N; k += 64) {
+  for (size_t k = threadIdx.x; k < N; k += blockDim.x) {
     const int i = (int)(k % nx), j = (int)(k / nx);
     double v = TaP[skpos(0, i, j, sg.R)];
     if (apply_mask && fabs(g.xd[i] - x0) < 1.5 * dxd && fabs(g.yd[j] - y0) < 1.5 * dyd) v = 0;
@@ -1123,13 +1330,16 @@ __device__ void adj_output(const Grid &g, const SkewGeom &sg, int lane, const do
   }
 }
 
-__global__ __launch_bounds__(64) void k_adj_strip(const SlotDesc *sd, int which) {
+template <bool TEAM>
+__global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_strip(const SlotDesc *sd, int which) {
+  extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (*D.status < 0) return;  // an earlier stage failed
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
+  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8, D.status);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
@@ -1141,14 +1351,15 @@ __global__ __launch_bounds__(64) void k_adj_strip(const SlotDesc *sd, int which)
   bool conv = false;
   for (; iter <= kAdjMaxIter; ++iter) {
     double s, m, a;
-    adj_iteration<double, true>(C, sg.ns, TaP, TaM, s, m, a);
+    adj_iteration<double, true>(C, T, TaP, TaM, s, m, a);
     const double l1 = s / (double)(g.nx * g.ny);
     if (fabs(l1) < kAdjTol && m < kAdjTol) { conv = true; break; }
   }
   if (!conv) iter = kAdjMaxIter;
-  adj_output(g, sg, lane, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
-  if (lane == 0) {
-    if (!conv && *D.status == kOk) *D.status = kWMaxIter;
+  adj_output(g, sg, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
+  if (threadIdx.x == 0) {
+    if (team_failed(T)) *D.status = kETeamWait;
+    else if (!conv && *D.status == kOk) *D.status = kWMaxIter;
     D.iters[1 + which] = iter;
   }
 }
@@ -1156,13 +1367,16 @@ __global__ __launch_bounds__(64) void k_adj_strip(const SlotDesc *sd, int which)
 // Mixed adjoint, FP32 phase: iterations on the float coefficients (float
 // skews from double skew 16: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
 // minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
-__global__ __launch_bounds__(64) void k_adj_mixed32(const SlotDesc *sd, int which) {
+template <bool TEAM>
+__global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_mixed32(const SlotDesc *sd, int which) {
+  extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (*D.status < 0) return;
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
+  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 4, D.status);
   double *sk0 = D.skew;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
   float *taP = f32 + 10 * SE, *taM = f32 + 11 * SE;
@@ -1175,7 +1389,7 @@ __global__ __launch_bounds__(64) void k_adj_mixed32(const SlotDesc *sd, int whic
   bool done = false;
   for (; iter <= kAdjMaxIter; ++iter) {
     double s, m, a;
-    adj_iteration<float, true>(C, sg.ns, taP, taM, s, m, a);
+    adj_iteration<float, true>(C, T, taP, taM, s, m, a);
     const double l1 = s / (double)(g.nx * g.ny);
     if ((fabs(l1) < kAdjTol && m < kAdjTol) || m <= (double)kMixRelStop * a) {
       done = true;
@@ -1184,22 +1398,26 @@ __global__ __launch_bounds__(64) void k_adj_mixed32(const SlotDesc *sd, int whic
   }
   if (!done) iter = kAdjMaxIter;
   double *TaP = sk0 + kSkTaP * SE;
-  for (size_t e = lane; e < SE; e += 64) TaP[e] = (double)taP[e];
-  if (lane == 0) {
-    if (!done && *D.status == kOk) *D.status = kWMaxIter;
+  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) TaP[e] = (double)taP[e];
+  if (threadIdx.x == 0) {
+    if (team_failed(T)) *D.status = kETeamWait;
+    else if (!done && *D.status == kOk) *D.status = kWMaxIter;
     D.iters[4 + which] = iter;
   }
 }
 
 // Mixed adjoint, FP64 polish: one iteration of the reference's update on the
 // FP64 coefficients, and the masked output.
-__global__ __launch_bounds__(64) void k_adj_polish(const SlotDesc *sd, int which) {
+template <bool TEAM>
+__global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_polish(const SlotDesc *sd, int which) {
+  extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (*D.status < 0) return;
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
+  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8, D.status);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
@@ -1208,9 +1426,12 @@ __global__ __launch_bounds__(64) void k_adj_polish(const SlotDesc *sd, int which
   C.is[1] = sk0 + kAF * SE;
   C.snap = nullptr;
   double s, m, a;
-  adj_iteration<double, false>(C, sg.ns, TaP, TaM, s, m, a);
-  adj_output(g, sg, lane, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
-  if (lane == 0) D.iters[1 + which] = 1;
+  adj_iteration<double, false>(C, T, TaP, TaM, s, m, a);
+  adj_output(g, sg, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
+  if (threadIdx.x == 0) {
+    if (team_failed(T)) *D.status = kETeamWait;
+    D.iters[1 + which] = 1;
+  }
 }
 
 // The item's own kernel, adj/svel**2 accumulated from zero
@@ -1256,16 +1477,41 @@ void launch_fwd_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, bo
     hipLaunchKernelGGL(k_fwd_setup<false>, skew_grid(gd), dim3(256), 0, s, sd);
 }
 
+namespace {
+// Block and LDS of a strip-kernel launch of the group (vbytes: the channel's
+// value size).
+dim3 team_block(const GroupDims &gd) { return dim3(64 * (unsigned)std::max(1, gd.waves)); }
+size_t team_lds(const GroupDims &gd, int vbytes) {
+  return team_lds_bytes(gd.waves, gd.max_ny, gd.max_ns, vbytes);
+}
+}  // namespace
+
+int team_waves(int want, int ny, int ns) {
+  int w = std::max(1, std::min({want, kMaxTeam, ns}));
+  while (w > 1 && team_lds_bytes(w, ny, ns, 8) > kTeamLdsMax) --w;
+  return w;
+}
+
 void launch_fwd_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
-  hipLaunchKernelGGL(k_fwd_strip, dim3(gd.count), dim3(64), 0, s, sd);
+  if (gd.waves > 1)
+    hipLaunchKernelGGL(k_fwd_strip<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd);
+  else
+    hipLaunchKernelGGL(k_fwd_strip<false>, dim3(gd.count), dim3(64), 0, s, sd);
 }
 
 void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
-  hipLaunchKernelGGL(k_fwd_mixed32, dim3(gd.count), dim3(64), 0, s, sd);
+  if (gd.waves > 1)
+    hipLaunchKernelGGL(k_fwd_mixed32<true>, dim3(gd.count), team_block(gd), team_lds(gd, 4), s,
+                       sd);
+  else
+    hipLaunchKernelGGL(k_fwd_mixed32<false>, dim3(gd.count), dim3(64), 0, s, sd);
 }
 
 void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
-  hipLaunchKernelGGL(k_fwd_polish, dim3(gd.count), dim3(64), 0, s, sd);
+  if (gd.waves > 1)
+    hipLaunchKernelGGL(k_fwd_polish<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd);
+  else
+    hipLaunchKernelGGL(k_fwd_polish<false>, dim3(gd.count), dim3(64), 0, s, sd);
 }
 
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
@@ -1281,15 +1527,27 @@ void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
 }
 
 void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
-  hipLaunchKernelGGL(k_adj_strip, dim3(gd.count), dim3(64), 0, s, sd, which);
+  if (gd.waves > 1)
+    hipLaunchKernelGGL(k_adj_strip<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd,
+                       which);
+  else
+    hipLaunchKernelGGL(k_adj_strip<false>, dim3(gd.count), dim3(64), 0, s, sd, which);
 }
 
 void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
-  hipLaunchKernelGGL(k_adj_mixed32, dim3(gd.count), dim3(64), 0, s, sd, which);
+  if (gd.waves > 1)
+    hipLaunchKernelGGL(k_adj_mixed32<true>, dim3(gd.count), team_block(gd), team_lds(gd, 4), s,
+                       sd, which);
+  else
+    hipLaunchKernelGGL(k_adj_mixed32<false>, dim3(gd.count), dim3(64), 0, s, sd, which);
 }
 
 void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
-  hipLaunchKernelGGL(k_adj_polish, dim3(gd.count), dim3(64), 0, s, sd, which);
+  if (gd.waves > 1)
+    hipLaunchKernelGGL(k_adj_polish<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd,
+                       which);
+  else
+    hipLaunchKernelGGL(k_adj_polish<false>, dim3(gd.count), dim3(64), 0, s, sd, which);
 }
 
 }  // namespace satt

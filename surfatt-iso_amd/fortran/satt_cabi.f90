@@ -11,7 +11,7 @@ module satt_cabi
   integer(c_int), parameter :: SATT_OK = 0, SATT_W_MAXITER = 1
   integer(c_int), parameter :: SATT_E_ARG = -1, SATT_E_SRC_ON_BOUNDARY = -2
   integer(c_int), parameter :: SATT_E_REC_OUT_OF_GRID = -3, SATT_E_NOMEM = -4
-  integer(c_int), parameter :: SATT_E_HIP = -5, SATT_E_NODEV = -6
+  integer(c_int), parameter :: SATT_E_HIP = -5, SATT_E_NODEV = -6, SATT_E_TEAM_WAIT = -7
   integer(c_int), parameter :: SATT_PREC_FP64 = 0, SATT_PREC_MIXED = 1
   integer(c_int), parameter :: SATT_LAUNCH_BATCHED = 0, SATT_LAUNCH_PER_SOLVE = 1
 

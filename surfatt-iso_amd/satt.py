@@ -21,6 +21,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "satt.h")
 
 OK, W_MAXITER = 0, 1
 E_ARG, E_SRC_ON_BOUNDARY, E_REC_OUT_OF_GRID, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
+E_TEAM_WAIT = -7
 PREC_FP64, PREC_MIXED = 0, 1
 LAUNCH_BATCHED, LAUNCH_PER_SOLVE = 0, 1
 FIELD_T, FIELD_TA, FIELD_TD = 0, 1, 2
@@ -123,6 +124,15 @@ class Context:
         if st != OK:
             raise SattError(st, "satt_create failed (no usable GPU?)")
         self.ptr = p
+
+    def tune_waves(self, nx, ny, nitems, precision=PREC_MIXED):
+        """Autotuner of the team size (waves per item) for an nx x ny grid and
+        a launch of nitems items; returns {"waves": W, "ms": [ms for W=1..4]}
+        and keeps W in the context's table."""
+        r = TuneResult()
+        self._check(self.lib.satt_tune_waves(self.ptr, int(nx), int(ny), int(nitems),
+                                             int(precision), ctypes.byref(r)))
+        return {"waves": r.waves, "ms": [float(v) for v in r.ms]}
 
     def close(self):
         if self.ptr:
@@ -276,6 +286,15 @@ class Plan:
         self.flags = (istotable, isadj)
         self.ctx._check(self.lib.satt_plan_run(self.ptr, int(istotable), int(isadj), int(launch)))
 
+    def set_waves(self, waves):
+        """Waves per item of the sweep kernels (capped per launch group)."""
+        self.ctx._check(self.lib.satt_plan_set_waves(self.ptr, int(waves)))
+
+    def set_graph(self, on=True):
+        """Replay form: batched runs capture their launch chain into a HIP
+        graph once per (istotable, isadj) and replay it afterwards."""
+        self.ctx._check(self.lib.satt_plan_set_graph(self.ptr, int(bool(on))))
+
     def wait(self):
         self.ctx._check(self.lib.satt_plan_wait(self.ptr))
 
@@ -320,9 +339,14 @@ class Plan:
 
 # ------------------------------------------------------------------ serving
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("waves", ctypes.c_int), ("ms", ctypes.c_double * 4)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("max_batch", ctypes.c_int),
-                ("streams", ctypes.c_int), ("device_bytes", ctypes.c_longlong)]
+                ("streams", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
+                ("waves", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -350,9 +374,11 @@ class Server:
     is a dict with nx, ny, xgrids, ygrids, svel (nx, ny), optional m11, m12,
     m22, srcx, srcy and receiver arrays recx, recy, tt, weight."""
 
-    def __init__(self, ctx, precision=PREC_FP64, max_batch=0, streams=0, device_bytes=0):
+    def __init__(self, ctx, precision=PREC_FP64, max_batch=0, streams=0, device_bytes=0,
+                 waves=0):
         self.ctx, self.lib = ctx, ctx.lib
-        cfg = ServerConfig(int(precision), int(max_batch), int(streams), int(device_bytes))
+        cfg = ServerConfig(int(precision), int(max_batch), int(streams), int(device_bytes),
+                           int(waves))
         p = ctypes.c_void_p()
         ctx._check(self.lib.satt_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(p)))
         self.ptr = p
