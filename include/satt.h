@@ -251,7 +251,7 @@ typedef struct satt_server satt_server;
 typedef struct {
   int precision;          /* SATT_PREC_FP64 or SATT_PREC_MIXED                 */
   int max_batch;          /* requests per launch group (0: 256)                */
-  int streams;            /* launch groups in flight (0: 4)                    */
+  int streams;            /* launch groups in flight (0: 6)                    */
   long long device_bytes; /* pool for requests in flight (0: half the free memory) */
   int waves;              /* waves per request in the sweep kernels, capped by
                              the grid's strips (0: the context's size tuned
