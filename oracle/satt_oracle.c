@@ -837,8 +837,8 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
  * One-sided candidates: u = (+-dis - tn*T0d)/p.  Causality tests, acceptance
  * rules, sweep order, tau init, frozen set and the stop rule (with the
  * residual |dtau|*T0 accumulated in double) are the reference's.  FP64
- * polish: tau = (double)tau32, then the reference's iterations (at least
- * one) with its stop rule; T = tau*T0.
+ * polish: the same upwind update in double on dtau = (double)dtau32, with
+ * the same stop rule (at least one iteration); T = (1 + dtau)*T0.
  *
  * Adjoint.  The face coefficients, d and delta are computed in FP64 as the
  * reference does.  FP32 phase: Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN
@@ -852,105 +852,116 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
 
 static const float MIX_REL_STOP = 4.76837158203125e-07f; /* 2^-21 */
 
-/* One cell of the FP32 forward sweep (see the note above) in the sweep
- * direction (xd, yd).  Only the candidates whose neighbours are upwind in
- * this sweep are evaluated: the quadrant of neighbours (i-xd, j) and
- * (i, j-yd), and the one-sided candidates from each of them.  Over the four
- * sweeps of an iteration every candidate of the reference's update is thus
- * evaluated once, with its neighbours' newest values; a fixed point is a
- * fixed point of the full update.  The field holds dtau = tau - 1: tau stays
- * near 1 wherever T0 is a good approximation, so the offset keeps 3-4 more
- * bits of tau than tau itself would in float. */
-static void fwd_cell32(int xd, int yd, int nx, int ny, int i, int j, float rdx,
-                       float rdy, const float *a, const float *b, const float *c,
-                       const float *f, const float *disx, const float *disy,
-                       const float *T0, const float *T0x, const float *T0y,
-                       float *dtau) {
-  const size_t k = IDX(i, j);
-  const float ak = a[k], bk = b[k], ck = c[k], fk = f[k];
-  const float t0 = T0[k], t0x = T0x[k], t0y = T0y[k];
-  const float t0dx = t0 * rdx, t0dy = t0 * rdy;
-  const int hasX = xd > 0 ? i > 0 : i < nx - 1;
-  const int hasY = yd > 0 ? j > 0 : j < ny - 1;
-  const size_t kX = hasX ? IDX(i - xd, j) : k, kY = hasY ? IDX(i, j - yd) : k;
-  const float dX = dtau[kX], dY = dtau[kY], zX = T0[kX], zY = T0[kY];
-  const float px = xd > 0 ? t0x + t0dx : t0x - t0dx;
-  const float py = yd > 0 ? t0y + t0dy : t0y - t0dy;
-  float best = dtau[k];
-  if (hasX && hasY) {
-    const float gx = t0x + t0x * dX;
-    const float gy = yd > 0 ? t0y + t0y * dX + t0dy * (dX - dY)
-                            : t0y + t0y * dX - t0dy * (dX - dY);
-    const float ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
-    const float eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));
-    const float ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;
-    const float disc = eb * eb - 4 * ea * ec;
-    if (disc >= 0) {
-      const float sq = sqrtf(disc);
-      const float qq = -0.5f * (eb >= 0 ? eb + sq : eb - sq);
-      for (int isol = 1; isol <= 2; ++isol) {
-        const float u = isol == 1 ? qq / ea : ec / qq;
-        const float dt = dX + u;
-        const float Tx = gx + px * u, Ty = gy + py * u;
-        const float chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;
-        const int ok = (xd > 0 ? chx >= 0 : chx <= 0) && (yd > 0 ? chy >= 0 : chy <= 0);
-        if (ok && dt < best) best = dt;
-      }
-    }
-  }
-  /* one-sided candidates (:996-1075): tt*T0 >= tn*T0n and tt > tn*0.5 in
-   * offsets; p is the quadrant's px (x neighbour) or py (y neighbour) */
-  for (int side = 0; side < 2; ++side) {
-    if (side == 0 ? !hasX : !hasY) continue;
-    const float p = side == 0 ? px : py, dis = side == 0 ? disx[k] : disy[k];
-    const float t0d = side == 0 ? t0x : t0y;
-    const float dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
-    const float rp = 1.0f / p, base = t0d + t0d * dn;
-    const float rhs = zn + dn * zn, half = 0.5f * dn - 0.5f;
-    float dt = dn + (dis - base) * rp;
-    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
-    dt = dn + (-dis - base) * rp;
-    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
-  }
-  dtau[k] = best;
-}
+/* One cell of the upwind forward sweep of the mixed mode (see the note
+ * above) in the sweep direction (xd, yd), in float (FP32 phase) or double
+ * (FP64 polish): the two are one text, instantiated per type below.  Only
+ * the candidates whose neighbours are upwind in this sweep are evaluated:
+ * the quadrant of neighbours (i-xd, j) and (i, j-yd), and the one-sided
+ * candidates from each of them.  Over the four sweeps of an iteration every
+ * candidate of the reference's update is thus evaluated once, with its
+ * neighbours' newest values; a fixed point is a fixed point of the full
+ * update.  The field holds dtau = tau - 1: tau stays near 1 wherever T0 is a
+ * good approximation, so the offset keeps 3-4 more bits of tau than tau
+ * itself would in float.  The iterations follow; they return the iteration
+ * count (FWD_MAXITER when the stop rule never held) and tell in *conv
+ * whether it held. */
+#define DEFINE_FWD_UP(SUF, V, SQRT) \
+static void fwd_cell_up_##SUF(int xd, int yd, int nx, int ny, int i, int j, V rdx,             \
+                       V rdy, const V *a, const V *b, const V *c,                              \
+                       const V *f, const V *disx, const V *disy,                               \
+                       const V *T0, const V *T0x, const V *T0y,                                \
+                       V *dtau) {                                                              \
+  const size_t k = IDX(i, j);                                                                  \
+  const V ak = a[k], bk = b[k], ck = c[k], fk = f[k];                                          \
+  const V t0 = T0[k], t0x = T0x[k], t0y = T0y[k];                                              \
+  const V t0dx = t0 * rdx, t0dy = t0 * rdy;                                                    \
+  const int hasX = xd > 0 ? i > 0 : i < nx - 1;                                                \
+  const int hasY = yd > 0 ? j > 0 : j < ny - 1;                                                \
+  const size_t kX = hasX ? IDX(i - xd, j) : k, kY = hasY ? IDX(i, j - yd) : k;                 \
+  const V dX = dtau[kX], dY = dtau[kY], zX = T0[kX], zY = T0[kY];                              \
+  const V px = xd > 0 ? t0x + t0dx : t0x - t0dx;                                               \
+  const V py = yd > 0 ? t0y + t0dy : t0y - t0dy;                                               \
+  V best = dtau[k];                                                                            \
+  if (hasX && hasY) {                                                                          \
+    const V gx = t0x + t0x * dX;                                                               \
+    const V gy = yd > 0 ? t0y + t0y * dX + t0dy * (dX - dY)                                    \
+                            : t0y + t0y * dX - t0dy * (dX - dY);                               \
+    const V ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;                           \
+    const V eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));                 \
+    const V ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;                 \
+    const V disc = eb * eb - 4 * ea * ec;                                                      \
+    if (disc >= 0) {                                                                           \
+      const V sq = SQRT(disc);                                                                 \
+      const V qq = (V)-0.5 * (eb >= 0 ? eb + sq : eb - sq);                                    \
+      for (int isol = 1; isol <= 2; ++isol) {                                                  \
+        const V u = isol == 1 ? qq / ea : ec / qq;                                             \
+        const V dt = dX + u;                                                                   \
+        const V Tx = gx + px * u, Ty = gy + py * u;                                            \
+        const V chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;                              \
+        const int ok = (xd > 0 ? chx >= 0 : chx <= 0) && (yd > 0 ? chy >= 0 : chy <= 0);       \
+        if (ok && dt < best) best = dt;                                                        \
+      }                                                                                        \
+    }                                                                                          \
+  }                                                                                            \
+  /* one-sided candidates (:996-1075): tt*T0 >= tn*T0n and tt > tn*0.5 in                      \
+   * offsets; p is the quadrant's px (x neighbour) or py (y neighbour) */                      \
+  for (int side = 0; side < 2; ++side) {                                                       \
+    if (side == 0 ? !hasX : !hasY) continue;                                                   \
+    const V p = side == 0 ? px : py, dis = side == 0 ? disx[k] : disy[k];                      \
+    const V t0d = side == 0 ? t0x : t0y;                                                       \
+    const V dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;                                \
+    const V rp = (V)1 / p, base = t0d + t0d * dn;                                              \
+    const V rhs = zn + dn * zn, half = (V)0.5 * dn - (V)0.5;                                   \
+    V dt = dn + (dis - base) * rp;                                                             \
+    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;                              \
+    dt = dn + (-dis - base) * rp;                                                              \
+    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;                              \
+  }                                                                                            \
+  dtau[k] = best;                                                                              \
+}                                                                                              \
+                                                                                               \
+static int fwd_iterate_up_##SUF(int nx, int ny, V rdx, V rdy, const V *fld,                    \
+                                const unsigned char *ischange, V *tau, V *old,                 \
+                                int *conv) {                                                   \
+  const size_t n = (size_t)nx * ny;                                                            \
+  const V *a = fld, *b = fld + n, *c = fld + 2 * n, *f = fld + 3 * n,                          \
+          *disx = fld + 4 * n, *disy = fld + 5 * n, *T0 = fld + 6 * n,                         \
+          *T0x = fld + 7 * n, *T0y = fld + 8 * n;                                              \
+  for (int iter = 1; iter <= FWD_MAXITER; ++iter) {                                            \
+    memcpy(old, tau, n * sizeof(V));                                                           \
+    for (int isw = 1; isw <= 4; ++isw) {                                                       \
+      const int xd = isw <= 2 ? 1 : -1;                                                        \
+      const int yd = (isw == 1 || isw == 3) ? 1 : -1;                                          \
+      for (int ii = 0; ii < nx; ++ii) {                                                        \
+        const int i = xd > 0 ? ii : nx - 1 - ii;                                               \
+        for (int jj = 0; jj < ny; ++jj) {                                                      \
+          const int j = yd > 0 ? jj : ny - 1 - jj;                                             \
+          if (ischange[IDX(i, j)])                                                             \
+            fwd_cell_up_##SUF(xd, yd, nx, ny, i, j, rdx, rdy, a, b, c, f, disx, disy, T0, T0x, \
+                              T0y, tau);                                                       \
+        }                                                                                      \
+      }                                                                                        \
+    }                                                                                          \
+    double l1 = 0, linf = 0;                                                                   \
+    for (int i = 0; i < nx; ++i)                                                               \
+      for (int j = 0; j < ny; ++j) {                                                           \
+        const size_t k = IDX(i, j);                                                            \
+        const double d = fabs((double)tau[k] - (double)old[k]) * (double)T0[k];                \
+        l1 = l1 + d;                                                                           \
+        linf = linf > d ? linf : d;                                                            \
+      }                                                                                        \
+    l1 = l1 / (double)nx / (double)ny;                                                         \
+    if (fabs(l1) < FWD_TOL && linf < FWD_TOL) {                                                \
+      *conv = 1;                                                                               \
+      return iter;                                                                             \
+    }                                                                                          \
+  }                                                                                            \
+  *conv = 0;                                                                                   \
+  return FWD_MAXITER;                                                                          \
+}                                                                                              \
 
-/* FP32 phase iterations; returns the iteration count (FWD_MAXITER when the
- * stop rule never held). */
-static int fwd_iterate32(int nx, int ny, float rdx, float rdy, const float *f32,
-                         const unsigned char *ischange, float *tau, float *old) {
-  const size_t n = (size_t)nx * ny;
-  const float *a = f32, *b = f32 + n, *c = f32 + 2 * n, *f = f32 + 3 * n,
-              *disx = f32 + 4 * n, *disy = f32 + 5 * n, *T0 = f32 + 6 * n,
-              *T0x = f32 + 7 * n, *T0y = f32 + 8 * n;
-  for (int iter = 1; iter <= FWD_MAXITER; ++iter) {
-    memcpy(old, tau, n * sizeof(float));
-    for (int isw = 1; isw <= 4; ++isw) {
-      const int xd = isw <= 2 ? 1 : -1;
-      const int yd = (isw == 1 || isw == 3) ? 1 : -1;
-      for (int ii = 0; ii < nx; ++ii) {
-        const int i = xd > 0 ? ii : nx - 1 - ii;
-        for (int jj = 0; jj < ny; ++jj) {
-          const int j = yd > 0 ? jj : ny - 1 - jj;
-          if (ischange[IDX(i, j)])
-            fwd_cell32(xd, yd, nx, ny, i, j, rdx, rdy, a, b, c, f, disx, disy, T0, T0x, T0y,
-                       tau);
-        }
-      }
-    }
-    double l1 = 0, linf = 0;
-    for (int i = 0; i < nx; ++i)
-      for (int j = 0; j < ny; ++j) {
-        const size_t k = IDX(i, j);
-        const double d = fabs((double)tau[k] - (double)old[k]) * (double)T0[k];
-        l1 = l1 + d;
-        linf = linf > d ? linf : d;
-      }
-    l1 = l1 / (double)nx / (double)ny;
-    if (fabs(l1) < FWD_TOL && linf < FWD_TOL) return iter;
-  }
-  return FWD_MAXITER;
-}
+DEFINE_FWD_UP(f32, float, sqrtf)
+DEFINE_FWD_UP(f64, double, sqrt)
 
 static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
                             int ny, const double *spha, const double *sphb,
@@ -1009,9 +1020,32 @@ static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
       f32[8 * n + k] = (float)T0y[k];
       tau32[k] = (float)(tau[k] - 1.0);
     }
-    it32 = fwd_iterate32(nx, ny, (float)(1.0 / dx), (float)(1.0 / dy), f32, isc, tau32, old32);
-    for (size_t k = 0; k < n; ++k) tau[k] = 1.0 + (double)tau32[k];
-    st = fwd_iterate(nx, ny, dx, dy, a, b, c, fun, T0, T0x, T0y, tau, isc, tau_old, T, &it64);
+    int c32 = 0, c64 = 0;
+    it32 = fwd_iterate_up_f32(nx, ny, (float)(1.0 / dx), (float)(1.0 / dy), f32, isc, tau32,
+                              old32, &c32);
+    /* polish on dtau in double: the same fields in FP64 */
+    double *f64 = malloc(sizeof(double) * n * 9);
+    if (!f64) {
+      st = ORC_E_NOMEM;
+    } else {
+      for (size_t k = 0; k < n; ++k) {
+        const double ak = a[k], bk = b[k], ck = c[k], fk = fun[k];
+        f64[k] = ak;
+        f64[n + k] = bk;
+        f64[2 * n + k] = ck;
+        f64[3 * n + k] = fk;
+        f64[4 * n + k] = sqrt(fk * fk * (bk / (ak * bk - ck * ck)));
+        f64[5 * n + k] = sqrt(fk * fk * (ak / (ak * bk - ck * ck)));
+        f64[6 * n + k] = T0[k];
+        f64[7 * n + k] = T0x[k];
+        f64[8 * n + k] = T0y[k];
+        tau[k] = (double)tau32[k];
+      }
+      it64 = fwd_iterate_up_f64(nx, ny, 1.0 / dx, 1.0 / dy, f64, isc, tau, tau_old, &c64);
+      for (size_t k = 0; k < n; ++k) T[k] = (1.0 + tau[k]) * T0[k];
+      st = c64 ? ORC_OK : ORC_W_MAXITER;
+      free(f64);
+    }
   }
   if (iters) { iters[0] = it32; iters[1] = it64; }
   free(xx); free(yy); free(buf); free(f32); free(isc);

@@ -48,8 +48,10 @@
 //   q = -(eb + sign(eb)*sqrt(disc))/2, u = q/ea and ec/q.  The field holds
 //   dtau = tau - 1, which keeps 3-4 more bits than tau near 1.  Causality
 //   tests, acceptance, sweep order and the stop rule (residual accumulated in
-//   double) are the reference's.  Polish: tau = 1 + dtau, then the
-//   reference's FP64 iterations to its stop rule.
+//   double) are the reference's.  Polish: the same upwind update in FP64 on
+//   dtau widened to double, iterated to the reference's stop rule (at least
+//   once); its fixed point is that of the reference's full update, and
+//   T = (1 + dtau)*T0.
 //   Adjoint FP32 phase: Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN with
 //   k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d
 //   formed in FP64 and rounded to float; stop when the reference's rule
@@ -64,6 +66,16 @@
 
 namespace satt {
 namespace {
+
+// Address-space-qualified pointers for the hot loops: fields in device
+// memory (global loads and stores rather than flat ones, which also count
+// against the LDS/scalar wait counter) and the teams' LDS channels.
+#define GMEM __attribute__((address_space(1)))
+#define LMEM __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ GMEM T *gm(T *p) { return (GMEM T *)p; }
+template <typename T>
+__device__ __forceinline__ const GMEM T *gm(const T *p) { return (const GMEM T *)p; }
 
 constexpr float kMixRelStop = 4.76837158203125e-07f;  // 2^-21
 
@@ -209,42 +221,42 @@ __device__ __forceinline__ double fwd_cell_r(int i, int j, int nx, int ny, doubl
   return tnew;
 }
 
-// One cell of the FP32 phase in the sweep direction (XD, YD) (see the
-// precision note at the top).  Only the candidates whose neighbours are
-// upwind in this sweep are evaluated: the quadrant of the neighbours
-// (i-XD, j) and (i, j-YD), and the one-sided candidates from each.  Over the
-// four sweeps of an iteration every candidate of the reference's update is
-// evaluated once with its neighbours' newest values, and a fixed point is a
-// fixed point of the full update.  Operands are offsets dtau = tau - 1; zX, zY
-// are the upwind neighbours' T0.
-template <int XD, int YD>
-__device__ __forceinline__ float fwd_cell32_r(int i, int j, int nx, int ny, float rdx, float rdy,
-                                              float ak, float bk, float ck, float fk, float disx,
-                                              float disy, float t0, float t0x, float t0y,
-                                              float dc, float dX, float dY, float zX, float zY) {
-  const float t0dx = t0 * rdx, t0dy = t0 * rdy;
+// One cell of the mixed mode's upwind update (FP32 phase, V = float; FP64
+// polish, V = double) in the sweep direction (XD, YD) (see the precision note
+// at the top).  Only the candidates whose neighbours are upwind in this sweep
+// are evaluated: the quadrant of the neighbours (i-XD, j) and (i, j-YD), and
+// the one-sided candidates from each.  Over the four sweeps of an iteration
+// every candidate of the reference's update is evaluated once with its
+// neighbours' newest values, and a fixed point is a fixed point of the full
+// update.  Operands are offsets dtau = tau - 1; zX, zY are the upwind
+// neighbours' T0.
+template <typename V, int XD, int YD>
+__device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, V rdy, V ak, V bk,
+                                           V ck, V fk, V disx, V disy, V t0, V t0x, V t0y, V dc,
+                                           V dX, V dY, V zX, V zY) {
+  const V t0dx = t0 * rdx, t0dy = t0 * rdy;
   const bool hasX = XD > 0 ? i > 0 : i < nx - 1;
   const bool hasY = YD > 0 ? j > 0 : j < ny - 1;
-  const float px = XD > 0 ? t0x + t0dx : t0x - t0dx;
-  const float py = YD > 0 ? t0y + t0dy : t0y - t0dy;
-  float best = dc;
+  const V px = XD > 0 ? t0x + t0dx : t0x - t0dx;
+  const V py = YD > 0 ? t0y + t0dy : t0y - t0dy;
+  V best = dc;
   if (hasX && hasY) {
-    const float gx = t0x + t0x * dX;
-    const float gy = YD > 0 ? t0y + t0y * dX + t0dy * (dX - dY)
+    const V gx = t0x + t0x * dX;
+    const V gy = YD > 0 ? t0y + t0y * dX + t0dy * (dX - dY)
                             : t0y + t0y * dX - t0dy * (dX - dY);
-    const float ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
-    const float eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));
-    const float ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;
-    const float disc = eb * eb - 4 * ea * ec;
+    const V ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
+    const V eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));
+    const V ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;
+    const V disc = eb * eb - 4 * ea * ec;
     if (disc >= 0) {
-      const float sq = sqrtf(disc);
-      const float qq = -0.5f * (eb >= 0 ? eb + sq : eb - sq);
+      const V sq = sqrt(disc);
+      const V qq = V(-0.5) * (eb >= 0 ? eb + sq : eb - sq);
 #pragma unroll
       for (int isol = 1; isol <= 2; ++isol) {
-        const float u = isol == 1 ? qq / ea : ec / qq;
-        const float dt = dX + u;
-        const float Tx = gx + px * u, Ty = gy + py * u;
-        const float chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;
+        const V u = isol == 1 ? qq / ea : ec / qq;
+        const V dt = dX + u;
+        const V Tx = gx + px * u, Ty = gy + py * u;
+        const V chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;
         const bool ok = (XD > 0 ? chx >= 0 : chx <= 0) && (YD > 0 ? chy >= 0 : chy <= 0);
         if (ok && dt < best) best = dt;
       }
@@ -255,12 +267,12 @@ __device__ __forceinline__ float fwd_cell32_r(int i, int j, int nx, int ny, floa
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     if (side == 0 ? !hasX : !hasY) continue;
-    const float p = side == 0 ? px : py, dis = side == 0 ? disx : disy;
-    const float t0d = side == 0 ? t0x : t0y;
-    const float dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
-    const float rp = 1.0f / p, base = t0d + t0d * dn;
-    const float rhs = zn + dn * zn, half = 0.5f * dn - 0.5f;
-    float dt = dn + (dis - base) * rp;
+    const V p = side == 0 ? px : py, dis = side == 0 ? disx : disy;
+    const V t0d = side == 0 ? t0x : t0y;
+    const V dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
+    const V rp = V(1) / p, base = t0d + t0d * dn;
+    const V rhs = zn + dn * zn, half = V(0.5) * dn - V(0.5);
+    V dt = dn + (dis - base) * rp;
     if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
     dt = dn + (-dis - base) * rp;
     if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
@@ -374,22 +386,27 @@ __device__ __forceinline__ size_t skpos(int sk, int i, int j, int R) {
 
 // The waves of one workgroup working on one item (a team of W waves).  The
 // k-th strip of a sweep goes to wave k mod W; the edge column a strip hands
-// to the next strip passes through LDS, with the count of rows published so
-// far, so that the next wave runs its strip as soon as the rows it needs are
-// out (at least L steps behind).  Each wave publishes only into its own
-// channel, and a channel row is overwritten (by the wave's next strip) only
-// after the consumer has read it, because that next strip runs behind the
-// consumer's strip.  W = 1: one wave does every strip in order; no LDS.
+// to the next strip (its final values, and their T0 in the forward) passes
+// through LDS, with the count of rows published so far, so that the next
+// wave runs its strip as soon as the rows it needs are out (at least L steps
+// behind).  Each wave publishes only into its own channel, and a channel row
+// is overwritten (by the wave's next strip) only after the consumer has read
+// it, because that next strip runs behind the consumer's strip.  W = 1: one
+// wave does every strip in order and the channel needs no counts: the
+// previous strip has finished, and the strip's own edge lane overwrites a
+// row after the strip's first lane has read it.  Either way the edge column
+// comes from LDS instead of one scattered global load per step.
 // A wait is bounded: past the bound the item gets kETeamWait and every later
 // wait of the team returns at once, so the kernel always ends.
 struct Team {
   int W, w;           // waves, this wave (wave-uniform)
   int ny, ns;
   int sweep;          // sweeps started (the count set in use is sweep & 1)
-  void *chan;         // [W][ny] edge values of the current strips (V)
-  volatile int *cnt;  // [2][W] rows published by each wave in the sweep
-  volatile int *stop; // set when a wait ran out
-  double *part;       // [ns][3] per-strip residual terms
+  LMEM char *chan;    // [W][ny] edge values of the current strips (V)
+  LMEM char *chan0;   // [W][ny] their T0 (forward; V)
+  volatile LMEM int *cnt;   // [2][W] rows published by each wave in the sweep
+  volatile LMEM int *stop;  // set when a wait ran out
+  LMEM double *part;        // [ns][3] per-strip residual terms
   int *status;        // the item's status
   __device__ bool multi() const { return W > 1; }
 };
@@ -398,8 +415,8 @@ constexpr int kTeamSpinLimit = 1 << 22;  // ~0.5 s of s_sleep per wait
 
 // LDS bytes of a team of W waves on grids up to ny rows and ns strips.
 __host__ __device__ inline size_t team_lds_bytes(int W, int ny, int ns, int vbytes) {
-  if (W <= 1) return 0;
-  return (size_t)ns * 3 * 8 + (size_t)W * ny * vbytes + (size_t)(2 * W + 2) * 4;
+  if (W <= 1) return (size_t)2 * ny * vbytes;  // the two channels only
+  return (size_t)ns * 3 * 8 + (size_t)2 * W * ny * vbytes + (size_t)(2 * W + 2) * 4;
 }
 
 // TEAM false: the one-wave form, with W = 1 known at compile time so that
@@ -413,10 +430,12 @@ __device__ inline Team team_init(void *lds, int ny, int ns, int vbytes, int *sta
   T.ns = ns;
   T.sweep = 0;
   T.status = status;
-  T.part = static_cast<double *>(lds);
-  char *b = static_cast<char *>(lds) + (size_t)ns * 3 * 8;
+  LMEM char *base = (LMEM char *)lds;
+  T.part = (LMEM double *)base;
+  LMEM char *b = base + (TEAM ? (size_t)ns * 3 * 8 : 0);
   T.chan = b;
-  T.cnt = reinterpret_cast<volatile int *>(b + (size_t)T.W * ny * vbytes);
+  T.chan0 = b + (size_t)T.W * ny * vbytes;
+  T.cnt = (volatile LMEM int *)(b + (size_t)2 * T.W * ny * vbytes);
   T.stop = T.cnt + 2 * T.W;
   if (TEAM) {
     if (threadIdx.x < 2 * (unsigned)T.W) T.cnt[threadIdx.x] = 0;
@@ -442,7 +461,7 @@ __device__ inline bool team_failed(const Team &T) { return T.multi() && *T.stop;
 
 // Wait (one lane) until wave pw has published `need` rows in this sweep.
 __device__ inline void team_wait(const Team &T, int pw, int need) {
-  volatile int *c = T.cnt + (T.sweep & 1) * T.W + pw;
+  volatile LMEM int *c = T.cnt + (T.sweep & 1) * T.W + pw;
   int spins = 0;
   while (*c < need && !*T.stop) {
     __builtin_amdgcn_s_sleep(2);
@@ -455,16 +474,26 @@ __device__ inline void team_wait(const Team &T, int pw, int need) {
 
 template <typename V>
 __device__ inline V team_read(const Team &T, int pw, int u) {
-  return static_cast<volatile V *>(T.chan)[(size_t)pw * T.ny + u];
+  return ((volatile LMEM V *)T.chan)[pw * T.ny + u];
+}
+template <typename V>
+__device__ inline V team_read0(const Team &T, int pw, int u) {
+  return ((volatile LMEM V *)T.chan0)[pw * T.ny + u];
 }
 
-// Publish (one lane) the edge value of row u, the `count`-th row of this
-// wave's strips in the sweep: the value first, then the count (LDS
-// operations of a wave complete in order).
+// Publish (one lane) the edge value of row u (and, forward, its T0), the
+// `count`-th row of this wave's strips in the sweep: the values first, then
+// the count (LDS operations of a wave complete in order).
 template <typename V>
 __device__ inline void team_publish(const Team &T, int u, V v, int count) {
-  static_cast<volatile V *>(T.chan)[(size_t)T.w * T.ny + u] = v;
-  T.cnt[(T.sweep & 1) * T.W + T.w] = count;
+  ((volatile LMEM V *)T.chan)[T.w * T.ny + u] = v;
+  if (T.multi()) T.cnt[(T.sweep & 1) * T.W + T.w] = count;
+}
+template <typename V>
+__device__ inline void team_publish2(const Team &T, int u, V v, V z, int count) {
+  ((volatile LMEM V *)T.chan)[T.w * T.ny + u] = v;
+  ((volatile LMEM V *)T.chan0)[T.w * T.ny + u] = z;
+  if (T.multi()) T.cnt[(T.sweep & 1) * T.W + T.w] = count;
 }
 
 // Reduction of a sweep's per-strip residual terms (sum, max, max) in the
@@ -504,7 +533,7 @@ __device__ inline StripAcc team_strip_total(const Team &T, const StripAcc &acc) 
 // lanes on the following rows, so it stays in cache while it is used.
 // Padding elements of the destination are left as they are.
 template <typename V, int FROM>
-__device__ void skew_copy(const Team &T, const V *__restrict__ src, V *__restrict__ dst, int nx,
+__device__ void skew_copy(const Team &T, const GMEM V *__restrict__ src, GMEM V *__restrict__ dst, int nx,
                           int ny, int R, int ns, int lane) {
   constexpr int TO = 1 - FROM;
   constexpr int kU = 8;
@@ -568,26 +597,27 @@ struct FwdCtx {
   int nx, ny, R, lane;
   size_t SE;
   double dx, dy;           // FP64 update
-  float rdx, rdy;          // FP32 update: 1/dx, 1/dy rounded to float
+  V rdx, rdy;              // upwind update: 1/dx, 1/dy in V
   int fi0, fi1, fj0, fj1;  // frozen rectangle
-  const V *ps[2];          // period skews (plus, minus), kPF fields each
-  const V *is[2];          // item T0 skews (plus, minus), kFF fields each
-  V *snap;                 // plus skew
+  const GMEM V *ps[2];     // period skews (plus, minus), kPF fields each
+  const GMEM V *is[2];     // item T0 skews (plus, minus), kFF fields each
+  GMEM V *snap;            // plus skew
 };
 
 
 // One sweep (XD, YD) over the k-th strip of the sweep, updating tau in place
-// in the sweep's own skew.  MODE 0: plain; 1: record the snapshot (first
-// sweep of an iteration); 2: the strip's sum and max of |tau-snap|*T0 in
-// double (last sweep) into ss, sm.
-template <typename V, int XD, int YD, int MODE>
-__device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, V *tau, double &ss,
+// in the sweep's own skew: the reference's update (UP false, FP64) or the
+// mixed mode's upwind update on dtau (UP true).  MODE 0: plain; 1: record
+// the snapshot (first sweep of an iteration); 2: the strip's sum and max of
+// |tau-snap|*T0 in double (last sweep) into ss, sm.
+template <typename V, int XD, int YD, int MODE, bool UP>
+__device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V *tau, double &ss,
                                double &sm) {
   constexpr int SK = XD == YD ? 0 : 1;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  const V *ps = C.ps[SK], *is = C.is[SK];
+  const GMEM V *ps = C.ps[SK], *is = C.is[SK];
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -600,15 +630,15 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, V *tau,
   const bool dn_edge = XD > 0 ? lane == L - 1 : lane == 0;
   const int iu = i - XD, id = i + XD;
   const bool up_ok = lv && up_edge && iu >= 0 && iu < nx;
-  // the FP32 update reads upwind neighbours only
-  constexpr bool kDown = std::is_same<V, double>::value;
+  // the upwind update reads upwind neighbours only
+  constexpr bool kDown = !UP;
+  static_assert(UP || std::is_same<V, double>::value, "the full update is FP64");
   const bool dn_ok = kDown && lv && dn_edge && id >= 0 && id < nx;
-  // team: the upstream strip's edge column comes from the previous wave's
-  // channel (rows counted over that wave's strips of the sweep); the lane
-  // next to the downstream strip publishes this strip's
-  const bool from_chan = T.multi() && k > 0;
+  // the upstream strip's edge column comes from the previous wave's channel
+  // (rows counted over that wave's strips of the sweep); the lane next to
+  // the downstream strip publishes this strip's
   const int pw = (T.w + T.W - 1) % T.W, base_in = ((k - 1) / T.W) * ny;
-  const bool pub = T.multi() && k < T.ns - 1 && lag == L - 1;
+  const bool pub = k < T.ns - 1 && lag == L - 1;
   const int base_out = (k / T.W) * ny;
   double acc_s = 0, acc_m = 0;
 
@@ -621,14 +651,10 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, V *tau,
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       q.t0 = is[e];
       q.tau = tau[e];
-      if (up_ok) {
-        if (from_chan) {
-          team_wait(T, pw, base_in + u + 1);
-          q.tup = team_read<V>(T, pw, u);
-        } else {
-          q.tup = tau[skpos(SK, iu, j, R)];
-        }
-        q.zup = is[skpos(SK, iu, j, R)];
+      if (up_ok) {  // k > 0: the upstream strip's edge
+        if (T.multi()) team_wait(T, pw, base_in + u + 1);
+        q.tup = team_read<V>(T, pw, u);
+        q.zup = team_read0<V>(T, pw, u);
       }
       if (kDown && dn_ok) {
         q.tdn = tau[skpos(SK, id, j, R)];
@@ -680,13 +706,13 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, V *tau,
                             fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
       } else {
         if (!frozen)
-          tnew = fwd_cell32_r<XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
+          tnew = fwd_cell_up_r<V, XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
                                       nc.dx_, nc.dy_, fc.t0, nc.t0x, nc.t0y, fc.tau, tUx, tprev,
                                       zUx, zprev);
       }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       tau[e] = tnew;
-      if (pub) team_publish<V>(T, t - lag, tnew, base_out + t - lag + 1);
+      if (pub) team_publish2<V>(T, t - lag, tnew, fc.t0, base_out + t - lag + 1);
       if (MODE == 1) C.snap[e] = fc.tau;
       if (MODE == 2) {
         const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
@@ -707,12 +733,12 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, V *tau,
   wave_fence();
 }
 
-template <typename V, int XD, int YD, int MODE>
-__device__ void fwd_sweep(const FwdCtx<V> &C, Team &T, V *tau, StripAcc &acc) {
+template <typename V, int XD, int YD, int MODE, bool UP>
+__device__ void fwd_sweep(const FwdCtx<V> &C, Team &T, GMEM V *tau, StripAcc &acc) {
   team_sweep_begin(T);
   for (int k = T.w; k < T.ns; k += T.W) {
     double ss = 0, sm = 0;
-    fwd_strip_pass<V, XD, YD, MODE>(C, T, k, tau, ss, sm);
+    fwd_strip_pass<V, XD, YD, MODE, UP>(C, T, k, tau, ss, sm);
     if (MODE == 2) team_strip_term(T, k, ss, sm, 0.0, acc);
   }
   team_sweep_end(T);
@@ -723,15 +749,15 @@ __device__ void fwd_sweep(const FwdCtx<V> &C, Team &T, V *tau, StripAcc &acc) {
 // the first sweep in the plus skew, the middle two in the minus skew, the
 // last in the plus skew, with the field copied between the skews at the
 // two family changes.  Returns whether the stop rule (:1091-1115) holds.
-template <typename V>
-__device__ bool fwd_iteration(const FwdCtx<V> &C, Team &T, V *tP, V *tM) {
+template <typename V, bool UP>
+__device__ bool fwd_iteration(const FwdCtx<V> &C, Team &T, GMEM V *tP, GMEM V *tM) {
   StripAcc acc;
-  fwd_sweep<V, 1, 1, 1>(C, T, tP, acc);
+  fwd_sweep<V, 1, 1, 1, UP>(C, T, tP, acc);
   skew_copy<V, 0>(T, tP, tM, C.nx, C.ny, C.R, T.ns, C.lane);
-  fwd_sweep<V, 1, -1, 0>(C, T, tM, acc);
-  fwd_sweep<V, -1, 1, 0>(C, T, tM, acc);
+  fwd_sweep<V, 1, -1, 0, UP>(C, T, tM, acc);
+  fwd_sweep<V, -1, 1, 0, UP>(C, T, tM, acc);
   skew_copy<V, 1>(T, tM, tP, C.nx, C.ny, C.R, T.ns, C.lane);
-  fwd_sweep<V, -1, -1, 2>(C, T, tP, acc);
+  fwd_sweep<V, -1, -1, 2, UP>(C, T, tP, acc);
   const StripAcc r = team_strip_total(T, acc);
   const double l1 = r.s / (double)C.nx / (double)C.ny;
   return fabs(l1) < kFwdTol && r.m < kFwdTol;
@@ -815,7 +841,7 @@ __global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd) {
   const int sk = blockIdx.y;
   if (e >= SE || *D.status < 0) return;
   const int nx = g.nx, ny = g.ny;
-  const double *aux = D.aux;
+  const GMEM double *aux = gm(D.aux);
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % sg.R);
   const int cc = (int)((e >> 6) / sg.R);
@@ -825,17 +851,18 @@ __global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd) {
   double t0 = 0, t0x = 0, t0y = 0;
   bool frz = false;
   if (valid) {
-    const double *tail = aux + 3 * nx + ny;
-    t0_cell_tab(g.sy[j], g.cy[j], aux[i], aux[nx + i], tail[1], tail[2], tail[0], t0, t0x, t0y);
+    const GMEM double *tail = aux + 3 * nx + ny;
+    t0_cell_tab(gm(g.sy)[j], gm(g.cy)[j], aux[i], aux[nx + i], tail[1], tail[2], tail[0], t0,
+                t0x, t0y);
     frz = aux[2 * nx + i] != 0.0 && aux[3 * nx + j] != 0.0;
     if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) *D.status = kESrcOnBoundary;
   }
-  double *sk0 = D.skew;
-  double *b0 = sk0 + (size_t)sk * kFF * SE;
+  GMEM double *sk0 = gm(D.skew);
+  GMEM double *b0 = sk0 + (size_t)sk * kFF * SE;
   b0[e] = t0; b0[SE + e] = t0x; b0[2 * SE + e] = t0y;
   if (MIXED) {
-    float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
-    float *b32 = f32 + (size_t)sk * kFF * SE;
+    GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkF32 * SE);
+    GMEM float *b32 = f32 + (size_t)sk * kFF * SE;
     b32[e] = (float)t0; b32[SE + e] = (float)t0x; b32[2 * SE + e] = (float)t0y;
     if (sk == 0) f32[6 * SE + e] = frz ? 0.0f : 9.0f;
   } else if (sk == 0) {
@@ -848,12 +875,15 @@ __device__ void fwd_ctx_init(FwdCtx<V> &C, const Grid &g, const SkewGeom &sg, in
                              const int rect[4]) {
   C.nx = g.nx; C.ny = g.ny; C.R = sg.R; C.lane = lane; C.SE = sg.skew_elems();
   C.dx = g.dx; C.dy = g.dy;
-  C.rdx = (float)(1.0 / g.dx); C.rdy = (float)(1.0 / g.dy);
+  C.rdx = (V)(1.0 / g.dx); C.rdy = (V)(1.0 / g.dy);
   C.fi0 = rect[0]; C.fi1 = rect[1]; C.fj0 = rect[2]; C.fj1 = rect[3];
 }
 
-// FP64 iterations from the tau in the plus skew, then T = tau*T0
-// (src/eikon_solver_2d.f90:1124) in both skews and in natural layout.
+// FP64 iterations from the field in the plus skew to the stop rule, then
+// T = tau*T0 (src/eikon_solver_2d.f90:1124) in both skews and in natural
+// layout: the reference's update on tau (UP false), or the mixed mode's
+// upwind update on dtau, with T = (1 + dtau)*T0 (UP true).
+template <bool UP>
 __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rect[4]) {
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
@@ -864,24 +894,25 @@ __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rec
   double *TP = sk0 + kSkTP * SE, *TM = sk0 + kSkTM * SE;
   FwdCtx<double> C;
   fwd_ctx_init(C, g, sg, lane, rect);
-  C.ps[0] = D.pskew;
-  C.ps[1] = D.pskew + (size_t)kPF * SE;
-  C.is[0] = sk0;
-  C.is[1] = sk0 + kFF * SE;
-  C.snap = sk0 + kSkSnapF * SE;
+  C.ps[0] = gm(D.pskew);
+  C.ps[1] = gm(D.pskew + (size_t)kPF * SE);
+  C.is[0] = gm(sk0);
+  C.is[1] = gm(sk0 + kFF * SE);
+  C.snap = gm(sk0 + kSkSnapF * SE);
   int iter = 1;
   bool conv = false;
   for (; iter <= kFwdMaxIter; ++iter)
-    if (fwd_iteration(C, T, tauP, tauM)) { conv = true; break; }
+    if (fwd_iteration<double, UP>(C, T, gm(tauP), gm(tauM))) { conv = true; break; }
   if (!conv) iter = kFwdMaxIter;
+  auto tau_of = [&](double v) { return UP ? 1.0 + v : v; };
   for (int cc = T.w; cc < ns; cc += T.W) {
     const int i = 64 * cc + lane;
     for (int r = 0; r < R; ++r) {
       const size_t e = skidx(cc, r, lane, R);
-      TP[e] = tauP[e] * C.is[0][e];
+      TP[e] = tau_of(tauP[e]) * C.is[0][e];
       const int j = r + lane - 63;
       double tm = 0;
-      if (i < nx && j >= 0 && j < ny) tm = tauP[skpos(0, i, j, R)] * C.is[1][e];
+      if (i < nx && j >= 0 && j < ny) tm = tau_of(tauP[skpos(0, i, j, R)]) * C.is[1][e];
       TM[e] = tm;
     }
   }
@@ -914,11 +945,11 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_strip(const SlotDesc *s
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
   Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8, D.status);
-  fwd_finish64(D, T, lane, rect);
+  fwd_finish64<false>(D, T, lane, rect);
 }
 
 // Mixed forward, FP32 phase: the FP32 iterations (count to iters[3]), then
-// tau = 1 + dtau into the FP64 plus skew for the polish.
+// dtau widened into the FP64 plus skew for the polish.
 template <bool TEAM>
 __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_mixed32(const SlotDesc *sd) {
   extern __shared__ double lds[];
@@ -934,26 +965,26 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_mixed32(const SlotDesc 
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
   FwdCtx<float> C;
   fwd_ctx_init(C, D.g, sg, lane, rect);
-  C.ps[0] = D.pskew32;
-  C.ps[1] = D.pskew32 + (size_t)kPF * SE;
-  C.is[0] = f32;
-  C.is[1] = f32 + (size_t)kFF * SE;
-  C.snap = f32 + 8 * SE;
+  C.ps[0] = gm(D.pskew32);
+  C.ps[1] = gm(D.pskew32 + (size_t)kPF * SE);
+  C.is[0] = gm(f32);
+  C.is[1] = gm(f32 + (size_t)kFF * SE);
+  C.snap = gm(f32 + 8 * SE);
   float *dP = f32 + 6 * SE, *dM = f32 + 7 * SE;
   int iter = 1;
   for (; iter <= kFwdMaxIter; ++iter)
-    if (fwd_iteration(C, T, dP, dM)) break;
+    if (fwd_iteration<float, true>(C, T, gm(dP), gm(dM))) break;
   if (iter > kFwdMaxIter) iter = kFwdMaxIter;
   double *tauP = sk0 + kSkTauP * SE;
-  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) tauP[e] = 1.0 + (double)dP[e];
+  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) tauP[e] = (double)dP[e];
   if (threadIdx.x == 0) {
     D.iters[3] = iter;
     if (team_failed(T)) *D.status = kETeamWait;
   }
 }
 
-// Mixed forward, FP64 polish: the reference's iterations to its stop rule
-// from the FP32 result, and the outputs.
+// Mixed forward, FP64 polish: the upwind update in FP64 from the FP32 result
+// to the reference's stop rule, and the outputs.
 template <bool TEAM>
 __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_polish(const SlotDesc *sd) {
   extern __shared__ double lds[];
@@ -963,7 +994,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_polish(const SlotDesc *
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
   Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8, D.status);
-  fwd_finish64(D, T, lane, rect);
+  fwd_finish64<true>(D, T, lane, rect);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -1039,10 +1070,10 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
   if (e >= SE || *D.status < 0) return;
   const int nx = g.nx, ny = g.ny, R = sg.R;
   const double dx = g.dx, dy = g.dy;
-  double *sk0 = D.skew;
-  const double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
-  const double *pa = D.pskew + (size_t)sk * kPF * SE;
-  const double *pb = pa + SE, *pc = pa + 2 * SE;
+  GMEM double *sk0 = gm(D.skew);
+  const GMEM double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
+  const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE;
+  const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % R);
   const int cc = (int)((e >> 6) / R);
@@ -1057,7 +1088,7 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
                Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
                pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
   }
-  double *b0 = sk0 + (size_t)sk * kAF * SE;
+  GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
 #pragma unroll
   for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
   if (KIND == 0) {
@@ -1073,8 +1104,8 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
       k[3] = (float)(b1p / dy / d);
       k[4] = (float)(-b2m / dy / d);
     }
-    float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
-    float *k32 = f32 + (size_t)sk * 5 * SE;
+    GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
+    GMEM float *k32 = f32 + (size_t)sk * 5 * SE;
 #pragma unroll
     for (int f = 0; f < 5; ++f) k32[f * SE + e] = k[f];
     if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
@@ -1141,12 +1172,12 @@ struct ANear<float> {
   float k0, kW, kE, kS, kN;
 };
 
-__device__ __forceinline__ void adj_load_near(const double *is, size_t SE, size_t e,
+__device__ __forceinline__ void adj_load_near(const GMEM double *is, size_t SE, size_t e,
                                               ANear<double> &q) {
   q.a1 = is[e]; q.a2 = is[SE + e]; q.b1 = is[2 * SE + e]; q.b2 = is[3 * SE + e];
   q.d = is[4 * SE + e]; q.dl = is[5 * SE + e];
 }
-__device__ __forceinline__ void adj_load_near(const float *is, size_t SE, size_t e,
+__device__ __forceinline__ void adj_load_near(const GMEM float *is, size_t SE, size_t e,
                                               ANear<float> &q) {
   q.k0 = is[e]; q.kW = is[SE + e]; q.kE = is[2 * SE + e]; q.kS = is[3 * SE + e];
   q.kN = is[4 * SE + e];
@@ -1171,21 +1202,21 @@ struct AdjCtx {
   int nx, ny, R, lane;
   size_t SE;
   double dx, dy;
-  const V *is[2];  // item coefficient skews
-  V *snap;         // plus skew
+  const GMEM V *is[2];  // item coefficient skews
+  GMEM V *snap;         // plus skew
 };
 
 // One adjoint sweep over strip c, in place in the sweep's own skew; MODE as
 // for the forward, and MODE 2 also accumulates max |Ta| (acc_a) for the
 // FP32 stop rule.
 template <typename V, int XD, int YD, int MODE>
-__device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, V *ta, double &ss,
+__device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V *ta, double &ss,
                                double &sm, double &sa) {
   constexpr int SK = XD == YD ? 0 : 1;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  const V *is = C.is[SK];
+  const GMEM V *is = C.is[SK];
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -1198,9 +1229,8 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, V *ta, 
   const bool dn_ok = lv && dn_edge && id >= 0 && id < nx;
   const bool interior_col = i >= 1 && i <= nx - 2;
   // team edges as in the forward pass
-  const bool from_chan = T.multi() && k > 0;
   const int pw = (T.w + T.W - 1) % T.W, base_in = ((k - 1) / T.W) * ny;
-  const bool pub = T.multi() && k < T.ns - 1 && lag == L - 1;
+  const bool pub = k < T.ns - 1 && lag == L - 1;
   const int base_out = (k / T.W) * ny;
   double acc_s = 0, acc_m = 0, acc_a = 0;
 
@@ -1215,15 +1245,9 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, V *ta, 
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
     const size_t e = skidx(c, r, lane, R);
     q.ta = ta[e];
-    if (up_ok) {
-      if (from_chan) {
-        if (q.act) {
-          team_wait(T, pw, base_in + u + 1);
-          q.tup = team_read<V>(T, pw, u);
-        }
-      } else {
-        q.tup = ta[skpos(SK, iu, jc, R)];
-      }
+    if (up_ok && q.act) {  // k > 0: the upstream strip's edge
+      if (T.multi()) team_wait(T, pw, base_in + u + 1);
+      q.tup = team_read<V>(T, pw, u);
     }
     if (dn_ok) q.tdn = ta[skpos(SK, id, jc, R)];
     if (MODE == 2) q.snap = C.snap[e];
@@ -1280,7 +1304,7 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, V *ta, 
 }
 
 template <typename V, int XD, int YD, int MODE>
-__device__ void adj_sweep(const AdjCtx<V> &C, Team &T, V *ta, StripAcc &acc) {
+__device__ void adj_sweep(const AdjCtx<V> &C, Team &T, GMEM V *ta, StripAcc &acc) {
   team_sweep_begin(T);
   for (int k = T.w; k < T.ns; k += T.W) {
     double ss = 0, sm = 0, sa = 0;
@@ -1294,7 +1318,7 @@ __device__ void adj_sweep(const AdjCtx<V> &C, Team &T, V *ta, StripAcc &acc) {
 // ascending first (src/eikon_solver_2d.f90:1383-1390); with SNAP the
 // snapshot and the residuals (:1412-1434) are taken.
 template <typename V, bool SNAP>
-__device__ void adj_iteration(const AdjCtx<V> &C, Team &T, V *tP, V *tM, double &s, double &m,
+__device__ void adj_iteration(const AdjCtx<V> &C, Team &T, GMEM V *tP, GMEM V *tM, double &s, double &m,
                               double &a) {
   StripAcc acc;
   adj_sweep<V, 1, 1, SNAP ? 1 : 0>(C, T, tP, acc);
@@ -1344,14 +1368,14 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_strip(const SlotDesc *s
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
   adj_ctx_init(C, g, sg, lane);
-  C.is[0] = sk0;
-  C.is[1] = sk0 + kAF * SE;
-  C.snap = sk0 + kSkSnapA * SE;
+  C.is[0] = gm(sk0);
+  C.is[1] = gm(sk0 + kAF * SE);
+  C.snap = gm(sk0 + kSkSnapA * SE);
   int iter = 1;
   bool conv = false;
   for (; iter <= kAdjMaxIter; ++iter) {
     double s, m, a;
-    adj_iteration<double, true>(C, T, TaP, TaM, s, m, a);
+    adj_iteration<double, true>(C, T, gm(TaP), gm(TaM), s, m, a);
     const double l1 = s / (double)(g.nx * g.ny);
     if (fabs(l1) < kAdjTol && m < kAdjTol) { conv = true; break; }
   }
@@ -1382,14 +1406,14 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_mixed32(const SlotDesc 
   float *taP = f32 + 10 * SE, *taM = f32 + 11 * SE;
   AdjCtx<float> C;
   adj_ctx_init(C, g, sg, lane);
-  C.is[0] = f32;
-  C.is[1] = f32 + 5 * SE;
-  C.snap = f32 + 12 * SE;
+  C.is[0] = gm(f32);
+  C.is[1] = gm(f32 + 5 * SE);
+  C.snap = gm(f32 + 12 * SE);
   int iter = 1;
   bool done = false;
   for (; iter <= kAdjMaxIter; ++iter) {
     double s, m, a;
-    adj_iteration<float, true>(C, T, taP, taM, s, m, a);
+    adj_iteration<float, true>(C, T, gm(taP), gm(taM), s, m, a);
     const double l1 = s / (double)(g.nx * g.ny);
     if ((fabs(l1) < kAdjTol && m < kAdjTol) || m <= (double)kMixRelStop * a) {
       done = true;
@@ -1422,11 +1446,11 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_polish(const SlotDesc *
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
   adj_ctx_init(C, g, sg, lane);
-  C.is[0] = sk0;
-  C.is[1] = sk0 + kAF * SE;
+  C.is[0] = gm(sk0);
+  C.is[1] = gm(sk0 + kAF * SE);
   C.snap = nullptr;
   double s, m, a;
-  adj_iteration<double, false>(C, T, TaP, TaM, s, m, a);
+  adj_iteration<double, false>(C, T, gm(TaP), gm(TaM), s, m, a);
   adj_output(g, sg, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
   if (threadIdx.x == 0) {
     if (team_failed(T)) *D.status = kETeamWait;
@@ -1496,7 +1520,7 @@ void launch_fwd_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
   if (gd.waves > 1)
     hipLaunchKernelGGL(k_fwd_strip<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd);
   else
-    hipLaunchKernelGGL(k_fwd_strip<false>, dim3(gd.count), dim3(64), 0, s, sd);
+    hipLaunchKernelGGL(k_fwd_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd);
 }
 
 void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
@@ -1504,14 +1528,14 @@ void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) 
     hipLaunchKernelGGL(k_fwd_mixed32<true>, dim3(gd.count), team_block(gd), team_lds(gd, 4), s,
                        sd);
   else
-    hipLaunchKernelGGL(k_fwd_mixed32<false>, dim3(gd.count), dim3(64), 0, s, sd);
+    hipLaunchKernelGGL(k_fwd_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, 4), s, sd);
 }
 
 void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
   if (gd.waves > 1)
     hipLaunchKernelGGL(k_fwd_polish<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd);
   else
-    hipLaunchKernelGGL(k_fwd_polish<false>, dim3(gd.count), dim3(64), 0, s, sd);
+    hipLaunchKernelGGL(k_fwd_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd);
 }
 
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
@@ -1531,7 +1555,7 @@ void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
     hipLaunchKernelGGL(k_adj_strip<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd,
                        which);
   else
-    hipLaunchKernelGGL(k_adj_strip<false>, dim3(gd.count), dim3(64), 0, s, sd, which);
+    hipLaunchKernelGGL(k_adj_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd, which);
 }
 
 void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
@@ -1539,7 +1563,8 @@ void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, 
     hipLaunchKernelGGL(k_adj_mixed32<true>, dim3(gd.count), team_block(gd), team_lds(gd, 4), s,
                        sd, which);
   else
-    hipLaunchKernelGGL(k_adj_mixed32<false>, dim3(gd.count), dim3(64), 0, s, sd, which);
+    hipLaunchKernelGGL(k_adj_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, 4), s, sd,
+                       which);
 }
 
 void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
@@ -1547,7 +1572,7 @@ void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, i
     hipLaunchKernelGGL(k_adj_polish<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd,
                        which);
   else
-    hipLaunchKernelGGL(k_adj_polish<false>, dim3(gd.count), dim3(64), 0, s, sd, which);
+    hipLaunchKernelGGL(k_adj_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd, which);
 }
 
 }  // namespace satt
