@@ -905,21 +905,23 @@ __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rec
     if (fwd_iteration<double, UP>(C, T, gm(tauP), gm(tauM))) { conv = true; break; }
   if (!conv) iter = kFwdMaxIter;
   auto tau_of = [&](double v) { return UP ? 1.0 + v : v; };
+  const GMEM double *gtau = gm(tauP);
+  GMEM double *gTP = gm(TP), *gTM = gm(TM), *gT = gm(Tnat);
   for (int cc = T.w; cc < ns; cc += T.W) {
     const int i = 64 * cc + lane;
     for (int r = 0; r < R; ++r) {
       const size_t e = skidx(cc, r, lane, R);
-      TP[e] = tau_of(tauP[e]) * C.is[0][e];
+      gTP[e] = tau_of(gtau[e]) * C.is[0][e];
       const int j = r + lane - 63;
       double tm = 0;
-      if (i < nx && j >= 0 && j < ny) tm = tau_of(tauP[skpos(0, i, j, R)]) * C.is[1][e];
-      TM[e] = tm;
+      if (i < nx && j >= 0 && j < ny) tm = tau_of(gtau[skpos(0, i, j, R)]) * C.is[1][e];
+      gTM[e] = tm;
     }
   }
   __syncthreads();
   for (size_t k = threadIdx.x; k < N; k += blockDim.x) {
     const int i = (int)(k % nx), j = (int)(k / nx);
-    Tnat[k] = TP[skpos(0, i, j, R)];
+    gT[k] = gTP[skpos(0, i, j, R)];
   }
   if (threadIdx.x == 0) {
     *D.status = team_failed(T) ? kETeamWait : conv ? kOk : kWMaxIter;
@@ -976,7 +978,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_mixed32(const SlotDesc 
     if (fwd_iteration<float, true>(C, T, gm(dP), gm(dM))) break;
   if (iter > kFwdMaxIter) iter = kFwdMaxIter;
   double *tauP = sk0 + kSkTauP * SE;
-  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) tauP[e] = (double)dP[e];
+  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) gm(tauP)[e] = (double)gm(dP)[e];
   if (threadIdx.x == 0) {
     D.iters[3] = iter;
     if (team_failed(T)) *D.status = kETeamWait;
@@ -1345,12 +1347,14 @@ __device__ void adj_output(const Grid &g, const SkewGeom &sg, const double *TaP,
                            double y0, int apply_mask, double *Tout) {
   const int nx = g.nx;
   const size_t N = (size_t)nx * g.ny;
-  const double dxd = g.xd[1] - g.xd[0], dyd = g.yd[1] - g.yd[0];
+  const GMEM double *xd = gm(g.xd), *yd = gm(g.yd), *src = gm(TaP);
+  GMEM double *out = gm(Tout);
+  const double dxd = xd[1] - xd[0], dyd = yd[1] - yd[0];
   for (size_t k = threadIdx.x; k < N; k += blockDim.x) {
     const int i = (int)(k % nx), j = (int)(k / nx);
-    double v = TaP[skpos(0, i, j, sg.R)];
-    if (apply_mask && fabs(g.xd[i] - x0) < 1.5 * dxd && fabs(g.yd[j] - y0) < 1.5 * dyd) v = 0;
-    Tout[k] = v;
+    double v = src[skpos(0, i, j, sg.R)];
+    if (apply_mask && fabs(xd[i] - x0) < 1.5 * dxd && fabs(yd[j] - y0) < 1.5 * dyd) v = 0;
+    out[k] = v;
   }
 }
 
@@ -1422,7 +1426,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_mixed32(const SlotDesc 
   }
   if (!done) iter = kAdjMaxIter;
   double *TaP = sk0 + kSkTaP * SE;
-  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) TaP[e] = (double)taP[e];
+  for (size_t e = threadIdx.x; e < SE; e += blockDim.x) gm(TaP)[e] = (double)gm(taP)[e];
   if (threadIdx.x == 0) {
     if (team_failed(T)) *D.status = kETeamWait;
     else if (!done && *D.status == kOk) *D.status = kWMaxIter;
