@@ -845,9 +845,12 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
  * with k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
  * kN = -b2m/dy/d rounded to float (all zero where |d| < eps), iterated until
  * the reference's absolute rule holds or the largest change is at most
- * 2^-21 of the largest |Ta| (float resolution).  FP64 polish: the
- * reference's update and stop rule from Ta = (double)Ta32, at least one
- * iteration.
+ * 2^-21 of the largest |Ta| (float resolution).  FP64 polish: one
+ * iteration of the reference's update from Ta = (double)Ta32.
+ *
+ * The upwind cell and the FP32 adjoint update are written with explicit
+ * fused multiply-adds (fmaf / fma, correctly rounded), as the device writes
+ * them; everything else is compiled without contraction.
  * ====================================================================== */
 
 static const float MIX_REL_STOP = 4.76837158203125e-07f; /* 2^-21 */
@@ -865,7 +868,7 @@ static const float MIX_REL_STOP = 4.76837158203125e-07f; /* 2^-21 */
  * itself would in float.  The iterations follow; they return the iteration
  * count (FWD_MAXITER when the stop rule never held) and tell in *conv
  * whether it held. */
-#define DEFINE_FWD_UP(SUF, V, SQRT) \
+#define DEFINE_FWD_UP(SUF, V, SQRT, FMA)                                                       \
 static void fwd_cell_up_##SUF(int xd, int yd, int nx, int ny, int i, int j, V rdx,             \
                        V rdy, const V *a, const V *b, const V *c,                              \
                        const V *f, const V *disx, const V *disy,                               \
@@ -874,30 +877,35 @@ static void fwd_cell_up_##SUF(int xd, int yd, int nx, int ny, int i, int j, V rd
   const size_t k = IDX(i, j);                                                                  \
   const V ak = a[k], bk = b[k], ck = c[k], fk = f[k];                                          \
   const V t0 = T0[k], t0x = T0x[k], t0y = T0y[k];                                              \
-  const V t0dx = t0 * rdx, t0dy = t0 * rdy;                                                    \
   const int hasX = xd > 0 ? i > 0 : i < nx - 1;                                                \
   const int hasY = yd > 0 ? j > 0 : j < ny - 1;                                                \
   const size_t kX = hasX ? IDX(i - xd, j) : k, kY = hasY ? IDX(i, j - yd) : k;                 \
   const V dX = dtau[kX], dY = dtau[kY], zX = T0[kX], zY = T0[kY];                              \
-  const V px = xd > 0 ? t0x + t0dx : t0x - t0dx;                                               \
-  const V py = yd > 0 ? t0y + t0dy : t0y - t0dy;                                               \
+  /* px = T0x +- T0/dx, py = T0y +- T0/dy (upwind neighbour side) */                           \
+  const V px = FMA(t0, xd > 0 ? rdx : -rdx, t0x);                                              \
+  const V py = FMA(t0, yd > 0 ? rdy : -rdy, t0y);                                              \
   V best = dtau[k];                                                                            \
   if (hasX && hasY) {                                                                          \
-    const V gx = t0x + t0x * dX;                                                               \
-    const V gy = yd > 0 ? t0y + t0y * dX + t0dy * (dX - dY)                                    \
-                            : t0y + t0y * dX - t0dy * (dX - dY);                               \
-    const V ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;                           \
-    const V eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));                 \
-    const V ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;                 \
-    const V disc = eb * eb - 4 * ea * ec;                                                      \
+    /* gx = T0x*tX, gy = T0y*tX +- (T0/dy)*(tX - tY) in offsets */                             \
+    const V t0dy = t0 * rdy;                                                                   \
+    const V gx = FMA(t0x, dX, t0x);                                                            \
+    const V gy = FMA(yd > 0 ? t0dy : -t0dy, dX - dY, FMA(t0y, dX, t0y));                       \
+    /* ea u^2 + eb u + ec = 0: ea = a px^2 + b py^2 - 2c px py,                                \
+     * eb = 2(gx (a px - c py) + gy (b py - c px)),                                            \
+     * ec = gx (a gx - 2c gy) + b gy^2 - f^2 */                                                \
+    const V apx = ak * px, bpy = bk * py, cpx = ck * px, cpy = ck * py;                        \
+    const V ea = FMA(apx, px, py * FMA((V)-2, cpx, bpy));                                      \
+    const V eb = (V)2 * FMA(gx, apx - cpy, gy * (bpy - cpx));                                  \
+    const V ec = FMA(gx, FMA(ak, gx, (V)-2 * (ck * gy)), FMA(bk * gy, gy, -(fk * fk)));        \
+    const V disc = FMA(eb, eb, (V)-4 * ea * ec);                                               \
     if (disc >= 0) {                                                                           \
       const V sq = SQRT(disc);                                                                 \
       const V qq = (V)-0.5 * (eb >= 0 ? eb + sq : eb - sq);                                    \
       for (int isol = 1; isol <= 2; ++isol) {                                                  \
         const V u = isol == 1 ? qq / ea : ec / qq;                                             \
         const V dt = dX + u;                                                                   \
-        const V Tx = gx + px * u, Ty = gy + py * u;                                            \
-        const V chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;                              \
+        const V Tx = FMA(px, u, gx), Ty = FMA(py, u, gy);                                      \
+        const V chx = FMA(ak, Tx, -(ck * Ty)), chy = FMA(bk, Ty, -(ck * Tx));                  \
         const int ok = (xd > 0 ? chx >= 0 : chx <= 0) && (yd > 0 ? chy >= 0 : chy <= 0);       \
         if (ok && dt < best) best = dt;                                                        \
       }                                                                                        \
@@ -910,12 +918,12 @@ static void fwd_cell_up_##SUF(int xd, int yd, int nx, int ny, int i, int j, V rd
     const V p = side == 0 ? px : py, dis = side == 0 ? disx[k] : disy[k];                      \
     const V t0d = side == 0 ? t0x : t0y;                                                       \
     const V dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;                                \
-    const V rp = (V)1 / p, base = t0d + t0d * dn;                                              \
-    const V rhs = zn + dn * zn, half = (V)0.5 * dn - (V)0.5;                                   \
-    V dt = dn + (dis - base) * rp;                                                             \
-    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;                              \
-    dt = dn + (-dis - base) * rp;                                                              \
-    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;                              \
+    const V rp = (V)1 / p, base = FMA(t0d, dn, t0d);                                           \
+    const V rhs = FMA(dn, zn, zn), half = FMA((V)0.5, dn, (V)-0.5);                            \
+    V dt = FMA(dis - base, rp, dn);                                                            \
+    if (FMA(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;                           \
+    dt = FMA(-dis - base, rp, dn);                                                             \
+    if (FMA(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;                           \
   }                                                                                            \
   dtau[k] = best;                                                                              \
 }                                                                                              \
@@ -960,8 +968,8 @@ static int fwd_iterate_up_##SUF(int nx, int ny, V rdx, V rdy, const V *fld,     
   return FWD_MAXITER;                                                                          \
 }                                                                                              \
 
-DEFINE_FWD_UP(f32, float, sqrtf)
-DEFINE_FWD_UP(f64, double, sqrt)
+DEFINE_FWD_UP(f32, float, sqrtf, fmaf)
+DEFINE_FWD_UP(f64, double, sqrt, fma)
 
 static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
                             int ny, const double *spha, const double *sphb,
@@ -1115,8 +1123,9 @@ static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
           for (int jj = 1; jj < ny - 1; ++jj) {
             const int j = ydir < 0 ? jj : ny - 1 - jj;
             const size_t k = IDX(i, j);
-            ta[k] = k0[k] + kW[k] * ta[IDX(i - 1, j)] + kE[k] * ta[IDX(i + 1, j)] +
-                    kS[k] * ta[IDX(i, j - 1)] + kN[k] * ta[IDX(i, j + 1)];
+            ta[k] = fmaf(kN[k], ta[IDX(i, j + 1)],
+                         fmaf(kS[k], ta[IDX(i, j - 1)],
+                              fmaf(kE[k], ta[IDX(i + 1, j)], fmaf(kW[k], ta[IDX(i - 1, j)], k0[k]))));
           }
         }
     double l1 = 0, linf = 0, amax = 0;

@@ -56,6 +56,9 @@
 //   k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d
 //   formed in FP64 and rounded to float; stop when the reference's rule
 //   holds or the largest change is at most 2^-21 of the largest |Ta|.
+//   The mixed mode's updates (upwind cell, FP32 adjoint) are written with
+//   explicit fused multiply-adds, which the oracle restates with fma/fmaf;
+//   the rest of the code is compiled without contraction.
 //   Polish: one FP64 iteration of the reference's update.
 #include <algorithm>
 #include <climits>
@@ -234,20 +237,25 @@ template <typename V, int XD, int YD>
 __device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, V rdy, V ak, V bk,
                                            V ck, V fk, V disx, V disy, V t0, V t0x, V t0y, V dc,
                                            V dX, V dY, V zX, V zY) {
-  const V t0dx = t0 * rdx, t0dy = t0 * rdy;
   const bool hasX = XD > 0 ? i > 0 : i < nx - 1;
   const bool hasY = YD > 0 ? j > 0 : j < ny - 1;
-  const V px = XD > 0 ? t0x + t0dx : t0x - t0dx;
-  const V py = YD > 0 ? t0y + t0dy : t0y - t0dy;
+  // px = T0x +- T0/dx, py = T0y +- T0/dy (upwind neighbour side)
+  const V px = fma(t0, XD > 0 ? rdx : -rdx, t0x);
+  const V py = fma(t0, YD > 0 ? rdy : -rdy, t0y);
   V best = dc;
   if (hasX && hasY) {
-    const V gx = t0x + t0x * dX;
-    const V gy = YD > 0 ? t0y + t0y * dX + t0dy * (dX - dY)
-                            : t0y + t0y * dX - t0dy * (dX - dY);
-    const V ea = ak * (px * px) + bk * (py * py) - 2 * ck * px * py;
-    const V eb = 2 * (ak * px * gx + bk * py * gy - ck * (px * gy + py * gx));
-    const V ec = ak * (gx * gx) + bk * (gy * gy) - 2 * ck * gx * gy - fk * fk;
-    const V disc = eb * eb - 4 * ea * ec;
+    // gx = T0x*tX, gy = T0y*tX +- (T0/dy)*(tX - tY) in offsets
+    const V t0dy = t0 * rdy;
+    const V gx = fma(t0x, dX, t0x);
+    const V gy = fma(YD > 0 ? t0dy : -t0dy, dX - dY, fma(t0y, dX, t0y));
+    // ea u^2 + eb u + ec = 0: ea = a px^2 + b py^2 - 2c px py,
+    // eb = 2(gx (a px - c py) + gy (b py - c px)),
+    // ec = gx (a gx - 2c gy) + b gy^2 - f^2
+    const V apx = ak * px, bpy = bk * py, cpx = ck * px, cpy = ck * py;
+    const V ea = fma(apx, px, py * fma(V(-2), cpx, bpy));
+    const V eb = V(2) * fma(gx, apx - cpy, gy * (bpy - cpx));
+    const V ec = fma(gx, fma(ak, gx, V(-2) * (ck * gy)), fma(bk * gy, gy, -(fk * fk)));
+    const V disc = fma(eb, eb, V(-4) * ea * ec);
     if (disc >= 0) {
       const V sq = sqrt(disc);
       const V qq = V(-0.5) * (eb >= 0 ? eb + sq : eb - sq);
@@ -255,8 +263,8 @@ __device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, 
       for (int isol = 1; isol <= 2; ++isol) {
         const V u = isol == 1 ? qq / ea : ec / qq;
         const V dt = dX + u;
-        const V Tx = gx + px * u, Ty = gy + py * u;
-        const V chx = ak * Tx - ck * Ty, chy = bk * Ty - ck * Tx;
+        const V Tx = fma(px, u, gx), Ty = fma(py, u, gy);
+        const V chx = fma(ak, Tx, -(ck * Ty)), chy = fma(bk, Ty, -(ck * Tx));
         const bool ok = (XD > 0 ? chx >= 0 : chx <= 0) && (YD > 0 ? chy >= 0 : chy <= 0);
         if (ok && dt < best) best = dt;
       }
@@ -270,12 +278,12 @@ __device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, 
     const V p = side == 0 ? px : py, dis = side == 0 ? disx : disy;
     const V t0d = side == 0 ? t0x : t0y;
     const V dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
-    const V rp = V(1) / p, base = t0d + t0d * dn;
-    const V rhs = zn + dn * zn, half = V(0.5) * dn - V(0.5);
-    V dt = dn + (dis - base) * rp;
-    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
-    dt = dn + (-dis - base) * rp;
-    if (t0 + dt * t0 >= rhs && dt > half && dt < best) best = dt;
+    const V rp = V(1) / p, base = fma(t0d, dn, t0d);
+    const V rhs = fma(dn, zn, zn), half = fma(V(0.5), dn, V(-0.5));
+    V dt = fma(dis - base, rp, dn);
+    if (fma(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;
+    dt = fma(-dis - base, rp, dn);
+    if (fma(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;
   }
   return best;
 }
@@ -673,15 +681,24 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     }
   };
 
-  FFar<V> fc{}, fn{}, f2{};
-  FNear<V> nc{}, nn{};
-  load_far(0, fc);
-  load_far(1, fn);
-  load_near(0, nc);
+  // Far operands are fetched two steps ahead, near ones two (float) or one
+  // (double) ahead.  The float loop cycles three operand sets, unrolled by
+  // three so that no set is copied (step t uses set t mod 3 and fills set
+  // (t+2) mod 3); up to two steps past the last run with every lane outside
+  // the grid, so they change nothing.  The double loop rotates its sets (the
+  // third near set would cost it a wave per SIMD of occupancy).
+  constexpr bool kUnroll = std::is_same<V, float>::value;
+  FFar<V> fA{}, fB{}, fC{};
+  FNear<V> nA{}, nB{}, nC{};
+  load_far(0, fA);
+  load_far(1, fB);
+  load_near(0, nA);
+  load_near(1, nB);
   V tprev = 0, zprev = 0;  // own result and T0 of the previous step
-  for (int t = 0; t < nsteps; ++t) {
+  auto step = [&](int t, const FFar<V> &fc, const FFar<V> &fn, FFar<V> &f2, const FNear<V> &nc,
+                  FNear<V> &nx2, int tnear) {
     load_far(t + 2, f2);
-    load_near(t + 1, nn);
+    load_near(tnear, nx2);
     // lane shifts read other lanes' registers: they run with every lane
     // enabled, outside the per-cell branch
     V tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
@@ -722,9 +739,20 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     }
     tprev = tnew;
     zprev = fc.t0;
-    fc = fn;
-    fn = f2;
-    nc = nn;
+  };
+  if constexpr (kUnroll) {
+    for (int t = 0; t < nsteps; t += 3) {
+      step(t, fA, fB, fC, nA, nC, t + 2);
+      step(t + 1, fB, fC, fA, nB, nA, t + 3);
+      step(t + 2, fC, fA, fB, nC, nB, t + 4);
+    }
+  } else {
+    for (int t = 0; t < nsteps; ++t) {
+      step(t, fA, fB, fC, nA, nB, t + 1);
+      fA = fB;
+      fB = fC;
+      nA = nB;
+    }
   }
   if (MODE == 2) {
     ss = wave_sum(acc_s);
@@ -1196,7 +1224,7 @@ __device__ __forceinline__ double adj_update(const ANear<double> &n, double tW, 
 }
 __device__ __forceinline__ float adj_update(const ANear<float> &n, float tW, float tE, float tS,
                                             float tN, double, double) {
-  return n.k0 + n.kW * tW + n.kE * tE + n.kS * tS + n.kN * tN;
+  return fmaf(n.kN, tN, fmaf(n.kS, tS, fmaf(n.kE, tE, fmaf(n.kW, tW, n.k0))));
 }
 
 template <typename V>
