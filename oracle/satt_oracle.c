@@ -848,9 +848,10 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
  * 2^-21 of the largest |Ta| (float resolution).  FP64 polish: one
  * iteration of the reference's update from Ta = (double)Ta32.
  *
- * The upwind cell and the FP32 adjoint update are written with explicit
- * fused multiply-adds (fmaf / fma, correctly rounded), as the device writes
- * them; everything else is compiled without contraction.
+ * The FP32 upwind cell and the FP32 adjoint update are written with
+ * explicit fused multiply-adds (fmaf, correctly rounded), as the device
+ * writes them; the FP64 polish evaluates the same expressions with separate
+ * roundings (muladd), and everything else is compiled without contraction.
  * ====================================================================== */
 
 static const float MIX_REL_STOP = 4.76837158203125e-07f; /* 2^-21 */
@@ -968,8 +969,11 @@ static int fwd_iterate_up_##SUF(int nx, int ny, V rdx, V rdy, const V *fld,     
   return FWD_MAXITER;                                                                          \
 }                                                                                              \
 
+/* a*b + c with two roundings, for the FP64 polish */
+static double muladd(double a, double b, double c) { return a * b + c; }
+
 DEFINE_FWD_UP(f32, float, sqrtf, fmaf)
-DEFINE_FWD_UP(f64, double, sqrt, fma)
+DEFINE_FWD_UP(f64, double, sqrt, muladd)
 
 static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
                             int ny, const double *spha, const double *sphb,

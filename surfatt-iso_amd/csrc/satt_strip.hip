@@ -56,9 +56,9 @@
 //   k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d
 //   formed in FP64 and rounded to float; stop when the reference's rule
 //   holds or the largest change is at most 2^-21 of the largest |Ta|.
-//   The mixed mode's updates (upwind cell, FP32 adjoint) are written with
-//   explicit fused multiply-adds, which the oracle restates with fma/fmaf;
-//   the rest of the code is compiled without contraction.
+//   The FP32 updates (upwind cell, adjoint) are written with explicit fused
+//   multiply-adds, which the oracle restates with fmaf; the rest of the
+//   code, the FP64 polish included, is compiled without contraction.
 //   Polish: one FP64 iteration of the reference's update.
 #include <algorithm>
 #include <climits>
@@ -224,6 +224,11 @@ __device__ __forceinline__ double fwd_cell_r(int i, int j, int nx, int ny, doubl
   return tnew;
 }
 
+// a*b + c: fused in float (the FP32 phase, where it saves issue slots), two
+// roundings in double (the FP64 polish, where the fused chains ran slower).
+__device__ __forceinline__ float fmav(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmav(double a, double b, double c) { return a * b + c; }
+
 // One cell of the mixed mode's upwind update (FP32 phase, V = float; FP64
 // polish, V = double) in the sweep direction (XD, YD) (see the precision note
 // at the top).  Only the candidates whose neighbours are upwind in this sweep
@@ -240,22 +245,22 @@ __device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, 
   const bool hasX = XD > 0 ? i > 0 : i < nx - 1;
   const bool hasY = YD > 0 ? j > 0 : j < ny - 1;
   // px = T0x +- T0/dx, py = T0y +- T0/dy (upwind neighbour side)
-  const V px = fma(t0, XD > 0 ? rdx : -rdx, t0x);
-  const V py = fma(t0, YD > 0 ? rdy : -rdy, t0y);
+  const V px = fmav(t0, XD > 0 ? rdx : -rdx, t0x);
+  const V py = fmav(t0, YD > 0 ? rdy : -rdy, t0y);
   V best = dc;
   if (hasX && hasY) {
     // gx = T0x*tX, gy = T0y*tX +- (T0/dy)*(tX - tY) in offsets
     const V t0dy = t0 * rdy;
-    const V gx = fma(t0x, dX, t0x);
-    const V gy = fma(YD > 0 ? t0dy : -t0dy, dX - dY, fma(t0y, dX, t0y));
+    const V gx = fmav(t0x, dX, t0x);
+    const V gy = fmav(YD > 0 ? t0dy : -t0dy, dX - dY, fmav(t0y, dX, t0y));
     // ea u^2 + eb u + ec = 0: ea = a px^2 + b py^2 - 2c px py,
     // eb = 2(gx (a px - c py) + gy (b py - c px)),
     // ec = gx (a gx - 2c gy) + b gy^2 - f^2
     const V apx = ak * px, bpy = bk * py, cpx = ck * px, cpy = ck * py;
-    const V ea = fma(apx, px, py * fma(V(-2), cpx, bpy));
-    const V eb = V(2) * fma(gx, apx - cpy, gy * (bpy - cpx));
-    const V ec = fma(gx, fma(ak, gx, V(-2) * (ck * gy)), fma(bk * gy, gy, -(fk * fk)));
-    const V disc = fma(eb, eb, V(-4) * ea * ec);
+    const V ea = fmav(apx, px, py * fmav(V(-2), cpx, bpy));
+    const V eb = V(2) * fmav(gx, apx - cpy, gy * (bpy - cpx));
+    const V ec = fmav(gx, fmav(ak, gx, V(-2) * (ck * gy)), fmav(bk * gy, gy, -(fk * fk)));
+    const V disc = fmav(eb, eb, V(-4) * ea * ec);
     if (disc >= 0) {
       const V sq = sqrt(disc);
       const V qq = V(-0.5) * (eb >= 0 ? eb + sq : eb - sq);
@@ -263,8 +268,8 @@ __device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, 
       for (int isol = 1; isol <= 2; ++isol) {
         const V u = isol == 1 ? qq / ea : ec / qq;
         const V dt = dX + u;
-        const V Tx = fma(px, u, gx), Ty = fma(py, u, gy);
-        const V chx = fma(ak, Tx, -(ck * Ty)), chy = fma(bk, Ty, -(ck * Tx));
+        const V Tx = fmav(px, u, gx), Ty = fmav(py, u, gy);
+        const V chx = fmav(ak, Tx, -(ck * Ty)), chy = fmav(bk, Ty, -(ck * Tx));
         const bool ok = (XD > 0 ? chx >= 0 : chx <= 0) && (YD > 0 ? chy >= 0 : chy <= 0);
         if (ok && dt < best) best = dt;
       }
@@ -278,12 +283,12 @@ __device__ __forceinline__ V fwd_cell_up_r(int i, int j, int nx, int ny, V rdx, 
     const V p = side == 0 ? px : py, dis = side == 0 ? disx : disy;
     const V t0d = side == 0 ? t0x : t0y;
     const V dn = side == 0 ? dX : dY, zn = side == 0 ? zX : zY;
-    const V rp = V(1) / p, base = fma(t0d, dn, t0d);
-    const V rhs = fma(dn, zn, zn), half = fma(V(0.5), dn, V(-0.5));
-    V dt = fma(dis - base, rp, dn);
-    if (fma(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;
-    dt = fma(-dis - base, rp, dn);
-    if (fma(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;
+    const V rp = V(1) / p, base = fmav(t0d, dn, t0d);
+    const V rhs = fmav(dn, zn, zn), half = fmav(V(0.5), dn, V(-0.5));
+    V dt = fmav(dis - base, rp, dn);
+    if (fmav(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;
+    dt = fmav(-dis - base, rp, dn);
+    if (fmav(dt, t0, t0) >= rhs && dt > half && dt < best) best = dt;
   }
   return best;
 }
@@ -1286,15 +1291,20 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
     const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
     adj_load_near(is, SE, skidx(c, r, lane, R), q);
   };
-  AFar<V> fc, fn, f2;
-  ANear<V> nc, nn;
-  load_far(0, fc);
-  load_far(1, fn);
-  load_near(0, nc);
+  // operand sets as in the forward pass: the float loop unrolled by three
+  // over three sets (far and near two steps ahead), the double loop rotated
+  constexpr bool kUnroll = std::is_same<V, float>::value;
+  AFar<V> fA, fB, fC;
+  ANear<V> nA, nB, nC;
+  load_far(0, fA);
+  load_far(1, fB);
+  load_near(0, nA);
+  load_near(1, nB);
   V tprev = 0;
-  for (int t = 0; t < nsteps; ++t) {
+  auto step = [&](int t, const AFar<V> &fc, const AFar<V> &fn, AFar<V> &f2, const ANear<V> &nc,
+                  ANear<V> &nx2, int tnear) {
     load_far(t + 2, f2);
-    load_near(t + 1, nn);
+    load_near(tnear, nx2);
     V tUx = lane_shift<-XD>(tprev);
     V tDx = lane_shift<XD>(fn.ta);
     if (up_ok) tUx = fc.tup;
@@ -1321,9 +1331,20 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
       }
     }
     tprev = tnew;
-    fc = fn;
-    fn = f2;
-    nc = nn;
+  };
+  if constexpr (kUnroll) {
+    for (int t = 0; t < nsteps; t += 3) {
+      step(t, fA, fB, fC, nA, nC, t + 2);
+      step(t + 1, fB, fC, fA, nB, nA, t + 3);
+      step(t + 2, fC, fA, fB, nC, nB, t + 4);
+    }
+  } else {
+    for (int t = 0; t < nsteps; ++t) {
+      step(t, fA, fB, fC, nA, nB, t + 1);
+      fA = fB;
+      fB = fC;
+      nA = nB;
+    }
   }
   if (MODE == 2) {
     ss = wave_sum(acc_s);
