@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the HIP-graph replay timing of the same step")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the run with autotuned team sizes")
     ap.add_argument("--workload", default="topo401", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--oracle-items", type=int, default=4,
@@ -367,6 +369,25 @@ def main():
                      "solves_per_s": round(args.steps * sum(len(b["item_pidx"]) for b in parts) / gdt, 3),
                      "results_same_as_eager": bool(same)}
             log(f"[bench] graph replay: {graph}")
+        # team size per launch group from the autotuner (one wave per item
+        # is the throughput form; the tuner measures it against teams)
+        tuned = None
+        if not args.no_tune:
+            t1 = time.perf_counter()
+            plan.set_waves(0)
+            ttune = time.perf_counter() - t1
+            plan.run(True, True)
+            plan.wait()
+            t1 = time.perf_counter()
+            plan.run(True, True)
+            plan.wait()
+            tdt = time.perf_counter() - t1
+            same = all(np.array_equal(plan.results(k)[key], outs[k][key])
+                       for k in range(len(parts)) for key in ("tt_fwd", "adj_s"))
+            plan.set_waves(1)
+            tuned = {"tuning_s": round(ttune, 2), "ms_per_step": round(tdt * 1e3, 3),
+                     "results_same_as_one_wave": bool(same)}
+            log(f"[bench] tuned teams: {tuned}")
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
     value = args.steps * nit / dt
     abytes = {}
@@ -485,6 +506,7 @@ def main():
                      "avg_launch_ms": kms / launches},
         "kernels": kernels,
         "graph_replay": graph,
+        "tuned_teams": tuned,
         "batched_fp64": None if fp64_value is None else round(fp64_value, 3),
         "mixed_error": mixed_err,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),
