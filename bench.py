@@ -9,8 +9,10 @@ and the kernel stacking.  Default workload: the 401 x 401 curved-surface
 topography grid at 0.005 deg, 200 sources x 20 periods = 4000 fwd+adj solves.
 
 Prints one JSON line (rank 0).  `value` is fwd+adj solves per second of the
-batched path; the plain FP64 one-solve-per-launch rate, the roofline of the
-dominant kernel and a host-core baseline are reported beside it.
+batched path (team sizes from the autotuner unless --no-tune); the plain FP64
+one-solve-per-launch rate, the roofline of the dominant kernel and a host-core
+baseline are reported beside it, and a model-side leg times the topographic
+metric and the steepest-descent update on the same grid.
 """
 import argparse
 import json
@@ -273,6 +275,81 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
             "plain_fp64_per_solve_solves_per_s": None if plain is None else round(plain, 3)}
 
 
+def window_nodes(xg, yg, sigma):
+    """Window nodes summed over the cells of one smoothing
+    (src/shared/utils.f90:1204-1219): the window is a rectangle per cell, so
+    the sum is (sum of x extents) x (sum of y extents)."""
+    nx, ny = len(xg), len(yg)
+    hx = int(sigma * 3 / (xg[1] - xg[0]))
+    i = np.arange(nx)
+    cx = np.sum(np.minimum(i + hx, nx - 1) - np.maximum(i - hx, 0) + 1)
+    cy = 0
+    for j in range(ny):
+        hy = int(sigma * 3 / np.cos(np.deg2rad(yg[j])) / (yg[1] - yg[0]))
+        cy += min(j + hy, ny - 1) - max(j - hy, 0) + 1
+    return float(cx) * float(cy)
+
+
+def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
+    """The model-side steps around the solver loop on the 401^2 workload grid:
+    get_topo's metric for all 20 periods (smoothing of the topography with
+    each period's sigma, src/grid.f90:114-123) and one optimize + steepest-
+    descent update of a stacked kernel (src/tomo2d.f90:203-251).  Device
+    time from HIP events around the kernels; the oracle times a few rows of
+    the smoothing on one host thread for the CPU rate."""
+    xg, yg, topo, periods, vref = W.topo_hills()
+    sig = [satt.topo_sigma(v, p, wavelen_factor) for v, p in zip(vref, periods)]
+    nodes = sum(window_nodes(xg, yg, s) for s in sig)
+    ctx.topo_metric(xg, yg, topo, sig)
+    t0 = time.perf_counter()
+    m11, m12, m22, ang = ctx.topo_metric(xg, yg, topo, sig)
+    wall = time.perf_counter() - t0
+    dev = ctx.last_device_ms()
+    n, npd = len(xg), len(periods)
+    rng = np.random.default_rng(5)
+    adj = np.asfortranarray(W.checkerboard_2d(n, n, 4, 4, 1e-3)[None] * rng.uniform(0.5, 1.5, (npd, 1, 1)))
+    svel = np.asfortranarray(vref[:, None, None] * np.ones((npd, n, n)))
+    ctx.steepest_descent(xg, yg, adj, sigma_2d, updatemax, svel)
+    t0 = time.perf_counter()
+    ctx.steepest_descent(xg, yg, adj, sigma_2d, updatemax, svel)
+    sd_wall = time.perf_counter() - t0
+    sd_dev = ctx.last_device_ms()
+    sd_nodes = npd * window_nodes(xg, yg, sigma_2d)
+    res = {
+        "workload": f"{n}x{n} @{xg[1] - xg[0]:.3f}deg, topography of 8 hills, {npd} periods "
+                    f"3-30 s, sigma = vel*T*{wavelen_factor}/(2 pi) "
+                    f"({min(sig):.3f}-{max(sig):.3f} deg); steepest descent sigma_2d {sigma_2d}",
+        "topo_metric": {"device_ms": round(dev, 3), "wall_ms": round(wall * 1e3, 3),
+                        "window_nodes": nodes, "gnodes_per_s": round(nodes / dev / 1e6, 1)},
+        "steepest_descent": {"device_ms": round(sd_dev, 3), "wall_ms": round(sd_wall * 1e3, 3),
+                             "window_nodes": sd_nodes,
+                             "gnodes_per_s": round(sd_nodes / sd_dev / 1e6, 1)},
+        "metric_range": {"m11": [float(m11.min()), float(m11.max())],
+                         "max_dip_deg": float(ang.max())},
+    }
+    if cpu:
+        import oracle as O
+        sample = [(sig[0], j) for j in (0, 200, 400)] + [(sig[npd // 2], 200), (sig[15], 200)]
+        snodes = 0.0
+        t0 = time.perf_counter()
+        for s_, j in sample:
+            O.smooth_geo2(xg, yg, topo, s_, rows=(j, j + 1))
+            hx = int(s_ * 3 / (xg[1] - xg[0]))
+            hy = int(s_ * 3 / np.cos(np.deg2rad(yg[j])) / (yg[1] - yg[0]))
+            i = np.arange(n)
+            snodes += float(np.sum(np.minimum(i + hx, n - 1) - np.maximum(i - hx, 0) + 1)) * \
+                (min(j + hy, n - 1) - max(j - hy, 0) + 1)
+        cdt = time.perf_counter() - t0
+        rate = snodes / cdt
+        res["cpu_baseline"] = {"value": round(rate / 1e6, 3), "unit": "Mnodes/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle gaussian_smooth_geo_2 on {len(sample)} rows "
+                                         f"({snodes:.3g} window nodes, {cdt:.1f} s)",
+                               "est_full_topo_metric_s": round(nodes / rate, 1)}
+        res["vs_cpu_core"] = round(nodes / dev / 1e3 / rate, 1)
+    return res
+
+
 def host_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -289,7 +366,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the HIP-graph replay timing of the same step")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the run with autotuned team sizes")
+                    help="one wave per item, no team-size tuning (and no comparison run)")
     ap.add_argument("--workload", default="topo401", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--oracle-items", type=int, default=4,
@@ -301,6 +378,8 @@ def main():
                     help="mixed mode: skip the FP64 batched run and the error report")
     ap.add_argument("--plain-items", type=int, default=48,
                     help="items timed in the one-solve-per-launch FP64 form")
+    ap.add_argument("--no-model-side", action="store_true",
+                    help="skip the model-side leg (topographic metric, steepest descent)")
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -340,6 +419,15 @@ def main():
     log("[bench] observed times computed")
 
     with ctx.plan_multi(parts, prec) as plan:
+        # team size per launch group from the autotuner (it times one wave
+        # per item against teams of 2-4 waves on a synthetic batch of the
+        # group's shape); the one-wave form is timed after, for comparison
+        ttune = None
+        if not args.no_tune:
+            t1 = time.perf_counter()
+            plan.set_waves(0)
+            ttune = time.perf_counter() - t1
+            log(f"[bench] team sizes tuned in {ttune:.1f}s")
         for _ in range(args.warmup):
             plan.run(True, True)
             plan.wait()
@@ -369,13 +457,9 @@ def main():
                      "solves_per_s": round(args.steps * sum(len(b["item_pidx"]) for b in parts) / gdt, 3),
                      "results_same_as_eager": bool(same)}
             log(f"[bench] graph replay: {graph}")
-        # team size per launch group from the autotuner (one wave per item
-        # is the throughput form; the tuner measures it against teams)
-        tuned = None
+        one_wave = None
         if not args.no_tune:
-            t1 = time.perf_counter()
-            plan.set_waves(0)
-            ttune = time.perf_counter() - t1
+            plan.set_waves(1)
             plan.run(True, True)
             plan.wait()
             t1 = time.perf_counter()
@@ -384,10 +468,9 @@ def main():
             tdt = time.perf_counter() - t1
             same = all(np.array_equal(plan.results(k)[key], outs[k][key])
                        for k in range(len(parts)) for key in ("tt_fwd", "adj_s"))
-            plan.set_waves(1)
-            tuned = {"tuning_s": round(ttune, 2), "ms_per_step": round(tdt * 1e3, 3),
-                     "results_same_as_one_wave": bool(same)}
-            log(f"[bench] tuned teams: {tuned}")
+            one_wave = {"tuning_s": round(ttune, 2), "one_wave_ms_per_step": round(tdt * 1e3, 3),
+                        "results_same_as_one_wave": bool(same)}
+            log(f"[bench] one wave per item: {one_wave}")
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
     value = args.steps * nit / dt
     abytes = {}
@@ -477,6 +560,11 @@ def main():
         cpu = cpu_baseline(parts, nits, max(args.cpu_sample, th), th)
         log(f"[bench] cpu baseline: {cpu['value']:.2f} solves/s on {th} threads")
 
+    side = None
+    if not args.no_model_side:
+        side = model_side(ctx, cpu=not args.no_cpu)
+        log(f"[bench] model side: {side}")
+
     out = outs[0]
     iters = out["iters"]
     line = {
@@ -495,6 +583,7 @@ def main():
         "config": {"workload": args.workload, "description": desc, "items": nit,
                    "grid": grids[0] if len(grids) == 1 else grids, "precision": args.precision,
                    "launch": "batched", "chunk": st["chunk"],
+                   "teams": "one wave per item" if args.no_tune else "autotuned per launch group",
                    "mean_iters_fwd": float(np.concatenate([o["iters"][:, 0] for o in outs]).mean()),
                    "mean_iters_adj": float(np.concatenate([o["iters"][:, 1] for o in outs]).mean()),
                    "mean_iters_fwd_fp32": float(np.concatenate([o["iters_fp32"][:, 0] for o in outs]).mean()),
@@ -506,7 +595,7 @@ def main():
                      "avg_launch_ms": kms / launches},
         "kernels": kernels,
         "graph_replay": graph,
-        "tuned_teams": tuned,
+        "team_tuning": one_wave,
         "batched_fp64": None if fp64_value is None else round(fp64_value, 3),
         "mixed_error": mixed_err,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),
@@ -514,6 +603,7 @@ def main():
         "serving": serving,
         "cpu_baseline": cpu,
         "vs_cpu": None if cpu is None else round(value / cpu["value"], 1),
+        "model_side": side,
     }
     print(json.dumps(line), flush=True)
 

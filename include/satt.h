@@ -16,6 +16,12 @@
  *                          (src/tomo2d.f90:145-156, src/acqui.f90:307-316)
  *   satt_plan_*            the same loop with the model, gathers and fields
  *                          resident in device memory across calls
+ *   satt_smooth_geo2       gaussian_smooth_geo_2  src/shared/utils.f90:1194
+ *   satt_gradient_geo2     gradient_2_geo         src/shared/utils.f90:1118
+ *   satt_topo_metric       the metric loop of att_grid%get_topo
+ *                          (src/grid.f90:114-123,137-141)
+ *   satt_steepest_descent  att_tomo_2d%optimize + steepest_descent
+ *                          (src/tomo2d.f90:203-251)
  *
  * Conventions
  *   - A grid field f(nx,ny) is column-major with x fastest: f[i + nx*j], as
@@ -103,6 +109,60 @@ int satt_mask(satt_ctx *ctx, const double *xx, const double *yy, int nx, int ny,
 int satt_interp2(satt_ctx *ctx, const double *x, const double *y, int nx,
                  int ny, const double *v, const double *xq, const double *yq,
                  int nq, double *vq);
+
+/* ------------------------------------------- model-side steps (per call) */
+/* Axes for these calls: lons(nx), lats(ny) in degrees, increasing, nx and ny
+ * >= 2, nx <= SATT_GEO_MAX_NX, the longitude axis uniform (as arange makes it,
+ * src/shared/utils.f90:645-653; to 1e-6 of its spacing). */
+#define SATT_GEO_MAX_NX 4096
+
+/* out = gaussian_smooth_geo_2(data, lons, lats, sigma) per field
+ * (src/shared/utils.f90:1194-1223), nfield fields at once.  data holds ndata
+ * fields, ndata = nfield or 1 (one field smoothed with every sigma); data and
+ * out are period-major, (ndata, nx, ny) and (nfield, nx, ny); sigma[nfield]
+ * in degrees, > 0 (the reference's tomo2d refuses sigma_2d = 0,
+ * src/tomo2d.f90:48-49).  Within 1e-12 of the reference relative to max|data|:
+ * a weight is evaluated per (row pair, column offset) rather than per node
+ * pair, which differs from the reference by the rounding of lon(i')-lon(i). */
+int satt_smooth_geo2(satt_ctx *ctx, const double *lons, const double *lats,
+                     int nx, int ny, int nfield, int ndata, const double *data,
+                     const double *sigma, double *out);
+
+/* gradient_2_geo(f, lon, lat, tx, ty) (src/shared/utils.f90:1118-1160) per
+ * field, nfield fields, period-major (nfield, nx, ny) in and out. */
+int satt_gradient_geo2(satt_ctx *ctx, const double *lons, const double *lats,
+                       int nx, int ny, int nfield, const double *f, double *tx,
+                       double *ty);
+
+/* The topographic metric of att_grid%get_topo (src/grid.f90:114-123,137-141)
+ * for nperiod periods: z(nx,ny) (km, on the model grid, as after
+ * att_topo%grid_topo) smoothed with sigma[p], then m11 = a, m22 = b,
+ * m12 = -c from its gradient, and the dip angle (src/topo.f90:123-130) in
+ * degrees unless angle is NULL.  Outputs period-major (nperiod, nx, ny). */
+int satt_topo_metric(satt_ctx *ctx, const double *lons, const double *lats,
+                     int nx, int ny, const double *z, int nperiod,
+                     const double *sigma, double *m11, double *m12,
+                     double *m22, double *angle);
+
+/* sigma of one period for satt_topo_metric, src/grid.f90:116:
+ * vel*period*wavelen_factor*km2deg/(2*pi), vel the period's velocity of the
+ * 1-D model (fwdsurf1d), in degrees. */
+double satt_topo_sigma(double vel, double period, double wavelen_factor);
+
+/* att_tomo_2d%optimize followed by steepest_descent (src/tomo2d.f90:203-251):
+ * ker_s(p,:,:) = gaussian_smooth_geo_2(adj_s(p,:,:), lons, lats, sigma_2d)
+ * for every period, then svel = svel*(1 + updatemax*ker_s/maxval(abs(ker_s)))
+ * in place.  The caller applies the misfit-increase shrink of updatemax
+ * (:238-246) first.  adj_s, svel, ker_s period-major (nperiod, nx, ny). */
+int satt_steepest_descent(satt_ctx *ctx, const double *lons, const double *lats,
+                          int nx, int ny, int nperiod, const double *adj_s,
+                          double sigma_2d, double updatemax, double *svel,
+                          double *ker_s);
+
+/* Device time in ms of the kernels of the last satt_smooth_geo2,
+ * satt_gradient_geo2, satt_topo_metric or satt_steepest_descent call on the
+ * context (HIP events on its stream; transfers excluded). */
+double satt_last_device_ms(const satt_ctx *ctx);
 
 /* ------------------------------------------------------------ batched form */
 

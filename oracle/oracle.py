@@ -52,6 +52,10 @@ def lib():
                      "orc_adj_mixed", "orc_adj_mixed_pre"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.orc_mask.restype = None
+        _lib.orc_gps2dist.restype = ctypes.c_double
+        _lib.orc_gps2dist.argtypes = [ctypes.c_double] * 4
+        _lib.orc_topo_sigma.restype = ctypes.c_double
+        _lib.orc_topo_sigma.argtypes = [ctypes.c_double] * 3
     return _lib
 
 
@@ -292,3 +296,86 @@ class RefSolver:
                        ctypes.byref(ctypes.c_int(len(yy))), _d(t),
                        ctypes.byref(ctypes.c_double(x0)), ctypes.byref(ctypes.c_double(y0)))
         return t
+
+
+# ---- model-side steps (satt_oracle_geo.c)
+
+def gps2dist(lat0, lon0, lat1, lon1):
+    """src/shared/utils.f90:1250-1262, km."""
+    return lib().orc_gps2dist(float(lat0), float(lon0), float(lat1), float(lon1))
+
+
+def smooth_halfwidths(lons, lats, sigma):
+    """(hx, [hy(j)]) of gaussian_smooth_geo_2 (src/shared/utils.f90:1204-1212)."""
+    lons = np.ascontiguousarray(lons, dtype=np.float64)
+    lats = np.ascontiguousarray(lats, dtype=np.float64)
+    L = lib()
+    hx = L.orc_smooth_halfwidth_x(_d(lons), ctypes.c_double(sigma))
+    hy = [L.orc_smooth_halfwidth_y(_d(lats), j, ctypes.c_double(sigma)) for j in range(len(lats))]
+    return hx, hy
+
+
+def smooth_geo2(lons, lats, data, sigma, rows=None):
+    """gaussian_smooth_geo_2 (src/shared/utils.f90:1194-1223) of one field
+    (nx, ny); ``rows`` = (j0, j1) restricts the output rows (others NaN)."""
+    lons = np.ascontiguousarray(lons, dtype=np.float64)
+    lats = np.ascontiguousarray(lats, dtype=np.float64)
+    sh = (len(lons), len(lats))
+    data = _f(data, sh)
+    out = np.full(sh, np.nan, order="F")
+    j0, j1 = rows if rows is not None else (0, sh[1])
+    st = lib().orc_smooth_geo2_rows(_d(lons), _d(lats), sh[0], sh[1], _d(data),
+                                    ctypes.c_double(sigma), int(j0), int(j1), _d(out))
+    if st != 0:
+        raise OracleError(st, "orc_smooth_geo2_rows")
+    return out
+
+
+def gradient_geo2(lons, lats, f):
+    """gradient_2_geo (src/shared/utils.f90:1118-1160): (tx, ty)."""
+    lons = np.ascontiguousarray(lons, dtype=np.float64)
+    lats = np.ascontiguousarray(lats, dtype=np.float64)
+    sh = (len(lons), len(lats))
+    f = _f(f, sh)
+    tx, ty = np.zeros(sh, order="F"), np.zeros(sh, order="F")
+    st = lib().orc_gradient_geo2(_d(f), _d(lons), _d(lats), sh[0], sh[1], _d(tx), _d(ty))
+    if st != 0:
+        raise OracleError(st, "orc_gradient_geo2")
+    return tx, ty
+
+
+def topo_sigma(vel, period, wavelen_factor):
+    """src/grid.f90:116, degrees."""
+    return lib().orc_topo_sigma(float(vel), float(period), float(wavelen_factor))
+
+
+def topo_metric(lons, lats, z, sigma):
+    """get_topo's metric (src/grid.f90:114-123,137-141) and dip angle
+    (src/topo.f90:123-130): (m11, m12, m22, angle), each (nperiod, nx, ny)."""
+    lons = np.ascontiguousarray(lons, dtype=np.float64)
+    lats = np.ascontiguousarray(lats, dtype=np.float64)
+    sh = (len(lons), len(lats))
+    z = _f(z, sh)
+    sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=np.float64)
+    outs = [np.zeros((len(sig),) + sh, order="F") for _ in range(4)]
+    st = lib().orc_topo_metric(_d(lons), _d(lats), sh[0], sh[1], _d(z), len(sig), _d(sig),
+                               *(_d(o) for o in outs))
+    if st != 0:
+        raise OracleError(st, "orc_topo_metric")
+    return tuple(outs)
+
+
+def steepest_descent(lons, lats, adj_s, sigma_2d, updatemax, svel):
+    """optimize + steepest_descent (src/tomo2d.f90:203-251): (svel_new, ker_s)."""
+    lons = np.ascontiguousarray(lons, dtype=np.float64)
+    lats = np.ascontiguousarray(lats, dtype=np.float64)
+    adj_s = np.asfortranarray(adj_s, dtype=np.float64)
+    sv = np.array(svel, dtype=np.float64, order="F", copy=True)
+    ker = np.zeros(adj_s.shape, order="F")
+    nf, nx, ny = adj_s.shape
+    st = lib().orc_steepest_descent(_d(lons), _d(lats), nx, ny, nf, _d(adj_s),
+                                    ctypes.c_double(sigma_2d), ctypes.c_double(updatemax),
+                                    _d(sv), _d(ker))
+    if st != 0:
+        raise OracleError(st, "orc_steepest_descent")
+    return sv, ker

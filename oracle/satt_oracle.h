@@ -128,6 +128,33 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
                          double *adj_density, double *item_chi, int *iters,
                          double *weight_out);
 
+/* ---- model-side steps (satt_oracle_geo.c) */
+/* gps2dist (src/shared/utils.f90:1250-1294), km */
+double orc_gps2dist(double lat0, double lon0, double lat1, double lon1);
+/* window half-widths of gaussian_smooth_geo_2 (src/shared/utils.f90:1204-1212) */
+int orc_smooth_halfwidth_x(const double *lons, double sigma);
+int orc_smooth_halfwidth_y(const double *lats, int j, double sigma);
+/* gaussian_smooth_geo_2 (src/shared/utils.f90:1194-1223), one field, rows
+ * j0..j1-1 (the others are left as they are) */
+int orc_smooth_geo2_rows(const double *lons, const double *lats, int nx, int ny,
+                         const double *data, double sigma, int j0, int j1, double *out);
+int orc_smooth_geo2(const double *lons, const double *lats, int nx, int ny, const double *data,
+                    double sigma, double *out);
+/* gradient_2_geo (src/shared/utils.f90:1118-1160) */
+int orc_gradient_geo2(const double *f, const double *lon, const double *lat, int nx, int ny,
+                      double *tx, double *ty);
+/* src/grid.f90:116 */
+double orc_topo_sigma(double vel, double period, double wavelen_factor);
+/* get_topo's metric and dip angle (src/grid.f90:114-123,137-141;
+ * src/topo.f90:123-130), outputs period-major; angle may be NULL */
+int orc_topo_metric(const double *lons, const double *lats, int nx, int ny, const double *z,
+                    int nperiod, const double *sigma, double *m11, double *m12, double *m22,
+                    double *angle);
+/* optimize + steepest_descent (src/tomo2d.f90:203-251), period-major */
+int orc_steepest_descent(const double *lons, const double *lats, int nx, int ny, int nperiod,
+                         const double *adj_s, double sigma_2d, double updatemax, double *svel,
+                         double *ker_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -30,6 +31,7 @@ struct satt_ctx {
   std::string err;
   std::mutex tune_mu;
   std::map<TuneKey, int> tuned;  // team sizes measured by satt_tune_waves
+  double last_ms = 0;            // device time of the last model-side call
 };
 
 int satt::ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k) {
@@ -362,6 +364,271 @@ extern "C" int satt_interp2(satt_ctx *ctx, const double *x, const double *y, int
 }
 
 // --------------------------------------------------------------- inspection
+
+// ------------------------------------------------------- model-side steps
+
+namespace {
+
+// cosd of the Fortran runtime: cos of the argument times pi/180 (true pi)
+double cosd_host(double x) { return std::cos(x * (3.14159265358979323846 / 180.0)); }
+
+int geo_axes_check(satt_ctx *ctx, const char *who, const double *lons, const double *lats,
+                   int nx, int ny) {
+  if (!lons || !lats || nx < 2 || ny < 2) return fail(ctx, SATT_E_ARG, "%s: bad grid axes", who);
+  if (nx > SATT_GEO_MAX_NX)
+    return fail(ctx, SATT_E_ARG, "%s: nx %d above %d", who, nx, SATT_GEO_MAX_NX);
+  const double d = lons[1] - lons[0];
+  if (!(d > 0) || !(lats[1] - lats[0] > 0))
+    return fail(ctx, SATT_E_ARG, "%s: axes must increase", who);
+  for (int i = 2; i < nx; ++i)
+    if (!(std::fabs((lons[i] - lons[0]) - i * d) <= 1e-6 * d))
+      return fail(ctx, SATT_E_ARG, "%s: longitude axis not uniform at %d", who, i);
+  for (int j = 1; j < ny; ++j)
+    if (!(lats[j] > lats[j - 1])) return fail(ctx, SATT_E_ARG, "%s: axes must increase", who);
+  if ((size_t)nx * ny > (size_t)INT_MAX) return fail(ctx, SATT_E_ARG, "%s: grid too large", who);
+  return SATT_OK;
+}
+
+// fields ride the grid's y dimension in the launches
+int geo_fields_check(satt_ctx *ctx, const char *who, int nfield) {
+  if (nfield < 1 || nfield > 65535)
+    return fail(ctx, SATT_E_ARG, "%s: %d fields (1..65535)", who, nfield);
+  return SATT_OK;
+}
+
+// Events around the kernels of one call; the elapsed time goes to last_ms.
+struct GeoTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~GeoTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+int geo_timer_start(satt_ctx *ctx, GeoTimer &t) {
+  HIPCHK(ctx, hipEventCreate(&t.a));
+  HIPCHK(ctx, hipEventCreate(&t.b));
+  HIPCHK(ctx, hipEventRecord(t.a, ctx->stream));
+  return SATT_OK;
+}
+
+int geo_timer_stop(satt_ctx *ctx, GeoTimer &t) {
+  HIPCHK(ctx, hipEventRecord(t.b, ctx->stream));
+  return SATT_OK;
+}
+
+int geo_timer_read(satt_ctx *ctx, GeoTimer &t) {
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, t.a, t.b));
+  ctx->last_ms = ms;
+  return SATT_OK;
+}
+
+// Device state of a smoothing: axes, s2, per-field sigma and window
+// half-widths (the reference's truncations, src/shared/utils.f90:1204-1212,
+// evaluated on the host), the natural-layout output.
+struct GeoSmoothBufs {
+  DBuf axes, tab, out;
+  GeoSmooth P{};
+  GeoGrad G{};
+};
+
+int geo_smooth_setup(satt_ctx *ctx, const char *who, const double *lons, const double *lats,
+                     int nx, int ny, int nfield, const double *sigma, GeoSmoothBufs &B) {
+  for (int f = 0; f < nfield; ++f)
+    if (!(sigma[f] > 0) || !std::isfinite(sigma[f]))
+      return fail(ctx, SATT_E_ARG, "%s: sigma[%d] must be positive", who, f);
+  const size_t N = (size_t)nx * ny;
+  const double dx = lons[1] - lons[0], dy = lats[1] - lats[0];
+  // doubles: lon, lat, cosd(lat), s2, sigma; ints: hx, hy
+  std::vector<double> hd(2 * (size_t)nx + 2 * (size_t)ny + nfield);
+  std::memcpy(hd.data(), lons, nx * 8);
+  std::memcpy(hd.data() + nx, lats, ny * 8);
+  for (int j = 0; j < ny; ++j) hd[nx + ny + j] = cosd_host(lats[j]);
+  std::memcpy(hd.data() + 2 * (size_t)nx + 2 * ny, sigma, nfield * 8);
+  std::vector<int> hi((size_t)nfield * (1 + ny));
+  for (int f = 0; f < nfield; ++f) {
+    // nx = sigma3/dx, ny = sigma3/cosd(lat)/dy, truncated; windows wider
+    // than the grid are the same as the grid
+    const double sx = sigma[f] * 3 / dx;
+    hi[f] = sx < nx ? (int)sx : nx;
+    for (int j = 0; j < ny; ++j) {
+      const double sy = sigma[f] * 3 / cosd_host(lats[j]) / dy;
+      hi[nfield + (size_t)f * ny + j] = sy >= 0 && sy < ny ? (int)sy : ny;
+    }
+  }
+  TRY(upload(ctx, B.axes, hd.data(), hd.size() * 8));
+  ALLOC(ctx, B.tab, (size_t)nx * 8 + hi.size() * sizeof(int));
+  double *ax = B.axes.as<double>();
+  double *s2 = B.tab.as<double>();
+  int *dh = reinterpret_cast<int *>(s2 + nx);
+  HIPCHK(ctx, hipMemcpyAsync(dh, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice,
+                             ctx->stream));
+  ALLOC(ctx, B.out, (size_t)nfield * N * 8);
+  launch_geo_s2(ctx->stream, ax, nx, s2);
+  B.P.nx = nx;
+  B.P.ny = ny;
+  B.P.lat = ax + nx;
+  B.P.s2 = s2;
+  B.P.sigma = ax + 2 * (size_t)nx + 2 * ny;
+  B.P.hx = dh;
+  B.P.hy = dh + nfield;
+  B.P.out = B.out.as<double>();
+  B.G.nx = nx;
+  B.G.ny = ny;
+  B.G.lon = ax;
+  B.G.lat = ax + nx;
+  B.G.cosd_lat = ax + nx + ny;
+  return SATT_OK;
+}
+
+}  // namespace
+
+extern "C" double satt_last_device_ms(const satt_ctx *ctx) { return ctx ? ctx->last_ms : 0.0; }
+
+extern "C" double satt_topo_sigma(double vel, double period, double wavelen_factor) {
+  const double pi = (double)3.14159265359f;
+  const double km2deg = 1.0 / (6371.0 * pi / 180.0);
+  return vel * period * wavelen_factor * km2deg / (2.0 * pi);
+}
+
+extern "C" int satt_smooth_geo2(satt_ctx *ctx, const double *lons, const double *lats, int nx,
+                                int ny, int nfield, int ndata, const double *data,
+                                const double *sigma, double *out) {
+  if (!ctx) return SATT_E_ARG;
+  TRY(geo_axes_check(ctx, "satt_smooth_geo2", lons, lats, nx, ny));
+  TRY(geo_fields_check(ctx, "satt_smooth_geo2", nfield));
+  if (nfield < 1 || (ndata != 1 && ndata != nfield) || !data || !sigma || !out)
+    return fail(ctx, SATT_E_ARG, "satt_smooth_geo2: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t N = (size_t)nx * ny;
+  GeoSmoothBufs B;
+  TRY(geo_smooth_setup(ctx, "satt_smooth_geo2", lons, lats, nx, ny, nfield, sigma, B));
+  DBuf din, dnat;
+  TRY(upload(ctx, din, data, (size_t)ndata * N * 8));
+  const double *nat = din.as<double>();
+  if (ndata > 1) {
+    ALLOC(ctx, dnat, (size_t)ndata * N * 8);
+    launch_pm_to_nat(ctx->stream, N, ndata, din.as<double>(), dnat.as<double>());
+    nat = dnat.as<double>();
+  }
+  B.P.ndata = ndata;
+  B.P.data = nat;
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_smooth_geo2(ctx->stream, B.P, nfield);
+  TRY(geo_timer_stop(ctx, t));
+  double *res = B.out.as<double>();
+  if (nfield > 1) {  // back to period-major in the input buffer's place
+    DBuf pm;
+    ALLOC(ctx, pm, (size_t)nfield * N * 8);
+    launch_to_period_major(ctx->stream, (int)N, nfield, res, pm.as<double>());
+    HIPCHK(ctx, hipMemcpyAsync(out, pm.p, (size_t)nfield * N * 8, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    TRY(sync_status(ctx));
+  } else {
+    HIPCHK(ctx, hipMemcpyAsync(out, res, N * 8, hipMemcpyDeviceToHost, ctx->stream));
+    TRY(sync_status(ctx));
+  }
+  return geo_timer_read(ctx, t);
+}
+
+extern "C" int satt_gradient_geo2(satt_ctx *ctx, const double *lons, const double *lats, int nx,
+                                  int ny, int nfield, const double *f, double *tx, double *ty) {
+  if (!ctx) return SATT_E_ARG;
+  TRY(geo_axes_check(ctx, "satt_gradient_geo2", lons, lats, nx, ny));
+  TRY(geo_fields_check(ctx, "satt_gradient_geo2", nfield));
+  if (nfield < 1 || !f || !tx || !ty)
+    return fail(ctx, SATT_E_ARG, "satt_gradient_geo2: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t N = (size_t)nx * ny, M = (size_t)nfield * N;
+  std::vector<double> hd(2 * (size_t)nx + 2 * (size_t)ny);
+  std::memcpy(hd.data(), lons, nx * 8);
+  std::memcpy(hd.data() + nx, lats, ny * 8);
+  for (int j = 0; j < ny; ++j) hd[nx + ny + j] = cosd_host(lats[j]);
+  DBuf ax, din, work;
+  TRY(upload(ctx, ax, hd.data(), hd.size() * 8));
+  TRY(upload(ctx, din, f, M * 8));
+  ALLOC(ctx, work, 3 * M * 8);
+  double *nat = work.as<double>(), *dtx = nat + M, *dty = dtx + M;
+  GeoGrad G{nx, ny, ax.as<double>(), ax.as<double>() + nx, ax.as<double>() + nx + ny};
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_pm_to_nat(ctx->stream, N, nfield, din.as<double>(), nat);
+  launch_gradient_geo2(ctx->stream, G, nfield, nat, dtx, dty);
+  TRY(geo_timer_stop(ctx, t));
+  HIPCHK(ctx, hipMemcpyAsync(tx, dtx, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ty, dty, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
+extern "C" int satt_topo_metric(satt_ctx *ctx, const double *lons, const double *lats, int nx,
+                                int ny, const double *z, int nperiod, const double *sigma,
+                                double *m11, double *m12, double *m22, double *angle) {
+  if (!ctx) return SATT_E_ARG;
+  TRY(geo_axes_check(ctx, "satt_topo_metric", lons, lats, nx, ny));
+  TRY(geo_fields_check(ctx, "satt_topo_metric", nperiod));
+  if (nperiod < 1 || !z || !sigma || !m11 || !m12 || !m22)
+    return fail(ctx, SATT_E_ARG, "satt_topo_metric: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t N = (size_t)nx * ny, M = (size_t)nperiod * N;
+  GeoSmoothBufs B;
+  TRY(geo_smooth_setup(ctx, "satt_topo_metric", lons, lats, nx, ny, nperiod, sigma, B));
+  DBuf dz, res;
+  TRY(upload(ctx, dz, z, N * 8));
+  ALLOC(ctx, res, (angle ? 4 : 3) * M * 8);
+  double *o11 = res.as<double>(), *o12 = o11 + M, *o22 = o12 + M;
+  double *oag = angle ? o22 + M : nullptr;
+  B.P.ndata = 1;
+  B.P.data = dz.as<double>();
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_smooth_geo2(ctx->stream, B.P, nperiod);
+  launch_topo_metric(ctx->stream, B.G, nperiod, B.out.as<double>(), o11, o12, o22, oag);
+  TRY(geo_timer_stop(ctx, t));
+  HIPCHK(ctx, hipMemcpyAsync(m11, o11, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(m12, o12, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(m22, o22, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (angle)
+    HIPCHK(ctx, hipMemcpyAsync(angle, oag, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
+extern "C" int satt_steepest_descent(satt_ctx *ctx, const double *lons, const double *lats,
+                                     int nx, int ny, int nperiod, const double *adj_s,
+                                     double sigma_2d, double updatemax, double *svel,
+                                     double *ker_s) {
+  if (!ctx) return SATT_E_ARG;
+  TRY(geo_axes_check(ctx, "satt_steepest_descent", lons, lats, nx, ny));
+  TRY(geo_fields_check(ctx, "satt_steepest_descent", nperiod));
+  if (nperiod < 1 || !adj_s || !svel || !ker_s || !std::isfinite(updatemax))
+    return fail(ctx, SATT_E_ARG, "satt_steepest_descent: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t N = (size_t)nx * ny, M = (size_t)nperiod * N;
+  std::vector<double> sig(nperiod, sigma_2d);
+  GeoSmoothBufs B;
+  TRY(geo_smooth_setup(ctx, "satt_steepest_descent", lons, lats, nx, ny, nperiod, sig.data(), B));
+  DBuf din, dsv, work;
+  TRY(upload(ctx, din, adj_s, M * 8));
+  TRY(upload(ctx, dsv, svel, M * 8));
+  ALLOC(ctx, work, (2 * M + absmax_parts(M) + 1) * 8);
+  double *nat = work.as<double>(), *kpm = nat + M, *part = kpm + M;
+  B.P.ndata = nperiod;
+  B.P.data = nat;
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_pm_to_nat(ctx->stream, N, nperiod, din.as<double>(), nat);
+  launch_smooth_geo2(ctx->stream, B.P, nperiod);
+  launch_to_period_major(ctx->stream, (int)N, nperiod, B.out.as<double>(), kpm);
+  launch_sd_update(ctx->stream, M, kpm, updatemax, dsv.as<double>(), part);
+  TRY(geo_timer_stop(ctx, t));
+  HIPCHK(ctx, hipMemcpyAsync(ker_s, kpm, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(svel, dsv.p, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
 
 extern "C" int satt_fwd_setup_fields(satt_ctx *ctx, const double *xx_deg,
                                      const double *yy_deg, int nx, int ny,

@@ -173,4 +173,36 @@ void launch_adj_delta(hipStream_t s, const Grid &g, const double *xr,
                       const double *yr, const double *rhs, int nr,
                       double *delta, int *status);
 
+// ---- geographic smoothing, gradient, metric (satt_geo.hip); all on `s`
+// Smoothing of nfield fields (natural layout [f][j][i], or one shared field
+// when ndata is 1) with per-field sigma (degrees); window half-widths hx[f]
+// and hy[f*ny + j] from the host; s2 from launch_geo_s2.
+struct GeoSmooth {
+  int nx, ny, ndata;
+  const double *lat;            // degrees
+  const double *s2;             // sin((lon(k)-lon(0))*deg2rad/2)**2, k < nx
+  const double *data, *sigma;
+  const int *hx, *hy;
+  double *out;                  // [f][j][i]
+};
+struct GeoGrad {
+  int nx, ny;
+  const double *lon, *lat;      // degrees
+  const double *cosd_lat;       // cosd(lat(j)), host-computed
+};
+size_t smooth_lds_bytes(int nx);
+void launch_geo_s2(hipStream_t s, const double *lon, int nx, double *s2);
+void launch_smooth_geo2(hipStream_t s, const GeoSmooth &P, int nfield);
+// tx, ty period-major (nf, nx, ny) of nf natural fields
+void launch_gradient_geo2(hipStream_t s, const GeoGrad &G, int nf, const double *f_nat,
+                          double *tx_pm, double *ty_pm);
+// m11, m12, m22 (and angle unless null) period-major from nf smoothed fields
+void launch_topo_metric(hipStream_t s, const GeoGrad &G, int nf, const double *sm_nat,
+                        double *m11, double *m12, double *m22, double *angle);
+void launch_pm_to_nat(hipStream_t s, size_t N, int nf, const double *pm, double *nat);
+// scratch `part` holds absmax_parts(n) + 1 doubles
+int absmax_parts(size_t n);
+void launch_sd_update(hipStream_t s, size_t n, const double *ker, double updatemax, double *svel,
+                      double *part);
+
 }  // namespace satt

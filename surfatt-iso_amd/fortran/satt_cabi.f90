@@ -92,6 +92,52 @@ module satt_cabi
       real(c_double), intent(out) :: vq(*)
     end function satt_interp2
 
+    integer(c_int) function satt_smooth_geo2(ctx, lons, lats, nx, ny, nfield, ndata, data, &
+                                             sigma, out) bind(C, name="satt_smooth_geo2")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nfield, ndata
+      real(c_double), intent(in) :: lons(*), lats(*), data(*), sigma(*)
+      real(c_double), intent(out) :: out(*)
+    end function satt_smooth_geo2
+
+    integer(c_int) function satt_gradient_geo2(ctx, lons, lats, nx, ny, nfield, f, tx, ty) &
+                                               bind(C, name="satt_gradient_geo2")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nfield
+      real(c_double), intent(in) :: lons(*), lats(*), f(*)
+      real(c_double), intent(out) :: tx(*), ty(*)
+    end function satt_gradient_geo2
+
+    integer(c_int) function satt_topo_metric(ctx, lons, lats, nx, ny, z, nperiod, sigma, &
+                                             m11, m12, m22, angle) &
+                                             bind(C, name="satt_topo_metric")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nperiod
+      real(c_double), intent(in) :: lons(*), lats(*), z(*), sigma(*)
+      real(c_double), intent(out) :: m11(*), m12(*), m22(*), angle(*)
+    end function satt_topo_metric
+
+    real(c_double) function satt_topo_sigma(vel, period, wavelen_factor) &
+                                            bind(C, name="satt_topo_sigma")
+      import :: c_double
+      real(c_double), value :: vel, period, wavelen_factor
+    end function satt_topo_sigma
+
+    integer(c_int) function satt_steepest_descent(ctx, lons, lats, nx, ny, nperiod, adj_s, &
+                                                  sigma_2d, updatemax, svel, ker_s) &
+                                                  bind(C, name="satt_steepest_descent")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nperiod
+      real(c_double), intent(in) :: lons(*), lats(*), adj_s(*)
+      real(c_double), value :: sigma_2d, updatemax
+      real(c_double), intent(inout) :: svel(*)
+      real(c_double), intent(out) :: ker_s(*)
+    end function satt_steepest_descent
+
     integer(c_int) function satt_forward_simulate(ctx, batch, istotable, isadj, precision, &
                                                   result) bind(C, name="satt_forward_simulate")
       import :: c_int, c_ptr, satt_batch, satt_result

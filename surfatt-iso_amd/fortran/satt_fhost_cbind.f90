@@ -34,3 +34,48 @@ subroutine satt_fhost_forward_simulate(dim, nx, ny, nperiod, xgrids, ygrids, per
                                   adj_density, st)
   status = st
 end subroutine satt_fhost_forward_simulate
+
+!=====================================================================
+! C-callable wrappers of the model-side host routines (for harnesses).
+!=====================================================================
+subroutine satt_fhost_topo_metric(nx, ny, nperiod, xgrids, ygrids, z, vel, periods, &
+                                  wavelen_factor, m11, m12, m22, topo_angle) &
+                                  bind(C, name="satt_fhost_topo_metric")
+  use, intrinsic :: iso_c_binding
+  use satt_host, only: satt_get_topo_metric
+  implicit none
+  integer(c_int), value :: nx, ny, nperiod
+  real(c_double), intent(in) :: xgrids(nx), ygrids(ny), z(nx,ny), vel(nperiod), periods(nperiod)
+  real(c_double), value :: wavelen_factor
+  real(c_double), intent(out) :: m11(nperiod,nx,ny), m12(nperiod,nx,ny), m22(nperiod,nx,ny)
+  real(c_double), intent(out) :: topo_angle(nperiod,nx,ny)
+  call satt_get_topo_metric(xgrids, ygrids, z, vel, periods, wavelen_factor, m11, m12, m22, &
+                            topo_angle)
+end subroutine satt_fhost_topo_metric
+
+subroutine satt_fhost_steepest_descent(nx, ny, nperiod, xgrids, ygrids, adj_s, sigma_2d, &
+                                       misfit_now, misfit_prev, maxshrink, updatemax, svel, &
+                                       ker_s) bind(C, name="satt_fhost_steepest_descent")
+  use, intrinsic :: iso_c_binding
+  use satt_host, only: satt_steepest_descent_host
+  implicit none
+  integer(c_int), value :: nx, ny, nperiod
+  real(c_double), intent(in) :: xgrids(nx), ygrids(ny), adj_s(nperiod,nx,ny)
+  real(c_double), value :: sigma_2d, misfit_now, misfit_prev, maxshrink
+  real(c_double), intent(inout) :: updatemax, svel(nperiod,nx,ny)
+  real(c_double), intent(out) :: ker_s(nperiod,nx,ny)
+  call satt_steepest_descent_host(xgrids, ygrids, adj_s, sigma_2d, misfit_now, misfit_prev, &
+                                  maxshrink, updatemax, svel, ker_s)
+end subroutine satt_fhost_steepest_descent
+
+subroutine satt_fhost_smooth(nx, ny, data, lons, lats, sigma, smdata) &
+                             bind(C, name="satt_fhost_smooth")
+  use, intrinsic :: iso_c_binding
+  use satt_host, only: satt_gaussian_smooth_geo_2
+  implicit none
+  integer(c_int), value :: nx, ny
+  real(c_double), intent(in) :: data(nx,ny), lons(nx), lats(ny)
+  real(c_double), value :: sigma
+  real(c_double), intent(out) :: smdata(nx,ny)
+  smdata = satt_gaussian_smooth_geo_2(data, lons, lats, sigma)
+end subroutine satt_fhost_smooth

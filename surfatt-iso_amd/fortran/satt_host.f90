@@ -16,6 +16,14 @@
 !                      the isrcs work list and the receiver gathers in the
 !                      reference's orders (src/src_rec.f90:162-217,252-283,
 !                      338-369) and runs every item in one device call.
+!  - satt_gaussian_smooth_geo_2(), satt_gradient_2_geo()  the reference's
+!                      gaussian_smooth_geo_2 and gradient_2_geo
+!                      (src/shared/utils.f90:1194,1118) with their arguments
+!  - satt_get_topo_metric()  the per-period metric loop of att_grid%get_topo
+!                      (src/grid.f90:114-123,137-141) over every period
+!  - satt_steepest_descent_host()  att_tomo_2d%optimize + steepest_descent
+!                      (src/tomo2d.f90:203-251), the misfit-increase shrink
+!                      of the step length included
 !=====================================================================
 module satt_host
   use, intrinsic :: iso_c_binding
@@ -23,6 +31,8 @@ module satt_host
   implicit none
   private
   public :: satt_host_ctx, satt_host_check, satt_host_release, satt_forward_simulate_host
+  public :: satt_gaussian_smooth_geo_2, satt_gradient_2_geo, satt_get_topo_metric
+  public :: satt_steepest_descent_host
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -188,6 +198,63 @@ contains
     end if
     chi = chi_c(1)
   end subroutine satt_forward_simulate_host
+
+  !> gaussian_smooth_geo_2(data, lons, lats, sigma) (src/shared/utils.f90:1194)
+  function satt_gaussian_smooth_geo_2(data, lons, lats, sigma) result(smdata)
+    real(c_double), intent(in) :: data(:,:), lons(:), lats(:), sigma
+    real(c_double), allocatable :: smdata(:,:)
+    real(c_double) :: sig(1), src(size(data,1), size(data,2))
+    allocate(smdata(size(data,1), size(data,2)))
+    sig(1) = sigma
+    src = data
+    call satt_host_check(satt_smooth_geo2(satt_host_ctx(), lons, lats, size(lons), size(lats), &
+                                          1, 1, src, sig, smdata), 'gaussian_smooth_geo_2')
+  end function satt_gaussian_smooth_geo_2
+
+  !> gradient_2_geo(f, lon, lat, tx, ty) (src/shared/utils.f90:1118)
+  subroutine satt_gradient_2_geo(f, lon, lat, tx, ty)
+    real(c_double), intent(in) :: f(:,:), lon(:), lat(:)
+    real(c_double), allocatable, intent(out) :: tx(:,:), ty(:,:)
+    real(c_double) :: src(size(f,1), size(f,2))
+    allocate(tx(size(f,1), size(f,2)), ty(size(f,1), size(f,2)))
+    src = f
+    call satt_host_check(satt_gradient_geo2(satt_host_ctx(), lon, lat, size(lon), size(lat), &
+                                            1, src, tx, ty), 'gradient_2_geo')
+  end subroutine satt_gradient_2_geo
+
+  !> The metric of att_grid%get_topo for every period (src/grid.f90:114-123,
+  !> 137-141): z is the topography on the model grid (km, as after
+  !> att_topo%grid_topo), vel(nperiod) the 1-D model's velocities from
+  !> fwdsurf1d; m11/m12/m22/topo_angle are (nperiod,nx,ny).
+  subroutine satt_get_topo_metric(xgrids, ygrids, z, vel, periods, wavelen_factor, &
+                                  m11, m12, m22, topo_angle)
+    real(c_double), intent(in) :: xgrids(:), ygrids(:), z(:,:), vel(:), periods(:)
+    real(c_double), intent(in) :: wavelen_factor
+    real(c_double), intent(out) :: m11(:,:,:), m12(:,:,:), m22(:,:,:), topo_angle(:,:,:)
+    real(c_double) :: sigma(size(periods))
+    integer :: ip
+    do ip = 1, size(periods)
+      sigma(ip) = satt_topo_sigma(vel(ip), periods(ip), wavelen_factor)
+    end do
+    call satt_host_check(satt_topo_metric(satt_host_ctx(), xgrids, ygrids, size(xgrids), &
+                                          size(ygrids), z, size(periods), sigma, m11, m12, &
+                                          m22, topo_angle), 'get_topo')
+  end subroutine satt_get_topo_metric
+
+  !> optimize + steepest_descent (src/tomo2d.f90:203-251): the step length
+  !> shrinks by maxshrink when the misfit rose (:238-246), then
+  !> svel = svel*(1 + updatemax*ker_s/maxval(abs(ker_s))).
+  subroutine satt_steepest_descent_host(xgrids, ygrids, adj_s, sigma_2d, misfit_now, &
+                                        misfit_prev, maxshrink, updatemax, svel, ker_s)
+    real(c_double), intent(in) :: xgrids(:), ygrids(:), adj_s(:,:,:), sigma_2d
+    real(c_double), intent(in) :: misfit_now, misfit_prev, maxshrink
+    real(c_double), intent(inout) :: updatemax, svel(:,:,:)
+    real(c_double), intent(out) :: ker_s(:,:,:)
+    if (misfit_now > misfit_prev) updatemax = updatemax*maxshrink
+    call satt_host_check(satt_steepest_descent(satt_host_ctx(), xgrids, ygrids, size(xgrids), &
+                                               size(ygrids), size(adj_s,1), adj_s, sigma_2d, &
+                                               updatemax, svel, ker_s), 'steepest_descent')
+  end subroutine satt_steepest_descent_host
 
   integer function find_name(names, n, s)
     character(len=*), intent(in) :: names(:), s

@@ -31,6 +31,12 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _lib = None
 
 
+def topo_sigma(vel, period, wavelen_factor):
+    """sigma (degrees) of one period for the topographic smoothing,
+    src/grid.f90:116."""
+    return float(load().satt_topo_sigma(float(vel), float(period), float(wavelen_factor)))
+
+
 class SattError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"status {status}: {msg}")
@@ -91,6 +97,9 @@ def load(path=LIB_PATH):
     L.satt_server_destroy.restype = None
     L.satt_last_error.restype = ctypes.c_char_p
     L.satt_version.restype = ctypes.c_char_p
+    L.satt_topo_sigma.restype = ctypes.c_double
+    L.satt_topo_sigma.argtypes = [ctypes.c_double] * 3
+    L.satt_last_device_ms.restype = ctypes.c_double
     _lib = L
     return L
 
@@ -202,6 +211,82 @@ class Context:
         self._check(self.lib.satt_interp2(self.ptr, _d(x), _d(y), len(x), len(y), _d(v), _d(xq),
                                           _d(yq), len(xq), _d(out)))
         return out
+
+    # ---- model-side steps (src/shared/utils.f90, src/grid.f90, src/tomo2d.f90)
+    @staticmethod
+    def _pm(v, nf, sh):
+        v = np.asarray(v, dtype=np.float64)
+        if v.shape == sh and nf == 1:
+            v = v.reshape((1,) + sh)
+        if v.shape != (nf,) + sh:
+            raise SattError(E_ARG, f"expected shape {(nf,) + sh}, got {v.shape}")
+        return np.asfortranarray(v)
+
+    def smooth_geo2(self, lons, lats, data, sigma):
+        """gaussian_smooth_geo_2 (src/shared/utils.f90:1194-1223) of each field of
+        ``data`` (nx, ny) or (nfield, nx, ny) with ``sigma`` (scalar or per field,
+        degrees); a single field with several sigmas is smoothed once per sigma.
+        Returns (nfield, nx, ny), or (nx, ny) for one field and one sigma."""
+        lons, lats = _c(lons), _c(lats)
+        sh = (len(lons), len(lats))
+        sig = np.atleast_1d(np.asarray(sigma, dtype=np.float64)).copy()
+        data = np.asarray(data, dtype=np.float64)
+        ndata = 1 if data.shape == sh else data.shape[0]
+        if sig.size == 1 and ndata > 1:
+            sig = np.full(ndata, sig[0])
+        nf = sig.size
+        src = self._pm(data, ndata, sh)
+        out = np.zeros((nf,) + sh, order="F")
+        self._check(self.lib.satt_smooth_geo2(self.ptr, _d(lons), _d(lats), sh[0], sh[1], nf,
+                                              ndata, _d(src), _d(sig), _d(out)))
+        return out[0] if (nf == 1 and data.shape == sh) else out
+
+    def gradient_geo2(self, lons, lats, f):
+        """gradient_2_geo (src/shared/utils.f90:1118-1160): (tx, ty) of ``f``,
+        (nx, ny) or (nfield, nx, ny)."""
+        lons, lats = _c(lons), _c(lats)
+        sh = (len(lons), len(lats))
+        f = np.asarray(f, dtype=np.float64)
+        one = f.shape == sh
+        nf = 1 if one else f.shape[0]
+        src = self._pm(f, nf, sh)
+        tx, ty = np.zeros((nf,) + sh, order="F"), np.zeros((nf,) + sh, order="F")
+        self._check(self.lib.satt_gradient_geo2(self.ptr, _d(lons), _d(lats), sh[0], sh[1], nf,
+                                                _d(src), _d(tx), _d(ty)))
+        return (tx[0], ty[0]) if one else (tx, ty)
+
+    def topo_metric(self, lons, lats, z, sigma, angle=True):
+        """The metric of att_grid%get_topo (src/grid.f90:114-123,137-141):
+        (m11, m12, m22[, angle]) of shape (nperiod, nx, ny) for topography ``z``
+        (nx, ny, km) and per-period ``sigma`` (degrees)."""
+        lons, lats = _c(lons), _c(lats)
+        sh = (len(lons), len(lats))
+        z = _f(z, sh)
+        sig = _c(np.atleast_1d(sigma))
+        nf = len(sig)
+        outs = [np.zeros((nf,) + sh, order="F") for _ in range(4 if angle else 3)]
+        self._check(self.lib.satt_topo_metric(self.ptr, _d(lons), _d(lats), sh[0], sh[1], _d(z),
+                                              nf, _d(sig), _d(outs[0]), _d(outs[1]),
+                                              _d(outs[2]), _d(outs[3]) if angle else None))
+        return tuple(outs)
+
+    def steepest_descent(self, lons, lats, adj_s, sigma_2d, updatemax, svel):
+        """att_tomo_2d%optimize + steepest_descent (src/tomo2d.f90:203-251):
+        returns (svel_new, ker_s), both (nperiod, nx, ny)."""
+        lons, lats = _c(lons), _c(lats)
+        sh = (len(lons), len(lats))
+        adj_s = np.asfortranarray(adj_s, dtype=np.float64)
+        nf = adj_s.shape[0]
+        sv = self._pm(svel, nf, sh).copy(order="F")
+        ker = np.zeros((nf,) + sh, order="F")
+        self._check(self.lib.satt_steepest_descent(
+            self.ptr, _d(lons), _d(lats), sh[0], sh[1], nf, _d(self._pm(adj_s, nf, sh)),
+            ctypes.c_double(sigma_2d), ctypes.c_double(updatemax), _d(sv), _d(ker)))
+        return sv, ker
+
+    def last_device_ms(self):
+        """Device time of the last model-side call's kernels (ms)."""
+        return float(self.lib.satt_last_device_ms(self.ptr))
 
     def fwd_setup_fields(self, xx, yy, spha, sphb, sphc, fun, x0, y0):
         xx, yy = _c(xx), _c(yy)

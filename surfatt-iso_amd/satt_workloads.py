@@ -247,6 +247,16 @@ def tomo2d_checkerboard(n=81, d=0.025, nsta=25, seed=0, pert=0.05, ncb=2, v0=3.0
     return bt, bi
 
 
+def topo_hills(n=401, d=0.005, nper=20, seed=3, nhill=8):
+    """The topography, periods and 1-D dispersion curve of the curved-surface
+    workload below (the same draws): (xg, yg, topo km, periods, vref)."""
+    xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
+    rng = np.random.default_rng(seed)
+    topo = gaussian_hills(xg, yg, nhill, rng)
+    periods = np.geomspace(3.0, 30.0, nper)
+    return xg, yg, topo, periods, 2.8 + 0.04 * periods
+
+
 def topo_checkerboard(n=401, d=0.005, nsrc=200, nper=20, seed=3, pert=0.08,
                       nhill=8, ncb=4, dim=2):
     """Curved-surface topography workload: n x n grid at d degrees, `nhill`

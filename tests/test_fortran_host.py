@@ -21,7 +21,8 @@ def test_fhost_exports_reference_names():
     assert os.path.exists(FHOST), "libsatt_fhost.so not built"
     lib = ctypes.CDLL(FHOST)
     for name in ("fsm_uw_ps_lonlat_2d_", "fsm_o1_jse_lonlat_2d_", "mask_",
-                 "satt_fhost_forward_simulate"):
+                 "satt_fhost_forward_simulate", "satt_fhost_smooth", "satt_fhost_topo_metric",
+                 "satt_fhost_steepest_descent"):
         assert hasattr(lib, name), name
 
 
@@ -108,3 +109,37 @@ def test_fortran_forward_simulate_matches_c_abi(ctx, dim):
         assert np.array_equal(adj_d, out["adj_density"])
     ref = O.forward_simulate(b)
     assert rel_err(tt_fwd, ref["tt_fwd"]) < 1e-9
+
+
+@pytest.mark.gpu
+def test_fortran_model_side_routines_match_c_abi(ctx):
+    """gaussian_smooth_geo_2, the get_topo metric loop and optimize +
+    steepest_descent called through the Fortran host equal the C-ABI calls
+    bit for bit; the step length shrinks when the misfit rose
+    (src/tomo2d.f90:238-246)."""
+    lib = ctypes.CDLL(FHOST)
+    nx, ny = 30, 24
+    xg = 100.0 + 0.03 * np.arange(nx); yg = 25.0 + 0.03 * np.arange(ny)
+    rng = np.random.default_rng(8)
+    z = np.asfortranarray(rng.uniform(0, 2, (nx, ny)))
+    sm = np.zeros((nx, ny), order="F")
+    lib.satt_fhost_smooth(nx, ny, _d(z), _d(xg), _d(yg), ctypes.c_double(0.08), _d(sm))
+    assert np.array_equal(sm, ctx.smooth_geo2(xg, yg, z, 0.08))
+    periods = np.array([5.0, 10.0, 20.0]); vel = np.array([3.0, 3.3, 3.6])
+    outs = [np.zeros((3, nx, ny), order="F") for _ in range(4)]
+    lib.satt_fhost_topo_metric(nx, ny, 3, _d(xg), _d(yg), _d(z), _d(vel), _d(periods),
+                               ctypes.c_double(2.5), *(_d(o) for o in outs))
+    sig = [satt.topo_sigma(v, p, 2.5) for v, p in zip(vel, periods)]
+    want = ctx.topo_metric(xg, yg, z, sig)
+    assert all(np.array_equal(a, b) for a, b in zip(outs, want))
+    adj = np.asfortranarray(rng.uniform(-1, 1, (3, nx, ny)))
+    svel = np.asfortranarray(rng.uniform(2.8, 3.6, (3, nx, ny)))
+    for now, prev, step in ((2.0, 1.0, 0.02 * 0.8), (1.0, 2.0, 0.02)):
+        sv = svel.copy(order="F"); ker = np.zeros((3, nx, ny), order="F")
+        upd = ctypes.c_double(0.02)
+        lib.satt_fhost_steepest_descent(nx, ny, 3, _d(xg), _d(yg), _d(adj), ctypes.c_double(0.07),
+                                        ctypes.c_double(now), ctypes.c_double(prev),
+                                        ctypes.c_double(0.8), ctypes.byref(upd), _d(sv), _d(ker))
+        assert upd.value == step
+        wsv, wker = ctx.steepest_descent(xg, yg, adj, 0.07, step, svel)
+        assert np.array_equal(sv, wsv) and np.array_equal(ker, wker)
