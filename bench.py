@@ -30,6 +30,8 @@ import satt  # noqa: E402
 import satt_workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# sweep stages with their own team size (satt_tune_result.stage_waves order)
+STAGES = ("fwd_fp32", "fwd_fp64", "adj_fp32", "adj_fp64")
 
 WORKLOADS = {
     # name: (builder, kwargs, description); a builder returns (batch_true,
@@ -346,7 +348,7 @@ def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
                                "sample": f"oracle gaussian_smooth_geo_2 on {len(sample)} rows "
                                          f"({snodes:.3g} window nodes, {cdt:.1f} s)",
                                "est_full_topo_metric_s": round(nodes / rate, 1)}
-        res["vs_cpu_core"] = round(nodes / dev / 1e3 / rate, 1)
+        res["vs_cpu_core"] = round(nodes / (dev * 1e-3) / rate, 1)
     return res
 
 
@@ -367,6 +369,9 @@ def main():
                     help="skip the HIP-graph replay timing of the same step")
     ap.add_argument("--no-tune", action="store_true",
                     help="one wave per item, no team-size tuning (and no comparison run)")
+    ap.add_argument("--stage-waves", default=None, metavar="W,W,W,W",
+                    help="team sizes of the four sweep stages instead of tuning them "
+                         "(forward FP32, forward FP64, adjoint FP32, adjoint FP64)")
     ap.add_argument("--workload", default="topo401", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--oracle-items", type=int, default=4,
@@ -422,12 +427,21 @@ def main():
         # team size per launch group from the autotuner (it times one wave
         # per item against teams of 2-4 waves on a synthetic batch of the
         # group's shape); the one-wave form is timed after, for comparison
-        ttune = None
-        if not args.no_tune:
+        ttune, tuned_w = None, None
+        if args.stage_waves:
+            sw = [int(v) for v in args.stage_waves.split(",")]
+            plan.set_stage_waves(sw)
+            tuned_w = {"stage_waves": sw, "stage_ms": [[0.0] * 4] * 4, "pinned": True}
+            ttune = 0.0
+            log(f"[bench] team sizes pinned: {sw}")
+        elif not args.no_tune:
             t1 = time.perf_counter()
-            plan.set_waves(0)
+            if len(parts) == 1:     # one grid: one launch group of every item
+                b0 = parts[0]
+                tuned_w = ctx.tune_waves(b0["nx"], b0["ny"], nit, prec)
+            plan.set_waves(0)       # the table's value (measured above, or now per group)
             ttune = time.perf_counter() - t1
-            log(f"[bench] team sizes tuned in {ttune:.1f}s")
+            log(f"[bench] team sizes tuned in {ttune:.1f}s: {tuned_w}")
         for _ in range(args.warmup):
             plan.run(True, True)
             plan.wait()
@@ -458,7 +472,7 @@ def main():
                      "results_same_as_eager": bool(same)}
             log(f"[bench] graph replay: {graph}")
         one_wave = None
-        if not args.no_tune:
+        if not args.no_tune and not args.stage_waves:
             plan.set_waves(1)
             plan.run(True, True)
             plan.wait()
@@ -470,6 +484,11 @@ def main():
                        for k in range(len(parts)) for key in ("tt_fwd", "adj_s"))
             one_wave = {"tuning_s": round(ttune, 2), "one_wave_ms_per_step": round(tdt * 1e3, 3),
                         "results_same_as_one_wave": bool(same)}
+            if tuned_w is not None:
+                one_wave["stage_waves"] = dict(zip(STAGES, tuned_w["stage_waves"]))
+                one_wave["tune_stage_ms_by_waves"] = {
+                    st: [round(tuned_w["stage_ms"][w][k], 2) for w in range(4)]
+                    for k, st in enumerate(STAGES)}
             log(f"[bench] one wave per item: {one_wave}")
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
     value = args.steps * nit / dt
@@ -583,7 +602,9 @@ def main():
         "config": {"workload": args.workload, "description": desc, "items": nit,
                    "grid": grids[0] if len(grids) == 1 else grids, "precision": args.precision,
                    "launch": "batched", "chunk": st["chunk"],
-                   "teams": "one wave per item" if args.no_tune else "autotuned per launch group",
+                   "teams": ("one wave per item" if args.no_tune else
+                             f"pinned per stage {args.stage_waves}" if args.stage_waves else
+                             "autotuned per launch group and stage"),
                    "mean_iters_fwd": float(np.concatenate([o["iters"][:, 0] for o in outs]).mean()),
                    "mean_iters_adj": float(np.concatenate([o["iters"][:, 1] for o in outs]).mean()),
                    "mean_iters_fwd_fp32": float(np.concatenate([o["iters_fp32"][:, 0] for o in outs]).mean()),

@@ -245,18 +245,29 @@ int satt_plan_set_graph(satt_plan *plan, int on);
  * most 4, at most the strip count, and the LDS of the edge channels).  The
  * results do not depend on W.  No counterpart in the reference. */
 int satt_plan_set_waves(satt_plan *plan, int waves);
-/* waves = 0: per launch group, the team size the autotuner measured for its
- * grid shape and size (tuned on the spot when the context has none yet). */
+/* waves = 0: per launch group and sweep stage, the team sizes the autotuner
+ * measured for its grid shape and size (tuned on the spot when the context
+ * has none yet). */
+/* Team sizes per sweep stage, waves[4] >= 1 in satt_tune_result.stage_waves
+ * order (forward FP32, forward FP64, adjoint FP32, adjoint FP64), each capped
+ * per launch group as above: pins a configuration the autotuner found.  No
+ * counterpart in the reference. */
+int satt_plan_set_stage_waves(satt_plan *plan, const int *waves);
 
 /* Autotuner of the team size: times a synthetic batch of `nitems` fwd+adj
  * solves on an nx x ny grid for each W in 1..4 (as far as the grid's strips
- * allow) and keeps the fastest in the context's table, keyed by strips, rows,
- * item count and precision, for plans (satt_plan_set_waves(plan, 0)) and
- * servers (waves 0).  ms[W-1] holds each measured run (0: not tried).  No
- * counterpart in the reference. */
+ * allow) and keeps, for each of the four sweep stages (forward FP32, forward
+ * FP64, adjoint FP32, adjoint FP64), the fastest W in the context's table,
+ * keyed by strips, rows, item count and precision, for plans
+ * (satt_plan_set_waves(plan, 0)) and servers (waves 0).  waves is the W of
+ * the fastest whole run and ms[W-1] that run's time; stage_waves[k] is stage
+ * k's choice and stage_ms[W-1][k] its time (0: not tried or not run in this
+ * precision).  No counterpart in the reference. */
 typedef struct {
   int waves;
   double ms[4];
+  int stage_waves[4];
+  double stage_ms[4][4];
 } satt_tune_result;
 int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int precision,
                     satt_tune_result *result);

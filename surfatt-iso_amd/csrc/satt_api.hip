@@ -30,17 +30,19 @@ struct satt_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::mutex tune_mu;
-  std::map<TuneKey, int> tuned;  // team sizes measured by satt_tune_waves
+  std::map<TuneKey, TuneWaves> tuned;  // team sizes measured by satt_tune_waves
   double last_ms = 0;            // device time of the last model-side call
 };
 
-int satt::ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k) {
+bool satt::ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k, TuneWaves &out) {
   std::lock_guard<std::mutex> lk(ctx->tune_mu);
   auto it = ctx->tuned.find(k);
-  return it == ctx->tuned.end() ? 0 : it->second;
+  if (it == ctx->tuned.end()) return false;
+  out = it->second;
+  return true;
 }
 
-void satt::ctx_tune_store(satt_ctx *ctx, const TuneKey &k, int waves) {
+void satt::ctx_tune_store(satt_ctx *ctx, const TuneKey &k, const TuneWaves &waves) {
   std::lock_guard<std::mutex> lk(ctx->tune_mu);
   ctx->tuned[k] = waves;
 }
@@ -1009,7 +1011,7 @@ int plan_groups(satt_plan *P) {
       gd.max_ny = std::max(gd.max_ny, Q.ny);
       gd.max_ns = std::max(gd.max_ns, Q.sg.ns);
     }
-    gd.waves = team_waves(P->waves, gd.max_ny, gd.max_ns);
+    for (int st = 0; st < kStages; ++st) gd.waves[st] = team_waves(P->waves, gd.max_ny, gd.max_ns);
     G.gd = gd;
     for (size_t q = 0; q < P->parts.size(); ++q) {
       const Part &Q = *P->parts[q];
@@ -1312,21 +1314,35 @@ extern "C" int satt_plan_set_waves(satt_plan *P, int waves) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   P->waves = waves;
   for (Group &G : P->groups) {
-    int w = waves;
-    if (w == 0) {  // tuned per launch group: its widest grid and its size
+    TuneWaves tw{{waves, waves, waves, waves}};
+    if (waves == 0) {  // tuned per launch group (its widest grid and its size) and stage
       const int nx = 64 * G.gd.max_ns;
       const TuneKey key = TuneKey::of(nx, G.gd.max_ny, G.gd.count, P->precision);
-      w = ctx_tune_lookup(ctx, key);
-      if (w == 0) {
+      if (!ctx_tune_lookup(ctx, key, tw)) {
         satt_tune_result r{};
         const int st = satt_tune_waves(ctx, nx, G.gd.max_ny, G.gd.count, P->precision, &r);
         if (st < 0) return st;
-        w = r.waves;
+        for (int s = 0; s < kStages; ++s) tw.w[s] = r.stage_waves[s];
       }
     }
-    G.gd.waves = team_waves(w, G.gd.max_ny, G.gd.max_ns);
+    for (int s = 0; s < kStages; ++s)
+      G.gd.waves[s] = team_waves(tw.w[s], G.gd.max_ny, G.gd.max_ns);
   }
   P->drop_graphs();  // captured with the old launch shapes
+  return SATT_OK;
+}
+
+extern "C" int satt_plan_set_stage_waves(satt_plan *P, const int *waves) {
+  if (!P || !waves) return SATT_E_ARG;
+  for (int s = 0; s < kStages; ++s)
+    if (waves[s] < 1) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (Group &G : P->groups)
+    for (int s = 0; s < kStages; ++s)
+      G.gd.waves[s] = team_waves(waves[s], G.gd.max_ny, G.gd.max_ns);
+  P->drop_graphs();
   return SATT_OK;
 }
 

@@ -28,10 +28,15 @@ struct TuneKey {
     return precision < o.precision;
   }
 };
-// The tuned team size of a key (0: not tuned), and storing one; both
+// Team sizes of the four sweep stages (forward FP32, forward FP64, adjoint
+// FP32, adjoint FP64).
+struct TuneWaves {
+  int w[4];
+};
+// The tuned team sizes of a key (false: not tuned), and storing them; both
 // thread-safe.
-int ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k);
-void ctx_tune_store(satt_ctx *ctx, const TuneKey &k, int waves);
+bool ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k, TuneWaves &out);
+void ctx_tune_store(satt_ctx *ctx, const TuneKey &k, const TuneWaves &waves);
 
 // Constants of the reference solvers, with their REAL(4) rounding kept:
 // src/eikon_solver_2d.f90:774-776,785 (forward) and :1293-1297 (adjoint).

@@ -256,13 +256,12 @@ bool satt_server::launch(Batch &b) {
     gd.max_ny = std::max(gd.max_ny, ny);
     gd.max_ns = std::max(gd.max_ns, sg.ns);
   }
-  int w = waves;
-  if (w == 0) {  // the team size tuned for one request of this shape, if any
+  TuneWaves tw{{waves, waves, waves, waves}};
+  if (waves == 0) {  // the team sizes tuned for one request of this shape, if any
     const TuneKey key = TuneKey::of(64 * gd.max_ns, gd.max_ny, 1, precision);
-    w = ctx_tune_lookup(ctx, key);
-    if (w == 0) w = 4;
+    if (!ctx_tune_lookup(ctx, key, tw)) tw = TuneWaves{{4, 4, 4, 4}};
   }
-  gd.waves = team_waves(w, gd.max_ny, gd.max_ns);
+  for (int st = 0; st < kStages; ++st) gd.waves[st] = team_waves(tw.w[st], gd.max_ny, gd.max_ns);
   std::memcpy(pin + b.desc_pin, desc.data(), desc.size() * sizeof(SlotDesc));
   SlotDesc *sd = reinterpret_cast<SlotDesc *>(dev + b.desc_dev);
   if (!hip_ok(hipMemcpyAsync(sd, pin + b.desc_pin, desc.size() * sizeof(SlotDesc),

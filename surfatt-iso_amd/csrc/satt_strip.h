@@ -42,15 +42,20 @@ inline size_t slot_doubles(int nx, int ny, int slot_skews) {
   return (n + 31) & ~(size_t)31;
 }
 
+// Sweep stages of the strip kernels; each has its own team size: the FP32
+// forward, the FP64 forward (polish, or the whole FP64 solve), the FP32
+// adjoint and the FP64 adjoint.
+enum { kStageFwd32 = 0, kStageFwd64 = 1, kStageAdj32 = 2, kStageAdj64 = 3, kStages = 4 };
+
 // Sizes of a launch group, for the grids of the element-wise kernels, and
-// the waves per item of its strip kernels.
+// the waves per item of its strip kernels in each stage.
 struct GroupDims {
   int count;        // slots
   size_t max_se;    // largest skewed field
   size_t max_n;     // largest grid
   int max_nxy;      // largest nx or ny
   int max_ny = 0, max_ns = 0;  // largest ny and strip count
-  int waves = 1;    // waves per item (a team; at most kMaxTeam)
+  int waves[kStages] = {1, 1, 1, 1};  // waves per item (a team; at most kMaxTeam)
 };
 
 // Waves per item of the strip kernels: each strip of a sweep goes to one

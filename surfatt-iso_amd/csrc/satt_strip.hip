@@ -1604,9 +1604,11 @@ void launch_fwd_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, bo
 namespace {
 // Block and LDS of a strip-kernel launch of the group (vbytes: the channel's
 // value size).
-dim3 team_block(const GroupDims &gd) { return dim3(64 * (unsigned)std::max(1, gd.waves)); }
-size_t team_lds(const GroupDims &gd, int vbytes) {
-  return team_lds_bytes(gd.waves, gd.max_ny, gd.max_ns, vbytes);
+dim3 team_block(const GroupDims &gd, int st) {
+  return dim3(64 * (unsigned)std::max(1, gd.waves[st]));
+}
+size_t team_lds(const GroupDims &gd, int st, int vbytes) {
+  return team_lds_bytes(gd.waves[st], gd.max_ny, gd.max_ns, vbytes);
 }
 }  // namespace
 
@@ -1617,25 +1619,31 @@ int team_waves(int want, int ny, int ns) {
 }
 
 void launch_fwd_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
-  if (gd.waves > 1)
-    hipLaunchKernelGGL(k_fwd_strip<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd);
+  const int st = kStageFwd64;
+  if (gd.waves[st] > 1)
+    hipLaunchKernelGGL(k_fwd_strip<true>, dim3(gd.count), team_block(gd, st), team_lds(gd, st, 8),
+                       s, sd);
   else
-    hipLaunchKernelGGL(k_fwd_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd);
+    hipLaunchKernelGGL(k_fwd_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd);
 }
 
 void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
-  if (gd.waves > 1)
-    hipLaunchKernelGGL(k_fwd_mixed32<true>, dim3(gd.count), team_block(gd), team_lds(gd, 4), s,
-                       sd);
+  const int st = kStageFwd32;
+  if (gd.waves[st] > 1)
+    hipLaunchKernelGGL(k_fwd_mixed32<true>, dim3(gd.count), team_block(gd, st),
+                       team_lds(gd, st, 4), s, sd);
   else
-    hipLaunchKernelGGL(k_fwd_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, 4), s, sd);
+    hipLaunchKernelGGL(k_fwd_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 4), s,
+                       sd);
 }
 
 void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
-  if (gd.waves > 1)
-    hipLaunchKernelGGL(k_fwd_polish<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd);
+  const int st = kStageFwd64;
+  if (gd.waves[st] > 1)
+    hipLaunchKernelGGL(k_fwd_polish<true>, dim3(gd.count), team_block(gd, st),
+                       team_lds(gd, st, 8), s, sd);
   else
-    hipLaunchKernelGGL(k_fwd_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd);
+    hipLaunchKernelGGL(k_fwd_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd);
 }
 
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
@@ -1651,28 +1659,33 @@ void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
 }
 
 void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
-  if (gd.waves > 1)
-    hipLaunchKernelGGL(k_adj_strip<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd,
-                       which);
+  const int st = kStageAdj64;
+  if (gd.waves[st] > 1)
+    hipLaunchKernelGGL(k_adj_strip<true>, dim3(gd.count), team_block(gd, st),
+                       team_lds(gd, st, 8), s, sd, which);
   else
-    hipLaunchKernelGGL(k_adj_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd, which);
+    hipLaunchKernelGGL(k_adj_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd,
+                       which);
 }
 
 void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
-  if (gd.waves > 1)
-    hipLaunchKernelGGL(k_adj_mixed32<true>, dim3(gd.count), team_block(gd), team_lds(gd, 4), s,
-                       sd, which);
+  const int st = kStageAdj32;
+  if (gd.waves[st] > 1)
+    hipLaunchKernelGGL(k_adj_mixed32<true>, dim3(gd.count), team_block(gd, st),
+                       team_lds(gd, st, 4), s, sd, which);
   else
-    hipLaunchKernelGGL(k_adj_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, 4), s, sd,
-                       which);
+    hipLaunchKernelGGL(k_adj_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 4), s,
+                       sd, which);
 }
 
 void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
-  if (gd.waves > 1)
-    hipLaunchKernelGGL(k_adj_polish<true>, dim3(gd.count), team_block(gd), team_lds(gd, 8), s, sd,
-                       which);
+  const int st = kStageAdj64;
+  if (gd.waves[st] > 1)
+    hipLaunchKernelGGL(k_adj_polish<true>, dim3(gd.count), team_block(gd, st),
+                       team_lds(gd, st, 8), s, sd, which);
   else
-    hipLaunchKernelGGL(k_adj_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, 8), s, sd, which);
+    hipLaunchKernelGGL(k_adj_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd,
+                       which);
 }
 
 }  // namespace satt

@@ -96,7 +96,7 @@ extern "C" int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int pr
   if (st < 0) return st;
   satt_tune_result r{};
   const int ns = make_skew_geom(nx, ny).ns;
-  double best = 0;
+  double best = 0, best_stage[kStages] = {0, 0, 0, 0};
   for (int W = 1; W <= kMaxTeam && st >= 0; ++W) {
     if (team_waves(W, ny, ns) != W) break;  // capped: the same launch as a smaller W
     st = satt_plan_set_waves(P, W);
@@ -112,10 +112,22 @@ extern "C" int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int pr
       best = s.ms_total;
       r.waves = W;
     }
+    // the stages run one after another on the stream, so each keeps its
+    // own fastest team size
+    const double sm[kStages] = {s.ms_fwd_fp32, s.ms_fwd_fp64, s.ms_adj_fp32, s.ms_adj_fp64};
+    for (int k = 0; k < kStages; ++k) {
+      r.stage_ms[W - 1][k] = sm[k];
+      if (r.stage_waves[k] == 0 || sm[k] < best_stage[k]) {
+        best_stage[k] = sm[k];
+        r.stage_waves[k] = W;
+      }
+    }
   }
   satt_plan_destroy(P);
   if (st < 0) return st;
-  ctx_tune_store(ctx, TuneKey::of(nx, ny, nitems, precision), r.waves);
+  TuneWaves tw;
+  for (int k = 0; k < kStages; ++k) tw.w[k] = r.stage_waves[k];
+  ctx_tune_store(ctx, TuneKey::of(nx, ny, nitems, precision), tw);
   if (out) *out = r;
   return SATT_OK;
 }

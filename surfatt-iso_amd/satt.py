@@ -141,7 +141,9 @@ class Context:
         r = TuneResult()
         self._check(self.lib.satt_tune_waves(self.ptr, int(nx), int(ny), int(nitems),
                                              int(precision), ctypes.byref(r)))
-        return {"waves": r.waves, "ms": [float(v) for v in r.ms]}
+        return {"waves": r.waves, "ms": [float(v) for v in r.ms],
+                "stage_waves": [int(v) for v in r.stage_waves],
+                "stage_ms": [[float(v) for v in row] for row in r.stage_ms]}
 
     def close(self):
         if self.ptr:
@@ -375,6 +377,14 @@ class Plan:
         """Waves per item of the sweep kernels (capped per launch group)."""
         self.ctx._check(self.lib.satt_plan_set_waves(self.ptr, int(waves)))
 
+    def set_stage_waves(self, waves):
+        """Team size per sweep stage (forward FP32, forward FP64, adjoint FP32,
+        adjoint FP64)."""
+        w = np.ascontiguousarray(waves, dtype=np.int32)
+        if w.shape != (4,):
+            raise SattError(E_ARG, "four stage team sizes expected")
+        self.ctx._check(self.lib.satt_plan_set_stage_waves(self.ptr, _i(w)))
+
     def set_graph(self, on=True):
         """Replay form: batched runs capture their launch chain into a HIP
         graph once per (istotable, isadj) and replay it afterwards."""
@@ -425,7 +435,8 @@ class Plan:
 # ------------------------------------------------------------------ serving
 
 class TuneResult(ctypes.Structure):
-    _fields_ = [("waves", ctypes.c_int), ("ms", ctypes.c_double * 4)]
+    _fields_ = [("waves", ctypes.c_int), ("ms", ctypes.c_double * 4),
+                ("stage_waves", ctypes.c_int * 4), ("stage_ms", (ctypes.c_double * 4) * 4)]
 
 
 class ServerConfig(ctypes.Structure):

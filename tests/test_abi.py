@@ -44,9 +44,10 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         pytest.skip("no C compiler")
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "satt.h"\nint main(void){printf('
-                   '"%zu %zu %zu %zu %zu %zu %zu", sizeof(satt_batch), sizeof(satt_result),'
+                   '"%zu %zu %zu %zu %zu %zu %zu %zu", sizeof(satt_batch), sizeof(satt_result),'
                    ' sizeof(satt_plan_stats), sizeof(satt_server_config), sizeof(satt_request),'
-                   ' sizeof(satt_response), sizeof(satt_server_stats));return 0;}\n')
+                   ' sizeof(satt_response), sizeof(satt_server_stats), sizeof(satt_tune_result));'
+                   'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run([cc, "-I", os.path.dirname(satt.HEADER), str(src), "-o", str(exe)], check=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
@@ -54,4 +55,4 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
     assert sizes == [ctypes.sizeof(satt.Batch), ctypes.sizeof(satt.Result),
                      ctypes.sizeof(satt.PlanStats), ctypes.sizeof(satt.ServerConfig),
                      ctypes.sizeof(satt.Request), ctypes.sizeof(satt.Response),
-                     ctypes.sizeof(satt.ServerStats)]
+                     ctypes.sizeof(satt.ServerStats), ctypes.sizeof(satt.TuneResult)]

@@ -57,6 +57,11 @@ def test_tuner_and_tuned_plan(ctx):
     assert 1 <= r["waves"] <= 4 and r["ms"][0] > 0
     tried = [m for m in r["ms"] if m > 0]
     assert r["ms"][r["waves"] - 1] == min(tried)
+    # each sweep stage keeps its own fastest team size
+    for k in range(4):
+        times = [r["stage_ms"][w][k] for w in range(len(tried))]
+        assert all(t > 0 for t in times), (k, times)
+        assert r["stage_ms"][r["stage_waves"][k] - 1][k] == min(times)
     b = shaped_batch(260, 60, 3, 2, seed=33)
     with ctx.plan(b, satt.PREC_MIXED) as p:
         p.run(True, True)
@@ -64,6 +69,18 @@ def test_tuner_and_tuned_plan(ctx):
         p.set_waves(0)                     # tuned per launch group
         p.run(True, True)
         same(p.results(), ref, 2, "tuned")
+
+
+def test_stage_team_sizes_pinned_equal_one_wave(ctx):
+    b = shaped_batch(200, 70, 3, 2, seed=35)
+    with ctx.plan(b, satt.PREC_MIXED) as p:
+        p.run(True, True)
+        ref = p.results()
+        p.set_stage_waves([1, 2, 3, 4])
+        p.run(True, True)
+        same(p.results(), ref, 2, "stage waves")
+        with pytest.raises(satt.SattError):
+            p.set_stage_waves([1, 0, 1, 1])
 
 
 def test_server_team_results_equal_one_wave(ctx):
