@@ -436,10 +436,7 @@ def main():
             log(f"[bench] team sizes pinned: {sw}")
         elif not args.no_tune:
             t1 = time.perf_counter()
-            if len(parts) == 1:     # one grid: one launch group of every item
-                b0 = parts[0]
-                tuned_w = ctx.tune_waves(b0["nx"], b0["ny"], nit, prec)
-            plan.set_waves(0)       # the table's value (measured above, or now per group)
+            tuned_w = plan.tune_waves()     # on the plan's own inputs
             ttune = time.perf_counter() - t1
             log(f"[bench] team sizes tuned in {ttune:.1f}s: {tuned_w}")
         for _ in range(args.warmup):

@@ -271,6 +271,13 @@ typedef struct {
 } satt_tune_result;
 int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int precision,
                     satt_tune_result *result);
+
+/* Autotunes the plan on its own inputs: runs it (istotable and isadj set) with
+ * every W in 1..4, keeps each sweep stage's fastest W and sets those (as
+ * satt_plan_set_stage_waves); result fields as satt_tune_waves.  The plan's
+ * results afterwards are those of a run (they do not depend on W).  No
+ * counterpart in the reference. */
+int satt_plan_tune_waves(satt_plan *plan, satt_tune_result *result);
 /* Wait for the plan's last run (no download). */
 int satt_plan_wait(satt_plan *plan);
 int satt_plan_results(satt_plan *plan, satt_result *result);

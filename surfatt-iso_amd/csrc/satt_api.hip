@@ -1346,6 +1346,47 @@ extern "C" int satt_plan_set_stage_waves(satt_plan *P, const int *waves) {
   return SATT_OK;
 }
 
+extern "C" int satt_plan_tune_waves(satt_plan *P, satt_tune_result *out) {
+  if (!P) return SATT_E_ARG;
+  satt_tune_result r{};
+  double best = 0, best_stage[kStages] = {0, 0, 0, 0};
+  const int graph = P->graph_on;  // stage times are split in the eager form only
+  P->graph_on = 0;
+  int st = SATT_OK;
+  for (int W = 1; W <= kMaxTeam && st >= 0; ++W) {
+    bool fresh = W == 1;  // a W that some group does not cap to a smaller one
+    for (const Group &G : P->groups) fresh = fresh || team_waves(W, G.gd.max_ny, G.gd.max_ns) == W;
+    if (!fresh) break;
+    const int w4[kStages] = {W, W, W, W};
+    st = satt_plan_set_stage_waves(P, w4);
+    for (int rep = 0; rep < 2 && st >= 0; ++rep) {  // the first run warms up
+      st = satt_plan_run(P, 1, 1, SATT_LAUNCH_BATCHED);
+      if (st >= 0) st = satt_plan_wait(P);
+    }
+    satt_plan_stats s{};
+    if (st >= 0) st = satt_plan_get_stats(P, &s);
+    if (st < 0) break;
+    r.ms[W - 1] = s.ms_total;
+    if (r.waves == 0 || s.ms_total < best) {
+      best = s.ms_total;
+      r.waves = W;
+    }
+    const double sm[kStages] = {s.ms_fwd_fp32, s.ms_fwd_fp64, s.ms_adj_fp32, s.ms_adj_fp64};
+    for (int k = 0; k < kStages; ++k) {
+      r.stage_ms[W - 1][k] = sm[k];
+      if (r.stage_waves[k] == 0 || sm[k] < best_stage[k]) {
+        best_stage[k] = sm[k];
+        r.stage_waves[k] = W;
+      }
+    }
+  }
+  P->graph_on = graph;
+  if (st < 0) return st;
+  TRY(satt_plan_set_stage_waves(P, r.stage_waves));
+  if (out) *out = r;
+  return SATT_OK;
+}
+
 extern "C" int satt_plan_set_graph(satt_plan *P, int on) {
   if (!P) return SATT_E_ARG;
   P->graph_on = on ? 1 : 0;

@@ -385,6 +385,14 @@ class Plan:
             raise SattError(E_ARG, "four stage team sizes expected")
         self.ctx._check(self.lib.satt_plan_set_stage_waves(self.ptr, _i(w)))
 
+    def tune_waves(self):
+        """Autotune the team size of each sweep stage on this plan's inputs."""
+        r = TuneResult()
+        self.ctx._check(self.lib.satt_plan_tune_waves(self.ptr, ctypes.byref(r)))
+        return {"waves": r.waves, "ms": [float(v) for v in r.ms],
+                "stage_waves": [int(v) for v in r.stage_waves],
+                "stage_ms": [[float(v) for v in row] for row in r.stage_ms]}
+
     def set_graph(self, on=True):
         """Replay form: batched runs capture their launch chain into a HIP
         graph once per (istotable, isadj) and replay it afterwards."""

@@ -71,6 +71,22 @@ def test_tuner_and_tuned_plan(ctx):
         same(p.results(), ref, 2, "tuned")
 
 
+def test_plan_tuned_on_its_own_inputs(ctx):
+    b = shaped_batch(200, 70, 4, 2, seed=36)
+    with ctx.plan(b, satt.PREC_MIXED) as p:
+        p.run(True, True)
+        ref = p.results()
+        r = p.tune_waves()
+        assert all(1 <= w <= 4 for w in r["stage_waves"])
+        for k in range(4):
+            tried = [r["stage_ms"][w][k] for w in range(4) if r["ms"][w] > 0]
+            assert r["stage_ms"][r["stage_waves"][k] - 1][k] == min(tried)
+        same(p.results(), ref, 2, "after tuning")
+        p.set_graph(True)
+        p.run(True, True)
+        same(p.results(), ref, 2, "tuned, replayed")
+
+
 def test_stage_team_sizes_pinned_equal_one_wave(ctx):
     b = shaped_batch(200, 70, 3, 2, seed=35)
     with ctx.plan(b, satt.PREC_MIXED) as p:
