@@ -385,6 +385,8 @@ def main():
                     help="items timed in the one-solve-per-launch FP64 form")
     ap.add_argument("--no-model-side", action="store_true",
                     help="skip the model-side leg (topographic metric, steepest descent)")
+    ap.add_argument("--model-side-only", action="store_true",
+                    help="run only the model-side leg and print its JSON")
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -399,6 +401,11 @@ def main():
               "(the (source, period) items shard across GPUs with one kernel all-reduce, "
               "which is not built here)", file=sys.stderr)
         sys.exit(2)
+
+    if args.model_side_only:
+        ctx = satt.Context(0)
+        print(json.dumps(model_side(ctx, cpu=not args.no_cpu)), flush=True)
+        return
 
     build, kw, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()

@@ -43,9 +43,11 @@ __global__ void k_geo_s2(const double *lon, int nx, double *s2) {
 }
 
 // One workgroup per (output row j, field f).  LDS: the row pair's weights
-// w[0..hxe] and data row j' (nx each).  CPT cells per thread: i = tid + q*bs.
-template <int CPT>
-__global__ __launch_bounds__(1024) void k_smooth_geo2(GeoSmooth P) {
+// w[0..hxe] and data row j' (nx each).  CPT cells per thread: i = tid + q*bs,
+// MAXT threads at most (512 when a thread has several cells, which leaves it
+// the registers of its accumulators).
+template <int CPT, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_smooth_geo2(GeoSmooth P) {
   extern __shared__ double lds[];
   const int j = blockIdx.x, f = blockIdx.y;
   const int nx = P.nx, ny = P.ny, bs = blockDim.x, tid = threadIdx.x;
@@ -77,10 +79,46 @@ __global__ __launch_bounds__(1024) void k_smooth_geo2(GeoSmooth P) {
       const int i = tid + q * bs;
       if (i < nx) {
         const int k0 = -hx > -i ? -hx : -i, k1 = hx < nx - 1 - i ? hx : nx - 1 - i;
+        const double *dp = dl + i;
         double a = swd[q], b = sw[q];
-        for (int k = k0; k <= k1; ++k) {
-          const double w = wl[k < 0 ? -k : k];
-          a += w * dl[i + k];
+        // offsets in ascending order, split at 0 so that both the weight and
+        // the data index run linearly; eight nodes' loads are issued before
+        // their sums, so one LDS round trip serves eight nodes
+        int k = k0;
+        for (; k + 8 <= 0; k += 8) {
+          double w[8], d[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            w[u] = wl[-(k + u)];
+            d[u] = dp[k + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a += w[u] * d[u];
+            b += w[u];
+          }
+        }
+        for (; k < 0; ++k) {
+          const double w = wl[-k];
+          a += w * dp[k];
+          b += w;
+        }
+        for (; k + 8 <= k1 + 1; k += 8) {
+          double w[8], d[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            w[u] = wl[k + u];
+            d[u] = dp[k + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a += w[u] * d[u];
+            b += w[u];
+          }
+        }
+        for (; k <= k1; ++k) {
+          const double w = wl[k];
+          a += w * dp[k];
           b += w;
         }
         swd[q] = a;
@@ -214,16 +252,15 @@ void launch_geo_s2(hipStream_t s, const double *lon, int nx, double *s2) {
 }
 
 void launch_smooth_geo2(hipStream_t s, const GeoSmooth &P, int nfield) {
-  const int bs = std::min(1024, (P.nx + 63) / 64 * 64);
-  const int cpt = (P.nx + bs - 1) / bs;
   const dim3 grid((unsigned)P.ny, (unsigned)nfield);
   const size_t lds = smooth_lds_bytes(P.nx);
-  if (cpt <= 1)
-    hipLaunchKernelGGL(k_smooth_geo2<1>, grid, dim3(bs), lds, s, P);
-  else if (cpt <= 2)
-    hipLaunchKernelGGL(k_smooth_geo2<2>, grid, dim3(bs), lds, s, P);
-  else
-    hipLaunchKernelGGL(k_smooth_geo2<4>, grid, dim3(bs), lds, s, P);
+  if (P.nx <= 1024) {  // one cell per thread
+    hipLaunchKernelGGL((k_smooth_geo2<1, 1024>), grid, dim3((P.nx + 63) / 64 * 64), lds, s, P);
+  } else if (P.nx <= 2048) {
+    hipLaunchKernelGGL((k_smooth_geo2<4, 512>), grid, dim3(512), lds, s, P);
+  } else {  // nx <= 4096 (SATT_GEO_MAX_NX)
+    hipLaunchKernelGGL((k_smooth_geo2<8, 512>), grid, dim3(512), lds, s, P);
+  }
 }
 
 void launch_gradient_geo2(hipStream_t s, const GeoGrad &G, int nf, const double *f_nat,

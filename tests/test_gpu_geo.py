@@ -35,8 +35,8 @@ def smooth_close(got, want, data):
     (37, 29, 0.05, [0.04, 0.3]),        # windows inside the grid, and wider than it
     (70, 45, 0.02, [0.05, 0.011]),
     (130, 9, 0.01, [0.02, 0.5]),
-    (1500, 3, 0.001, [0.01]),           # two cells per thread
-    (2100, 2, 0.001, [0.004]),          # four cells per thread
+    (1500, 3, 0.001, [0.01]),           # three cells per thread
+    (2100, 2, 0.001, [0.004]),          # five cells per thread
 ])
 def test_smoothing_matches_oracle(ctx, nx, ny, d, sigmas):
     lons, lats = axes(nx, ny, d)
