@@ -438,7 +438,7 @@ def main():
         if args.stage_waves:
             sw = [int(v) for v in args.stage_waves.split(",")]
             plan.set_stage_waves(sw)
-            tuned_w = {"stage_waves": sw, "stage_ms": [[0.0] * 4] * 4, "pinned": True}
+            tuned_w = {"stage_waves": sw, "stage_ms": [[0.0] * 4] * 8, "pinned": True}
             ttune = 0.0
             log(f"[bench] team sizes pinned: {sw}")
         elif not args.no_tune:
@@ -491,7 +491,7 @@ def main():
             if tuned_w is not None:
                 one_wave["stage_waves"] = dict(zip(STAGES, tuned_w["stage_waves"]))
                 one_wave["tune_stage_ms_by_waves"] = {
-                    st: [round(tuned_w["stage_ms"][w][k], 2) for w in range(4)]
+                    st: [round(tuned_w["stage_ms"][w][k], 2) for w in range(len(tuned_w["stage_ms"]))]
                     for k, st in enumerate(STAGES)}
             log(f"[bench] one wave per item: {one_wave}")
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")

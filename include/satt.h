@@ -255,7 +255,7 @@ int satt_plan_set_waves(satt_plan *plan, int waves);
 int satt_plan_set_stage_waves(satt_plan *plan, const int *waves);
 
 /* Autotuner of the team size: times a synthetic batch of `nitems` fwd+adj
- * solves on an nx x ny grid for each W in 1..4 (as far as the grid's strips
+ * solves on an nx x ny grid for each W in 1..8 (as far as the grid's strips
  * allow) and keeps, for each of the four sweep stages (forward FP32, forward
  * FP64, adjoint FP32, adjoint FP64), the fastest W in the context's table,
  * keyed by strips, rows, item count and precision, for plans
@@ -265,15 +265,15 @@ int satt_plan_set_stage_waves(satt_plan *plan, const int *waves);
  * precision).  No counterpart in the reference. */
 typedef struct {
   int waves;
-  double ms[4];
+  double ms[8];
   int stage_waves[4];
-  double stage_ms[4][4];
+  double stage_ms[8][4];
 } satt_tune_result;
 int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int precision,
                     satt_tune_result *result);
 
 /* Autotunes the plan on its own inputs: runs it (istotable and isadj set) with
- * every W in 1..4, keeps each sweep stage's fastest W and sets those (as
+ * every W in 1..8, keeps each sweep stage's fastest W and sets those (as
  * satt_plan_set_stage_waves); result fields as satt_tune_waves.  The plan's
  * results afterwards are those of a run (they do not depend on W).  No
  * counterpart in the reference. */

@@ -1011,7 +1011,8 @@ int plan_groups(satt_plan *P) {
       gd.max_ny = std::max(gd.max_ny, Q.ny);
       gd.max_ns = std::max(gd.max_ns, Q.sg.ns);
     }
-    for (int st = 0; st < kStages; ++st) gd.waves[st] = team_waves(P->waves, gd.max_ny, gd.max_ns);
+    for (int st = 0; st < kStages; ++st)
+      gd.waves[st] = team_waves(P->waves, gd.max_ny, gd.max_ns, st);
     G.gd = gd;
     for (size_t q = 0; q < P->parts.size(); ++q) {
       const Part &Q = *P->parts[q];
@@ -1326,7 +1327,7 @@ extern "C" int satt_plan_set_waves(satt_plan *P, int waves) {
       }
     }
     for (int s = 0; s < kStages; ++s)
-      G.gd.waves[s] = team_waves(tw.w[s], G.gd.max_ny, G.gd.max_ns);
+      G.gd.waves[s] = team_waves(tw.w[s], G.gd.max_ny, G.gd.max_ns, s);
   }
   P->drop_graphs();  // captured with the old launch shapes
   return SATT_OK;
@@ -1341,7 +1342,7 @@ extern "C" int satt_plan_set_stage_waves(satt_plan *P, const int *waves) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   for (Group &G : P->groups)
     for (int s = 0; s < kStages; ++s)
-      G.gd.waves[s] = team_waves(waves[s], G.gd.max_ny, G.gd.max_ns);
+      G.gd.waves[s] = team_waves(waves[s], G.gd.max_ny, G.gd.max_ns, s);
   P->drop_graphs();
   return SATT_OK;
 }
@@ -1355,10 +1356,11 @@ extern "C" int satt_plan_tune_waves(satt_plan *P, satt_tune_result *out) {
   int st = SATT_OK;
   for (int W = 1; W <= kMaxTeam && st >= 0; ++W) {
     bool fresh = W == 1;  // a W that some group does not cap to a smaller one
-    for (const Group &G : P->groups) fresh = fresh || team_waves(W, G.gd.max_ny, G.gd.max_ns) == W;
+    for (const Group &G : P->groups)
+      fresh = fresh || team_waves(W, G.gd.max_ny, G.gd.max_ns, kStageFwd32) == W;
     if (!fresh) break;
-    const int w4[kStages] = {W, W, W, W};
-    st = satt_plan_set_stage_waves(P, w4);
+    const int wW[kStages] = {W, W, W, W};
+    st = satt_plan_set_stage_waves(P, wW);
     for (int rep = 0; rep < 2 && st >= 0; ++rep) {  // the first run warms up
       st = satt_plan_run(P, 1, 1, SATT_LAUNCH_BATCHED);
       if (st >= 0) st = satt_plan_wait(P);

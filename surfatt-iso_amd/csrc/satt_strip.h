@@ -62,10 +62,11 @@ struct GroupDims {
 // wave of the item's team, the next strip's wave following at least one
 // strip width behind.  More waves cut an item's latency on wide grids; one
 // wave per item is the throughput form when items fill the device.
-constexpr int kMaxTeam = 4;
+constexpr int kMaxTeam = 8;
 constexpr size_t kTeamLdsMax = 64 * 1024;
-// The largest team (<= want) whose LDS fits for grids up to ny rows, ns strips.
-int team_waves(int want, int ny, int ns);
+// The largest team (<= want) of a sweep stage whose LDS (edge channels of
+// the stage's value type) fits for grids up to ny rows, ns strips.
+int team_waves(int want, int ny, int ns, int stage);
 
 // Batched kernels over the slots of a launch group (satt_strip.hip); all
 // asynchronous on `s`.  Forward: setup (per-item tables, then T0 skews and

@@ -1019,7 +1019,7 @@ __device__ bool fwd_item_rect(const SlotDesc &D, int lane, int rect[4]) {
 // The strip kernels run one item per workgroup of W = blockDim.x/64 waves
 // (a team), with team_lds_bytes of dynamic LDS when W > 1.
 template <bool TEAM>
-__global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_strip(const SlotDesc *sd) {
+__global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_strip(const SlotDesc *sd) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -1033,7 +1033,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_strip(const SlotDesc *s
 // Mixed forward, FP32 phase: the FP32 iterations (count to iters[3]), then
 // dtau widened into the FP64 plus skew for the polish.
 template <bool TEAM>
-__global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_mixed32(const SlotDesc *sd) {
+__global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_mixed32(const SlotDesc *sd) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_mixed32(const SlotDesc 
 // Mixed forward, FP64 polish: the upwind update in FP64 from the FP32 result
 // to the reference's stop rule, and the outputs.
 template <bool TEAM>
-__global__ __launch_bounds__(TEAM ? 256 : 64) void k_fwd_polish(const SlotDesc *sd) {
+__global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_polish(const SlotDesc *sd) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -1455,7 +1455,7 @@ __device__ void adj_output(const Grid &g, const SkewGeom &sg, const double *TaP,
 }
 
 template <bool TEAM>
-__global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_strip(const SlotDesc *sd, int which) {
+__global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_strip(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -1492,7 +1492,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_strip(const SlotDesc *s
 // skews from double skew 16: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
 // minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
 template <bool TEAM>
-__global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_mixed32(const SlotDesc *sd, int which) {
+__global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -1533,7 +1533,7 @@ __global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_mixed32(const SlotDesc 
 // Mixed adjoint, FP64 polish: one iteration of the reference's update on the
 // FP64 coefficients, and the masked output.
 template <bool TEAM>
-__global__ __launch_bounds__(TEAM ? 256 : 64) void k_adj_polish(const SlotDesc *sd, int which) {
+__global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -1612,9 +1612,10 @@ size_t team_lds(const GroupDims &gd, int st, int vbytes) {
 }
 }  // namespace
 
-int team_waves(int want, int ny, int ns) {
+int team_waves(int want, int ny, int ns, int stage) {
+  const int vbytes = stage == kStageFwd32 || stage == kStageAdj32 ? 4 : 8;
   int w = std::max(1, std::min({want, kMaxTeam, ns}));
-  while (w > 1 && team_lds_bytes(w, ny, ns, 8) > kTeamLdsMax) --w;
+  while (w > 1 && team_lds_bytes(w, ny, ns, vbytes) > kTeamLdsMax) --w;
   return w;
 }
 

@@ -98,7 +98,7 @@ extern "C" int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int pr
   const int ns = make_skew_geom(nx, ny).ns;
   double best = 0, best_stage[kStages] = {0, 0, 0, 0};
   for (int W = 1; W <= kMaxTeam && st >= 0; ++W) {
-    if (team_waves(W, ny, ns) != W) break;  // capped: the same launch as a smaller W
+    if (team_waves(W, ny, ns, kStageFwd32) != W) break;  // capped: as a smaller W
     st = satt_plan_set_waves(P, W);
     for (int rep = 0; rep < 2 && st >= 0; ++rep) {  // the first run warms up
       st = satt_plan_run(P, 1, 1, SATT_LAUNCH_BATCHED);

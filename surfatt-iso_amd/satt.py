@@ -136,7 +136,8 @@ class Context:
 
     def tune_waves(self, nx, ny, nitems, precision=PREC_MIXED):
         """Autotuner of the team size (waves per item) for an nx x ny grid and
-        a launch of nitems items; returns {"waves": W, "ms": [ms for W=1..4]}
+        a launch of nitems items; returns {"waves": W, "ms": [ms for W=1..8], "stage_waves":
+        [W per sweep stage], "stage_ms": [[ms per stage] for W=1..8]}
         and keeps W in the context's table."""
         r = TuneResult()
         self._check(self.lib.satt_tune_waves(self.ptr, int(nx), int(ny), int(nitems),
@@ -443,8 +444,8 @@ class Plan:
 # ------------------------------------------------------------------ serving
 
 class TuneResult(ctypes.Structure):
-    _fields_ = [("waves", ctypes.c_int), ("ms", ctypes.c_double * 4),
-                ("stage_waves", ctypes.c_int * 4), ("stage_ms", (ctypes.c_double * 4) * 4)]
+    _fields_ = [("waves", ctypes.c_int), ("ms", ctypes.c_double * 8),
+                ("stage_waves", ctypes.c_int * 4), ("stage_ms", (ctypes.c_double * 4) * 8)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -28,7 +28,7 @@ def test_team_equals_one_wave(ctx, prec, nx, ny, dim):
     with ctx.plan(b, prec) as p:
         p.run(True, True)
         ref = p.results()
-        for W in (2, 3, 4):
+        for W in (2, 3, 4, 5, 8):
             p.set_waves(W)
             p.run(True, True)
             same(p.results(), ref, dim, W)
@@ -54,7 +54,7 @@ def test_team_multi_grid_and_graph(ctx):
 
 def test_tuner_and_tuned_plan(ctx):
     r = ctx.tune_waves(260, 60, 2, satt.PREC_MIXED)
-    assert 1 <= r["waves"] <= 4 and r["ms"][0] > 0
+    assert 1 <= r["waves"] <= 8 and r["ms"][0] > 0
     tried = [m for m in r["ms"] if m > 0]
     assert r["ms"][r["waves"] - 1] == min(tried)
     # each sweep stage keeps its own fastest team size
@@ -77,9 +77,9 @@ def test_plan_tuned_on_its_own_inputs(ctx):
         p.run(True, True)
         ref = p.results()
         r = p.tune_waves()
-        assert all(1 <= w <= 4 for w in r["stage_waves"])
+        assert all(1 <= w <= 8 for w in r["stage_waves"])
         for k in range(4):
-            tried = [r["stage_ms"][w][k] for w in range(4) if r["ms"][w] > 0]
+            tried = [r["stage_ms"][w][k] for w in range(8) if r["ms"][w] > 0]
             assert r["stage_ms"][r["stage_waves"][k] - 1][k] == min(tried)
         same(p.results(), ref, 2, "after tuning")
         p.set_graph(True)
