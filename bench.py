@@ -329,6 +329,40 @@ def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
         "metric_range": {"m11": [float(m11.min()), float(m11.max())],
                          "max_dip_deg": float(ang.max())},
     }
+    # surface-wave dispersion of the 3-D stage's model (fwdsurf, before the loop)
+    vs3, depz, per3 = W.acqui3d_model()
+    ncol = vs3.shape[0] * vs3.shape[1]
+    disp = {}
+    for iw, igr, name in ((2, 0, "rayleigh_phase"), (1, 1, "love_group")):
+        ctx.fwdsurf3d(vs3, depz, per3, iw, igr)
+        t0 = time.perf_counter()
+        sv = ctx.fwdsurf3d(vs3, depz, per3, iw, igr)
+        disp[name] = {"device_ms": round(ctx.last_device_ms(), 3),
+                      "wall_ms": round((time.perf_counter() - t0) * 1e3, 3),
+                      "columns_per_s": round(ncol / (ctx.last_device_ms() * 1e-3), 1)}
+        if cpu:
+            import oracle as O
+            if O.RefSolver.available():
+                ref = O.RefSolver()
+                cols = np.linspace(0, ncol - 1, 200).round().astype(int)
+                t0 = time.perf_counter()
+                worst = 0.0
+                for c in cols:
+                    i, j = c % vs3.shape[0], c // vs3.shape[0]
+                    want = O.fwdsurf1d(ref, vs3[i, j], iw, igr, per3, depz)
+                    worst = max(worst, float(np.max(np.abs(sv[:, i, j] - want) / want)))
+                cdt = time.perf_counter() - t0
+                disp[name]["cpu_baseline"] = {
+                    "value": round(len(cols) / cdt, 1), "unit": "columns/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": f"{len(cols)} of the {ncol} columns, the reference's surfdisp96 "
+                              f"(oracle/_ref) through the oracle's fwdsurf1d, {cdt:.2f} s"}
+                disp[name]["max_rel_diff_vs_reference"] = worst
+                disp[name]["vs_cpu_core"] = round(disp[name]["columns_per_s"] / (len(cols) / cdt), 1)
+    res["dispersion"] = {"workload": f"{vs3.shape[0]}x{vs3.shape[1]} columns x {vs3.shape[2]} "
+                                     f"depths 0-60 km, Vs 3.0-4.5 km/s +-6% checkerboard, "
+                                     f"{len(per3)} periods 4-40 s, spherical earth, fundamental mode",
+                         **disp}
     if cpu:
         import oracle as O
         sample = [(sig[0], j) for j in (0, 200, 400)] + [(sig[npd // 2], 200), (sig[15], 200)]

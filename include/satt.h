@@ -22,6 +22,9 @@
  *                          (src/grid.f90:114-123,137-141)
  *   satt_steepest_descent  att_tomo_2d%optimize + steepest_descent
  *                          (src/tomo2d.f90:203-251)
+ *   satt_fwdsurf3d         fwdsurf3d_decomp / fwdsurf1d with surfdisp96
+ *                          (src/surfker/surfker.f90:200-254,
+ *                          src/surfker/surfdisp96.f:57)
  *
  * Conventions
  *   - A grid field f(nx,ny) is column-major with x fastest: f[i + nx*j], as
@@ -58,6 +61,10 @@ extern "C" {
 #define SATT_E_TEAM_WAIT (-7)     /* a multi-wave solve's wait for another wave
                                      ran past its bound; the item's fields are
                                      not valid                               */
+#define SATT_E_DISPERSION (-8)    /* no root of the period equation for a
+                                     column (the reference stops with
+                                     'Dispersion calculation failed',
+                                     src/surfker/surfdisp96.f:343)           */
 
 /* precision modes for the batched path */
 #define SATT_PREC_FP64 0  /* the reference's FP64 arithmetic, sweep for sweep */
@@ -158,6 +165,23 @@ int satt_steepest_descent(satt_ctx *ctx, const double *lons, const double *lats,
                           int nx, int ny, int nperiod, const double *adj_s,
                           double sigma_2d, double updatemax, double *svel,
                           double *ker_s);
+
+/* The surface-wave velocity maps of att_grid%fwdsurf (src/grid.f90:150-163):
+ * for every column of vs3d(nx,ny,nz) (km/s, Fortran order) on depths depz(nz)
+ * (km), fwdsurf1d (src/surfker/surfker.f90:224-254: Vp and density from Vs by
+ * Brocher's relations, layers between the depths over a half-space) and
+ * surfdisp96 (src/surfker/surfdisp96.f:57-377) with a spherical earth and the
+ * fundamental mode: the phase (igr 0) or group (igr > 0) velocity of Love
+ * (iwave 1) or Rayleigh (iwave 2) waves at periods(nperiod) (s, in the order
+ * given; the search of each period starts from the previous one's root).
+ * svel is period-major (nperiod, nx, ny), as att_grid%svel.  A column whose
+ * search finds no root gets zeros from that period on and the call returns
+ * SATT_E_DISPERSION (the reference stops the run there); col_status (nx*ny,
+ * may be NULL) holds each column's code.  nz <= SATT_DISP_MAX_NZ. */
+#define SATT_DISP_MAX_NZ 1024
+int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
+                   const double *depz, int nperiod, const double *periods,
+                   int iwave, int igr, double *svel, int *col_status);
 
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
  * satt_gradient_geo2, satt_topo_metric or satt_steepest_descent call on the

@@ -288,6 +288,23 @@ class RefSolver:
 
         return self._run(call)
 
+    def surfdisp96(self, thk, vp, vs, rho, iwave, igr, periods, iflsph=1, mode=1):
+        """surfdisp96 (src/surfker/surfdisp96.f:57) on one layered model given
+        in REAL(4) (thickness, Vp, Vs, density); returns the velocities at
+        `periods` (REAL(8)).  The reference stops the process when it finds no
+        root, so callers pass models with velocity rising with depth.  Its
+        search state lives in COMMON blocks: one call at a time per process."""
+        f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+        thk, vp, vs, rho = f(thk), f(vp), f(vs), f(rho)
+        t = np.ascontiguousarray(periods, dtype=np.float64)
+        cg = np.zeros(len(t))
+        fp = ctypes.POINTER(ctypes.c_float)
+        i = lambda v: ctypes.byref(ctypes.c_int(int(v)))
+        self.lib.surfdisp96_(thk.ctypes.data_as(fp), vp.ctypes.data_as(fp),
+                             vs.ctypes.data_as(fp), rho.ctypes.data_as(fp), i(len(thk)),
+                             i(iflsph), i(iwave), i(mode), i(igr), i(len(t)), _d(t), _d(cg))
+        return cg
+
     def mask(self, xx, yy, table, x0, y0):
         xx = np.ascontiguousarray(xx, dtype=np.float64)
         yy = np.ascontiguousarray(yy, dtype=np.float64)
@@ -379,3 +396,32 @@ def steepest_descent(lons, lats, adj_s, sigma_2d, updatemax, svel):
     if st != 0:
         raise OracleError(st, "orc_steepest_descent")
     return sv, ker
+
+
+# ---- surface-wave dispersion of a column (fwdsurf1d, src/surfker/surfker.f90:224-254)
+
+def brocher_vp_rho(vs):
+    """get_vprho (src/surfker/surfker.f90:306-320) in REAL(4): Vp and density
+    from Vs by Brocher's (2005) polynomials, integer powers by repeated
+    squaring (x**3 = x*(x*x), x**4 = (x*x)**2, x**5 = x*x**4)."""
+    f32 = np.float32
+    x = np.asarray(vs, dtype=f32)
+    x2 = x * x; x3 = x * x2; x4 = x2 * x2
+    vp = f32(0.9409) + f32(2.0947) * x - f32(0.8206) * x2 + f32(0.2683) * x3 - f32(0.0251) * x4
+    p2 = vp * vp; p3 = vp * p2; p4 = p2 * p2; p5 = vp * p4
+    rho = (f32(1.6612) * vp - f32(0.4721) * p2 + f32(0.0671) * p3 - f32(0.0043) * p4
+           + f32(0.000106) * p5)
+    return vp.astype(f32), rho.astype(f32)
+
+
+def fwdsurf1d(ref, vs, iwave, igr, periods, depz):
+    """fwdsurf1d: REAL(4) Vs, Vp and density, layer thicknesses
+    depz(k+1)-depz(k) over a half-space, then the reference's surfdisp96
+    with a spherical earth and the fundamental mode (iflsph = 1, mode = 1,
+    src/surfker/surfker.f90:8)."""
+    vs32 = np.asarray(vs, dtype=np.float64).astype(np.float32)
+    vp, rho = brocher_vp_rho(vs32)
+    depz = np.asarray(depz, dtype=np.float64)
+    thk = np.zeros(len(depz), dtype=np.float32)
+    thk[:-1] = (depz[1:] - depz[:-1]).astype(np.float32)
+    return ref.surfdisp96(thk, vp, vs32, rho, iwave, igr, periods)

@@ -632,6 +632,63 @@ extern "C" int satt_steepest_descent(satt_ctx *ctx, const double *lons, const do
   return geo_timer_read(ctx, t);
 }
 
+extern "C" int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
+                              const double *depz, int nperiod, const double *periods, int iwave,
+                              int igr, double *svel, int *col_status) {
+  if (!ctx) return SATT_E_ARG;
+  if (!vs3d || !depz || !periods || !svel || nx < 1 || ny < 1 || nz < 2 ||
+      nz > SATT_DISP_MAX_NZ || nperiod < 1 || (iwave != 1 && iwave != 2) || igr < 0 ||
+      (size_t)nx * ny > (size_t)INT_MAX)
+    return fail(ctx, SATT_E_ARG, "satt_fwdsurf3d: bad arguments");
+  for (int k = 0; k < nz; ++k) {
+    if (!std::isfinite(depz[k]) || (k > 0 && !(depz[k] > depz[k - 1])))
+      return fail(ctx, SATT_E_ARG, "satt_fwdsurf3d: depths must increase");
+  }
+  for (int p = 0; p < nperiod; ++p)
+    if (!(periods[p] > 0) || !std::isfinite(periods[p]))
+      return fail(ctx, SATT_E_ARG, "satt_fwdsurf3d: periods must be positive");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int ncol = nx * ny;
+  const size_t nv = (size_t)ncol * nz, no = (size_t)ncol * nperiod;
+  DBuf dvs, dsmall, dwork, dout;
+  TRY(upload(ctx, dvs, vs3d, nv * 8));
+  std::vector<double> hs((size_t)nz + nperiod);
+  std::memcpy(hs.data(), depz, nz * 8);
+  std::memcpy(hs.data() + nz, periods, nperiod * 8);
+  TRY(upload(ctx, dsmall, hs.data(), hs.size() * 8));
+  ALLOC(ctx, dwork, fwdsurf_work_floats(ncol, nz) * sizeof(float));
+  ALLOC(ctx, dout, no * 8 + (size_t)ncol * sizeof(int));
+  DispJob J{};
+  J.ncol = ncol;
+  J.nz = nz;
+  J.kmax = nperiod;
+  J.iwave = iwave;
+  J.igr = igr;
+  J.vs = dvs.as<double>();
+  J.depz = dsmall.as<double>();
+  J.periods = dsmall.as<double>() + nz;
+  J.work = dwork.as<float>();
+  J.out = dout.as<double>();
+  J.status = reinterpret_cast<int *>(dout.as<double>() + no);
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_fwdsurf(ctx->stream, J);
+  TRY(geo_timer_stop(ctx, t));
+  std::vector<int> st(ncol);
+  HIPCHK(ctx, hipMemcpyAsync(svel, J.out, no * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(st.data(), J.status, (size_t)ncol * sizeof(int),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  TRY(geo_timer_read(ctx, t));
+  if (col_status) std::memcpy(col_status, st.data(), (size_t)ncol * sizeof(int));
+  for (int c = 0; c < ncol; ++c)
+    if (st[c] != SATT_OK)
+      return fail(ctx, SATT_E_DISPERSION,
+                  "satt_fwdsurf3d: no root of the period equation in column (%d, %d)",
+                  c % nx, c / nx);
+  return SATT_OK;
+}
+
 extern "C" int satt_fwd_setup_fields(satt_ctx *ctx, const double *xx_deg,
                                      const double *yy_deg, int nx, int ny,
                                      const double *spha, const double *sphb,

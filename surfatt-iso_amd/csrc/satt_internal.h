@@ -50,7 +50,8 @@ constexpr int kAdjMaxIter = 1000;
 
 // status codes (include/satt.h)
 constexpr int kOk = 0, kWMaxIter = 1, kEArg = -1, kESrcOnBoundary = -2,
-              kERecOutOfGrid = -3, kENoMem = -4, kEHip = -5, kETeamWait = -7;
+              kERecOutOfGrid = -3, kENoMem = -4, kEHip = -5, kETeamWait = -7,
+              kEDispersion = -8;
 
 // The lon/lat grid shared by every item of a batch.  Radian axes are made on
 // the host with the reference's expression xx_deg/180.0*pi.
@@ -209,5 +210,18 @@ void launch_pm_to_nat(hipStream_t s, size_t N, int nf, const double *pm, double 
 int absmax_parts(size_t n);
 void launch_sd_update(hipStream_t s, size_t n, const double *ker, double updatemax, double *svel,
                       double *part);
+
+// ---- surface-wave dispersion per column (satt_disp.hip)
+struct DispJob {
+  int ncol, nz, kmax, iwave, igr;
+  const double *vs;        // [nz][ncol]: the 3-D model vs3d(nx,ny,nz)
+  const double *depz;      // [nz] km
+  const double *periods;   // [kmax] s
+  float *work;             // fwdsurf_work_floats(ncol, nz)
+  double *out;             // [col][kmax]: period-major (kmax, nx, ny)
+  int *status;             // [col]
+};
+size_t fwdsurf_work_floats(int ncol, int nz);
+void launch_fwdsurf(hipStream_t s, const DispJob &J);
 
 }  // namespace satt

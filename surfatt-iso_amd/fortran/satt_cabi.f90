@@ -12,6 +12,7 @@ module satt_cabi
   integer(c_int), parameter :: SATT_E_ARG = -1, SATT_E_SRC_ON_BOUNDARY = -2
   integer(c_int), parameter :: SATT_E_REC_OUT_OF_GRID = -3, SATT_E_NOMEM = -4
   integer(c_int), parameter :: SATT_E_HIP = -5, SATT_E_NODEV = -6, SATT_E_TEAM_WAIT = -7
+  integer(c_int), parameter :: SATT_E_DISPERSION = -8
   integer(c_int), parameter :: SATT_PREC_FP64 = 0, SATT_PREC_MIXED = 1
   integer(c_int), parameter :: SATT_LAUNCH_BATCHED = 0, SATT_LAUNCH_PER_SOLVE = 1
 
@@ -119,6 +120,17 @@ module satt_cabi
       real(c_double), intent(in) :: lons(*), lats(*), z(*), sigma(*)
       real(c_double), intent(out) :: m11(*), m12(*), m22(*), angle(*)
     end function satt_topo_metric
+
+    integer(c_int) function satt_fwdsurf3d(ctx, vs3d, nx, ny, nz, depz, nperiod, periods, &
+                                           iwave, igr, svel, col_status) &
+                                           bind(C, name="satt_fwdsurf3d")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nz, nperiod, iwave, igr
+      real(c_double), intent(in) :: vs3d(*), depz(*), periods(*)
+      real(c_double), intent(out) :: svel(*)
+      integer(c_int), intent(out) :: col_status(*)
+    end function satt_fwdsurf3d
 
     real(c_double) function satt_topo_sigma(vel, period, wavelen_factor) &
                                             bind(C, name="satt_topo_sigma")

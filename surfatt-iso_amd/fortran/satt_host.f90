@@ -24,6 +24,9 @@
 !  - satt_steepest_descent_host()  att_tomo_2d%optimize + steepest_descent
 !                      (src/tomo2d.f90:203-251), the misfit-increase shrink
 !                      of the step length included
+!  - satt_fwdsurf3d_host()  fwdsurf3d_decomp (src/surfker/surfker.f90:200-222)
+!                      over the whole grid: the velocity maps of
+!                      att_grid%fwdsurf (src/grid.f90:150-163)
 !=====================================================================
 module satt_host
   use, intrinsic :: iso_c_binding
@@ -32,7 +35,7 @@ module satt_host
   private
   public :: satt_host_ctx, satt_host_check, satt_host_release, satt_forward_simulate_host
   public :: satt_gaussian_smooth_geo_2, satt_gradient_2_geo, satt_get_topo_metric
-  public :: satt_steepest_descent_host
+  public :: satt_steepest_descent_host, satt_fwdsurf3d_host
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -255,6 +258,24 @@ contains
                                                size(ygrids), size(adj_s,1), adj_s, sigma_2d, &
                                                updatemax, svel, ker_s), 'steepest_descent')
   end subroutine satt_steepest_descent_host
+
+  !> fwdsurf3d_decomp(vel, ..., iwave, igr, tRc, depz, svel) for every column:
+  !> svel(nperiod,nx,ny) phase (igr 0) or group velocities of Love (iwave 1)
+  !> or Rayleigh (iwave 2) waves.  A column without a root stops the run, as
+  !> the reference's surfdisp96 does.
+  subroutine satt_fwdsurf3d_host(vel, iwave, igr, tRc, depz, svel)
+    real(c_double), intent(in) :: vel(:,:,:), tRc(:), depz(:)
+    integer, intent(in) :: iwave, igr
+    real(c_double), intent(out) :: svel(:,:,:)
+    integer(c_int) :: cs(size(vel,1)*size(vel,2)), st
+    st = satt_fwdsurf3d(satt_host_ctx(), vel, size(vel,1), size(vel,2), size(vel,3), depz, &
+                        size(tRc), tRc, iwave, igr, svel, cs)
+    if (st == SATT_E_DISPERSION) then
+      write(*,*) 'WARNING:improper initial value in disper - no zero found'
+      stop 'Dispersion calculation failed'
+    end if
+    call satt_host_check(st, 'fwdsurf3d')
+  end subroutine satt_fwdsurf3d_host
 
   integer function find_name(names, n, s)
     character(len=*), intent(in) :: names(:), s

@@ -22,6 +22,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "satt.h")
 OK, W_MAXITER = 0, 1
 E_ARG, E_SRC_ON_BOUNDARY, E_REC_OUT_OF_GRID, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
 E_TEAM_WAIT = -7
+E_DISPERSION = -8
 PREC_FP64, PREC_MIXED = 0, 1
 LAUNCH_BATCHED, LAUNCH_PER_SOLVE = 0, 1
 FIELD_T, FIELD_TA, FIELD_TD = 0, 1, 2
@@ -286,6 +287,24 @@ class Context:
             self.ptr, _d(lons), _d(lats), sh[0], sh[1], nf, _d(self._pm(adj_s, nf, sh)),
             ctypes.c_double(sigma_2d), ctypes.c_double(updatemax), _d(sv), _d(ker)))
         return sv, ker
+
+    def fwdsurf3d(self, vs3d, depz, periods, iwave=2, igr=0, status=False):
+        """Surface-wave velocity maps of att_grid%fwdsurf (src/grid.f90:150-163):
+        (nperiod, nx, ny) phase (igr 0) or group (igr > 0) velocities of Love
+        (iwave 1) or Rayleigh (iwave 2) waves for vs3d (nx, ny, nz) on depths
+        depz (nz).  status=True also returns the per-column codes and does not
+        raise on columns without a root."""
+        vs = np.asfortranarray(vs3d, dtype=np.float64)
+        nx, ny, nz = vs.shape
+        depz, periods = _c(depz), _c(periods)
+        out = np.zeros((len(periods), nx, ny), order="F")
+        cs = np.zeros((nx, ny), dtype=np.int32, order="F")
+        st = self.lib.satt_fwdsurf3d(self.ptr, _d(vs), nx, ny, nz, _d(depz), len(periods),
+                                     _d(periods), int(iwave), int(igr), _d(out), _i(cs))
+        if status and st == E_DISPERSION:
+            return out, cs
+        self._check(st)
+        return (out, cs) if status else out
 
     def last_device_ms(self):
         """Device time of the last model-side call's kernels (ms)."""

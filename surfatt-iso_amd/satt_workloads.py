@@ -405,6 +405,19 @@ def phase_velocity_maps(vs3d, depths, periods):
     return out
 
 
+def acqui3d_model(n=41, nz=31, zmax=60.0, nper=16, pert=0.06, ncb=(2, 3, 2)):
+    """The 3-D stage's true model: Vs (n, n, nz) on depths 0..zmax km rising
+    linearly from 3.0 to 4.5 km/s times (1 + the reference's 3-D checkerboard,
+    src/model.f90:276-300), and nper periods log-spaced 4-40 s."""
+    depths = np.linspace(0.0, zmax, nz)
+    vs1 = 3.0 + 1.5 * depths / zmax
+    xp = np.sin(ncb[0] * PI_F * np.arange(1, n + 1) / n)
+    yp = np.sin(ncb[1] * PI_F * np.arange(1, n + 1) / n)
+    zp = np.sin(ncb[2] * PI_F * np.arange(1, nz + 1) / nz)
+    vs = vs1[None, None, :] * (1 + pert * xp[:, None, None] * yp[None, :, None] * zp[None, None, :])
+    return vs, depths, np.geomspace(4.0, 40.0, nper)
+
+
 def acqui3d_checkerboard(n=41, d=0.05, nz=31, zmax=60.0, nsta=100, nper=16, seed=1,
                          pert=0.06, ncb=(2, 3, 2)):
     """The 3-D inversion's 2-D stage (att_acqui%forward_simulate, dim 3):
@@ -416,14 +429,9 @@ def acqui3d_checkerboard(n=41, d=0.05, nz=31, zmax=60.0, nsta=100, nper=16, seed
     (nsta*nper items).  The current model is the unperturbed 1-D Vs.
     Returns (batch_true, batch_current)."""
     xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
-    depths = np.linspace(0.0, zmax, nz)
+    vs_true, depths, periods = acqui3d_model(n, nz, zmax, nper, pert, ncb)
     vs1 = 3.0 + 1.5 * depths / zmax
-    xp = np.sin(ncb[0] * PI_F * np.arange(1, n + 1) / n)
-    yp = np.sin(ncb[1] * PI_F * np.arange(1, n + 1) / n)
-    zp = np.sin(ncb[2] * PI_F * np.arange(1, nz + 1) / nz)
-    vs_true = vs1[None, None, :] * (1 + pert * xp[:, None, None] * yp[None, :, None] * zp[None, None, :])
     vs_cur = np.broadcast_to(vs1[None, None, :], (n, n, nz))
-    periods = np.geomspace(4.0, 40.0, nper)
     ct = phase_velocity_maps(vs_true, depths, periods)
     cc = phase_velocity_maps(np.ascontiguousarray(vs_cur), depths, periods)
     rng = np.random.default_rng(seed)

@@ -227,3 +227,4 @@ def test_steepest_descent_scales_by_the_global_max():
     assert np.array_equal(sv, want)
     per_period = svel * (1 + 0.02 * (adj / np.max(np.abs(adj), axis=(1, 2), keepdims=True)))
     assert np.max(np.abs(sv - per_period)) > 1e-3
+
