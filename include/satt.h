@@ -178,7 +178,7 @@ int satt_steepest_descent(satt_ctx *ctx, const double *lons, const double *lats,
  * search finds no root gets zeros from that period on and the call returns
  * SATT_E_DISPERSION (the reference stops the run there); col_status (nx*ny,
  * may be NULL) holds each column's code.  nz <= SATT_DISP_MAX_NZ. */
-#define SATT_DISP_MAX_NZ 1024
+#define SATT_DISP_MAX_NZ 256
 int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
                    const double *depz, int nperiod, const double *periods,
                    int iwave, int igr, double *svel, int *col_status);

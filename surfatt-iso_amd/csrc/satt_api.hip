@@ -650,13 +650,12 @@ extern "C" int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny,
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int ncol = nx * ny;
   const size_t nv = (size_t)ncol * nz, no = (size_t)ncol * nperiod;
-  DBuf dvs, dsmall, dwork, dout;
+  DBuf dvs, dsmall, dout;
   TRY(upload(ctx, dvs, vs3d, nv * 8));
   std::vector<double> hs((size_t)nz + nperiod);
   std::memcpy(hs.data(), depz, nz * 8);
   std::memcpy(hs.data() + nz, periods, nperiod * 8);
   TRY(upload(ctx, dsmall, hs.data(), hs.size() * 8));
-  ALLOC(ctx, dwork, fwdsurf_work_floats(ncol, nz) * sizeof(float));
   ALLOC(ctx, dout, no * 8 + (size_t)ncol * sizeof(int));
   DispJob J{};
   J.ncol = ncol;
@@ -667,7 +666,6 @@ extern "C" int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny,
   J.vs = dvs.as<double>();
   J.depz = dsmall.as<double>();
   J.periods = dsmall.as<double>() + nz;
-  J.work = dwork.as<float>();
   J.out = dout.as<double>();
   J.status = reinterpret_cast<int *>(dout.as<double>() + no);
   GeoTimer t;

@@ -7,12 +7,15 @@
 //   get_vprho (Brocher 2005)      src/surfker/surfker.f90:306-320
 //   surfdisp96 (Herrmann's CPS)   src/surfker/surfdisp96.f:57-1098
 //
-// One thread per column.  A column's periods are searched in order (each
-// search starts from the previous period's root) and every search evaluates
-// the period equation layer by layer, so the column is sequential work; the
-// columns are independent.  The layered model and the roots live in a
-// workspace laid out [field][layer][column], so the lanes of a wave (adjacent
-// columns) touch adjacent words.
+// One wave per column.  A column's periods are searched in order (each
+// search starts from the previous period's root) and each step of a search
+// evaluates the period equation, a chain of per-layer propagators from the
+// half-space up.  The per-layer terms (the trigonometric and exponential
+// functions of each layer, Dunkin's 5x5 compound matrix for Rayleigh waves)
+// are independent, so the lanes compute one layer each into LDS; then every
+// lane runs the short chain itself, in the reference's order, so that all
+// lanes hold the same value and the search runs wave-uniformly.  The column's
+// layered model sits in LDS beside the terms.
 //
 // The arithmetic keeps the reference's types variable by variable: the model
 // (thickness, Vp, Vs, density, the earth-flattening factors), the starting
@@ -31,14 +34,16 @@ namespace {
 constexpr double kTwoPiD = 2.0 * 3.141592653589793;  // 2.d0*3.141592653589793d0
 constexpr int kSearchCap = 1 << 16;  // bound on bracketing steps (the search ends long before)
 
-// One column's layered model in the workspace: layer m of field f at
-// base[(f * nz + m) * ncol + col].
+// One column's layered model in LDS (layer m of field f at w[f * n + m])
+// and the per-layer terms of the period equation (kTermWords doubles per
+// layer).
+constexpr int kTermWords = 25;
 struct Layers {
   float *w;
-  int nz, ncol, col;
+  double *terms;
   int n;    // layers in the model (mmax)
   int llw;  // first layer below the water (1-based, as the reference's llw)
-  __device__ float &at(int f, int m) const { return w[((size_t)f * nz + m) * ncol + col]; }
+  __device__ float &at(int f, int m) const { return w[f * n + m]; }
   __device__ float &d(int m) const { return at(0, m); }
   __device__ float &a(int m) const { return at(1, m); }
   __device__ float &b(int m) const { return at(2, m); }
@@ -52,19 +57,15 @@ constexpr int kLayerFields = 7;
 __device__ inline double sgn(double x) { return copysign(1.0, x); }
 
 // SH period equation, Haskell-Thomson from the half-space up
-// (surfdisp96.f:737-799).
+// (surfdisp96.f:737-799).  Lanes fill one layer's (cosq, y, z, mu) each;
+// every lane then runs the chain.
 __device__ double love_det(const Layers &L, double wvno, double omega) {
   const int n = L.n;
-  double beta1 = (double)L.b(n - 1), rho1 = (double)L.rho(n - 1);
-  double xkb = omega / beta1;
-  double rb = sqrt((wvno + xkb) * fabs(wvno - xkb));
-  double e1 = rho1 * rb, e2 = 1.0 / (beta1 * beta1);
-  for (int m = n - 2; m >= L.llw - 1; --m) {
-    beta1 = (double)L.b(m);
-    rho1 = (double)L.rho(m);
+  for (int m = L.llw - 1 + (int)threadIdx.x; m <= n - 2; m += (int)blockDim.x) {
+    const double beta1 = (double)L.b(m), rho1 = (double)L.rho(m);
     const double xmu = rho1 * beta1 * beta1;
-    xkb = omega / beta1;
-    rb = sqrt((wvno + xkb) * fabs(wvno - xkb));
+    const double xkb = omega / beta1;
+    const double rb = sqrt((wvno + xkb) * fabs(wvno - xkb));
     const double q = (double)L.d(m) * rb;
     double cosq, y, z;
     if (wvno < xkb) {
@@ -83,6 +84,20 @@ __device__ double love_det(const Layers &L, double wvno, double omega) {
       y = sinq / rb;
       z = rb * sinq;
     }
+    double *t = L.terms + (size_t)m * kTermWords;
+    t[0] = cosq;
+    t[1] = y;
+    t[2] = z;
+    t[3] = xmu;
+  }
+  __syncthreads();
+  const double beta1 = (double)L.b(n - 1), rho1 = (double)L.rho(n - 1);
+  const double xkb = omega / beta1;
+  const double rb = sqrt((wvno + xkb) * fabs(wvno - xkb));
+  double e1 = rho1 * rb, e2 = 1.0 / (beta1 * beta1);
+  for (int m = n - 2; m >= L.llw - 1; --m) {
+    const double *t = L.terms + (size_t)m * kTermWords;
+    const double cosq = t[0], y = t[1], z = t[2], xmu = t[3];
     const double e10 = e1 * cosq + e2 * xmu * z;
     const double e20 = e1 * y / xmu + e2 * cosq;
     double xnor = fabs(e10);
@@ -92,6 +107,7 @@ __device__ double love_det(const Layers &L, double wvno, double omega) {
     e1 = e10 / xnor;
     e2 = e20 / xnor;
   }
+  __syncthreads();  // the terms are rewritten by the next evaluation
   return e1;
 }
 
@@ -154,10 +170,10 @@ __device__ LayerTerms layer_terms(double p, double q, double ra, double rb, doub
   return t;
 }
 
-// e <- e * Dunkin's compound matrix of one layer, then normalised by its
-// largest element (surfdisp96.f:858-872, 1025-1098).
-__device__ void dunkin_step(double e[5], const LayerTerms &t, double wvno2, double gam,
-                            double gammk, double rho) {
+// Dunkin's compound matrix of one layer (surfdisp96.f:1054-1098), ca[i][j]
+// for the reference's ca(i+1, j+1), into 25 words.
+__device__ void dunkin_matrix(double *out, const LayerTerms &t, double wvno2, double gam,
+                              double gammk, double rho) {
   const double gamm1 = gam - 1.0, twgm1 = gam + gamm1, gmgmk = gam * gammk;
   const double gmgm1 = gam * gamm1, gm1sq = gamm1 * gamm1, rho2 = rho * rho;
   const double a0pq = t.a0 - t.cpcq;
@@ -189,12 +205,21 @@ __device__ void dunkin_step(double e[5], const LayerTerms &t, double wvno2, doub
   ca[2][2] = t.a0 + 2.0 * (t.cpcq - ca[0][0]);
   ca[2][3] = tt * ca[1][2];
   ca[2][4] = tt * ca[0][2];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) out[5 * i + j] = ca[i][j];
+}
+
+// e <- e * ca, then normalised by its largest element (surfdisp96.f:
+// 858-872, 1025-1052).
+__device__ void chain_step(double e[5], const double *ca) {
   double ee[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
     double cr = 0.0;
 #pragma unroll
-    for (int j = 0; j < 5; ++j) cr = cr + e[j] * ca[j][i];
+    for (int j = 0; j < 5; ++j) cr = cr + e[j] * ca[5 * j + i];
     ee[i] = cr;
   }
   double big = 0.0;
@@ -226,19 +251,18 @@ __device__ double rayleigh_det(const Layers &L, double wvno, double omga) {
   e[2] = rho1 * (gamm1 - gammk * ra * rb);
   e[3] = rho1 * rb;
   e[4] = wvno2 - ra * rb;
-  for (int m = n - 2; m >= L.llw - 1; --m) {
-    xka = omega / (double)L.a(m);
-    xkb = omega / (double)L.b(m);
-    t = (double)L.b(m) / omega;
-    gammk = 2.0 * t * t;
-    gam = gammk * wvno2;
-    ra = radical(wvno, xka);
-    rb = radical(wvno, xkb);
+  for (int m = L.llw - 1 + (int)threadIdx.x; m <= n - 2; m += (int)blockDim.x) {
+    const double xkam = omega / (double)L.a(m), xkbm = omega / (double)L.b(m);
+    const double tm = (double)L.b(m) / omega;
+    const double gammkm = 2.0 * tm * tm, gamm = gammkm * wvno2;
+    const double ram = radical(wvno, xkam), rbm = radical(wvno, xkbm);
     const double dpth = (double)L.d(m);
-    rho1 = (double)L.rho(m);
-    const LayerTerms lt = layer_terms(ra * dpth, rb * dpth, ra, rb, wvno, xka, xkb, dpth);
-    dunkin_step(e, lt, wvno2, gam, gammk, rho1);
+    const LayerTerms lt = layer_terms(ram * dpth, rbm * dpth, ram, rbm, wvno, xkam, xkbm, dpth);
+    dunkin_matrix(L.terms + (size_t)m * kTermWords, lt, wvno2, gamm, gammkm, (double)L.rho(m));
   }
+  __syncthreads();
+  for (int m = n - 2; m >= L.llw - 1; --m) chain_step(e, L.terms + (size_t)m * kTermWords);
+  __syncthreads();  // the terms are rewritten by the next evaluation
   if (L.llw != 1) {  // the water layer on top
     xka = omega / (double)L.a(0);
     ra = radical(wvno, xka);
@@ -385,7 +409,7 @@ __device__ float halfspace_rayleigh(float a, float b) {
 
 // Earth flattening (surfdisp96.f:502-576): thicknesses and velocities to the
 // flat-earth equivalents, the factors kept for the density correction.
-__device__ float flatten(const Layers &L) {
+__device__ void flatten(const Layers &L) {
   const int n = L.n;
   const double ar = 6370.0;
   double dr = 0.0, r0 = ar;
@@ -405,15 +429,14 @@ __device__ float flatten(const Layers &L) {
     L.btp(i) = (float)tmp;
     r0 = r1;
   }
-  const float dhalf = L.d(n - 1);
   L.d(n - 1) = 0.0f;
-  return dhalf;
 }
 
-__device__ void flatten_density(const Layers &L, int ifunc, float dhalf) {
+// The density factors of the flattening, one layer per lane (the half-space
+// thickness the reference restores around them is 0 again afterwards).
+__device__ void flatten_density(const Layers &L, int ifunc) {
   const int n = L.n;
-  L.d(n - 1) = dhalf;
-  for (int i = 0; i < n; ++i) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const float bt = L.btp(i);
     if (ifunc == 1) {
       const float b2 = bt * bt;
@@ -422,7 +445,6 @@ __device__ void flatten_density(const Layers &L, int ifunc, float dhalf) {
       L.rho(i) = L.rtp(i) * powf(bt, -2.275f);
     }
   }
-  L.d(n - 1) = 0.0f;
 }
 
 // Vp and density from Vs (Brocher 2005), single precision
@@ -435,14 +457,13 @@ __device__ inline void brocher(float vs, float &vp, float &rho) {
 }
 
 __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= J.ncol) return;
+  extern __shared__ double lds_terms[];
+  const int col = blockIdx.x, lane = threadIdx.x;
   const int nz = J.nz, kmax = J.kmax;
-  Layers L{J.work, nz, J.ncol, col, nz, 1};
-  // fwdsurf1d: Vs to REAL(4), Vp and density, layer thicknesses with a
+  Layers L{reinterpret_cast<float *>(lds_terms + (size_t)nz * kTermWords), lds_terms, nz, 1};
+  // fwdsurf1d: Vs to REAL(4), Vp and density, layer thicknesses over a
   // half-space at the bottom
-  float vmin = 0.0f;
-  for (int k = 0; k < nz; ++k) {
+  for (int k = lane; k < nz; k += 64) {
     const float vs = (float)J.vs[col + (size_t)J.ncol * k];
     float vp, rho;
     brocher(vs, vp, rho);
@@ -450,14 +471,18 @@ __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
     L.a(k) = vp;
     L.rho(k) = rho;
     L.d(k) = k < nz - 1 ? (float)(J.depz[k + 1] - J.depz[k]) : 0.0f;
-    vmin = k == 0 || vs < vmin ? vs : vmin;
   }
+  __syncthreads();
+  float vmin = L.b(0);
+  for (int k = 1; k < nz; ++k) vmin = L.b(k) < vmin ? L.b(k) : vmin;
   // surfdisp96 with a spherical earth (iflsph 1) and the fundamental mode
   const float ddc = vmin < 1.0f ? 0.001f : (vmin < 2.0f ? 0.002f : 0.005f);
   const float sone = 1.5f, h = 0.005f;
   L.llw = L.b(0) <= 0.0f ? 2 : 1;
   const double one = 1.0e-2;
-  const float dhalf = flatten(L);
+  __syncthreads();
+  if (lane == 0) flatten(L);  // a running depth: one lane
+  __syncthreads();
   float betmx = -1.0e20f, betmn = 1.0e20f;
   int jmn = 0, jsol = 1;
   for (int i = 0; i < nz; ++i) {
@@ -474,7 +499,8 @@ __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
     if (bi > betmx) betmx = bi;
   }
   const int ifunc = J.iwave;
-  flatten_density(L, ifunc, dhalf);
+  flatten_density(L, ifunc);
+  __syncthreads();
   const double onea = (double)sone;
   float cc1 = jsol == 0 ? betmn : halfspace_rayleigh(L.a(jmn), L.b(jmn));
   cc1 = 0.95f * cc1;
@@ -504,7 +530,8 @@ __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
     }
     if (!find_root(L, ifunc, t1, c1, clow, dc, cm, betmx, k == 0, S)) {
       status = kEDispersion;  // the reference stops: 'Dispersion calculation failed'
-      for (int q = k; q < kmax; ++q) J.out[q + (size_t)kmax * col] = 0.0;
+      if (lane == 0)
+        for (int q = k; q < kmax; ++q) J.out[q + (size_t)kmax * col] = 0.0;
       break;
     }
     const double ck = c1;
@@ -525,17 +552,19 @@ __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
       const float gvel = (1.0f / t1a - 1.0f / t1b) / (1.0f / (t1a * cc0) - 1.0f / (t1b * cc1k));
       out = (double)gvel;
     }
-    J.out[k + (size_t)kmax * col] = out;
+    if (lane == 0) J.out[k + (size_t)kmax * col] = out;
   }
-  J.status[col] = status;
+  if (lane == 0) J.status[col] = status;
 }
 
 }  // namespace
 
-void launch_fwdsurf(hipStream_t s, const DispJob &J) {
-  hipLaunchKernelGGL(k_fwdsurf, dim3((unsigned)((J.ncol + 63) / 64)), dim3(64), 0, s, J);
+size_t fwdsurf_lds_bytes(int nz) {
+  return (size_t)nz * (kTermWords * sizeof(double) + kLayerFields * sizeof(float));
 }
 
-size_t fwdsurf_work_floats(int ncol, int nz) { return (size_t)kLayerFields * nz * ncol; }
+void launch_fwdsurf(hipStream_t s, const DispJob &J) {
+  hipLaunchKernelGGL(k_fwdsurf, dim3((unsigned)J.ncol), dim3(64), fwdsurf_lds_bytes(J.nz), s, J);
+}
 
 }  // namespace satt

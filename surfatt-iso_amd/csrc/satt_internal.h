@@ -217,11 +217,11 @@ struct DispJob {
   const double *vs;        // [nz][ncol]: the 3-D model vs3d(nx,ny,nz)
   const double *depz;      // [nz] km
   const double *periods;   // [kmax] s
-  float *work;             // fwdsurf_work_floats(ncol, nz)
   double *out;             // [col][kmax]: period-major (kmax, nx, ny)
   int *status;             // [col]
 };
-size_t fwdsurf_work_floats(int ncol, int nz);
+// LDS of one column's wave (the layered model and the per-layer terms)
+size_t fwdsurf_lds_bytes(int nz);
 void launch_fwdsurf(hipStream_t s, const DispJob &J);
 
 }  // namespace satt
