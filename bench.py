@@ -218,34 +218,6 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
-def hbm_probe(nbytes=1 << 30, reps=5):
-    """Achievable HBM bandwidth on this device: the runtime's device-to-device
-    copy of nbytes (reads nbytes, writes nbytes), best of `reps`, timed with
-    HIP events; GB/s counts both directions."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
-    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
-    best = None
-    for _ in range(reps + 1):   # the first copy warms up
-        assert hip.hipEventRecord(e0, None) == 0
-        assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None) == 0
-        assert hip.hipEventRecord(e1, None) == 0
-        assert hip.hipEventSynchronize(e1) == 0
-        ms = ctypes.c_float()
-        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
-        best = ms.value if best is None or ms.value < best else best
-    hip.hipEventDestroy(e0)
-    hip.hipEventDestroy(e1)
-    hip.hipFree(a)
-    hip.hipFree(b)
-    return 2 * nbytes / (best * 1e-3) / 1e9
-
-
 def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
     """The serving mix (grid sizes 101/201/401/801 squared with probabilities
     0.4/0.3/0.2/0.1, 100 receivers) as a Poisson stream of `nreq` requests at
@@ -450,7 +422,7 @@ def main():
     ap.add_argument("--model-side-only", action="store_true",
                     help="run only the model-side leg and print its JSON")
     ap.add_argument("--no-hbm-probe", action="store_true",
-                    help="skip the device-to-device copy that measures achievable HBM bandwidth")
+                    help="skip the streaming kernels that measure achievable HBM bandwidth")
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -647,9 +619,9 @@ def main():
         cpu = cpu_baseline(parts, nits, max(args.cpu_sample, th), th)
         log(f"[bench] cpu baseline: {cpu['value']:.2f} solves/s on {th} threads")
 
-    hbm = None if args.no_hbm_probe else hbm_probe()
+    hbm = None if args.no_hbm_probe else ctx.hbm_probe(1 << 31)
     if hbm is not None:
-        log(f"[bench] device-to-device copy: {hbm:.0f} GB/s")
+        log(f"[bench] streaming kernels: read {hbm[0]:.0f} GB/s, copy {hbm[1]:.0f} GB/s")
     side = None
     if not args.no_model_side:
         side = model_side(ctx, cpu=not args.no_cpu)
@@ -683,7 +655,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "achievable_copy_GBs": None if hbm is None else round(hbm, 1),
+                     "achievable_GBs": None if hbm is None else
+                     {"stream_read": round(hbm[0], 1), "stream_copy": round(hbm[1], 1)},
                      "measured_traffic_GBs": None if traffic is None else
                      round(traffic / (kms / launches * 1e-3) / 1e9, 1),
                      "alg_bytes_per_launch": kbytes / launches,

@@ -183,6 +183,12 @@ int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
                    const double *depz, int nperiod, const double *periods,
                    int iwave, int igr, double *svel, int *col_status);
 
+/* Achievable HBM bandwidth: the best of five runs of a streaming read and of a
+ * streaming copy over nbytes (1 MiB to 64 GiB; allocates 2*nbytes), in GB/s
+ * (the copy counts bytes read and written).  A measurement aid for the
+ * roofline of the kernels above; no counterpart in the reference. */
+int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_gbs);
+
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
  * satt_gradient_geo2, satt_topo_metric or satt_steepest_descent call on the
  * context (HIP events on its stream; transfers excluded). */

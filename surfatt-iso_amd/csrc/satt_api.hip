@@ -488,6 +488,17 @@ int geo_smooth_setup(satt_ctx *ctx, const char *who, const double *lons, const d
 
 extern "C" double satt_last_device_ms(const satt_ctx *ctx) { return ctx ? ctx->last_ms : 0.0; }
 
+extern "C" int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_gbs) {
+  if (!ctx || !read_gbs || !copy_gbs || !(nbytes >= (1 << 20)) || nbytes > 64.0 * (1ull << 30))
+    return SATT_E_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int cus = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int st = stream_probe(ctx->stream, cus, (size_t)nbytes, 5, *read_gbs, *copy_gbs);
+  if (st < 0) return fail(ctx, st, "satt_hbm_probe: %s", st == SATT_E_NOMEM ? "no memory" : "HIP error");
+  return SATT_OK;
+}
+
 extern "C" double satt_topo_sigma(double vel, double period, double wavelen_factor) {
   const double pi = (double)3.14159265359f;
   const double km2deg = 1.0 / (6371.0 * pi / 180.0);

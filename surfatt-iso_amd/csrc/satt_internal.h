@@ -220,6 +220,9 @@ struct DispJob {
   double *out;             // [col][kmax]: period-major (kmax, nx, ny)
   int *status;             // [col]
 };
+// Streaming read and copy bandwidth of the device (satt_probe.hip), GB/s
+int stream_probe(hipStream_t s, int cus, size_t nbytes, int reps, double &read_gbs,
+                 double &copy_gbs);
 // LDS of one column's wave (the layered model and the per-layer terms)
 size_t fwdsurf_lds_bytes(int nz);
 void launch_fwdsurf(hipStream_t s, const DispJob &J);

@@ -63,7 +63,7 @@ struct GroupDims {
 // strip width behind.  More waves cut an item's latency on wide grids; one
 // wave per item is the throughput form when items fill the device.
 constexpr int kMaxTeam = 8;
-constexpr size_t kTeamLdsMax = 64 * 1024;
+constexpr size_t kTeamLdsMax = 160 * 1024;  // a gfx950 workgroup's LDS
 // The largest team (<= want) of a sweep stage whose LDS (edge channels of
 // the stage's value type) fits for grids up to ny rows, ns strips.
 int team_waves(int want, int ny, int ns, int stage);

@@ -306,6 +306,13 @@ class Context:
         self._check(st)
         return (out, cs) if status else out
 
+    def hbm_probe(self, nbytes=1 << 30):
+        """(read GB/s, copy GB/s) of streaming kernels over nbytes."""
+        r, c = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.satt_hbm_probe(self.ptr, ctypes.c_double(nbytes), ctypes.byref(r),
+                                            ctypes.byref(c)))
+        return r.value, c.value
+
     def last_device_ms(self):
         """Device time of the last model-side call's kernels (ms)."""
         return float(self.lib.satt_last_device_ms(self.ptr))
