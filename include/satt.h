@@ -22,6 +22,7 @@
  *                          (src/grid.f90:114-123,137-141)
  *   satt_steepest_descent  att_tomo_2d%optimize + steepest_descent
  *                          (src/tomo2d.f90:203-251)
+ *   satt_plan_linesearch   att_tomo_2d%linesearch (src/tomo2d.f90:253-288)
  *   satt_fwdsurf3d         fwdsurf3d_decomp / fwdsurf1d with surfdisp96
  *                          (src/surfker/surfker.f90:200-254,
  *                          src/surfker/surfdisp96.f:57)
@@ -272,7 +273,7 @@ int satt_plan_set_graph(satt_plan *plan, int on);
  * sweep following the previous one at least a strip width behind, which
  * cuts an item's latency on wide grids; one wave per item is the throughput
  * form when the items fill the device.  W is capped per launch group (at
- * most 4, at most the strip count, and the LDS of the edge channels).  The
+ * most 8, at most the strip count, and the LDS of the edge channels).  The
  * results do not depend on W.  No counterpart in the reference. */
 int satt_plan_set_waves(satt_plan *plan, int waves);
 /* waves = 0: per launch group and sweep stage, the team sizes the autotuner
@@ -313,6 +314,22 @@ int satt_plan_wait(satt_plan *plan);
 int satt_plan_results(satt_plan *plan, satt_result *result);
 /* The results of part `part` (satt_plan_results: part 0). */
 int satt_plan_results_part(satt_plan *plan, int part, satt_result *result);
+/* att_tomo_2d%linesearch (src/tomo2d.f90:253-288) on a one-grid plan, with
+ * prepare_fwd_linesearch (src/acqui_2d.f90:173-183).  For sit = 1 ..
+ * max_sub_niter: the plan's velocity becomes
+ * svel*(1 + (*updatemax)*ker_s/maxval(abs(ker_s))) (formed on the device),
+ * the plan runs forward only (forward_simulate(chi, .false., .false.)) and
+ * if its misfit exceeds chi0 the step shrinks (*updatemax *= maxshrink) and
+ * the search goes on, else it stops.  svel_out gets the last velocity tried
+ * (acqui%svel = acqui%ag%svel, also when every sub-iteration raised the
+ * misfit), *chi_out the last misfit, *nsub the sub-iterations run; the plan
+ * keeps that velocity as its model (m11, m12, m22 unchanged).  svel, ker_s,
+ * svel_out are (nperiod,nx,ny); with max_sub_niter < 1 svel_out = svel and
+ * *chi_out = chi0.  ker_s must have a nonzero finite maximum. */
+int satt_plan_linesearch(satt_plan *plan, const double *svel, const double *ker_s,
+                         double chi0, double maxshrink, int max_sub_niter,
+                         double *updatemax, double *svel_out, double *chi_out,
+                         int *nsub);
 void satt_plan_destroy(satt_plan *plan);
 
 /* Sizing and timing hooks of a plan (for measurement harnesses). */

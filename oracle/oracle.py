@@ -382,6 +382,30 @@ def topo_metric(lons, lats, z, sigma):
     return tuple(outs)
 
 
+def linesearch(batch, ker_s, chi0, updatemax, maxshrink, max_sub_niter):
+    """att_tomo_2d%linesearch (src/tomo2d.f90:253-288): each sub-iteration
+    forms prepare_fwd_linesearch's trial model (src/acqui_2d.f90:179-180,
+    svel*(1 + updatemax*ker_s/maxval(abs(ker_s))), the product first) from
+    the batch's svel, runs forward_simulate forward only (:271) and shrinks
+    the step while the misfit exceeds chi0 (:272-283); the model kept is the
+    last one tried (:285-287).  Returns svel, updatemax, chi, nsub."""
+    svel = np.asfortranarray(batch["svel"], dtype=np.float64)
+    ker = np.asfortranarray(ker_s, dtype=np.float64)
+    mx = np.max(np.abs(ker))
+    trial, chi, nsub = svel.copy(order="F"), chi0, 0
+    for sit in range(1, max_sub_niter + 1):
+        trial = np.asfortranarray(svel * (1 + updatemax * ker / mx))
+        b = dict(batch)
+        b["svel"] = trial
+        chi = forward_simulate(b, istotable=False, isadj=False)["chi"]
+        nsub = sit
+        if chi > chi0:
+            updatemax = updatemax * maxshrink
+        else:
+            break
+    return {"svel": trial, "updatemax": updatemax, "chi": chi, "nsub": nsub}
+
+
 def steepest_descent(lons, lats, adj_s, sigma_2d, updatemax, svel):
     """optimize + steepest_descent (src/tomo2d.f90:203-251): (svel_new, ker_s)."""
     lons = np.ascontiguousarray(lons, dtype=np.float64)

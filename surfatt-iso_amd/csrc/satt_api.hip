@@ -1575,6 +1575,67 @@ extern "C" int satt_plan_results(satt_plan *P, satt_result *R) {
   return satt_plan_results_part(P, 0, R);
 }
 
+// att_tomo_2d%linesearch (src/tomo2d.f90:253-284): each sub-iteration forms
+// the trial model of prepare_fwd_linesearch (src/acqui_2d.f90:173-183) on
+// the device, in the plan's own model buffer, and runs the plan forward only
+// (forward_simulate(chi, .false., .false.), :271); the base model and the
+// kernel stay resident over the sub-iterations.
+extern "C" int satt_plan_linesearch(satt_plan *P, const double *svel, const double *ker_s,
+                                    double chi0, double maxshrink, int max_sub_niter,
+                                    double *updatemax, double *svel_out, double *chi_out,
+                                    int *nsub) {
+  if (!P) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  if (P->parts.size() != 1)
+    return fail(ctx, SATT_E_ARG, "satt_plan_linesearch: the plan must hold one grid");
+  if (!svel || !ker_s || !updatemax || !svel_out || !chi_out || !nsub ||
+      !std::isfinite(*updatemax) || !std::isfinite(maxshrink) || std::isnan(chi0))
+    return fail(ctx, SATT_E_ARG, "satt_plan_linesearch: bad arguments");
+  Part &Q = *P->parts[0];
+  const size_t M = (size_t)Q.np * Q.N;
+  *nsub = 0;
+  *chi_out = chi0;
+  if (max_sub_niter < 1) {  // the loop does not run: the model stays svel
+    std::memcpy(svel_out, svel, M * 8);
+    return SATT_OK;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DBuf work;
+  ALLOC(ctx, work, (2 * M + absmax_parts(M) + 1) * 8);
+  double *base = work.as<double>(), *ker = base + M, *part = ker + M;
+  HIPCHK(ctx, hipMemcpyAsync(base, svel, M * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ker, ker_s, M * 8, hipMemcpyHostToDevice, ctx->stream));
+  const double *mx = launch_absmax(ctx->stream, M, ker, part);
+  double mxh = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&mxh, mx, 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  if (!(mxh > 0) || !std::isfinite(mxh))
+    return fail(ctx, SATT_E_ARG, "satt_plan_linesearch: ker_s is zero or not finite");
+  int warn = SATT_OK;
+  for (int sit = 1; sit <= max_sub_niter; ++sit) {
+    launch_ls_model(ctx->stream, M, base, ker, mx, *updatemax, Q.model.as<double>());
+    HIPCHK(ctx, hipGetLastError());
+    TRY(part_period_fields(ctx, Q));
+    TRY(satt_plan_run(P, 0, 0, SATT_LAUNCH_BATCHED));
+    double chi = 0;
+    satt_result R{};
+    R.chi = &chi;
+    const int st = satt_plan_results_part(P, 0, &R);
+    if (st < 0) return st;
+    if (st > 0) warn = st;
+    *nsub = sit;
+    *chi_out = chi;
+    if (chi > chi0)
+      *updatemax = *updatemax * maxshrink;  // :275-278
+    else
+      break;  // :279-282
+  }
+  // acqui%svel = acqui%ag%svel (:285-287): the last model tried
+  HIPCHK(ctx, hipMemcpyAsync(svel_out, Q.model.p, M * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return warn;
+}
+
 extern "C" int satt_plan_item_field(satt_plan *P, int item, int which, double *out) {
   if (!P) return SATT_E_ARG;
   satt_ctx *ctx = P->ctx;

@@ -241,6 +241,16 @@ __global__ void k_sd_update(size_t n, const double *ker, const double *mx, doubl
   svel[k] = svel[k] * (1 + updatemax * (ker[k] / *mx));
 }
 
+// out = base*(1 + update), update = updatemax*ker/maxval(abs(ker)) with the
+// product formed first, as prepare_fwd_linesearch writes it
+// (src/acqui_2d.f90:179-180); out may be base
+__global__ void k_ls_model(size_t n, const double *base, const double *ker, const double *mx,
+                           double updatemax, double *out) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  out[k] = base[k] * (1 + updatemax * ker[k] / *mx);
+}
+
 dim3 flat_grid(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
@@ -282,13 +292,22 @@ void launch_pm_to_nat(hipStream_t s, size_t N, int nf, const double *pm, double 
 
 int absmax_parts(size_t n) { return (int)std::min<size_t>(kMaxBlocks, (n + 255) / 256); }
 
+const double *launch_absmax(hipStream_t s, size_t n, const double *v, double *part) {
+  const int nb = absmax_parts(n);
+  hipLaunchKernelGGL(k_absmax_part, dim3((unsigned)nb), dim3(256), 0, s, v, n, part);
+  hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(256), 0, s, part, nb);
+  return part + nb;
+}
+
 void launch_sd_update(hipStream_t s, size_t n, const double *ker, double updatemax, double *svel,
                       double *part) {
-  const int nb = absmax_parts(n);
-  hipLaunchKernelGGL(k_absmax_part, dim3((unsigned)nb), dim3(256), 0, s, ker, n, part);
-  hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(256), 0, s, part, nb);
-  hipLaunchKernelGGL(k_sd_update, flat_grid(n), dim3(256), 0, s, n, ker, part + nb, updatemax,
-                     svel);
+  const double *mx = launch_absmax(s, n, ker, part);
+  hipLaunchKernelGGL(k_sd_update, flat_grid(n), dim3(256), 0, s, n, ker, mx, updatemax, svel);
+}
+
+void launch_ls_model(hipStream_t s, size_t n, const double *base, const double *ker,
+                     const double *mx, double updatemax, double *out) {
+  hipLaunchKernelGGL(k_ls_model, flat_grid(n), dim3(256), 0, s, n, base, ker, mx, updatemax, out);
 }
 
 }  // namespace satt

@@ -210,6 +210,11 @@ void launch_pm_to_nat(hipStream_t s, size_t N, int nf, const double *pm, double 
 int absmax_parts(size_t n);
 void launch_sd_update(hipStream_t s, size_t n, const double *ker, double updatemax, double *svel,
                       double *part);
+// max |v| into part[absmax_parts(n)]; returns that address
+const double *launch_absmax(hipStream_t s, size_t n, const double *v, double *part);
+// out = base*(1 + updatemax*ker/(*mx)), the line search's trial model
+void launch_ls_model(hipStream_t s, size_t n, const double *base, const double *ker,
+                     const double *mx, double updatemax, double *out);
 
 // ---- surface-wave dispersion per column (satt_disp.hip)
 struct DispJob {

@@ -188,6 +188,19 @@ module satt_cabi
       type(satt_result), intent(inout) :: result
     end function satt_plan_results
 
+    integer(c_int) function satt_plan_linesearch(plan, svel, ker_s, chi0, maxshrink, &
+                                                 max_sub_niter, updatemax, svel_out, chi_out, &
+                                                 nsub) bind(C, name="satt_plan_linesearch")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: svel(*), ker_s(*)
+      real(c_double), value :: chi0, maxshrink
+      integer(c_int), value :: max_sub_niter
+      real(c_double), intent(inout) :: updatemax
+      real(c_double), intent(out) :: svel_out(*), chi_out
+      integer(c_int), intent(out) :: nsub
+    end function satt_plan_linesearch
+
     subroutine satt_plan_destroy(plan) bind(C, name="satt_plan_destroy")
       import :: c_ptr
       type(c_ptr), value :: plan

@@ -35,6 +35,40 @@ subroutine satt_fhost_forward_simulate(dim, nx, ny, nperiod, xgrids, ygrids, per
   status = st
 end subroutine satt_fhost_forward_simulate
 
+subroutine satt_fhost_linesearch(dim, nx, ny, nperiod, xgrids, ygrids, periods, svel, m11, &
+                                 m12, m22, npath, evt_id, sta_id, evla, evlo, stla, stlo, &
+                                 periods_all, tt, weight, ker_s, chi0, maxshrink, &
+                                 max_sub_niter, updatemax, chi, nsub, status) &
+                                 bind(C, name="satt_fhost_linesearch")
+  use, intrinsic :: iso_c_binding
+  use satt_host, only: satt_linesearch_host
+  implicit none
+  integer(c_int), value :: dim, nx, ny, nperiod, npath, max_sub_niter
+  real(c_double), intent(in) :: xgrids(nx), ygrids(ny), periods(nperiod)
+  real(c_double), intent(inout) :: svel(nperiod,nx,ny)
+  real(c_double), intent(in) :: m11(nperiod,nx,ny), m12(nperiod,nx,ny), m22(nperiod,nx,ny)
+  integer(c_int), intent(in) :: evt_id(npath), sta_id(npath)
+  real(c_double), intent(in) :: evla(npath), evlo(npath), stla(npath), stlo(npath)
+  real(c_double), intent(in) :: periods_all(npath), tt(npath), weight(npath)
+  real(c_double), intent(in) :: ker_s(nperiod,nx,ny)
+  real(c_double), value :: chi0, maxshrink
+  real(c_double), intent(inout) :: updatemax
+  real(c_double), intent(out) :: chi
+  integer(c_int), intent(out) :: nsub, status
+  character(len=30), allocatable :: evtname(:), staname(:)
+  integer :: i, st, ns
+  allocate(evtname(npath), staname(npath))
+  do i = 1, npath
+    write(evtname(i), '(a,i6.6)') 'S', evt_id(i)
+    write(staname(i), '(a,i6.6)') 'S', sta_id(i)
+  end do
+  call satt_linesearch_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, evtname, &
+                            staname, evla, evlo, stla, stlo, periods_all, tt, weight, ker_s, &
+                            chi0, maxshrink, max_sub_niter, updatemax, chi, ns, st)
+  nsub = ns
+  status = st
+end subroutine satt_fhost_linesearch
+
 !=====================================================================
 ! C-callable wrappers of the model-side host routines (for harnesses).
 !=====================================================================

@@ -24,6 +24,10 @@
 !  - satt_steepest_descent_host()  att_tomo_2d%optimize + steepest_descent
 !                      (src/tomo2d.f90:203-251), the misfit-increase shrink
 !                      of the step length included
+!  - satt_linesearch_host()  att_tomo_2d%linesearch (src/tomo2d.f90:253-288)
+!                      with prepare_fwd_linesearch (src/acqui_2d.f90:173-183):
+!                      the trial models and the forward-only runs on the
+!                      device, over the same arrays as forward_simulate
 !  - satt_fwdsurf3d_host()  fwdsurf3d_decomp (src/surfker/surfker.f90:200-222)
 !                      over the whole grid: the velocity maps of
 !                      att_grid%fwdsurf (src/grid.f90:150-163)
@@ -35,9 +39,18 @@ module satt_host
   private
   public :: satt_host_ctx, satt_host_check, satt_host_release, satt_forward_simulate_host
   public :: satt_gaussian_smooth_geo_2, satt_gradient_2_geo, satt_get_topo_metric
-  public :: satt_steepest_descent_host, satt_fwdsurf3d_host
+  public :: satt_steepest_descent_host, satt_fwdsurf3d_host, satt_linesearch_host
 
   type(c_ptr), save :: ctx = c_null_ptr
+
+  ! The C-ABI batch of one forward_simulate call and the arrays it points to
+  ! (the work list and the receiver gathers built from the path columns).
+  type :: host_batch
+    integer(c_int), allocatable :: item_pidx(:), rec_off(:), idx_path(:)
+    real(c_double), allocatable :: item_srcx(:), item_srcy(:), recx(:), recy(:)
+    real(c_double), allocatable :: rtt(:), rweight(:)
+    type(satt_batch) :: b
+  end type host_batch
 
 contains
 
@@ -101,17 +114,97 @@ contains
     real(c_double), intent(out), target, contiguous :: adj_s(:,:,:), adj_density(:,:,:)
     integer, intent(out), optional :: status
 
+    type(host_batch), target :: hb
+    real(c_double), target :: chi_c(1)
+    type(satt_result) :: r
+    integer(c_int) :: st
+
+    call build_batch(dim, xgrids, ygrids, periods, svel, m11, m12, m22, evtname, staname, &
+                     evla, evlo, stla, stlo, periods_all, tt, weight, hb)
+    r%tt_fwd = c_loc(tt_fwd)
+    r%chi = c_loc(chi_c)
+    r%adj_s = c_loc(adj_s)
+    if (dim == 3) r%adj_density = c_loc(adj_density)
+    st = satt_forward_simulate(satt_host_ctx(), hb%b, merge(1_c_int, 0_c_int, istotable), &
+                               merge(1_c_int, 0_c_int, isadj), SATT_PREC_FP64, r)
+    if (present(status)) then
+      status = st
+    else
+      call satt_host_check(st, 'forward_simulate')
+    end if
+    chi = chi_c(1)
+  end subroutine satt_forward_simulate_host
+
+  !> att_tomo_2d%linesearch (src/tomo2d.f90:253-288) over forward_simulate's
+  !> arrays: chi0 is acqui%misfits(acqui%iter), ker_s the smoothed kernel of
+  !> optimize, updatemax the step length (shrunk in place, :275-278).  On
+  !> return svel is the last model tried (acqui%svel = acqui%ag%svel,
+  !> :285-287), chi its misfit and nsub the sub-iterations run.  The trial
+  !> models and the forward-only runs stay on the device.
+  subroutine satt_linesearch_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, &
+                                  evtname, staname, evla, evlo, stla, stlo, periods_all, &
+                                  tt, weight, ker_s, chi0, maxshrink, max_sub_niter, &
+                                  updatemax, chi, nsub, status)
+    integer, intent(in) :: dim
+    real(c_double), intent(in), target, contiguous :: xgrids(:), ygrids(:), periods(:)
+    real(c_double), intent(inout), target, contiguous :: svel(:,:,:)
+    real(c_double), intent(in), target, contiguous :: m11(:,:,:), m12(:,:,:), m22(:,:,:)
+    character(len=*), intent(in) :: evtname(:), staname(:)
+    real(c_double), intent(in) :: evla(:), evlo(:), stla(:), stlo(:), periods_all(:)
+    real(c_double), intent(in) :: tt(:), weight(:)
+    real(c_double), intent(in), contiguous :: ker_s(:,:,:)
+    real(c_double), intent(in) :: chi0, maxshrink
+    integer, intent(in) :: max_sub_niter
+    real(c_double), intent(inout) :: updatemax
+    real(c_double), intent(out) :: chi
+    integer, intent(out) :: nsub
+    integer, intent(out), optional :: status
+
+    type(host_batch), target :: hb
+    real(c_double), allocatable :: svel_out(:,:,:)
+    type(c_ptr) :: plan
+    integer(c_int) :: st, ns
+
+    call build_batch(dim, xgrids, ygrids, periods, svel, m11, m12, m22, evtname, staname, &
+                     evla, evlo, stla, stlo, periods_all, tt, weight, hb)
+    allocate(svel_out(size(svel,1), size(svel,2), size(svel,3)))
+    chi = chi0
+    nsub = 0
+    plan = c_null_ptr
+    st = satt_plan_create(satt_host_ctx(), hb%b, SATT_PREC_FP64, plan)
+    if (st >= 0) then
+      st = satt_plan_linesearch(plan, svel, ker_s, chi0, maxshrink, &
+                                int(max_sub_niter, c_int), updatemax, svel_out, chi, ns)
+      if (st >= 0) then
+        svel = svel_out
+        nsub = ns
+      end if
+    end if
+    if (c_associated(plan)) call satt_plan_destroy(plan)
+    if (present(status)) then
+      status = st
+    else
+      call satt_host_check(st, 'linesearch')
+    end if
+  end subroutine satt_linesearch_host
+
+  ! The isrcs work list and the receiver gathers of forward_simulate's path
+  ! columns, and the C-ABI batch pointing at them and at the model.
+  subroutine build_batch(dim, xgrids, ygrids, periods, svel, m11, m12, m22, evtname, staname, &
+                         evla, evlo, stla, stlo, periods_all, tt, weight, hb)
+    integer, intent(in) :: dim
+    real(c_double), intent(in), target, contiguous :: xgrids(:), ygrids(:), periods(:)
+    real(c_double), intent(in), target, contiguous :: svel(:,:,:), m11(:,:,:), m12(:,:,:)
+    real(c_double), intent(in), target, contiguous :: m22(:,:,:)
+    character(len=*), intent(in) :: evtname(:), staname(:)
+    real(c_double), intent(in) :: evla(:), evlo(:), stla(:), stlo(:), periods_all(:)
+    real(c_double), intent(in) :: tt(:), weight(:)
+    type(host_batch), intent(inout), target :: hb
+
     integer :: npath, nsta, i, j, k, np, nitems, n, p
     character(len=len(staname)), allocatable :: names(:)
     real(c_double), allocatable :: slat(:), slon(:)
     integer, allocatable :: evt_sta(:), rec_sta(:), item_pidx_f(:), item_sta(:), plist(:)
-    integer(c_int), allocatable, target :: item_pidx(:), rec_off(:), idx_path(:)
-    real(c_double), allocatable, target :: item_srcx(:), item_srcy(:), recx(:), recy(:)
-    real(c_double), allocatable, target :: rtt(:), rweight(:)
-    real(c_double), target :: chi_c(1)
-    type(satt_batch) :: b
-    type(satt_result) :: r
-    integer(c_int) :: st
     logical :: seen
 
     npath = size(evtname)
@@ -159,48 +252,40 @@ contains
       end do
     end do
     ! receiver gathers in path order (get_evt_gather, src/src_rec.f90:252-283)
-    allocate(item_pidx(nitems), item_srcx(nitems), item_srcy(nitems), rec_off(nitems + 1))
-    allocate(recx(npath), recy(npath), rtt(npath), rweight(npath), idx_path(npath))
-    rec_off(1) = 0
+    allocate(hb%item_pidx(nitems), hb%item_srcx(nitems), hb%item_srcy(nitems))
+    allocate(hb%rec_off(nitems + 1), hb%recx(npath), hb%recy(npath), hb%rtt(npath))
+    allocate(hb%rweight(npath), hb%idx_path(npath))
+    hb%rec_off(1) = 0
     n = 0
     do k = 1, nitems
       p = item_pidx_f(k)
-      item_pidx(k) = p - 1
-      item_srcx(k) = slon(item_sta(k))
-      item_srcy(k) = slat(item_sta(k))
+      hb%item_pidx(k) = p - 1
+      hb%item_srcx(k) = slon(item_sta(k))
+      hb%item_srcy(k) = slat(item_sta(k))
       do i = 1, npath
         if (evt_sta(i) == item_sta(k) .and. periods_all(i) == periods(p)) then
           n = n + 1
-          recx(n) = slon(rec_sta(i)); recy(n) = slat(rec_sta(i))
-          rtt(n) = tt(i); rweight(n) = weight(i); idx_path(n) = i - 1
+          hb%recx(n) = slon(rec_sta(i)); hb%recy(n) = slat(rec_sta(i))
+          hb%rtt(n) = tt(i); hb%rweight(n) = weight(i); hb%idx_path(n) = i - 1
         end if
       end do
-      rec_off(k + 1) = n
+      hb%rec_off(k + 1) = n
     end do
 
-    b%dim = dim
-    b%nx = size(xgrids); b%ny = size(ygrids); b%nperiod = np
-    b%xgrids = c_loc(xgrids); b%ygrids = c_loc(ygrids)
-    b%svel = c_loc(svel); b%m11 = c_loc(m11); b%m12 = c_loc(m12); b%m22 = c_loc(m22)
-    b%nitems = nitems
-    b%item_pidx = c_loc(item_pidx); b%item_srcx = c_loc(item_srcx); b%item_srcy = c_loc(item_srcy)
-    b%rec_off = c_loc(rec_off)
-    b%recx = c_loc(recx); b%recy = c_loc(recy); b%tt = c_loc(rtt); b%weight = c_loc(rweight)
-    b%idx_path = c_loc(idx_path)
-    b%npath = npath
-    r%tt_fwd = c_loc(tt_fwd)
-    r%chi = c_loc(chi_c)
-    r%adj_s = c_loc(adj_s)
-    if (dim == 3) r%adj_density = c_loc(adj_density)
-    st = satt_forward_simulate(satt_host_ctx(), b, merge(1_c_int, 0_c_int, istotable), &
-                               merge(1_c_int, 0_c_int, isadj), SATT_PREC_FP64, r)
-    if (present(status)) then
-      status = st
-    else
-      call satt_host_check(st, 'forward_simulate')
-    end if
-    chi = chi_c(1)
-  end subroutine satt_forward_simulate_host
+    hb%b%dim = dim
+    hb%b%nx = size(xgrids); hb%b%ny = size(ygrids); hb%b%nperiod = np
+    hb%b%xgrids = c_loc(xgrids); hb%b%ygrids = c_loc(ygrids)
+    hb%b%svel = c_loc(svel); hb%b%m11 = c_loc(m11); hb%b%m12 = c_loc(m12)
+    hb%b%m22 = c_loc(m22)
+    hb%b%nitems = nitems
+    hb%b%item_pidx = c_loc(hb%item_pidx)
+    hb%b%item_srcx = c_loc(hb%item_srcx); hb%b%item_srcy = c_loc(hb%item_srcy)
+    hb%b%rec_off = c_loc(hb%rec_off)
+    hb%b%recx = c_loc(hb%recx); hb%b%recy = c_loc(hb%recy)
+    hb%b%tt = c_loc(hb%rtt); hb%b%weight = c_loc(hb%rweight)
+    hb%b%idx_path = c_loc(hb%idx_path)
+    hb%b%npath = npath
+  end subroutine build_batch
 
   !> gaussian_smooth_geo_2(data, lons, lats, sigma) (src/shared/utils.f90:1194)
   function satt_gaussian_smooth_geo_2(data, lons, lats, sigma) result(smdata)

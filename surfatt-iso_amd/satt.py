@@ -396,6 +396,21 @@ class Plan:
         self.ctx._check(self.lib.satt_plan_set_model_part(self.ptr, int(part),
                                                           *(_d(a) for a in arrs)))
 
+    def linesearch(self, svel, ker_s, chi0, updatemax, maxshrink, max_sub_niter):
+        """att_tomo_2d%linesearch (src/tomo2d.f90:253-288) on this one-grid
+        plan: returns {"svel": the last model tried, "updatemax": the step
+        after the search, "chi": the last misfit, "nsub": sub-iterations run,
+        "status"}.  The plan keeps the returned model."""
+        sv, ker = _f(svel), _f(ker_s)
+        out = np.zeros(sv.shape, order="F")
+        upd, chi, nsub = ctypes.c_double(updatemax), ctypes.c_double(0), ctypes.c_int(0)
+        st = self.ctx._check(self.lib.satt_plan_linesearch(
+            self.ptr, _d(sv), _d(ker), ctypes.c_double(chi0), ctypes.c_double(maxshrink),
+            int(max_sub_niter), ctypes.byref(upd), _d(out), ctypes.byref(chi),
+            ctypes.byref(nsub)))
+        return {"svel": out, "updatemax": upd.value, "chi": chi.value, "nsub": nsub.value,
+                "status": st}
+
     def run(self, istotable=True, isadj=True, launch=LAUNCH_BATCHED):
         self.flags = (istotable, isadj)
         self.ctx._check(self.lib.satt_plan_run(self.ptr, int(istotable), int(isadj), int(launch)))

@@ -22,7 +22,7 @@ def test_fhost_exports_reference_names():
     lib = ctypes.CDLL(FHOST)
     for name in ("fsm_uw_ps_lonlat_2d_", "fsm_o1_jse_lonlat_2d_", "mask_",
                  "satt_fhost_forward_simulate", "satt_fhost_smooth", "satt_fhost_topo_metric",
-                 "satt_fhost_steepest_descent"):
+                 "satt_fhost_steepest_descent", "satt_fhost_linesearch"):
         assert hasattr(lib, name), name
 
 
@@ -143,3 +143,42 @@ def test_fortran_model_side_routines_match_c_abi(ctx):
         assert upd.value == step
         wsv, wker = ctx.steepest_descent(xg, yg, adj, 0.07, step, svel)
         assert np.array_equal(sv, wsv) and np.array_equal(ker, wker)
+
+
+@pytest.mark.gpu
+def test_fortran_linesearch_matches_c_abi(ctx):
+    """att_tomo_2d%linesearch through the Fortran host (it builds the batch
+    from the path columns and runs satt_plan_linesearch on a plan of its
+    own) equals the Python plan's line search bit for bit."""
+    lib = ctypes.CDLL(FHOST)
+    n, d = 33, 0.03
+    xg = 100.0 + d * np.arange(n); yg = 20.0 + d * np.arange(n)
+    rng = np.random.default_rng(23)
+    sx, sy = W.stations_uniform(rng, 4, xg, yg)
+    periods = np.array([8.0, 16.0])
+    paths = W.all_pairs_paths(sx, sy, periods)
+    cb = W.checkerboard_2d(n, n, 2, 2, 0.06)
+    ker = np.asfortranarray(np.stack([cb, 0.5 * cb]))
+    v0 = np.stack([(2.9 + 0.03 * p) * np.ones((n, n)) for p in periods])
+    one = np.ones((2, n, n)); zero = np.zeros((2, n, n))
+    bt = W.make_batch(2, xg, yg, periods, v0 * (1 + ker), one, zero, one, paths)
+    paths["tt"] = O.forward_simulate(bt, isadj=False)["tt_fwd"]
+    b = W.make_batch(2, xg, yg, periods, v0, one, zero, one, paths)
+    chi0 = O.forward_simulate(b, istotable=False, isadj=False)["chi"]
+    step = 4 * float(np.max(np.abs(cb)))
+    with ctx.plan(b) as p:
+        want = p.linesearch(b["svel"], ker, chi0, step, 0.25, 4)
+    ids = lambda names: np.array([int(s[1:]) for s in names], dtype=np.int32)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    sv = np.asfortranarray(b["svel"]).copy(order="F")
+    upd, chi = ctypes.c_double(step), ctypes.c_double(0)
+    nsub, st = ctypes.c_int(0), ctypes.c_int(0)
+    args = [c(paths[k]) for k in ("evla", "evlo", "stla", "stlo", "period", "tt", "weight")]
+    lib.satt_fhost_linesearch(
+        2, n, n, 2, _d(xg), _d(yg), _d(c(periods)), _d(sv), _d(b["m11"]), _d(b["m12"]),
+        _d(b["m22"]), len(paths["tt"]), O._i(ids(paths["evtname"])), O._i(ids(paths["staname"])),
+        *(_d(a) for a in args), _d(ker), ctypes.c_double(chi0), ctypes.c_double(0.25), 4,
+        ctypes.byref(upd), ctypes.byref(chi), ctypes.byref(nsub), ctypes.byref(st))
+    assert st.value >= 0
+    assert nsub.value == want["nsub"] == 2 and upd.value == want["updatemax"]
+    assert chi.value == want["chi"] and np.array_equal(sv, want["svel"])
