@@ -363,6 +363,7 @@ def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
                                      f"depths 0-60 km, Vs 3.0-4.5 km/s +-6% checkerboard, "
                                      f"{len(per3)} periods 4-40 s, spherical earth, fundamental mode",
                          **disp}
+    res["ker3d"] = ker3d_leg(ctx, cpu)
     if cpu:
         import oracle as O
         sample = [(sig[0], j) for j in (0, 200, 400)] + [(sig[npd // 2], 200), (sig[15], 200)]
@@ -384,6 +385,62 @@ def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
                                "est_full_topo_metric_s": round(nodes / rate, 1)}
         res["vs_cpu_core"] = round(nodes / (dev * 1e-3) / rate, 1)
     return res
+
+
+def ker3d_leg(ctx, cpu, n_inv=(8, 9, 8), nset=5, chi0=2.5, weight=1.0):
+    """The 3-D stage's kernel steps after the loop on its 41x41x31 model with
+    16 periods: combine_kernels through the Vs chain rule (src/acqui.f90:
+    224-238) and post_processing_for_kernel onto 5 inversion-grid sets of
+    8x9x8 nodes (the reference's example 00 settings), with synthetic
+    stacked kernels and depth sensitivities of the right shapes.  The CPU
+    leg runs the oracle's combination and the reference's own inv_grid_iso /
+    inv2fwd_iso (oracle/_ref) on one host thread."""
+    vs3, depz, per3 = W.acqui3d_model()
+    nx, ny, nz = vs3.shape
+    npd = len(per3)
+    vx, vy = np.linspace(100.0, 102.0, nx), np.linspace(30.0, 32.0, ny)
+    xi, yi, zi = W.inv_grids(vx, vy, depz, n_inv, nset)
+    rng = np.random.default_rng(6)
+    f = lambda *sh: np.asfortranarray(rng.standard_normal(sh))
+    adj, adj_d = f(npd, nx, ny), f(npd, nx, ny)
+    svs, svp, srho = f(npd, nx, ny, nz), f(npd, nx, ny, nz), f(npd, nx, ny, nz)
+    vs = np.asfortranarray(vs3)
+    vp = np.asfortranarray(0.9409 + 2.0947 * vs - 0.8206 * vs ** 2 + 0.2683 * vs ** 3 - 0.0251 * vs ** 4)
+    grad0 = np.zeros((1, nx, ny, nz), order="F")
+    out = {"workload": f"{nx}x{ny}x{nz} model, {npd} periods; {nset} inversion-grid sets of "
+                       f"{n_inv[0]}x{n_inv[1]}x{n_inv[2]} nodes; combine through the Vs chain "
+                       f"rule, no density preconditioning"}
+    for rep in range(2):
+        t0 = time.perf_counter()
+        ker, kd = ctx.combine_kernels(adj, adj_d, svs, svp, srho, vs, vp, False, True)
+        c_dev, c_wall = ctx.last_device_ms(), time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, grad = ctx.kernel_postproc(vx, vy, depz, xi, yi, zi, ker, kd, 0.0, chi0, weight, grad0)
+        p_dev, p_wall = ctx.last_device_ms(), time.perf_counter() - t0
+    out["combine"] = {"device_ms": round(c_dev, 4), "wall_ms": round(c_wall * 1e3, 3)}
+    out["postproc"] = {"device_ms": round(p_dev, 4), "wall_ms": round(p_wall * 1e3, 3)}
+    if cpu:
+        import oracle as O
+        t0 = time.perf_counter()
+        wk, wkd = O.combine_kernels(adj, adj_d, svs, svp, srho, vs, vp, False, True)
+        c_cpu = time.perf_counter() - t0
+        out["combine"]["bitwise_vs_oracle"] = bool(np.array_equal(wk, ker))
+        out["combine"]["cpu_baseline"] = {"value_ms": round(c_cpu * 1e3, 3), "cores": 1,
+                                          "kind": "port", "sample": "the whole step"}
+        if O.RefSolver.available():
+            ref = O.RefSolver()
+            t0 = time.perf_counter()
+            gk = ref.inv_grid_iso(xi, yi, zi, wk[0], vx, vy, depz)
+            up = ref.inv2fwd_iso(xi, yi, zi, gk, vx, vy, depz)
+            p_cpu = time.perf_counter() - t0
+            want = grad0[0] + up / nset / chi0 * weight
+            out["postproc"]["bitwise_vs_reference"] = bool(np.array_equal(grad[0], want))
+            out["postproc"]["cpu_baseline"] = {
+                "value_ms": round(p_cpu * 1e3, 3), "cores": 1, "kind": "reference",
+                "sample": "the whole step: the reference's inv_grid_iso + inv2fwd_iso (oracle/_ref)"}
+            out["postproc"]["vs_cpu_core"] = round(p_cpu * 1e3 / p_dev, 1)
+        out["combine"]["vs_cpu_core"] = round(c_cpu * 1e3 / c_dev, 1)
+    return out
 
 
 def host_threads():

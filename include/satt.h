@@ -26,6 +26,11 @@
  *   satt_fwdsurf3d         fwdsurf3d_decomp / fwdsurf1d with surfdisp96
  *                          (src/surfker/surfker.f90:200-254,
  *                          src/surfker/surfdisp96.f:57)
+ *   satt_combine_kernels   att_acqui%combine_kernels (src/acqui.f90:197-245)
+ *   satt_inv_grid_iso      inv_grid_iso           src/eikon_solver_2d.f90:1538
+ *   satt_inv2fwd_iso       inv2fwd_iso            src/eikon_solver_2d.f90:1613
+ *   satt_kernel_postproc   att_acqui%post_processing_for_kernel
+ *                          (src/acqui.f90:247-304)
  *
  * Conventions
  *   - A grid field f(nx,ny) is column-major with x fastest: f[i + nx*j], as
@@ -184,6 +189,51 @@ int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
                    const double *depz, int nperiod, const double *periods,
                    int iwave, int igr, double *svel, int *col_status);
 
+/* ---- 3-D kernel steps of the att_acqui path (after the stacking) ---- */
+
+/* combine_kernels (src/acqui.f90:197-245): the stacked period kernels
+ * adj_s, adj_density (nperiod,nx,ny) times the depth sensitivities sen_vs,
+ * sen_vp, sen_rho (nperiod,nx,ny,nz), summed over periods into ker and
+ * ker_density (nker,nx,ny,nz), which are accumulated into (the reference
+ * zeroes them first, prepare_inv :73-88).  use_alpha_beta_rho: nker = 3
+ * (Vs, Vp, density components; with rho_scaling the density component is
+ * 0.33 times the Vs one and sen_rho may be NULL); else nker = 1 through the
+ * chain rule of Brocher's relations on vs3d, vp3d (nx,ny,nz)
+ * (dalpha_dbeta, drho_dalpha, src/shared/utils.f90:99-145). */
+int satt_combine_kernels(satt_ctx *ctx, int nperiod, int nx, int ny, int nz,
+                         const double *adj_s, const double *adj_density,
+                         const double *sen_vs, const double *sen_vp, const double *sen_rho,
+                         const double *vs3d, const double *vp3d, int use_alpha_beta_rho,
+                         int rho_scaling, double *ker, double *ker_density);
+
+/* inv_grid_iso (src/eikon_solver_2d.f90:1538-1610), the reference's
+ * arguments: gk(nset*nxinv*nyinv*nzinv) = the interior points of adj
+ * (nx,ny,nz) spread trilinearly onto the inversion grids xinv(nxinv,nset),
+ * yinv, zinv, times dx*dy*dz.  A point outside a set's grid is SATT_E_ARG
+ * (the reference stops). */
+int satt_inv_grid_iso(satt_ctx *ctx, const double *xinv, const double *yinv,
+                      const double *zinv, int nxinv, int nyinv, int nzinv, int nset,
+                      const double *adj, int nx, int ny, int nz, double *gk,
+                      const double *vel_x, const double *vel_y, const double *vel_z);
+/* inv2fwd_iso (src/eikon_solver_2d.f90:1613-1691): update(nx,ny,nz) = the
+ * trilinear interpolation of gk at every model point, summed over sets. */
+int satt_inv2fwd_iso(satt_ctx *ctx, const double *xinv, const double *yinv,
+                     const double *zinv, int nxinv, int nyinv, int nzinv, int nset,
+                     int nx, int ny, int nz, const double *gk, const double *vel_x,
+                     const double *vel_y, const double *vel_z, double *update);
+/* post_processing_for_kernel (src/acqui.f90:247-285) for every component
+ * iker of ker (nker,nx,ny,nz): with kdensity_coe > 0 the component is first
+ * preconditioned in place by regularize_ker_density (:287-304) from
+ * ker_density (nker,nx,ny,nz; may be NULL otherwise); then projected to the
+ * inversion grids and back, and gradient_s(iker,:,:,:) +=
+ * update/nset/chi0*weight.  The projections stay on the device. */
+int satt_kernel_postproc(satt_ctx *ctx, int nker, int nx, int ny, int nz,
+                         const double *vel_x, const double *vel_y, const double *vel_z,
+                         int nset, int nxinv, int nyinv, int nzinv, const double *xinv,
+                         const double *yinv, const double *zinv, double *ker,
+                         const double *ker_density, double kdensity_coe, double chi0,
+                         double weight, double *gradient_s);
+
 /* Achievable HBM bandwidth: the best of five runs of a streaming read and of a
  * streaming copy over nbytes (1 MiB to 64 GiB; allocates 2*nbytes), in GB/s
  * (the copy counts bytes read and written).  A measurement aid for the
@@ -191,8 +241,10 @@ int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
 int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_gbs);
 
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
- * satt_gradient_geo2, satt_topo_metric or satt_steepest_descent call on the
- * context (HIP events on its stream; transfers excluded). */
+ * satt_gradient_geo2, satt_topo_metric, satt_steepest_descent,
+ * satt_combine_kernels, satt_inv_grid_iso, satt_inv2fwd_iso or
+ * satt_kernel_postproc call on the context (HIP events on its stream;
+ * transfers excluded). */
 double satt_last_device_ms(const satt_ctx *ctx);
 
 /* ------------------------------------------------------------ batched form */

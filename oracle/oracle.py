@@ -56,6 +56,10 @@ def lib():
         _lib.orc_gps2dist.argtypes = [ctypes.c_double] * 4
         _lib.orc_topo_sigma.restype = ctypes.c_double
         _lib.orc_topo_sigma.argtypes = [ctypes.c_double] * 3
+        for name in ("orc_drho_dalpha", "orc_dalpha_dbeta"):
+            getattr(_lib, name).restype = ctypes.c_double
+            getattr(_lib, name).argtypes = [ctypes.c_double]
+        _lib.orc_combine_kernels.restype = None
     return _lib
 
 
@@ -314,6 +318,34 @@ class RefSolver:
                        ctypes.byref(ctypes.c_double(x0)), ctypes.byref(ctypes.c_double(y0)))
         return t
 
+    def inv_grid_iso(self, xinv, yinv, zinv, adj, vx, vy, vz):
+        """inv_grid_iso (src/eikon_solver_2d.f90:1538): gk(nset*nxinv*nyinv*nzinv).
+        The reference stops the process for a point outside the inversion
+        grid: callers pass grids that cover the model."""
+        xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)
+        (nxi, ns), nyi, nzi = xinv.shape, yinv.shape[0], zinv.shape[0]
+        adj = _f(adj)
+        nx, ny, nz = adj.shape
+        vx, vy, vz = (np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz))
+        gk = np.zeros(ns * nxi * nyi * nzi)
+        i = lambda v: ctypes.byref(ctypes.c_int(int(v)))
+        self.lib.inv_grid_iso_(_d(xinv), _d(yinv), _d(zinv), i(nxi), i(nyi), i(nzi), i(ns),
+                               _d(adj), i(nx), i(ny), i(nz), _d(gk), _d(vx), _d(vy), _d(vz))
+        return gk
+
+    def inv2fwd_iso(self, xinv, yinv, zinv, gk, vx, vy, vz):
+        """inv2fwd_iso (src/eikon_solver_2d.f90:1613): update(nx,ny,nz)."""
+        xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)
+        (nxi, ns), nyi, nzi = xinv.shape, yinv.shape[0], zinv.shape[0]
+        vx, vy, vz = (np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz))
+        gk = np.ascontiguousarray(gk, dtype=np.float64)
+        up = np.zeros((len(vx), len(vy), len(vz)), order="F")
+        i = lambda v: ctypes.byref(ctypes.c_int(int(v)))
+        self.lib.inv2fwd_iso_(_d(xinv), _d(yinv), _d(zinv), i(nxi), i(nyi), i(nzi), i(ns),
+                              i(len(vx)), i(len(vy)), i(len(vz)), _d(gk), _d(vx), _d(vy),
+                              _d(vz), _d(up))
+        return up
+
 
 # ---- model-side steps (satt_oracle_geo.c)
 
@@ -380,6 +412,78 @@ def topo_metric(lons, lats, z, sigma):
     if st != 0:
         raise OracleError(st, "orc_topo_metric")
     return tuple(outs)
+
+
+# ---- 3-D kernel steps (satt_oracle_ker3d.c)
+
+def drho_dalpha(vp):
+    return lib().orc_drho_dalpha(float(vp))
+
+
+def dalpha_dbeta(vs):
+    return lib().orc_dalpha_dbeta(float(vs))
+
+
+def combine_kernels(adj_s, adj_den, sen_vs, sen_vp, sen_rho, vs3d, vp3d, use_alpha_beta_rho,
+                    rho_scaling, ker=None, ker_den=None):
+    """combine_kernels (src/acqui.f90:197-245): (ker, ker_den), each
+    (nker, nx, ny, nz) with nker 3 (use_alpha_beta_rho) or 1, accumulated
+    into ker/ker_den when given, else into zeros."""
+    adj_s, adj_den = _f(adj_s), _f(adj_den)
+    sen_vs, sen_vp, sen_rho, vs3d, vp3d = (_f(v) for v in (sen_vs, sen_vp, sen_rho, vs3d, vp3d))
+    npd, nx, ny, nz = sen_vs.shape
+    nker = 3 if use_alpha_beta_rho else 1
+    ker = np.zeros((nker, nx, ny, nz), order="F") if ker is None else _f(ker).copy(order="F")
+    ker_den = (np.zeros((nker, nx, ny, nz), order="F") if ker_den is None
+               else _f(ker_den).copy(order="F"))
+    lib().orc_combine_kernels(npd, nx, ny, nz, _d(adj_s), _d(adj_den), _d(sen_vs), _d(sen_vp),
+                              _d(sen_rho), _d(vs3d), _d(vp3d), int(bool(use_alpha_beta_rho)),
+                              int(bool(rho_scaling)), _d(ker), _d(ker_den))
+    return ker, ker_den
+
+
+def inv_grid_iso(xinv, yinv, zinv, adj, vx, vy, vz):
+    xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)
+    (nxi, ns), nyi, nzi = xinv.shape, yinv.shape[0], zinv.shape[0]
+    adj = _f(adj)
+    nx, ny, nz = adj.shape
+    vx, vy, vz = (np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz))
+    gk = np.zeros(ns * nxi * nyi * nzi)
+    if lib().orc_inv_grid_iso(_d(xinv), _d(yinv), _d(zinv), nxi, nyi, nzi, ns, _d(adj), nx, ny,
+                              nz, _d(gk), _d(vx), _d(vy), _d(vz)) != 0:
+        raise OracleError(-1, "orc_inv_grid_iso: a point is outside the inversion grid")
+    return gk
+
+
+def inv2fwd_iso(xinv, yinv, zinv, gk, vx, vy, vz):
+    xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)
+    (nxi, ns), nyi, nzi = xinv.shape, yinv.shape[0], zinv.shape[0]
+    vx, vy, vz = (np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz))
+    gk = np.ascontiguousarray(gk, dtype=np.float64)
+    up = np.zeros((len(vx), len(vy), len(vz)), order="F")
+    if lib().orc_inv2fwd_iso(_d(xinv), _d(yinv), _d(zinv), nxi, nyi, nzi, ns, len(vx), len(vy),
+                             len(vz), _d(gk), _d(vx), _d(vy), _d(vz), _d(up)) != 0:
+        raise OracleError(-1, "orc_inv2fwd_iso: a point is outside the inversion grid")
+    return up
+
+
+def kernel_postproc(vx, vy, vz, xinv, yinv, zinv, ker, ker_den, kdensity_coe, chi0, weight,
+                    gradient_s):
+    """post_processing_for_kernel (src/acqui.f90:247-285): returns (ker
+    after the density preconditioning, gradient_s after the update)."""
+    xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)
+    (nxi, ns), nyi, nzi = xinv.shape, yinv.shape[0], zinv.shape[0]
+    ker, g = _f(ker).copy(order="F"), _f(gradient_s).copy(order="F")
+    ker_den = _f(ker_den)
+    nker, nx, ny, nz = ker.shape
+    vx, vy, vz = (np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz))
+    work = np.zeros(ns * nxi * nyi * nzi + 2 * nx * ny * nz)
+    if lib().orc_kernel_postproc(nker, nx, ny, nz, _d(vx), _d(vy), _d(vz), ns, nxi, nyi, nzi,
+                                 _d(xinv), _d(yinv), _d(zinv), _d(ker), _d(ker_den),
+                                 ctypes.c_double(kdensity_coe), ctypes.c_double(chi0),
+                                 ctypes.c_double(weight), _d(g), _d(work)) != 0:
+        raise OracleError(-1, "orc_kernel_postproc: a point is outside the inversion grid")
+    return ker, g
 
 
 def linesearch(batch, ker_s, chi0, updatemax, maxshrink, max_sub_niter):

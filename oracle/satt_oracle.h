@@ -155,6 +155,35 @@ int orc_steepest_descent(const double *lons, const double *lats, int nx, int ny,
                          const double *adj_s, double sigma_2d, double updatemax, double *svel,
                          double *ker_s);
 
+/* ---- 3-D kernel steps (satt_oracle_ker3d.c) */
+/* src/shared/utils.f90:99-145 */
+double orc_drho_dalpha(double vp);
+double orc_dalpha_dbeta(double vs);
+/* combine_kernels (src/acqui.f90:197-245), accumulating into ker/ker_den
+ * (nker = 3 with use_alpha_beta_rho, else 1) */
+void orc_combine_kernels(int nperiod, int nx, int ny, int nz, const double *adj_s,
+                         const double *adj_den, const double *sen_vs, const double *sen_vp,
+                         const double *sen_rho, const double *vs3d, const double *vp3d,
+                         int use_alpha_beta_rho, int rho_scaling, double *ker,
+                         double *ker_den);
+/* src/eikon_solver_2d.f90:1470-1691; -1 when a point is outside the grid */
+int orc_locate_within_cube(double xx, double yy, double zz, int nx, int ny, int nz,
+                           const double *xv, const double *yv, const double *zv, int *i1,
+                           int *j1, int *k1, double *wt);
+int orc_inv_grid_iso(const double *xinv, const double *yinv, const double *zinv, int nxinv,
+                     int nyinv, int nzinv, int nset, const double *adj, int nx, int ny, int nz,
+                     double *gk, const double *vx, const double *vy, const double *vz);
+int orc_inv2fwd_iso(const double *xinv, const double *yinv, const double *zinv, int nxinv,
+                    int nyinv, int nzinv, int nset, int nx, int ny, int nz, const double *gk,
+                    const double *vx, const double *vy, const double *vz, double *update);
+/* post_processing_for_kernel + regularize_ker_density (src/acqui.f90:247-304);
+ * work: nset*nxinv*nyinv*nzinv + 2*nx*ny*nz doubles */
+int orc_kernel_postproc(int nker, int nx, int ny, int nz, const double *vx, const double *vy,
+                        const double *vz, int nset, int nxinv, int nyinv, int nzinv,
+                        const double *xinv, const double *yinv, const double *zinv, double *ker,
+                        const double *ker_den, double kdensity_coe, double chi0, double weight,
+                        double *gradient_s, double *work);
+
 #ifdef __cplusplus
 }
 #endif

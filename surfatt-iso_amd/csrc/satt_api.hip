@@ -643,6 +643,222 @@ extern "C" int satt_steepest_descent(satt_ctx *ctx, const double *lons, const do
   return geo_timer_read(ctx, t);
 }
 
+namespace {
+
+bool increasing(const double *v, int n) {
+  for (int k = 0; k < n; ++k)
+    if (!std::isfinite(v[k]) || (k > 0 && !(v[k] > v[k - 1]))) return false;
+  return true;
+}
+
+// Arguments of inv_grid_iso / inv2fwd_iso: increasing axes, and every model
+// point the reference's loops visit (the interior ones for inv_grid_iso)
+// inside every set's grid, where the reference would stop with "I cannot
+// locate the receiver in the domain" (src/eikon_solver_2d.f90:1604-1607).
+int inv_check(satt_ctx *ctx, const char *who, const double *xinv, const double *yinv,
+              const double *zinv, int nxi, int nyi, int nzi, int nset, int nx, int ny, int nz,
+              const double *vx, const double *vy, const double *vz, bool interior) {
+  if (!xinv || !yinv || !zinv || !vx || !vy || !vz || nxi < 2 || nyi < 2 || nzi < 2 ||
+      nxi > 1024 || nyi > 1024 || nzi > 1024 ||
+      nset < 1 || nx < 2 || ny < 2 || nz < 2 || (size_t)nx * ny * nz > (size_t)INT_MAX ||
+      (size_t)nxi * nyi * nzi * nset > (size_t)INT_MAX)
+    return fail(ctx, SATT_E_ARG, "%s: bad arguments", who);
+  if (!increasing(vx, nx) || !increasing(vy, ny) || !increasing(vz, nz))
+    return fail(ctx, SATT_E_ARG, "%s: model axes must increase", who);
+  const int lo = interior ? 1 : 0;
+  const bool visits = !interior || (nx > 2 && ny > 2 && nz > 2);
+  const double *v[3] = {vx, vy, vz}, *inv[3] = {xinv, yinv, zinv};
+  const int n[3] = {nx, ny, nz}, ni[3] = {nxi, nyi, nzi};
+  for (int s = 0; s < nset; ++s)
+    for (int d = 0; d < 3; ++d) {
+      const double *X = inv[d] + (size_t)ni[d] * s;
+      if (!increasing(X, ni[d]))
+        return fail(ctx, SATT_E_ARG, "%s: inversion grid set %d: axis %d must increase", who,
+                    s + 1, d + 1);
+      if (visits && !(v[d][lo] >= X[0] && v[d][n[d] - 1 - lo] <= X[ni[d] - 1]))
+        return fail(ctx, SATT_E_ARG,
+                    "%s: a model point lies outside inversion grid set %d (axis %d)", who,
+                    s + 1, d + 1);
+    }
+  return SATT_OK;
+}
+
+// The model axes and the inversion grids on the device.
+int inv_upload(satt_ctx *ctx, DBuf &buf, InvGrid &G, const double *xinv, const double *yinv,
+               const double *zinv, int nxi, int nyi, int nzi, int nset, int nx, int ny, int nz,
+               const double *vx, const double *vy, const double *vz) {
+  const size_t na = (size_t)nx + ny + nz, ni = (size_t)(nxi + nyi + nzi) * nset;
+  std::vector<double> h(na + ni);
+  double *q = h.data();
+  std::memcpy(q, vx, nx * 8);
+  std::memcpy(q + nx, vy, ny * 8);
+  std::memcpy(q + nx + ny, vz, nz * 8);
+  std::memcpy(q + na, xinv, (size_t)nxi * nset * 8);
+  std::memcpy(q + na + (size_t)nxi * nset, yinv, (size_t)nyi * nset * 8);
+  std::memcpy(q + na + (size_t)(nxi + nyi) * nset, zinv, (size_t)nzi * nset * 8);
+  TRY(upload(ctx, buf, h.data(), h.size() * 8));
+  const double *d = buf.as<double>();
+  G = InvGrid{nx, ny, nz, nxi, nyi, nzi, nset, d, d + nx, d + nx + ny, d + na,
+              d + na + (size_t)nxi * nset, d + na + (size_t)(nxi + nyi) * nset};
+  return SATT_OK;
+}
+
+}  // namespace
+
+extern "C" int satt_combine_kernels(satt_ctx *ctx, int nperiod, int nx, int ny, int nz,
+                                    const double *adj_s, const double *adj_density,
+                                    const double *sen_vs, const double *sen_vp,
+                                    const double *sen_rho, const double *vs3d,
+                                    const double *vp3d, int use_alpha_beta_rho, int rho_scaling,
+                                    double *ker, double *ker_density) {
+  if (!ctx) return SATT_E_ARG;
+  const bool abr = use_alpha_beta_rho != 0, rs = rho_scaling != 0;
+  if (nperiod < 1 || nx < 1 || ny < 1 || nz < 1 || !adj_s || !adj_density || !sen_vs ||
+      !sen_vp || (!sen_rho && !(abr && rs)) || (!abr && (!vs3d || !vp3d)) || !ker ||
+      !ker_density || (size_t)nperiod * nx * ny * nz > (size_t)INT_MAX)
+    return fail(ctx, SATT_E_ARG, "satt_combine_kernels: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t n2 = (size_t)nx * ny, n3 = n2 * nz, M2 = (size_t)nperiod * n2;
+  const size_t M3 = (size_t)nperiod * n3, nk = (abr ? 3 : 1) * n3;
+  DBuf b2, bs, bm, bk;
+  ALLOC(ctx, b2, 2 * M2 * 8);
+  ALLOC(ctx, bs, 3 * M3 * 8);
+  ALLOC(ctx, bk, 2 * nk * 8);
+  double *a = b2.as<double>(), *sv = bs.as<double>(), *kk = bk.as<double>();
+  auto h2d = [&](double *d, const double *h, size_t n) {
+    return hipMemcpyAsync(d, h, n * 8, hipMemcpyHostToDevice, ctx->stream);
+  };
+  HIPCHK(ctx, h2d(a, adj_s, M2));
+  HIPCHK(ctx, h2d(a + M2, adj_density, M2));
+  HIPCHK(ctx, h2d(sv, sen_vs, M3));
+  HIPCHK(ctx, h2d(sv + M3, sen_vp, M3));
+  if (sen_rho) HIPCHK(ctx, h2d(sv + 2 * M3, sen_rho, M3));
+  HIPCHK(ctx, h2d(kk, ker, nk));
+  HIPCHK(ctx, h2d(kk + nk, ker_density, nk));
+  Combine C{nperiod, abr, rs, n2, n3, a, a + M2, sv, sv + M3, sen_rho ? sv + 2 * M3 : nullptr,
+            nullptr, nullptr, kk, kk + nk};
+  if (!abr) {
+    ALLOC(ctx, bm, 2 * n3 * 8);
+    HIPCHK(ctx, h2d(bm.as<double>(), vs3d, n3));
+    HIPCHK(ctx, h2d(bm.as<double>() + n3, vp3d, n3));
+    C.vs3d = bm.as<double>();
+    C.vp3d = bm.as<double>() + n3;
+  }
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_combine(ctx->stream, C);
+  HIPCHK(ctx, hipGetLastError());
+  TRY(geo_timer_stop(ctx, t));
+  HIPCHK(ctx, hipMemcpyAsync(ker, kk, nk * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ker_density, kk + nk, nk * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
+extern "C" int satt_inv_grid_iso(satt_ctx *ctx, const double *xinv, const double *yinv,
+                                 const double *zinv, int nxinv, int nyinv, int nzinv, int nset,
+                                 const double *adj, int nx, int ny, int nz, double *gk,
+                                 const double *vel_x, const double *vel_y,
+                                 const double *vel_z) {
+  if (!ctx) return SATT_E_ARG;
+  TRY(inv_check(ctx, "satt_inv_grid_iso", xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx, ny,
+                nz, vel_x, vel_y, vel_z, true));
+  if (!adj || !gk) return fail(ctx, SATT_E_ARG, "satt_inv_grid_iso: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DBuf axes, din, dout;
+  InvGrid G;
+  TRY(inv_upload(ctx, axes, G, xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx, ny, nz, vel_x,
+                 vel_y, vel_z));
+  const size_t N = (size_t)nx * ny * nz, nb = (size_t)nxinv * nyinv * nzinv * nset;
+  TRY(upload(ctx, din, adj, N * 8));
+  ALLOC(ctx, dout, nb * 8 + inv_grid_scratch_bytes(G));
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_inv_grid(ctx->stream, G, din.as<double>(), dout.as<double>(),
+                  dout.as<char>() + nb * 8);
+  HIPCHK(ctx, hipGetLastError());
+  TRY(geo_timer_stop(ctx, t));
+  HIPCHK(ctx, hipMemcpyAsync(gk, dout.p, nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
+extern "C" int satt_inv2fwd_iso(satt_ctx *ctx, const double *xinv, const double *yinv,
+                                const double *zinv, int nxinv, int nyinv, int nzinv, int nset,
+                                int nx, int ny, int nz, const double *gk, const double *vel_x,
+                                const double *vel_y, const double *vel_z, double *update) {
+  if (!ctx) return SATT_E_ARG;
+  TRY(inv_check(ctx, "satt_inv2fwd_iso", xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx, ny,
+                nz, vel_x, vel_y, vel_z, false));
+  if (!gk || !update) return fail(ctx, SATT_E_ARG, "satt_inv2fwd_iso: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DBuf axes, din, dout;
+  InvGrid G;
+  TRY(inv_upload(ctx, axes, G, xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx, ny, nz, vel_x,
+                 vel_y, vel_z));
+  const size_t N = (size_t)nx * ny * nz, nb = (size_t)nxinv * nyinv * nzinv * nset;
+  TRY(upload(ctx, din, gk, nb * 8));
+  ALLOC(ctx, dout, N * 8);
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_inv2fwd(ctx->stream, G, din.as<double>(), dout.as<double>());
+  HIPCHK(ctx, hipGetLastError());
+  TRY(geo_timer_stop(ctx, t));
+  HIPCHK(ctx, hipMemcpyAsync(update, dout.p, N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
+extern "C" int satt_kernel_postproc(satt_ctx *ctx, int nker, int nx, int ny, int nz,
+                                    const double *vel_x, const double *vel_y,
+                                    const double *vel_z, int nset, int nxinv, int nyinv,
+                                    int nzinv, const double *xinv, const double *yinv,
+                                    const double *zinv, double *ker, const double *ker_density,
+                                    double kdensity_coe, double chi0, double weight,
+                                    double *gradient_s) {
+  if (!ctx) return SATT_E_ARG;
+  // every model point inside every set (inv2fwd_iso's condition, which
+  // covers inv_grid_iso's interior one)
+  TRY(inv_check(ctx, "satt_kernel_postproc", xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx,
+                ny, nz, vel_x, vel_y, vel_z, false));
+  const bool pre = kdensity_coe > 0;
+  if (nker < 1 || !ker || !gradient_s || (pre && !ker_density) || !std::isfinite(chi0) ||
+      !std::isfinite(weight) || !std::isfinite(kdensity_coe) ||
+      (size_t)nker * nx * ny * nz > (size_t)INT_MAX)
+    return fail(ctx, SATT_E_ARG, "satt_kernel_postproc: bad arguments");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DBuf axes, dk, dkd, dg, work;
+  InvGrid G;
+  TRY(inv_upload(ctx, axes, G, xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx, ny, nz, vel_x,
+                 vel_y, vel_z));
+  const size_t N = (size_t)nx * ny * nz, nb = (size_t)nxinv * nyinv * nzinv * nset;
+  TRY(upload(ctx, dk, ker, nker * N * 8));
+  if (pre) TRY(upload(ctx, dkd, ker_density, nker * N * 8));
+  TRY(upload(ctx, dg, gradient_s, nker * N * 8));
+  ALLOC(ctx, work, (3 * N + nb + absmax_parts(N) + 1) * 8 + inv_grid_scratch_bytes(G));
+  double *kc = work.as<double>(), *kdc = kc + N, *up = kdc + N, *gk = up + N, *part = gk + nb;
+  void *scratch = part + absmax_parts(N) + 1;
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  for (int q = 0; q < nker; ++q) {
+    launch_component(ctx->stream, N, nker, q, dk.as<double>(), pre ? dkd.as<double>() : nullptr,
+                     kc, pre ? kdc : nullptr);
+    if (pre) {
+      const double *mx = launch_absmax(ctx->stream, N, kdc, part);
+      launch_precond(ctx->stream, N, nker, q, kdc, mx, kdensity_coe, dk.as<double>(), kc);
+    }
+    launch_inv_grid(ctx->stream, G, kc, gk, scratch);
+    launch_inv2fwd(ctx->stream, G, gk, up);
+    launch_grad_update(ctx->stream, N, nker, q, up, nset, chi0, weight, dg.as<double>());
+  }
+  HIPCHK(ctx, hipGetLastError());
+  TRY(geo_timer_stop(ctx, t));
+  if (pre) HIPCHK(ctx, hipMemcpyAsync(ker, dk.p, nker * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(gradient_s, dg.p, nker * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
 extern "C" int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
                               const double *depz, int nperiod, const double *periods, int iwave,
                               int igr, double *svel, int *col_status) {

@@ -216,6 +216,33 @@ const double *launch_absmax(hipStream_t s, size_t n, const double *v, double *pa
 void launch_ls_model(hipStream_t s, size_t n, const double *base, const double *ker,
                      const double *mx, double updatemax, double *out);
 
+// ---- 3-D kernel combination and inversion-grid projection (satt_ker3d.hip)
+struct Combine {
+  int nperiod, abr, rho_scaling;
+  size_t n2, n3;  // nx*ny, nx*ny*nz
+  const double *adj_s, *adj_den;              // (nperiod, nx, ny)
+  const double *sen_vs, *sen_vp, *sen_rho;    // (nperiod, nx, ny, nz)
+  const double *vs3d, *vp3d;                  // (nx, ny, nz)
+  double *ker, *ker_den;                      // (3 or 1, nx, ny, nz)
+};
+struct InvGrid {
+  int nx, ny, nz, nxi, nyi, nzi, nset;
+  const double *vx, *vy, *vz;           // model axes
+  const double *xinv, *yinv, *zinv;     // (n?i, nset)
+};
+void launch_combine(hipStream_t s, const Combine &C);
+// scratch: inv_grid_scratch_bytes(G) bytes (cube and terms of every point)
+size_t inv_grid_scratch_bytes(const InvGrid &G);
+void launch_inv_grid(hipStream_t s, const InvGrid &G, const double *adj, double *gk,
+                     void *scratch);
+void launch_inv2fwd(hipStream_t s, const InvGrid &G, const double *gk, double *update);
+void launch_component(hipStream_t s, size_t n, int nker, int q, const double *ker,
+                      const double *kd, double *kc, double *kdc);
+void launch_precond(hipStream_t s, size_t n, int nker, int q, const double *kdc,
+                    const double *mx, double coe, double *ker, double *kc);
+void launch_grad_update(hipStream_t s, size_t n, int nker, int q, const double *update,
+                        int nset, double chi0, double weight, double *grad);
+
 // ---- surface-wave dispersion per column (satt_disp.hip)
 struct DispJob {
   int ncol, nz, kmax, iwave, igr;

@@ -132,6 +132,53 @@ module satt_cabi
       integer(c_int), intent(out) :: col_status(*)
     end function satt_fwdsurf3d
 
+    integer(c_int) function satt_combine_kernels(ctx, nperiod, nx, ny, nz, adj_s, adj_density, &
+                                                 sen_vs, sen_vp, sen_rho, vs3d, vp3d, &
+                                                 use_alpha_beta_rho, rho_scaling, ker, &
+                                                 ker_density) bind(C, name="satt_combine_kernels")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nperiod, nx, ny, nz, use_alpha_beta_rho, rho_scaling
+      real(c_double), intent(in) :: adj_s(*), adj_density(*), sen_vs(*), sen_vp(*), sen_rho(*)
+      real(c_double), intent(in) :: vs3d(*), vp3d(*)
+      real(c_double), intent(inout) :: ker(*), ker_density(*)
+    end function satt_combine_kernels
+
+    integer(c_int) function satt_inv_grid_iso(ctx, xinv, yinv, zinv, nxinv, nyinv, nzinv, &
+                                              nset, adj, nx, ny, nz, gk, vel_x, vel_y, vel_z) &
+                                              bind(C, name="satt_inv_grid_iso")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nxinv, nyinv, nzinv, nset, nx, ny, nz
+      real(c_double), intent(in) :: xinv(*), yinv(*), zinv(*), adj(*), vel_x(*), vel_y(*)
+      real(c_double), intent(in) :: vel_z(*)
+      real(c_double), intent(out) :: gk(*)
+    end function satt_inv_grid_iso
+
+    integer(c_int) function satt_inv2fwd_iso(ctx, xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, &
+                                             nx, ny, nz, gk, vel_x, vel_y, vel_z, update) &
+                                             bind(C, name="satt_inv2fwd_iso")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nxinv, nyinv, nzinv, nset, nx, ny, nz
+      real(c_double), intent(in) :: xinv(*), yinv(*), zinv(*), gk(*), vel_x(*), vel_y(*)
+      real(c_double), intent(in) :: vel_z(*)
+      real(c_double), intent(out) :: update(*)
+    end function satt_inv2fwd_iso
+
+    integer(c_int) function satt_kernel_postproc(ctx, nker, nx, ny, nz, vel_x, vel_y, vel_z, &
+                                                 nset, nxinv, nyinv, nzinv, xinv, yinv, zinv, &
+                                                 ker, ker_density, kdensity_coe, chi0, weight, &
+                                                 gradient_s) bind(C, name="satt_kernel_postproc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nker, nx, ny, nz, nset, nxinv, nyinv, nzinv
+      real(c_double), intent(in) :: vel_x(*), vel_y(*), vel_z(*), xinv(*), yinv(*), zinv(*)
+      real(c_double), intent(inout) :: ker(*), gradient_s(*)
+      real(c_double), intent(in) :: ker_density(*)
+      real(c_double), value :: kdensity_coe, chi0, weight
+    end function satt_kernel_postproc
+
     real(c_double) function satt_topo_sigma(vel, period, wavelen_factor) &
                                             bind(C, name="satt_topo_sigma")
       import :: c_double

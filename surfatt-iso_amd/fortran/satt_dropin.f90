@@ -3,7 +3,8 @@
 ! subroutine names and argument lists as src/eikon_solver_2d.f90, so a
 ! reference build that links this object (and libsatt.so) instead of
 ! eikon_solver_2d.f90 runs its solves on the GPU with no source change at
-! the call sites (src/measadj.f90:68-70,85-89,99-103).
+! the call sites (src/measadj.f90:68-70,85-89,99-103), and its inversion-grid
+! projections too (src/acqui.f90:274-279).
 !=====================================================================
 
 ! src/eikon_solver_2d.f90:764  FSM_UW_PS_lonlat_2d
@@ -48,3 +49,35 @@ subroutine mask(xx, yy, nx, ny, TableAdj, x0, y0)
   double precision :: xx(nx), yy(ny), TableAdj(nx,ny), x0, y0
   call satt_host_check(satt_mask(satt_host_ctx(), xx, yy, nx, ny, TableAdj, x0, y0), 'mask')
 end subroutine mask
+
+! src/eikon_solver_2d.f90:1538  inv_grid_iso
+subroutine inv_grid_iso(xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, adj, nx, ny, nz_3d, gk, &
+                        vel_x, vel_y, vel_z_3d)
+  use, intrinsic :: iso_c_binding
+  use satt_cabi, only: satt_inv_grid_iso
+  use satt_host, only: satt_host_ctx, satt_host_check
+  implicit none
+  integer :: nxinv, nyinv, nzinv, nset, nx, ny, nz_3d
+  double precision :: xinv(nxinv, nset), yinv(nyinv, nset), zinv(nzinv, nset)
+  double precision :: adj(nx, ny, nz_3d), gk(nset*nxinv*nyinv*nzinv)
+  double precision :: vel_x(nx), vel_y(ny), vel_z_3d(nz_3d)
+  call satt_host_check(satt_inv_grid_iso(satt_host_ctx(), xinv, yinv, zinv, nxinv, nyinv, &
+                                         nzinv, nset, adj, nx, ny, nz_3d, gk, vel_x, vel_y, &
+                                         vel_z_3d), 'inv_grid_iso')
+end subroutine inv_grid_iso
+
+! src/eikon_solver_2d.f90:1613  inv2fwd_iso
+subroutine inv2fwd_iso(xinv, yinv, zinv, nxinv, nyinv, nzinv, nset, nx, ny, nz_3d, gk, &
+                       vel_x, vel_y, vel_z_3d, update)
+  use, intrinsic :: iso_c_binding
+  use satt_cabi, only: satt_inv2fwd_iso
+  use satt_host, only: satt_host_ctx, satt_host_check
+  implicit none
+  integer :: nxinv, nyinv, nzinv, nset, nx, ny, nz_3d
+  double precision :: xinv(nxinv, nset), yinv(nyinv, nset), zinv(nzinv, nset)
+  double precision :: gk(nset*nxinv*nyinv*nzinv), update(nx, ny, nz_3d)
+  double precision :: vel_x(nx), vel_y(ny), vel_z_3d(nz_3d)
+  call satt_host_check(satt_inv2fwd_iso(satt_host_ctx(), xinv, yinv, zinv, nxinv, nyinv, &
+                                        nzinv, nset, nx, ny, nz_3d, gk, vel_x, vel_y, vel_z_3d, &
+                                        update), 'inv2fwd_iso')
+end subroutine inv2fwd_iso

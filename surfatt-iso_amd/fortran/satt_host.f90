@@ -28,6 +28,11 @@
 !                      with prepare_fwd_linesearch (src/acqui_2d.f90:173-183):
 !                      the trial models and the forward-only runs on the
 !                      device, over the same arrays as forward_simulate
+!  - satt_combine_kernels_host()  att_acqui%combine_kernels
+!                      (src/acqui.f90:197-245) over its arrays
+!  - satt_post_processing_host()  att_acqui%post_processing_for_kernel
+!                      (src/acqui.f90:247-304): preconditioning and both
+!                      inversion-grid projections on the device
 !  - satt_fwdsurf3d_host()  fwdsurf3d_decomp (src/surfker/surfker.f90:200-222)
 !                      over the whole grid: the velocity maps of
 !                      att_grid%fwdsurf (src/grid.f90:150-163)
@@ -40,6 +45,7 @@ module satt_host
   public :: satt_host_ctx, satt_host_check, satt_host_release, satt_forward_simulate_host
   public :: satt_gaussian_smooth_geo_2, satt_gradient_2_geo, satt_get_topo_metric
   public :: satt_steepest_descent_host, satt_fwdsurf3d_host, satt_linesearch_host
+  public :: satt_combine_kernels_host, satt_post_processing_host
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -361,6 +367,42 @@ contains
     end if
     call satt_host_check(st, 'fwdsurf3d')
   end subroutine satt_fwdsurf3d_host
+
+  !> combine_kernels over att_acqui's arrays: ker and ker_density
+  !> (nker,nx,ny,nz) accumulate -adj*sen over the periods.
+  subroutine satt_combine_kernels_host(adj_s, adj_density, sen_vs, sen_vp, sen_rho, vs3d, &
+                                       vp3d, use_alpha_beta_rho, rho_scaling, ker, ker_density)
+    real(c_double), intent(in), contiguous :: adj_s(:,:,:), adj_density(:,:,:)
+    real(c_double), intent(in), contiguous :: sen_vs(:,:,:,:), sen_vp(:,:,:,:), sen_rho(:,:,:,:)
+    real(c_double), intent(in), contiguous :: vs3d(:,:,:), vp3d(:,:,:)
+    logical, intent(in) :: use_alpha_beta_rho, rho_scaling
+    real(c_double), intent(inout), contiguous :: ker(:,:,:,:), ker_density(:,:,:,:)
+    call satt_host_check(satt_combine_kernels(satt_host_ctx(), size(sen_vs,1), size(sen_vs,2), &
+                                              size(sen_vs,3), size(sen_vs,4), adj_s, &
+                                              adj_density, sen_vs, sen_vp, sen_rho, vs3d, vp3d, &
+                                              merge(1_c_int, 0_c_int, use_alpha_beta_rho), &
+                                              merge(1_c_int, 0_c_int, rho_scaling), ker, &
+                                              ker_density), 'combine_kernels')
+  end subroutine satt_combine_kernels_host
+
+  !> post_processing_for_kernel: every component of ker (nker,nx,ny,nz)
+  !> preconditioned (kdensity_coe > 0, in place), projected to the inversion
+  !> grids xinv(nxinv,nset), yinv, zinv and back, and added to gradient_s as
+  !> update/nset/chi0*weight.
+  subroutine satt_post_processing_host(xgrids, ygrids, zgrids, xinv, yinv, zinv, ker, &
+                                       ker_density, kdensity_coe, chi0, weight, gradient_s)
+    real(c_double), intent(in), contiguous :: xgrids(:), ygrids(:), zgrids(:)
+    real(c_double), intent(in), contiguous :: xinv(:,:), yinv(:,:), zinv(:,:)
+    real(c_double), intent(inout), contiguous :: ker(:,:,:,:), gradient_s(:,:,:,:)
+    real(c_double), intent(in), contiguous :: ker_density(:,:,:,:)
+    real(c_double), intent(in) :: kdensity_coe, chi0, weight
+    call satt_host_check(satt_kernel_postproc(satt_host_ctx(), size(ker,1), size(ker,2), &
+                                              size(ker,3), size(ker,4), xgrids, ygrids, zgrids, &
+                                              size(xinv,2), size(xinv,1), size(yinv,1), &
+                                              size(zinv,1), xinv, yinv, zinv, ker, ker_density, &
+                                              kdensity_coe, chi0, weight, gradient_s), &
+                         'post_processing_for_kernel')
+  end subroutine satt_post_processing_host
 
   integer function find_name(names, n, s)
     character(len=*), intent(in) :: names(:), s

@@ -288,6 +288,67 @@ class Context:
             ctypes.c_double(sigma_2d), ctypes.c_double(updatemax), _d(sv), _d(ker)))
         return sv, ker
 
+    def combine_kernels(self, adj_s, adj_density, sen_vs, sen_vp, sen_rho, vs3d=None,
+                        vp3d=None, use_alpha_beta_rho=False, rho_scaling=True, ker=None,
+                        ker_density=None):
+        """att_acqui%combine_kernels (src/acqui.f90:197-245): (ker,
+        ker_density), each (nker, nx, ny, nz), accumulated into the given
+        arrays (else zeros)."""
+        sv = _f(sen_vs)
+        npd, nx, ny, nz = sv.shape
+        nker = 3 if use_alpha_beta_rho else 1
+        k = np.zeros((nker, nx, ny, nz), order="F") if ker is None else _f(ker).copy(order="F")
+        kd = (np.zeros((nker, nx, ny, nz), order="F") if ker_density is None
+              else _f(ker_density).copy(order="F"))
+        opt = lambda a: None if a is None else _f(a)
+        args = [_f(adj_s), _f(adj_density), sv, _f(sen_vp), opt(sen_rho), opt(vs3d), opt(vp3d)]
+        self._check(self.lib.satt_combine_kernels(
+            self.ptr, npd, nx, ny, nz, *(_d(a) for a in args), int(bool(use_alpha_beta_rho)),
+            int(bool(rho_scaling)), _d(k), _d(kd)))
+        return k, kd
+
+    @staticmethod
+    def _inv(xinv, yinv, zinv):
+        xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)
+        if xinv.ndim != 2 or yinv.shape[1:] != xinv.shape[1:] or zinv.shape[1:] != xinv.shape[1:]:
+            raise SattError(E_ARG, "inversion grids are (n, nset) arrays with one nset")
+        return xinv, yinv, zinv, xinv.shape[0], yinv.shape[0], zinv.shape[0], xinv.shape[1]
+
+    def inv_grid_iso(self, xinv, yinv, zinv, adj, vel_x, vel_y, vel_z):
+        """inv_grid_iso (src/eikon_solver_2d.f90:1538): gk (nset*nxinv*nyinv*nzinv)."""
+        xinv, yinv, zinv, nxi, nyi, nzi, ns = self._inv(xinv, yinv, zinv)
+        adj = _f(adj)
+        nx, ny, nz = adj.shape
+        gk = np.zeros(ns * nxi * nyi * nzi)
+        self._check(self.lib.satt_inv_grid_iso(
+            self.ptr, _d(xinv), _d(yinv), _d(zinv), nxi, nyi, nzi, ns, _d(adj), nx, ny, nz,
+            _d(gk), _d(_c(vel_x)), _d(_c(vel_y)), _d(_c(vel_z))))
+        return gk
+
+    def inv2fwd_iso(self, xinv, yinv, zinv, gk, vel_x, vel_y, vel_z):
+        """inv2fwd_iso (src/eikon_solver_2d.f90:1613): update (nx, ny, nz)."""
+        xinv, yinv, zinv, nxi, nyi, nzi, ns = self._inv(xinv, yinv, zinv)
+        vx, vy, vz = _c(vel_x), _c(vel_y), _c(vel_z)
+        up = np.zeros((len(vx), len(vy), len(vz)), order="F")
+        self._check(self.lib.satt_inv2fwd_iso(
+            self.ptr, _d(xinv), _d(yinv), _d(zinv), nxi, nyi, nzi, ns, len(vx), len(vy), len(vz),
+            _d(_c(gk)), _d(vx), _d(vy), _d(vz), _d(up)))
+        return up
+
+    def kernel_postproc(self, vel_x, vel_y, vel_z, xinv, yinv, zinv, ker, ker_density,
+                        kdensity_coe, chi0, weight, gradient_s):
+        """att_acqui%post_processing_for_kernel (src/acqui.f90:247-285): returns
+        (ker after the density preconditioning, gradient_s after the update)."""
+        xinv, yinv, zinv, nxi, nyi, nzi, ns = self._inv(xinv, yinv, zinv)
+        k, g = _f(ker).copy(order="F"), _f(gradient_s).copy(order="F")
+        nker, nx, ny, nz = k.shape
+        kd = None if ker_density is None else _f(ker_density)
+        self._check(self.lib.satt_kernel_postproc(
+            self.ptr, nker, nx, ny, nz, _d(_c(vel_x)), _d(_c(vel_y)), _d(_c(vel_z)), ns, nxi,
+            nyi, nzi, _d(xinv), _d(yinv), _d(zinv), _d(k), _d(kd), ctypes.c_double(kdensity_coe),
+            ctypes.c_double(chi0), ctypes.c_double(weight), _d(g)))
+        return k, g
+
     def fwdsurf3d(self, vs3d, depz, periods, iwave=2, igr=0, status=False):
         """Surface-wave velocity maps of att_grid%fwdsurf (src/grid.f90:150-163):
         (nperiod, nx, ny) phase (igr 0) or group (igr > 0) velocities of Love

@@ -440,3 +440,41 @@ def acqui3d_checkerboard(n=41, d=0.05, nz=31, zmax=60.0, nsta=100, nper=16, seed
     ones = np.ones((nper, n, n)); zeros = np.zeros((nper, n, n))
     bt = make_batch(3, xg, yg, periods, ct, ones, zeros, ones, paths)
     return bt, with_model(bt, cc)
+
+
+def linspace_ref(first, last, n):
+    """linspace_r8r8 (src/shared/utils.f90:592-604): first + step*(i-1), the
+    last node set to `last`."""
+    step = (last - first) / (n - 1)
+    out = first + step * np.arange(n, dtype=np.float64)
+    out[-1] = last
+    return out
+
+
+def inv_grids(vx, vy, vz, n_inv, nset):
+    """The inversion grids of att_model%get_inv_grids (src/model.f90:127-168)
+    for model axes vx, vy, vz and n_inv = (ninvx, ninvy, ninvz): (xinv, yinv,
+    zinv), each (ninv, nset) in Fortran order, the sets shifted as there."""
+    nxi, nyi, nzi = n_inv
+    x1, xn, y1, yn, z1, zn = vx[0], vx[-1], vy[0], vy[-1], vz[0], vz[-1]
+    ix = linspace_ref(x1 - (xn - x1) / (nxi - 2) / 2.0, xn + (xn - x1) / (nxi - 2) / 2.0, nxi)
+    iy = linspace_ref(y1 - (yn - y1) / (nyi - 2), yn, nyi)
+    iz = linspace_ref(z1 - 0.5 * (zn - z1) / (nzi - 2), zn + 0.5 * (zn - z1) / (nzi - 2), nzi)
+    zref = np.empty(nzi)
+    zref[:-1] = iz[1:]
+    zref[-1] = iz[-1]
+    zadd = (zref - iz) / (nset + 1)
+    zadd[0] = 0
+    xinv = np.zeros((nxi, nset), order="F")
+    yinv = np.zeros((nyi, nset), order="F")
+    zinv = np.zeros((nzi, nset), order="F")
+    polar = 1
+    for i in range(nset):
+        if i == 0:
+            xinv[:, 0] = ix
+        else:
+            xinv[:, i] = xinv[:, i - 1] + polar * i * (ix[1] - ix[0]) / (nset + 1)
+            polar = -polar
+        yinv[:, i] = iy + i * (iy[1] - iy[0]) / (nset + 1)
+        zinv[:, i] = iz + i * zadd
+    return xinv, yinv, zinv

@@ -22,7 +22,8 @@ def test_fhost_exports_reference_names():
     lib = ctypes.CDLL(FHOST)
     for name in ("fsm_uw_ps_lonlat_2d_", "fsm_o1_jse_lonlat_2d_", "mask_",
                  "satt_fhost_forward_simulate", "satt_fhost_smooth", "satt_fhost_topo_metric",
-                 "satt_fhost_steepest_descent", "satt_fhost_linesearch"):
+                 "satt_fhost_steepest_descent", "satt_fhost_linesearch", "inv_grid_iso_",
+                 "inv2fwd_iso_"):
         assert hasattr(lib, name), name
 
 
@@ -182,3 +183,18 @@ def test_fortran_linesearch_matches_c_abi(ctx):
     assert st.value >= 0
     assert nsub.value == want["nsub"] == 2 and upd.value == want["updatemax"]
     assert chi.value == want["chi"] and np.array_equal(sv, want["svel"])
+
+
+@pytest.mark.gpu
+def test_projection_dropins_called_like_the_reference(ctx):
+    """inv_grid_iso_ / inv2fwd_iso_ of libsatt_fhost.so called with the
+    reference's by-reference argument lists (src/acqui.f90:274-279) equal
+    the C-ABI calls bit for bit."""
+    fh = O.RefSolver(FHOST)            # the same Fortran ABI as the reference build
+    vx, vy, vz = np.linspace(100, 101, 21), np.linspace(30, 31, 19), np.linspace(0, 40, 11)
+    xi, yi, zi = W.inv_grids(vx, vy, vz, (7, 6, 5), 3)
+    adj = np.asfortranarray(np.random.default_rng(2).standard_normal((21, 19, 11)))
+    gk = fh.inv_grid_iso(xi, yi, zi, adj, vx, vy, vz)
+    assert np.array_equal(gk, ctx.inv_grid_iso(xi, yi, zi, adj, vx, vy, vz))
+    up = fh.inv2fwd_iso(xi, yi, zi, gk, vx, vy, vz)
+    assert np.array_equal(up, ctx.inv2fwd_iso(xi, yi, zi, gk, vx, vy, vz))
