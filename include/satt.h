@@ -31,6 +31,7 @@
  *   satt_inv2fwd_iso       inv2fwd_iso            src/eikon_solver_2d.f90:1613
  *   satt_kernel_postproc   att_acqui%post_processing_for_kernel
  *                          (src/acqui.f90:247-304)
+ *   satt_correct_depth     correct_depth          src/surfker/surfker.f90:260
  *
  * Conventions
  *   - A grid field f(nx,ny) is column-major with x fastest: f[i + nx*j], as
@@ -206,6 +207,17 @@ int satt_combine_kernels(satt_ctx *ctx, int nperiod, int nx, int ny, int nz,
                          const double *vs3d, const double *vp3d, int use_alpha_beta_rho,
                          int rho_scaling, double *ker, double *ker_density);
 
+/* correct_depth (src/surfker/surfker.f90:260-283), the topography step of
+ * att_acqui%depthkernel (src/acqui.f90:183-187): every (period, column)
+ * profile of the depth kernels ker_vs, ker_vp, ker_rho (nperiod,nx,ny,nz),
+ * in place, becomes its linear interpolation (interp1, src/shared/
+ * utils.f90:786-805) at depz/cosd(dip) for the column's dip angle
+ * dip_angle (nperiod,nx,ny, degrees in [0, 90); satt_topo_metric's angle).
+ * depz (nz) increases from >= 0. */
+int satt_correct_depth(satt_ctx *ctx, int nperiod, int nx, int ny, int nz,
+                       double *ker_vs, double *ker_vp, double *ker_rho,
+                       const double *dip_angle, const double *depz);
+
 /* inv_grid_iso (src/eikon_solver_2d.f90:1538-1610), the reference's
  * arguments: gk(nset*nxinv*nyinv*nzinv) = the interior points of adj
  * (nx,ny,nz) spread trilinearly onto the inversion grids xinv(nxinv,nset),
@@ -242,8 +254,8 @@ int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_
 
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
  * satt_gradient_geo2, satt_topo_metric, satt_steepest_descent,
- * satt_combine_kernels, satt_inv_grid_iso, satt_inv2fwd_iso or
- * satt_kernel_postproc call on the context (HIP events on its stream;
+ * satt_combine_kernels, satt_inv_grid_iso, satt_inv2fwd_iso,
+ * satt_kernel_postproc or satt_correct_depth call on the context (HIP events on its stream;
  * transfers excluded). */
 double satt_last_device_ms(const satt_ctx *ctx);
 

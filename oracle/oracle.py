@@ -60,6 +60,8 @@ def lib():
             getattr(_lib, name).restype = ctypes.c_double
             getattr(_lib, name).argtypes = [ctypes.c_double]
         _lib.orc_combine_kernels.restype = None
+        _lib.orc_correct_depth.restype = None
+        _lib.orc_interp1.restype = ctypes.c_double
     return _lib
 
 
@@ -440,6 +442,25 @@ def combine_kernels(adj_s, adj_den, sen_vs, sen_vp, sen_rho, vs3d, vp3d, use_alp
                               _d(sen_rho), _d(vs3d), _d(vp3d), int(bool(use_alpha_beta_rho)),
                               int(bool(rho_scaling)), _d(ker), _d(ker_den))
     return ker, ker_den
+
+
+def interp1(x, v, xq):
+    """interp1_0 (src/shared/utils.f90:786-805)."""
+    x, v = np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_interp1(_d(x), _d(v), len(x), ctypes.c_double(xq))
+
+
+def correct_depth(ker_vs, ker_vp, ker_rho, dip, depz):
+    """correct_depth (src/surfker/surfker.f90:260-283) on (nperiod, nx, ny, nz)
+    kernels and (nperiod, nx, ny) dip angles: the corrected copies."""
+    ks = [_f(k).copy(order="F") for k in (ker_vs, ker_vp, ker_rho)]
+    npd, nx, ny, nz = ks[0].shape
+    dip = _f(dip)
+    depz = np.ascontiguousarray(depz, dtype=np.float64)
+    work = np.zeros(2 * nz)
+    lib().orc_correct_depth(npd, nx, ny, nz, _d(ks[0]), _d(ks[1]), _d(ks[2]), _d(dip), _d(depz),
+                            _d(work))
+    return tuple(ks)
 
 
 def inv_grid_iso(xinv, yinv, zinv, adj, vx, vy, vz):

@@ -176,6 +176,11 @@ int orc_inv_grid_iso(const double *xinv, const double *yinv, const double *zinv,
 int orc_inv2fwd_iso(const double *xinv, const double *yinv, const double *zinv, int nxinv,
                     int nyinv, int nzinv, int nset, int nx, int ny, int nz, const double *gk,
                     const double *vx, const double *vy, const double *vz, double *update);
+/* interp1_0 (src/shared/utils.f90:786-805) */
+double orc_interp1(const double *x, const double *v, int n, double xq);
+/* correct_depth (src/surfker/surfker.f90:260-283); work: 2*nz doubles */
+void orc_correct_depth(int nperiod, int nx, int ny, int nz, double *ker_vs, double *ker_vp,
+                       double *ker_rho, const double *dip, const double *depz, double *work);
 /* post_processing_for_kernel + regularize_ker_density (src/acqui.f90:247-304);
  * work: nset*nxinv*nyinv*nzinv + 2*nx*ny*nz doubles */
 int orc_kernel_postproc(int nker, int nx, int ny, int nz, const double *vx, const double *vy,

@@ -12,6 +12,8 @@
  *   orc_inv2fwd_iso         inv2fwd_iso            src/eikon_solver_2d.f90:1613-1691
  *   orc_kernel_postproc     post_processing_for_kernel + regularize_ker_density
  *                                                  src/acqui.f90:247-304
+ *   orc_interp1             interp1_0              src/shared/utils.f90:786-805
+ *   orc_correct_depth       correct_depth          src/surfker/surfker.f90:260-283
  *
  * Statement order and evaluation order follow the reference (built with
  * -ffp-contract=off).  Array layouts are the reference's Fortran ones:
@@ -245,4 +247,39 @@ int orc_kernel_postproc(int nker, int nx, int ny, int nz, const double *vx, cons
     }
   }
   return 0;
+}
+
+/* interp1_0 (utils.f90:786-805): x ascending; x1 the last node <= xq, x2 the
+ * first node > xq; past the last node the last value. */
+double orc_interp1(const double *x, const double *v, int n, double xq) {
+  int x1 = -1, x2 = -1;
+  for (int i = 0; i < n; ++i) {
+    if (xq >= x[i]) x1 = i;
+    else if (x2 < 0) x2 = i;
+  }
+  if (x2 < 0) return v[n - 1];
+  if (x1 < 0) return NAN;  /* below the first node: the reference reads x(0) */
+  const double vn = fabs(x[x2] - x[x1]);
+  const double vq = v[x1] * (x[x2] - xq) + v[x2] * (xq - x[x1]);
+  return vq / vn;
+}
+
+/* correct_depth (surfker.f90:260-283): every (period, column) of the depth
+ * kernels (nperiod,nx,ny,nz) resampled at depz/cosd(dip) from the dip
+ * angles (nperiod,nx,ny); cosd(x) = cos(x*pi/180) with the true pi.
+ * work: 2*nz doubles. */
+void orc_correct_depth(int nperiod, int nx, int ny, int nz, double *ker_vs, double *ker_vp,
+                       double *ker_rho, const double *dip, const double *depz, double *work) {
+  static const double PI_TRUE = 3.14159265358979323846;
+  double *col = work, *newz = work + nz;
+  double *ks[3] = {ker_vs, ker_vp, ker_rho};
+  const size_t n3 = (size_t)nperiod * nx * ny;
+  for (size_t c = 0; c < n3; ++c) {
+    const double cd = cos(dip[c] * (PI_TRUE / 180.0));
+    for (int k = 0; k < nz; ++k) newz[k] = depz[k] / cd;
+    for (int q = 0; q < 3; ++q) {
+      for (int k = 0; k < nz; ++k) col[k] = ks[q][c + n3 * k];
+      for (int k = 0; k < nz; ++k) ks[q][c + n3 * k] = orc_interp1(depz, col, nz, newz[k]);
+    }
+  }
 }

@@ -859,6 +859,44 @@ extern "C" int satt_kernel_postproc(satt_ctx *ctx, int nker, int nx, int ny, int
   return geo_timer_read(ctx, t);
 }
 
+extern "C" int satt_correct_depth(satt_ctx *ctx, int nperiod, int nx, int ny, int nz,
+                                  double *ker_vs, double *ker_vp, double *ker_rho,
+                                  const double *dip_angle, const double *depz) {
+  if (!ctx) return SATT_E_ARG;
+  if (nperiod < 1 || nx < 1 || ny < 1 || nz < 2 || !ker_vs || !ker_vp || !ker_rho ||
+      !dip_angle || !depz || (size_t)nperiod * nx * ny * nz > (size_t)INT_MAX)
+    return fail(ctx, SATT_E_ARG, "satt_correct_depth: bad arguments");
+  if (!increasing(depz, nz) || depz[0] < 0)
+    return fail(ctx, SATT_E_ARG, "satt_correct_depth: depths must increase from >= 0");
+  const size_t n3 = (size_t)nperiod * nx * ny, M = n3 * nz;
+  for (size_t c = 0; c < n3; ++c)
+    if (!(dip_angle[c] >= 0 && dip_angle[c] < 90))
+      return fail(ctx, SATT_E_ARG, "satt_correct_depth: dip angles must lie in [0, 90)");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DBuf bk, bo, ba;
+  ALLOC(ctx, bk, 3 * M * 8);
+  ALLOC(ctx, bo, 3 * M * 8);
+  double *k = bk.as<double>();
+  double *const ks[3] = {ker_vs, ker_vp, ker_rho};
+  for (int q = 0; q < 3; ++q)
+    HIPCHK(ctx, hipMemcpyAsync(k + q * M, ks[q], M * 8, hipMemcpyHostToDevice, ctx->stream));
+  std::vector<double> h(n3 + nz);
+  std::memcpy(h.data(), dip_angle, n3 * 8);
+  std::memcpy(h.data() + n3, depz, (size_t)nz * 8);
+  TRY(upload(ctx, ba, h.data(), h.size() * 8));
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_correct_depth(ctx->stream, n3, nz, k, bo.as<double>(), ba.as<double>(),
+                       ba.as<double>() + n3);
+  HIPCHK(ctx, hipGetLastError());
+  TRY(geo_timer_stop(ctx, t));
+  for (int q = 0; q < 3; ++q)
+    HIPCHK(ctx, hipMemcpyAsync(ks[q], bo.as<double>() + q * M, M * 8, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  TRY(sync_status(ctx));
+  return geo_timer_read(ctx, t);
+}
+
 extern "C" int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
                               const double *depz, int nperiod, const double *periods, int iwave,
                               int igr, double *svel, int *col_status) {

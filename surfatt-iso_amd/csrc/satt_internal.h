@@ -240,6 +240,10 @@ void launch_component(hipStream_t s, size_t n, int nker, int q, const double *ke
                       const double *kd, double *kc, double *kdc);
 void launch_precond(hipStream_t s, size_t n, int nker, int q, const double *kdc,
                     const double *mx, double coe, double *ker, double *kc);
+// the three kernels (vs, vp, rho) consecutive in `in` and `out`, each
+// (n3 = nperiod*nx*ny, nz)
+void launch_correct_depth(hipStream_t s, size_t n3, int nz, const double *in, double *out,
+                          const double *dip, const double *depz);
 void launch_grad_update(hipStream_t s, size_t n, int nker, int q, const double *update,
                         int nset, double chi0, double weight, double *grad);
 

@@ -8,6 +8,8 @@
 //   inv_grid_iso, inv2fwd_iso    src/eikon_solver_2d.f90:1538-1691
 //                                (locate_within_cube :1470-1535)
 //   post_processing_for_kernel   src/acqui.f90:247-304
+//   correct_depth (interp1_0)    src/surfker/surfker.f90:260-283
+//                                (src/shared/utils.f90:786-805)
 //
 // Locating a point.  The reference walks from the cube the previous point
 // of its (j, i, k) loop ended in, so a point on a face of the inversion grid
@@ -308,6 +310,33 @@ __global__ void k_grad_update(size_t n, int nker, int q, const double *update, i
   grad[q + (size_t)nker * c] = grad[q + (size_t)nker * c] + u * weight;
 }
 
+// correct_depth: one thread per (period, column); each kernel's depth
+// profile is resampled at depz/cosd(dip) by interp1_0: the last node <= z
+// and the first node > z, past the last node the last value.
+__global__ void k_correct_depth(size_t n3, int nz, const double *in, double *out,
+                                const double *dip, const double *depz) {
+  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (c >= n3) return;
+  const int q = blockIdx.y;  // vs, vp, rho
+  const double *v = in + (size_t)q * n3 * nz;
+  double *o = out + (size_t)q * n3 * nz;
+  const double cd = cos(dip[c] * (3.14159265358979323846 / 180.0));
+  int x1 = 0;
+  for (int k = 0; k < nz; ++k) {
+    const double z = depz[k] / cd;
+    while (x1 + 1 < nz && depz[x1 + 1] <= z) ++x1;  // newz ascends with k
+    double r;
+    if (x1 + 1 >= nz || z < depz[0]) {
+      r = v[c + n3 * (nz - 1)];
+    } else {
+      const int x2 = x1 + 1;
+      const double vn = fabs(depz[x2] - depz[x1]);
+      r = (v[c + n3 * x1] * (depz[x2] - z) + v[c + n3 * x2] * (z - depz[x1])) / vn;
+    }
+    o[c + n3 * k] = r;
+  }
+}
+
 __global__ void k_fill(size_t n, double *v) {
   const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (c < n) v[c] = 0.0;
@@ -355,6 +384,12 @@ void launch_component(hipStream_t s, size_t n, int nker, int q, const double *ke
 void launch_precond(hipStream_t s, size_t n, int nker, int q, const double *kdc,
                     const double *mx, double coe, double *ker, double *kc) {
   hipLaunchKernelGGL(k_precond, blocks(n), dim3(256), 0, s, n, nker, q, kdc, mx, coe, ker, kc);
+}
+
+void launch_correct_depth(hipStream_t s, size_t n3, int nz, const double *in, double *out,
+                          const double *dip, const double *depz) {
+  hipLaunchKernelGGL(k_correct_depth, dim3((unsigned)((n3 + 255) / 256), 3), dim3(256), 0, s, n3,
+                     nz, in, out, dip, depz);
 }
 
 void launch_grad_update(hipStream_t s, size_t n, int nker, int q, const double *update,

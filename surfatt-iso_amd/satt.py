@@ -307,6 +307,16 @@ class Context:
             int(bool(rho_scaling)), _d(k), _d(kd)))
         return k, kd
 
+    def correct_depth(self, ker_vs, ker_vp, ker_rho, dip_angle, depz):
+        """correct_depth (src/surfker/surfker.f90:260-283): corrected copies of
+        the (nperiod, nx, ny, nz) depth kernels for dip angles (nperiod, nx, ny)."""
+        ks = [_f(k).copy(order="F") for k in (ker_vs, ker_vp, ker_rho)]
+        npd, nx, ny, nz = ks[0].shape
+        self._check(self.lib.satt_correct_depth(
+            self.ptr, npd, nx, ny, nz, _d(ks[0]), _d(ks[1]), _d(ks[2]), _d(_f(dip_angle)),
+            _d(_c(depz))))
+        return tuple(ks)
+
     @staticmethod
     def _inv(xinv, yinv, zinv):
         xinv, yinv, zinv = _f(xinv), _f(yinv), _f(zinv)

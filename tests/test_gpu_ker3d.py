@@ -80,3 +80,20 @@ def test_point_outside_is_an_error(ctx):
                          vx, vy, vz)
     # still usable
     assert ctx.inv2fwd_iso(xi, yi, zi, np.ones(2 * 6 * 5 * 5), vx, vy, vz).shape == (13, 11, 9)
+
+
+def test_correct_depth_matches_oracle(ctx):
+    depz = np.linspace(0.0, 60.0, 31)
+    r = np.random.default_rng(21)
+    sh = (5, 7, 6, 31)
+    ks = [np.asfortranarray(r.standard_normal(sh)) for _ in range(3)]
+    dip = np.asfortranarray(r.uniform(0.0, 40.0, sh[:3]))
+    dip[0, 0, 0] = 0.0
+    got = ctx.correct_depth(*ks, dip, depz)
+    want = O.correct_depth(*ks, dip, depz)
+    for g, w in zip(got, want):
+        assert np.max(np.abs(g - w)) <= 1e-13 * np.max(np.abs(w))
+    assert np.array_equal(got[0][0, 0, 0], ks[0][0, 0, 0]) or \
+        np.allclose(got[0][0, 0, 0], ks[0][0, 0, 0], rtol=1e-15)
+    with pytest.raises(satt.SattError):
+        ctx.correct_depth(*ks, dip + 90.0, depz)

@@ -166,3 +166,38 @@ def test_postproc_composition_and_preconditioning():
     small = np.abs(kd[0]) / 1e3 < 1e-4
     assert np.all(k2[0][small] == 0) and small.sum() > 10
     assert np.all(k2[0][~small] != 0)
+
+
+def test_interp1_definition():
+    """interp1_0: linear between the bracketing nodes, the node value at a
+    node, the last value past the last node."""
+    x = np.array([0.0, 2.0, 3.0, 7.0])
+    v = np.array([1.0, -3.0, 4.0, 8.0])
+    assert O.interp1(x, v, 2.5) == (-3.0 * 0.5 + 4.0 * 0.5) / 1.0
+    assert O.interp1(x, v, 1.0) == (1.0 * 1.0 + -3.0 * 1.0) / 2.0
+    assert O.interp1(x, v, 3.0) == 4.0 and O.interp1(x, v, 0.0) == 1.0
+    assert O.interp1(x, v, 9.5) == 8.0 and O.interp1(x, v, 7.0) == 8.0
+    lin = 0.3 * x - 2.0
+    for q in (0.1, 2.9, 5.5):
+        assert abs(O.interp1(x, lin, q) - (0.3 * q - 2.0)) < 1e-15
+
+
+def test_correct_depth_stretches_profiles():
+    """dip 0 keeps every profile; dip 60 degrees (cos 1/2) moves the value at
+    depth 2z up to z, a linear profile stays linear until 2z leaves the grid,
+    then holds the last value."""
+    depz = np.linspace(0.0, 40.0, 21)
+    npd, nx, ny = 2, 3, 2
+    r = np.random.default_rng(3)
+    k = np.asfortranarray(r.standard_normal((npd, nx, ny, 21)))
+    lin = np.asfortranarray(np.broadcast_to(1.0 + 0.05 * depz, (npd, nx, ny, 21)))
+    dip = np.zeros((npd, nx, ny), order="F")
+    out = O.correct_depth(k, lin, k, dip, depz)
+    assert np.allclose(out[0], k, rtol=1e-15, atol=1e-15) and np.allclose(out[1], lin, rtol=1e-15)
+    dip[:] = 60.0
+    out = O.correct_depth(k, lin, k, dip, depz)
+    z2 = depz / np.cos(np.deg2rad(60.0))
+    want = np.where(z2 <= 40.0, 1.0 + 0.05 * z2, 1.0 + 0.05 * 40.0)
+    assert np.allclose(out[1], np.broadcast_to(want, out[1].shape), rtol=0, atol=1e-13)
+    # node k goes to node 2k where 2k is a node (cos 60 = 1/2 to rounding)
+    assert np.allclose(out[0][..., 3], k[..., 6], rtol=0, atol=1e-12)
