@@ -6,12 +6,12 @@
 subroutine satt_fhost_forward_simulate(dim, nx, ny, nperiod, xgrids, ygrids, periods, svel, &
                                        m11, m12, m22, npath, evt_id, sta_id, evla, evlo, &
                                        stla, stlo, periods_all, tt, weight, istotable, isadj, &
-                                       tt_fwd, chi, adj_s, adj_density, status) &
+                                       tt_fwd, chi, adj_s, adj_density, status, precision) &
                                        bind(C, name="satt_fhost_forward_simulate")
   use, intrinsic :: iso_c_binding
   use satt_host, only: satt_forward_simulate_host
   implicit none
-  integer(c_int), value :: dim, nx, ny, nperiod, npath, istotable, isadj
+  integer(c_int), value :: dim, nx, ny, nperiod, npath, istotable, isadj, precision
   real(c_double), intent(in) :: xgrids(nx), ygrids(ny), periods(nperiod)
   real(c_double), intent(in) :: svel(nperiod,nx,ny), m11(nperiod,nx,ny), m12(nperiod,nx,ny)
   real(c_double), intent(in) :: m22(nperiod,nx,ny)
@@ -31,7 +31,7 @@ subroutine satt_fhost_forward_simulate(dim, nx, ny, nperiod, xgrids, ygrids, per
   call satt_forward_simulate_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, &
                                   evtname, staname, evla, evlo, stla, stlo, periods_all, tt, &
                                   weight, istotable /= 0, isadj /= 0, tt_fwd, chi, adj_s, &
-                                  adj_density, st)
+                                  adj_density, st, precision)
   status = st
 end subroutine satt_fhost_forward_simulate
 

@@ -106,7 +106,7 @@ contains
   subroutine satt_forward_simulate_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, &
                                         evtname, staname, evla, evlo, stla, stlo, &
                                         periods_all, tt, weight, istotable, isadj, &
-                                        tt_fwd, chi, adj_s, adj_density, status)
+                                        tt_fwd, chi, adj_s, adj_density, status, precision)
     integer, intent(in) :: dim
     real(c_double), intent(in), target, contiguous :: xgrids(:), ygrids(:), periods(:)
     real(c_double), intent(in), target, contiguous :: svel(:,:,:), m11(:,:,:), m12(:,:,:)
@@ -119,9 +119,11 @@ contains
     real(c_double), intent(out) :: chi
     real(c_double), intent(out), target, contiguous :: adj_s(:,:,:), adj_density(:,:,:)
     integer, intent(out), optional :: status
+    integer, intent(in), optional :: precision   ! SATT_PREC_FP64 (default) or SATT_PREC_MIXED
 
     type(host_batch), target :: hb
     real(c_double), target :: chi_c(1)
+    integer(c_int) :: prec
     type(satt_result) :: r
     integer(c_int) :: st
 
@@ -131,8 +133,10 @@ contains
     r%chi = c_loc(chi_c)
     r%adj_s = c_loc(adj_s)
     if (dim == 3) r%adj_density = c_loc(adj_density)
+    prec = SATT_PREC_FP64
+    if (present(precision)) prec = int(precision, c_int)
     st = satt_forward_simulate(satt_host_ctx(), hb%b, merge(1_c_int, 0_c_int, istotable), &
-                               merge(1_c_int, 0_c_int, isadj), SATT_PREC_FP64, r)
+                               merge(1_c_int, 0_c_int, isadj), prec, r)
     if (present(status)) then
       status = st
     else
@@ -150,7 +154,7 @@ contains
   subroutine satt_linesearch_host(dim, xgrids, ygrids, periods, svel, m11, m12, m22, &
                                   evtname, staname, evla, evlo, stla, stlo, periods_all, &
                                   tt, weight, ker_s, chi0, maxshrink, max_sub_niter, &
-                                  updatemax, chi, nsub, status)
+                                  updatemax, chi, nsub, status, precision)
     integer, intent(in) :: dim
     real(c_double), intent(in), target, contiguous :: xgrids(:), ygrids(:), periods(:)
     real(c_double), intent(inout), target, contiguous :: svel(:,:,:)
@@ -165,11 +169,12 @@ contains
     real(c_double), intent(out) :: chi
     integer, intent(out) :: nsub
     integer, intent(out), optional :: status
+    integer, intent(in), optional :: precision   ! SATT_PREC_FP64 (default) or SATT_PREC_MIXED
 
     type(host_batch), target :: hb
     real(c_double), allocatable :: svel_out(:,:,:)
     type(c_ptr) :: plan
-    integer(c_int) :: st, ns
+    integer(c_int) :: st, ns, prec
 
     call build_batch(dim, xgrids, ygrids, periods, svel, m11, m12, m22, evtname, staname, &
                      evla, evlo, stla, stlo, periods_all, tt, weight, hb)
@@ -177,7 +182,9 @@ contains
     chi = chi0
     nsub = 0
     plan = c_null_ptr
-    st = satt_plan_create(satt_host_ctx(), hb%b, SATT_PREC_FP64, plan)
+    prec = SATT_PREC_FP64
+    if (present(precision)) prec = int(precision, c_int)
+    st = satt_plan_create(satt_host_ctx(), hb%b, prec, plan)
     if (st >= 0) then
       st = satt_plan_linesearch(plan, svel, ker_s, chi0, maxshrink, &
                                 int(max_sub_niter, c_int), updatemax, svel_out, chi, ns)

@@ -64,7 +64,7 @@ def test_dropin_solvers_called_like_the_reference(ctx, golden, name):
     assert np.array_equal(Ta, O.mask(c["xx"], c["yy"], Tac, c["x0"], c["y0"]))
 
 
-def fhost_forward_simulate(b, paths, dim):
+def fhost_forward_simulate(b, paths, dim, precision=0):
     lib = ctypes.CDLL(FHOST)
     ids = lambda names: np.array([int(s[1:]) for s in names], dtype=np.int32)
     npd, nx, ny = b["nperiod"], b["nx"], b["ny"]
@@ -78,7 +78,7 @@ def fhost_forward_simulate(b, paths, dim):
         dim, nx, ny, npd, _d(b["xgrids"]), _d(b["ygrids"]), _d(c(b["periods"])), _d(b["svel"]),
         _d(b["m11"]), _d(b["m12"]), _d(b["m22"]), npath, O._i(evt), O._i(sta),
         *(_d(a) for a in args), 1, 1, _d(tt_fwd), ctypes.byref(chi), _d(adj_s), _d(adj_d),
-        ctypes.byref(st))
+        ctypes.byref(st), int(precision))
     return st.value, tt_fwd, chi.value, adj_s, adj_d
 
 
@@ -110,6 +110,10 @@ def test_fortran_forward_simulate_matches_c_abi(ctx, dim):
         assert np.array_equal(adj_d, out["adj_density"])
     ref = O.forward_simulate(b)
     assert rel_err(tt_fwd, ref["tt_fwd"]) < 1e-9
+    # the mixed-precision form through the same host routine
+    st, tt_m, chi_m, adj_m, _ = fhost_forward_simulate(b, paths, dim, satt.PREC_MIXED)
+    out_m = ctx.forward_simulate(b, precision=satt.PREC_MIXED)
+    assert st >= 0 and np.array_equal(tt_m, out_m["tt_fwd"]) and np.array_equal(adj_m, out_m["adj_s"])
 
 
 @pytest.mark.gpu
