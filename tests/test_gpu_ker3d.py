@@ -97,3 +97,26 @@ def test_correct_depth_matches_oracle(ctx):
         np.allclose(got[0][0, 0, 0], ks[0][0, 0, 0], rtol=1e-15)
     with pytest.raises(satt.SattError):
         ctx.correct_depth(*ks, dip + 90.0, depz)
+
+
+def test_projection_edge_grids(ctx):
+    """Inversion nodes on every model node (a tie in every axis at every
+    point), a two-node inversion grid, and a model with no interior point
+    (inv_grid_iso then gives zeros)."""
+    vx, vy, vz = np.arange(9.0), np.arange(7.0), np.arange(5.0)
+    cases = [(vx[:, None].copy(order="F"), vy[:, None].copy(order="F"), vz[:, None].copy(order="F")),
+             (np.asfortranarray([[-1.0], [9.0]]), np.asfortranarray([[0.0], [6.0]]),
+              np.asfortranarray([[-0.5], [4.5]]))]
+    rng = np.random.default_rng(31)
+    adj = np.asfortranarray(rng.standard_normal((9, 7, 5)))
+    for xi, yi, zi in cases:
+        gk = ctx.inv_grid_iso(xi, yi, zi, adj, vx, vy, vz)
+        assert np.array_equal(gk, O.inv_grid_iso(xi, yi, zi, adj, vx, vy, vz))
+        g = rng.standard_normal(gk.shape)
+        assert np.array_equal(ctx.inv2fwd_iso(xi, yi, zi, g, vx, vy, vz),
+                              O.inv2fwd_iso(xi, yi, zi, g, vx, vy, vz))
+    flat = np.asfortranarray(rng.standard_normal((9, 7, 2)))
+    xi, yi, zi = cases[1]
+    zi2 = np.asfortranarray([[-0.5], [1.5]])
+    gk = ctx.inv_grid_iso(xi, yi, zi2, flat, vx, vy, vz[:2])
+    assert np.all(gk == 0) and np.array_equal(gk, O.inv_grid_iso(xi, yi, zi2, flat, vx, vy, vz[:2]))
