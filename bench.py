@@ -390,9 +390,10 @@ def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
 def ker3d_leg(ctx, cpu, n_inv=(8, 9, 8), nset=5, chi0=2.5, weight=1.0):
     """The 3-D stage's kernel steps after the loop on its 41x41x31 model with
     16 periods: combine_kernels through the Vs chain rule (src/acqui.f90:
-    224-238) and post_processing_for_kernel onto 5 inversion-grid sets of
-    8x9x8 nodes (the reference's example 00 settings), with synthetic
-    stacked kernels and depth sensitivities of the right shapes.  The CPU
+    224-238), post_processing_for_kernel onto 5 inversion-grid sets of 8x9x8
+    nodes (the reference's example 00 settings) and correct_depth
+    (src/surfker/surfker.f90:260-283), with synthetic stacked kernels, depth
+    sensitivities and dip angles of the right shapes.  The CPU
     leg runs the oracle's combination and the reference's own inv_grid_iso /
     inv2fwd_iso (oracle/_ref) on one host thread."""
     vs3, depz, per3 = W.acqui3d_model()
@@ -419,6 +420,22 @@ def ker3d_leg(ctx, cpu, n_inv=(8, 9, 8), nset=5, chi0=2.5, weight=1.0):
         p_dev, p_wall = ctx.last_device_ms(), time.perf_counter() - t0
     out["combine"] = {"device_ms": round(c_dev, 4), "wall_ms": round(c_wall * 1e3, 3)}
     out["postproc"] = {"device_ms": round(p_dev, 4), "wall_ms": round(p_wall * 1e3, 3)}
+    # correct_depth (depthkernel with topography) on the depth kernels, dip angles up to 30 deg
+    dip = np.asfortranarray(rng.uniform(0.0, 30.0, (npd, nx, ny)))
+    for rep in range(2):
+        cd = ctx.correct_depth(svs, svp, srho, dip, depz)
+        d_dev = ctx.last_device_ms()
+    out["correct_depth"] = {"device_ms": round(d_dev, 4)}
+    if cpu:
+        import oracle as O
+        t0 = time.perf_counter()
+        wcd = O.correct_depth(svs, svp, srho, dip, depz)
+        d_cpu = time.perf_counter() - t0
+        out["correct_depth"]["max_rel_diff_vs_oracle"] = float(
+            max(np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(cd, wcd)))
+        out["correct_depth"]["cpu_baseline"] = {"value_ms": round(d_cpu * 1e3, 3), "cores": 1,
+                                                "kind": "port", "sample": "the whole step"}
+        out["correct_depth"]["vs_cpu_core"] = round(d_cpu * 1e3 / d_dev, 1)
     if cpu:
         import oracle as O
         t0 = time.perf_counter()
