@@ -135,6 +135,130 @@ __global__ __launch_bounds__(MAXT) void k_smooth_geo2(GeoSmooth P) {
   }
 }
 
+// Two output rows (j, j+1) per workgroup, one cell per thread (nx <= 1024):
+// each staged data row serves both rows, so a node costs three LDS reads for
+// two cells (the two rows' weights and the shared datum) instead of four, and
+// a window row takes one barrier for two output rows.  Each cell's sums keep
+// the single-row kernel's order (window rows ascending, offsets ascending).
+__global__ __launch_bounds__(1024) void k_smooth_geo2_pair(GeoSmooth P) {
+  extern __shared__ double lds[];
+  const int j0 = 2 * blockIdx.x, f = blockIdx.y;
+  const int nx = P.nx, ny = P.ny, tid = threadIdx.x;
+  const bool two = j0 + 1 < ny;
+  const int hx = P.hx[f];
+  const int hy0 = P.hy[(size_t)f * ny + j0], hy1 = two ? P.hy[(size_t)f * ny + j0 + 1] : 0;
+  const int hxe = hx < nx - 1 ? hx : nx - 1;
+  double *w0 = lds, *w1 = lds + nx, *dl = lds + 2 * nx;
+  const double *data = P.data + (P.ndata == 1 ? 0 : (size_t)f * nx * ny);
+  const double sig = P.sigma[f], two_sig_sq = 2.0 * (sig * sig);
+  const int a1 = j0 - hy0 > 0 ? j0 - hy0 : 0, a2 = j0 + hy0 < ny - 1 ? j0 + hy0 : ny - 1;
+  const int b1 = two ? (j0 + 1 - hy1 > 0 ? j0 + 1 - hy1 : 0) : a1;
+  const int b2 = two ? (j0 + 1 + hy1 < ny - 1 ? j0 + 1 + hy1 : ny - 1) : a2;
+  const int n1 = a1 < b1 ? a1 : b1, n2 = a2 > b2 ? a2 : b2;
+  const double c00 = cos(P.lat[j0] * kGeoDeg2Rad);
+  const double c01 = two ? cos(P.lat[j0 + 1] * kGeoDeg2Rad) : c00;
+  const int i = tid;
+  const int k0 = -hx > -i ? -hx : -i, k1 = hx < nx - 1 - i ? hx : nx - 1 - i;
+  double s0 = 0.0, t0 = 0.0, s1 = 0.0, t1 = 0.0;
+  for (int jj = n1; jj <= n2; ++jj) {
+    const bool in0 = jj >= a1 && jj <= a2, in1 = two && jj >= b1 && jj <= b2;
+    const double c1 = cos(P.lat[jj] * kGeoDeg2Rad);
+    // the weights of both rows: lanes 0..hxe for row j0, the next hxe+1 for j0+1
+    for (int q = tid; q <= 2 * hxe + 1; q += blockDim.x) {
+      const int r = q > hxe, k = r ? q - hxe - 1 : q;
+      if (r ? !in1 : !in0) continue;
+      const double latj = P.lat[j0 + r], cj = r ? c01 : c00;
+      const double sl = sin((latj - P.lat[jj]) * kGeoDeg2Rad * 0.5);
+      const double a = sl * sl + P.s2[k] * cj * c1;
+      const double deg = 2.0 * atan2(sqrt(a), sqrt(1.0 - a));
+      const double delta = kGeoRadius * deg * kGeoKm2Deg;
+      (r ? w1 : w0)[k] = exp(-(delta * delta) / two_sig_sq);
+    }
+    const double *row = data + (size_t)jj * nx;
+    for (int q = tid; q < nx; q += blockDim.x) dl[q] = row[q];
+    __syncthreads();
+    if (i < nx) {
+      const double *dp = dl + i;
+      if (in0 && in1) {
+        // four nodes' loads (twelve LDS reads) are issued before their sums
+        int k = k0;
+        for (; k + 4 <= 0; k += 4) {
+          double d[4], x0[4], x1[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            d[u] = dp[k + u];
+            x0[u] = w0[-(k + u)];
+            x1[u] = w1[-(k + u)];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            s0 += x0[u] * d[u];
+            t0 += x0[u];
+            s1 += x1[u] * d[u];
+            t1 += x1[u];
+          }
+        }
+        for (; k < 0; ++k) {
+          const double d = dp[k], x0 = w0[-k], x1 = w1[-k];
+          s0 += x0 * d;
+          t0 += x0;
+          s1 += x1 * d;
+          t1 += x1;
+        }
+        for (; k + 4 <= k1 + 1; k += 4) {
+          double d[4], x0[4], x1[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            d[u] = dp[k + u];
+            x0[u] = w0[k + u];
+            x1[u] = w1[k + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            s0 += x0[u] * d[u];
+            t0 += x0[u];
+            s1 += x1[u] * d[u];
+            t1 += x1[u];
+          }
+        }
+        for (; k <= k1; ++k) {
+          const double d = dp[k], x0 = w0[k], x1 = w1[k];
+          s0 += x0 * d;
+          t0 += x0;
+          s1 += x1 * d;
+          t1 += x1;
+        }
+      } else {
+        const double *w = in0 ? w0 : w1;
+        double s = in0 ? s0 : s1, t = in0 ? t0 : t1;
+        int k = k0;
+        for (; k < 0; ++k) {
+          const double x = w[-k];
+          s += x * dp[k];
+          t += x;
+        }
+        for (; k <= k1; ++k) {
+          const double x = w[k];
+          s += x * dp[k];
+          t += x;
+        }
+        if (in0) {
+          s0 = s;
+          t0 = t;
+        } else {
+          s1 = s;
+          t1 = t;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (i < nx) {
+    P.out[(size_t)f * nx * ny + (size_t)j0 * nx + i] = (0.0 + s0) / t0;
+    if (two) P.out[(size_t)f * nx * ny + (size_t)(j0 + 1) * nx + i] = (0.0 + s1) / t1;
+  }
+}
+
 // gradient_2_geo of field p (natural layout) at cell k; cosd of each row
 // comes from the host.
 __device__ __forceinline__ void grad_geo_cell(const GeoGrad &G, const double *f, int i, int j,
@@ -264,8 +388,9 @@ void launch_geo_s2(hipStream_t s, const double *lon, int nx, double *s2) {
 void launch_smooth_geo2(hipStream_t s, const GeoSmooth &P, int nfield) {
   const dim3 grid((unsigned)P.ny, (unsigned)nfield);
   const size_t lds = smooth_lds_bytes(P.nx);
-  if (P.nx <= 1024) {  // one cell per thread
-    hipLaunchKernelGGL((k_smooth_geo2<1, 1024>), grid, dim3((P.nx + 63) / 64 * 64), lds, s, P);
+  if (P.nx <= 1024) {  // one cell per thread, two output rows per workgroup
+    hipLaunchKernelGGL(k_smooth_geo2_pair, dim3((unsigned)((P.ny + 1) / 2), (unsigned)nfield),
+                       dim3((P.nx + 63) / 64 * 64), 3 * (size_t)P.nx * sizeof(double), s, P);
   } else if (P.nx <= 2048) {
     hipLaunchKernelGGL((k_smooth_geo2<4, 512>), grid, dim3(512), lds, s, P);
   } else {  // nx <= 4096 (SATT_GEO_MAX_NX)
