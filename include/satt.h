@@ -87,6 +87,12 @@ extern "C" {
 typedef struct satt_ctx satt_ctx;
 typedef struct satt_plan satt_plan;
 
+/* The batched form, the plans and the server take grids of at most
+ * SATT_MAX_NY rows (latitudes): a solve passes two columns of FP64 values per
+ * row between its strips through the workgroup's 160 KB of LDS.  Larger
+ * grids are SATT_E_ARG. */
+#define SATT_MAX_NY 10240
+
 /* Create a context on HIP device `device` (its own stream). */
 int satt_create(satt_ctx **ctx, int device);
 void satt_destroy(satt_ctx *ctx);

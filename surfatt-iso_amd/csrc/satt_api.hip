@@ -1116,6 +1116,9 @@ int check_batch(satt_ctx *ctx, const satt_batch *B, int k) {
       B->nitems < 0 || !B->xgrids || !B->ygrids || !B->svel || !B->m11 || !B->m12 || !B->m22 ||
       (B->nitems > 0 && (!B->item_pidx || !B->item_srcx || !B->item_srcy || !B->rec_off)))
     return fail(ctx, SATT_E_ARG, "batch %d: bad arguments", k);
+  if (B->ny > SATT_MAX_NY)
+    return fail(ctx, SATT_E_ARG, "batch %d: %d rows, more than SATT_MAX_NY (%d)", k, B->ny,
+                SATT_MAX_NY);
   const int nrec = B->nitems > 0 ? B->rec_off[B->nitems] : 0;
   if (nrec < 0 || (nrec > 0 && (!B->recx || !B->recy || !B->tt || !B->weight || !B->idx_path)))
     return fail(ctx, SATT_E_ARG, "batch %d: bad receiver gather", k);

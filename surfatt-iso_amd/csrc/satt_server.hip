@@ -126,7 +126,7 @@ struct Batch {
   int stream = 0;
   std::vector<Req *> reqs;
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
-  size_t desc_dev = 0, desc_pin = 0, desc_bytes = 0;
+  size_t desc_dev = 0, desc_pin = 0;  // the stream's descriptor area
   bool failed = false;
 };
 
@@ -139,6 +139,10 @@ struct satt_server {
   char *dev = nullptr;     // device pool
   char *pin = nullptr;     // pinned staging pool
   size_t dev_cap = 0, pin_cap = 0;
+  // Each stream owns a descriptor area of max_batch descriptors, carved from
+  // both pools at creation, so that admitting a group needs only the
+  // requests' own blocks; dev_avail/pin_avail are what requests can use.
+  size_t desc_stride = 0, desc_dev0 = 0, desc_pin0 = 0, dev_avail = 0, pin_avail = 0;
   Pool dev_pool, pin_pool;
   std::vector<hipStream_t> streams;
   std::vector<bool> stream_busy;
@@ -292,6 +296,10 @@ bool satt_server::launch(Batch &b) {
     const size_t N = (size_t)r->q.nx * r->q.ny;
     // tekio (nr) | item record (32) | kernel (N) | T (N, when asked for)
     const Layout &L = r->L;
+    // until the copy below lands, the request's record reads as a HIP error
+    double *po_item = reinterpret_cast<double *>(po + (size_t)r->q.nrec * 8);
+    std::memset(po_item, 0, 32 * 8);
+    *reinterpret_cast<int *>(po_item + 4) = SATT_E_HIP;
     if (!hip_ok(hipMemcpyAsync(po, blk + L.rec_out, (size_t)r->q.nrec * 8,
                                hipMemcpyDeviceToHost, s)) ||
         !hip_ok(hipMemcpyAsync(po + (size_t)r->q.nrec * 8, blk + L.in_item, 32 * 8,
@@ -310,10 +318,10 @@ bool satt_server::launch(Batch &b) {
 // Results of a finished group into the callers' responses.
 void satt_server::finish(Batch &b) {
   float ms = 0;
-  if (!b.failed && hipEventElapsedTime(&ms, b.ev_start, b.ev_end) == hipSuccess)
-    st.device_busy_ms += ms;
+  const bool timed = !b.failed && hipEventElapsedTime(&ms, b.ev_start, b.ev_end) == hipSuccess;
   const Clock::time_point now = Clock::now();
   std::lock_guard<std::mutex> lk(mu);
+  if (timed) st.device_busy_ms += ms;
   for (Req *r : b.reqs) {
     const char *po = pin + r->pin_out;
     const int nr = r->q.nrec;
@@ -343,8 +351,6 @@ void satt_server::finish(Batch &b) {
     in_device--;
     st.completed++;
   }
-  dev_pool.release(b.desc_dev, b.desc_bytes);
-  pin_pool.release(b.desc_pin, b.desc_bytes);
   stream_busy[b.stream] = false;
   spare_events.push_back(b.ev_start);
   spare_events.push_back(b.ev_end);
@@ -373,6 +379,9 @@ void satt_server::run() {
       const hipError_t q = hipEventQuery(flying[k].ev_end);
       if (q == hipErrorNotReady) { ++k; continue; }
       if (q != hipSuccess) {
+        // the group's outputs were never copied back: every request in it
+        // reports the error instead of the staging pool's stale contents
+        flying[k].failed = true;
         std::lock_guard<std::mutex> lk(mu);
         hip_error = SATT_E_HIP;
       }
@@ -393,15 +402,11 @@ void satt_server::run() {
       if (si < 0) break;
       Batch b;
       b.stream = si;
+      b.desc_dev = desc_dev0 + (size_t)si * desc_stride;
+      b.desc_pin = desc_pin0 + (size_t)si * desc_stride;
       {
         std::lock_guard<std::mutex> lk(mu);
         const size_t most = std::min<size_t>(wq.size(), (size_t)max_batch);
-        b.desc_bytes = most * sizeof(SlotDesc);
-        if (!dev_pool.alloc(b.desc_bytes, &b.desc_dev)) break;
-        if (!pin_pool.alloc(b.desc_bytes, &b.desc_pin)) {
-          dev_pool.release(b.desc_dev, b.desc_bytes);
-          break;
-        }
         while (!wq.empty() && b.reqs.size() < most) {
           Req *r = wq.front();
           size_t d, pi, po;
@@ -422,11 +427,7 @@ void satt_server::run() {
           b.reqs.push_back(r);
           wq.pop_front();
         }
-        if (b.reqs.empty()) {
-          dev_pool.release(b.desc_dev, b.desc_bytes);
-          pin_pool.release(b.desc_pin, b.desc_bytes);
-          break;
-        }
+        if (b.reqs.empty()) break;
         in_device += (long long)b.reqs.size();
         st.max_in_flight = std::max(st.max_in_flight, in_device);
         st.launches++;
@@ -489,6 +490,16 @@ extern "C" int satt_server_create(satt_ctx *ctx, const satt_server_config *cfg,
   }
   S->dev_pool.reset(S->dev_cap);
   S->pin_pool.reset(S->pin_cap);
+  S->desc_stride = ((size_t)S->max_batch * sizeof(SlotDesc) + 255) & ~(size_t)255;
+  const size_t desc_all = (size_t)S->nstreams * S->desc_stride;
+  if (desc_all * 4 > S->dev_cap || desc_all * 4 > S->pin_cap ||
+      !S->dev_pool.alloc(desc_all, &S->desc_dev0) || !S->pin_pool.alloc(desc_all, &S->desc_pin0)) {
+    (void)hipFree(S->dev);
+    (void)hipHostFree(S->pin);
+    return SATT_E_ARG;  // a pool too small for its own descriptors
+  }
+  S->dev_avail = S->dev_cap - desc_all;
+  S->pin_avail = S->pin_cap - desc_all;
   S->streams.resize(S->nstreams);
   S->stream_busy.assign(S->nstreams, false);
   for (auto &st : S->streams)
@@ -503,7 +514,7 @@ extern "C" int satt_server_submit(satt_server *S, const satt_request *q, satt_re
                                   long long *ticket) {
   if (!S || !q || !resp) return SATT_E_ARG;
   const bool no_metric = !q->m11 && !q->m12 && !q->m22;
-  if (q->nx < 3 || q->ny < 3 || !q->xgrids || !q->ygrids || !q->svel || q->nrec < 0 ||
+  if (q->nx < 3 || q->ny < 3 || q->ny > SATT_MAX_NY || !q->xgrids || !q->ygrids || !q->svel || q->nrec < 0 ||
       (!no_metric && (!q->m11 || !q->m12 || !q->m22)) ||
       (q->nrec > 0 && (!q->recx || !q->recy || !q->tt || !q->weight)))
     return SATT_E_ARG;
@@ -541,7 +552,7 @@ extern "C" int satt_server_submit(satt_server *S, const satt_request *q, satt_re
   resp->status = 0;
   std::lock_guard<std::mutex> lk(S->mu);
   if (S->hip_error) return S->hip_error;
-  if ((r->L.total + 32) * 8 > S->dev_cap || (r->L.in_end * 8 + r->out_bytes) > S->pin_cap / 2)
+  if (r->L.total * 8 > S->dev_avail || (r->L.in_end * 8 + r->out_bytes) > S->pin_avail / 2)
     return SATT_E_ARG;  // larger than the pool
   r->ticket = S->next_ticket++;
   if (ticket) *ticket = r->ticket;

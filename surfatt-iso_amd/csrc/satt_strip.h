@@ -1,6 +1,7 @@
 // Strip-pipeline kernels (satt_strip.hip): geometry of the skewed layouts
 // and the launch entry points used by the batched plan.
 #pragma once
+#include "../../include/satt.h"
 #include "satt_internal.h"
 
 namespace satt {
@@ -64,6 +65,8 @@ struct GroupDims {
 // wave per item is the throughput form when items fill the device.
 constexpr int kMaxTeam = 8;
 constexpr size_t kTeamLdsMax = 160 * 1024;  // a gfx950 workgroup's LDS
+// one wave's two FP64 edge channels of SATT_MAX_NY rows fill it
+static_assert(2 * 8 * (size_t)SATT_MAX_NY <= kTeamLdsMax, "SATT_MAX_NY exceeds the LDS");
 // The largest team (<= want) of a sweep stage whose LDS (edge channels of
 // the stage's value type) fits for grids up to ny rows, ns strips.
 int team_waves(int want, int ny, int ns, int stage);
