@@ -574,3 +574,54 @@ def fwdsurf1d(ref, vs, iwave, igr, periods, depz):
     thk = np.zeros(len(depz), dtype=np.float32)
     thk[:-1] = (depz[1:] - depz[:-1]).astype(np.float32)
     return ref.surfdisp96(thk, vp, vs32, rho, iwave, igr, periods)
+
+
+# ---- Rayleigh-wave depth kernels (depthkernel1d, src/surfker/surfker.f90:70-103)
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def refinegrid(vel, depz):
+    """depthkernel1d's layered model: REAL(4) Vs, get_vprho and refinegrid
+    (src/surfker/surfker.f90:285-321); (thk, vp, vs, rho), nz + 1 layers."""
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    depz = np.ascontiguousarray(depz, dtype=np.float64)
+    nz = len(vel)
+    out = [np.zeros(nz + 1, dtype=np.float32) for _ in range(4)]
+    lib().orc_refinegrid(_d(vel), _d(depz), nz, *(o.ctypes.data_as(_fp) for o in out))
+    return tuple(out)
+
+
+def sregn96(thk, vp, vs, rho, t, cp, iflsph=1):
+    """CPS sregn96 (src/surfker/sregn96.f90:1630-1731) on one REAL(4) layered
+    model at period t and phase velocity cp: the spherical phase and group
+    velocities, the eigenfunctions and dc/dVp, dc/dVs, dc/dh, dc/drho per
+    layer."""
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    thk, vp, vs, rho = f(thk), f(vp), f(vs), f(rho)
+    n = len(thk)
+    o = {k: np.zeros(n) for k in ("ur", "uz", "tr", "tz", "dcda", "dcdb", "dcdh", "dcdr")}
+    c, cg = ctypes.c_double(cp), ctypes.c_double(0)
+    lib().orc_sregn96(*(a.ctypes.data_as(_fp) for a in (thk, vp, vs, rho)), n,
+                      ctypes.c_double(t), ctypes.byref(c), ctypes.byref(cg),
+                      *(_d(o[k]) for k in ("ur", "uz", "tr", "tz", "dcda", "dcdb", "dcdh", "dcdr")),
+                      int(iflsph))
+    o["cp"], o["cg"] = c.value, cg.value
+    return o
+
+
+def depthkernel_rayleigh(vel, depz, periods, cp):
+    """The Rayleigh phase branch of depthkernel1d for one column, given the
+    phase velocities surfdisp96 finds on the refinegrid model: sen_vs,
+    sen_vp, sen_rho of shape (kmax, nz)."""
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    depz = np.ascontiguousarray(depz, dtype=np.float64)
+    t = np.ascontiguousarray(periods, dtype=np.float64)
+    cp = np.ascontiguousarray(cp, dtype=np.float64)
+    nz, km = len(vel), len(t)
+    out = [np.zeros((km, nz), order="F") for _ in range(3)]
+    st = lib().orc_depthkernel_rayleigh(_d(vel), _d(depz), nz, _d(t), _d(cp), km,
+                                        *(_d(o) for o in out))
+    if st != 0:
+        raise OracleError(st, "orc_depthkernel_rayleigh")
+    return tuple(out)
