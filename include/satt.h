@@ -121,6 +121,22 @@ int satt_adj_lonlat(satt_ctx *ctx, const double *xx_deg, const double *yy_deg,
                     const double *yrec_deg, const double *src_adj, int nr,
                     double *Ta, int *iters);
 
+/* satt_fwd_lonlat without any device allocation: the same kernels on the same
+ * operands, hence the same T bit for bit.  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its work
+ * space is one static device array per GPU (about 28 MB, held until the
+ * process ends), so grids are limited to SATT_NOALLOC_MAX_CELLS cells and
+ * SATT_NOALLOC_MAX_AXIS nodes per axis (larger ones are SATT_E_ARG).  All
+ * calls on one GPU share that array: calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller. */
+#define SATT_NOALLOC_MAX_CELLS 262144 /* 512 x 512 */
+#define SATT_NOALLOC_MAX_AXIS 8192
+int satt_fwd_lonlat_noalloc(satt_ctx *ctx, const double *xx_deg, const double *yy_deg,
+                            int nx, int ny, const double *spha, const double *sphb,
+                            const double *sphc, const double *fun, double x0_deg,
+                            double y0_deg, double *T, int *iters);
+
 /* mask(xx,yy,nx,ny,TableAdj,x0,y0)  src/eikon_solver_2d.f90:1449 (in place). */
 int satt_mask(satt_ctx *ctx, const double *xx, const double *yy, int nx, int ny,
               double *table, double x0, double y0);

@@ -164,6 +164,16 @@ const char *status_text(int st) {
 // ------------------------------------------------------------------ context
 
 int satt::ctx_device(const satt_ctx *ctx) { return ctx->device; }
+hipStream_t satt::ctx_stream(const satt_ctx *ctx) { return ctx->stream; }
+int satt::ctx_fail(satt_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
 
 extern "C" const char *satt_version(void) { return "satt 0.1 (gfx950)"; }
 
@@ -177,6 +187,8 @@ extern "C" int satt_create(satt_ctx **out, int device) {
   }
   if (device < 0 || device >= n) return SATT_E_ARG;
   if (hipSetDevice(device) != hipSuccess) return SATT_E_HIP;
+  // the allocation-free entry point's scratch address on this device
+  if (noalloc_init(device) != SATT_OK) return SATT_E_HIP;
   satt_ctx *ctx = new satt_ctx();
   ctx->device = device;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -8,8 +8,15 @@ struct satt_ctx;
 
 namespace satt {
 
-// The device of a context (satt_api.hip).
+// The device and the stream of a context, and recording a failure (its
+// message and code) on it (satt_api.hip).
 int ctx_device(const satt_ctx *ctx);
+hipStream_t ctx_stream(const satt_ctx *ctx);
+int ctx_fail(satt_ctx *ctx, int code, const char *fmt, ...);
+
+// Looks up the allocation-free entry point's scratch address on `device`,
+// which must be current (satt_noalloc.hip); once per device, thread-safe.
+int noalloc_init(int device);
 
 // Key of the context's tuning table (satt_tune.hip): strips, rows rounded
 // up to 32, the launch group's item count rounded up to a power of two,

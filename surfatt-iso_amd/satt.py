@@ -26,6 +26,7 @@ E_DISPERSION = -8
 PREC_FP64, PREC_MIXED = 0, 1
 LAUNCH_BATCHED, LAUNCH_PER_SOLVE = 0, 1
 FIELD_T, FIELD_TA, FIELD_TD = 0, 1, 2
+NOALLOC_MAX_CELLS, NOALLOC_MAX_AXIS = 262144, 8192   # satt_fwd_lonlat_noalloc's scratch
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -183,6 +184,19 @@ class Context:
         st = self.lib.satt_fwd_lonlat(self.ptr, _d(xx), _d(yy), sh[0], sh[1], _d(spha), _d(sphb),
                                       _d(sphc), _d(fun), ctypes.c_double(x0), ctypes.c_double(y0),
                                       _d(T), ctypes.byref(it))
+        self._check(st)
+        return T, it.value, st
+
+    def fwd_noalloc(self, xx, yy, spha, sphb, sphc, fun, x0, y0):
+        """satt_fwd_lonlat_noalloc (no device allocation) -> (T, iters, status)."""
+        xx, yy = _c(xx), _c(yy)
+        sh = (len(xx), len(yy))
+        spha, sphb, sphc, fun = (_f(v, sh) for v in (spha, sphb, sphc, fun))
+        T = np.zeros(sh, order="F")
+        it = ctypes.c_int(0)
+        st = self.lib.satt_fwd_lonlat_noalloc(
+            self.ptr, _d(xx), _d(yy), sh[0], sh[1], _d(spha), _d(sphb), _d(sphc), _d(fun),
+            ctypes.c_double(x0), ctypes.c_double(y0), _d(T), ctypes.byref(it))
         self._check(st)
         return T, it.value, st
 
