@@ -277,6 +277,32 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
             "plain_fp64_per_solve_solves_per_s": None if plain is None else round(plain, 3)}
 
 
+def dropin_leg(ctx, part, prec, calls=3):
+    """satt_forward_simulate as the reference's caller uses it (the Fortran host
+    calls it once per forward_simulate): host arrays in, results back to the
+    host, PCIe both ways.  The first call creates the plan (device memory,
+    uploads, team tuning on its own inputs); later calls with the same
+    gathers and a new model upload only the model (src/tomo2d.f90:104,272)."""
+    n = len(part["item_pidx"])
+    t0 = time.perf_counter()
+    ctx.forward_simulate_dropin(part, precision=prec)
+    first = time.perf_counter() - t0
+    times = []
+    for k in range(calls):
+        b = dict(part, svel=np.asfortranarray(part["svel"] * (1 + 1e-3 * (k + 1))))
+        t1 = time.perf_counter()
+        ctx.forward_simulate_dropin(b, precision=prec)
+        times.append(time.perf_counter() - t1)
+    ctx.forward_simulate_release()
+    steady = float(np.median(times))
+    model_mb = 4 * part["svel"].size * 8 / 1e6
+    out_mb = (part["npath"] + part["svel"].size * (2 if part["dim"] == 3 else 1)) * 8 / 1e6
+    return {"first_call_s": round(first, 3), "steady_s_per_call": round(steady, 4),
+            "steady_solves_per_s": round(n / steady, 1), "calls": calls,
+            "host_to_device_MB_per_call": round(model_mb, 1),
+            "device_to_host_MB_per_call": round(out_mb, 1)}
+
+
 def window_nodes(xg, yg, sigma):
     """Window nodes summed over the cells of one smoothing
     (src/shared/utils.f90:1204-1219): the window is a rectangle per cell, so
@@ -497,6 +523,8 @@ def main():
                     help="run only the model-side leg and print its JSON")
     ap.add_argument("--no-hbm-probe", action="store_true",
                     help="skip the streaming kernels that measure achievable HBM bandwidth")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the end-to-end satt_forward_simulate calls (host arrays, PCIe)")
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -668,6 +696,12 @@ def main():
                                           "tt_max_ms": worst * 1e3, "fp64_tt_max_rel": worst64}
         log(f"[bench] mixed error: {mixed_err}")
 
+    # the drop-in call, host arrays in and out (one grid only)
+    dropin = None
+    if len(parts) == 1 and not args.no_dropin:
+        dropin = dropin_leg(ctx, parts[0], prec)
+        log(f"[bench] drop-in satt_forward_simulate: {dropin}")
+
     # plain FP64 form: one solve per launch, resident inputs, a sample of
     # items spread over the parts in proportion
     plain = None
@@ -740,6 +774,7 @@ def main():
         "team_tuning": one_wave,
         "batched_fp64": None if fp64_value is None else round(fp64_value, 3),
         "mixed_error": mixed_err,
+        "dropin_end_to_end": dropin,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),
         "batched_vs_plain": None if plain is None else round(value / plain, 2),
         "serving": serving,

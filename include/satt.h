@@ -276,9 +276,9 @@ int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_
 
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
  * satt_gradient_geo2, satt_topo_metric, satt_steepest_descent,
- * satt_combine_kernels, satt_inv_grid_iso, satt_inv2fwd_iso,
- * satt_kernel_postproc or satt_correct_depth call on the context (HIP events on its stream;
- * transfers excluded). */
+ * satt_fwdsurf3d, satt_combine_kernels, satt_inv_grid_iso,
+ * satt_inv2fwd_iso, satt_kernel_postproc or satt_correct_depth call on the
+ * context (HIP events on its stream; transfers excluded). */
 double satt_last_device_ms(const satt_ctx *ctx);
 
 /* ------------------------------------------------------------ batched form */
@@ -320,11 +320,21 @@ typedef struct {
                           FP64 mode); `iters` then counts the FP64 polish  */
 } satt_result;
 
-/* One call of forward_simulate(chi, istotable, isadj): upload, run, download.
- * Equivalent to satt_plan_create + satt_plan_run + satt_plan_results. */
+/* One call of forward_simulate(chi, istotable, isadj): upload, run, download,
+ * as satt_plan_create + satt_plan_run + satt_plan_results.  The plan stays
+ * resident in the context: the reference calls forward_simulate once per
+ * iteration and per line-search step with the same gathers and a new model
+ * (src/tomo2d.f90:104,272), so a call whose batch equals the previous one in
+ * everything but svel/m11/m12/m22 (and the precision) only uploads the model
+ * (satt_plan_set_model) before running.  A new plan has the team size of
+ * each sweep stage tuned on its own inputs once (satt_plan_tune_waves).  A
+ * call with other gathers replaces the plan; its device memory is held until
+ * then, satt_forward_simulate_release or satt_destroy. */
 int satt_forward_simulate(satt_ctx *ctx, const satt_batch *batch,
                           int istotable, int isadj, int precision,
                           satt_result *result);
+/* Frees satt_forward_simulate's resident plan (no-op without one). */
+void satt_forward_simulate_release(satt_ctx *ctx);
 
 /* Resident form: the plan keeps the grid, the model, the gathers and every
  * field in device memory.  satt_plan_run is asynchronous on the context's

@@ -10,6 +10,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -25,6 +26,41 @@
 
 using namespace satt;
 
+namespace {
+// Everything of a satt_batch except the model (svel, m11, m12, m22), with the
+// precision: what satt_forward_simulate's resident plan is keyed on.
+struct FsKey {
+  int dim = 0, nx = 0, ny = 0, nperiod = 0, nitems = 0, npath = 0, precision = -1;
+  std::vector<double> axes, items, recs;
+  std::vector<int> ints;
+  static FsKey of(const satt_batch *B, int precision) {
+    FsKey k;
+    k.dim = B->dim; k.nx = B->nx; k.ny = B->ny; k.nperiod = B->nperiod;
+    k.nitems = B->nitems; k.npath = B->npath; k.precision = precision;
+    const int ni = B->nitems > 0 ? B->nitems : 0;
+    const int nr = ni > 0 ? B->rec_off[ni] : 0;
+    k.axes.assign(B->xgrids, B->xgrids + B->nx);
+    k.axes.insert(k.axes.end(), B->ygrids, B->ygrids + B->ny);
+    if (ni) {
+      k.items.assign(B->item_srcx, B->item_srcx + ni);
+      k.items.insert(k.items.end(), B->item_srcy, B->item_srcy + ni);
+      k.ints.assign(B->item_pidx, B->item_pidx + ni);
+      k.ints.insert(k.ints.end(), B->rec_off, B->rec_off + ni + 1);
+    }
+    if (nr) {
+      for (const double *a : {B->recx, B->recy, B->tt, B->weight}) k.recs.insert(k.recs.end(), a, a + nr);
+      k.ints.insert(k.ints.end(), B->idx_path, B->idx_path + nr);
+    }
+    return k;
+  }
+  bool operator==(const FsKey &o) const {
+    return dim == o.dim && nx == o.nx && ny == o.ny && nperiod == o.nperiod &&
+           nitems == o.nitems && npath == o.npath && precision == o.precision &&
+           axes == o.axes && items == o.items && recs == o.recs && ints == o.ints;
+  }
+};
+}  // namespace
+
 struct satt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -32,6 +68,12 @@ struct satt_ctx {
   std::mutex tune_mu;
   std::map<TuneKey, TuneWaves> tuned;  // team sizes measured by satt_tune_waves
   double last_ms = 0;            // device time of the last model-side call
+  // satt_forward_simulate's resident plan and what it was made from: the
+  // reference calls forward_simulate once per iteration and per line-search
+  // step with the same gathers and a new model (src/tomo2d.f90:104,272), so
+  // a call whose batch differs only in the model refreshes it in place
+  satt_plan *fs_plan = nullptr;
+  FsKey fs_key;
 };
 
 bool satt::ctx_tune_lookup(satt_ctx *ctx, const TuneKey &k, TuneWaves &out) {
@@ -201,6 +243,7 @@ extern "C" int satt_create(satt_ctx **out, int device) {
 
 extern "C" void satt_destroy(satt_ctx *ctx) {
   if (!ctx) return;
+  satt_forward_simulate_release(ctx);
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1289,9 +1332,56 @@ SlotDesc make_desc(const Part &Q, int k, double *base, int slot_skews, int mixed
   return D;
 }
 
+// XCD-aware slot order of one launch group.  The device deals workgroups
+// to its kXcds chiplets round robin (workgroup w to XCD w mod kXcds), and each
+// chiplet has its own L2.  Items of one period read the same per-period
+// fields (metric, slowness and the one-sided constants: six of the ten
+// fields a forward step loads), so the slots are ordered to give every XCD
+// a contiguous run of periods: each grid's items (the grids arrive most
+// expensive first) are sorted by period and cut into kXcds balanced runs,
+// run x goes to XCD x, and slot w takes the next item of XCD w mod kXcds.
+// Items that run together on one XCD then stream the same rows of their
+// period's fields through its L2.  Every item still has its own slot and
+// scratch, so the order changes no result.
+constexpr int kXcds = 8;
+void xcd_order(const satt_plan *P, std::vector<std::pair<int, int>> &slots) {
+  const size_t n = slots.size();
+  if (n < 2 * (size_t)kXcds) return;
+  std::vector<std::deque<std::pair<int, int>>> q(kXcds);
+  int rot = 0;  // where the next grid's remainder starts, so that it spreads
+  for (size_t a = 0; a < n;) {
+    size_t b = a;
+    while (b < n && slots[b].first == slots[a].first) ++b;
+    std::vector<std::pair<int, int>> run(slots.begin() + a, slots.begin() + b);
+    const Part &Q = *P->parts[run[0].first];
+    std::stable_sort(run.begin(), run.end(),
+                     [&](const std::pair<int, int> &u, const std::pair<int, int> &v) {
+                       return Q.h_pidx[u.second] < Q.h_pidx[v.second];
+                     });
+    const size_t m = run.size(), rem = m % kXcds;
+    size_t at = 0;
+    for (int k = 0; k < kXcds; ++k) {
+      const int x = (rot + k) % kXcds;
+      const size_t cnt = m / kXcds + ((size_t)k < rem ? 1 : 0);
+      for (size_t c = 0; c < cnt; ++c) q[x].push_back(run[at++]);
+    }
+    rot = (int)((rot + rem) % kXcds);
+    a = b;
+  }
+  for (size_t w = 0; w < n; ++w) {
+    int x = (int)(w % kXcds);
+    if (q[x].empty())  // the queues differ by a few items at the end
+      for (int y = 0; y < kXcds; ++y)
+        if (q[y].size() > q[x].size()) x = y;
+    slots[w] = q[x].front();
+    q[x].pop_front();
+  }
+}
+
 // Launch groups: the slots in order (one grid: item order; several grids:
 // the most expensive grid first, each grid's items in item order), as many
-// per group as the scratch budget holds.
+// per group as the scratch budget holds, then each group's slots in the
+// XCD-aware order above.
 int plan_groups(satt_plan *P) {
   satt_ctx *ctx = P->ctx;
   const bool mixed = P->precision == SATT_PREC_MIXED;
@@ -1324,6 +1414,7 @@ int plan_groups(satt_plan *P) {
       G.slots.push_back(order[a++]);
     }
     biggest = std::max(biggest, used);
+    xcd_order(P, G.slots);
     P->groups.push_back(std::move(G));
   }
   if (P->strip.alloc(std::max<size_t>(biggest, 1) * 8) != hipSuccess)
@@ -1361,8 +1452,11 @@ int plan_groups(satt_plan *P) {
       }
       slots.assign(off[Q.np], 0);
       std::vector<int> fill(off.begin(), off.end() - 1);
-      for (size_t s = 0; s < G.slots.size(); ++s)  // item order within each period
-        if (G.slots[s].first == (int)q) slots[fill[Q.h_pidx[G.slots[s].second]]++] = (int)s;
+      std::vector<std::pair<int, int>> by_item;  // (item, slot): item order within each period
+      for (size_t s = 0; s < G.slots.size(); ++s)
+        if (G.slots[s].first == (int)q) by_item.push_back({G.slots[s].second, (int)s});
+      std::sort(by_item.begin(), by_item.end());
+      for (const auto &is : by_item) slots[fill[Q.h_pidx[is.first]]++] = is.second;
       list_at.push_back(hl.size());
       hl.insert(hl.end(), off.begin(), off.end());
       hl.insert(hl.end(), slots.begin(), slots.end());
@@ -1950,13 +2044,38 @@ extern "C" void satt_plan_destroy(satt_plan *P) {
   delete P;
 }
 
+extern "C" void satt_forward_simulate_release(satt_ctx *ctx) {
+  if (!ctx || !ctx->fs_plan) return;
+  satt_plan_destroy(ctx->fs_plan);
+  ctx->fs_plan = nullptr;
+  ctx->fs_key = FsKey{};
+}
+
 extern "C" int satt_forward_simulate(satt_ctx *ctx, const satt_batch *B, int istotable,
                                      int isadj, int precision, satt_result *R) {
-  satt_plan *P = nullptr;
-  int st = satt_plan_create(ctx, B, precision, &P);
-  if (st < 0) return st;
-  st = satt_plan_run(P, istotable, isadj, SATT_LAUNCH_BATCHED);
-  if (st >= 0) st = satt_plan_results(P, R);
-  satt_plan_destroy(P);
+  if (!ctx) return SATT_E_ARG;
+  TRY(check_batch(ctx, B, 0));
+  if (precision != SATT_PREC_FP64 && precision != SATT_PREC_MIXED)
+    return fail(ctx, SATT_E_ARG, "satt_forward_simulate: unknown precision %d", precision);
+  FsKey key = FsKey::of(B, precision);
+  int st;
+  if (ctx->fs_plan && ctx->fs_key == key) {
+    // the same gathers: only the model is new
+    TRY(satt_plan_set_model(ctx->fs_plan, B->svel, B->m11, B->m12, B->m22));
+  } else {
+    satt_forward_simulate_release(ctx);  // the old plan's memory goes first
+    satt_plan *P = nullptr;
+    TRY(satt_plan_create(ctx, B, precision, &P));
+    // the team size of each sweep stage, tuned once on the plan's own inputs
+    satt_tune_result tr{};
+    if ((st = satt_plan_tune_waves(P, &tr)) < 0) {
+      satt_plan_destroy(P);
+      return st;
+    }
+    ctx->fs_plan = P;
+    ctx->fs_key = std::move(key);
+  }
+  st = satt_plan_run(ctx->fs_plan, istotable, isadj, SATT_LAUNCH_BATCHED);
+  if (st >= 0) st = satt_plan_results(ctx->fs_plan, R);
   return st;
 }

@@ -96,6 +96,7 @@ def load(path=LIB_PATH):
         fn.restype = ctypes.c_int
     L.satt_destroy.restype = None
     L.satt_plan_destroy.restype = None
+    L.satt_forward_simulate_release.restype = None
     L.satt_server_destroy.restype = None
     L.satt_last_error.restype = ctypes.c_char_p
     L.satt_version.restype = ctypes.c_char_p
@@ -436,6 +437,44 @@ class Context:
             p.run(istotable, isadj)
             return p.results()
 
+    def forward_simulate_dropin(self, batch, istotable=True, isadj=True, precision=PREC_FP64):
+        """satt_forward_simulate, the drop-in call (its plan stays resident in
+        the context for the next call with the same gathers)."""
+        out, r = _result_arrays(batch)
+        st = self.lib.satt_forward_simulate(self.ptr, ctypes.byref(_batch_struct(batch)),
+                                            int(istotable), int(isadj), int(precision),
+                                            ctypes.byref(r))
+        self._check(st)
+        out["status"] = st
+        out["chi"] = float(out["chi"][0])
+        return out
+
+    def forward_simulate_release(self):
+        self.lib.satt_forward_simulate_release(self.ptr)
+
+
+def _result_arrays(b):
+    """Output arrays of one part's results and the satt_result pointing at them."""
+    nit, npd, nx, ny = len(b["item_pidx"]), b["nperiod"], b["nx"], b["ny"]
+    out = {
+        "tt_fwd": np.zeros(b["npath"]),
+        "chi": np.zeros(1),
+        "adj_s": np.zeros((npd, nx, ny), order="F"),
+        "adj_density": np.zeros((npd, nx, ny), order="F") if b["dim"] == 3 else None,
+        "item_chi": np.zeros(nit),
+        "iters": np.zeros((nit, 3), dtype=np.int32),
+        "iters_fp32": np.zeros((nit, 3), dtype=np.int32),
+        "item_status": np.zeros(nit, dtype=np.int32),
+        "weight_out": np.zeros(len(b["recx"])),
+    }
+    r = Result()
+    r.tt_fwd, r.chi, r.adj_s = _d(out["tt_fwd"]), _d(out["chi"]), _d(out["adj_s"])
+    r.adj_density = _d(out["adj_density"])
+    r.item_chi, r.iters, r.item_status = _d(out["item_chi"]), _i(out["iters"]), _i(out["item_status"])
+    r.weight_out = _d(out["weight_out"])
+    r.iters_fp32 = _i(out["iters_fp32"])
+    return out, r
+
 
 def _batch_struct(b):
     s = Batch()
@@ -529,25 +568,7 @@ class Plan:
         self.ctx._check(self.lib.satt_plan_wait(self.ptr))
 
     def results(self, part=0):
-        b = self.parts[part]
-        nit, npd, nx, ny = len(b["item_pidx"]), b["nperiod"], b["nx"], b["ny"]
-        out = {
-            "tt_fwd": np.zeros(b["npath"]),
-            "chi": np.zeros(1),
-            "adj_s": np.zeros((npd, nx, ny), order="F"),
-            "adj_density": np.zeros((npd, nx, ny), order="F") if b["dim"] == 3 else None,
-            "item_chi": np.zeros(nit),
-            "iters": np.zeros((nit, 3), dtype=np.int32),
-            "iters_fp32": np.zeros((nit, 3), dtype=np.int32),
-            "item_status": np.zeros(nit, dtype=np.int32),
-            "weight_out": np.zeros(len(b["recx"])),
-        }
-        r = Result()
-        r.tt_fwd, r.chi, r.adj_s = _d(out["tt_fwd"]), _d(out["chi"]), _d(out["adj_s"])
-        r.adj_density = _d(out["adj_density"])
-        r.item_chi, r.iters, r.item_status = _d(out["item_chi"]), _i(out["iters"]), _i(out["item_status"])
-        r.weight_out = _d(out["weight_out"])
-        r.iters_fp32 = _i(out["iters_fp32"])
+        out, r = _result_arrays(self.parts[part])
         st = self.lib.satt_plan_results_part(self.ptr, int(part), ctypes.byref(r))
         out["status"] = st
         out["chi"] = float(out["chi"][0])
