@@ -385,6 +385,37 @@ def model_side(ctx, cpu, sigma_2d=0.2, wavelen_factor=2.5, updatemax=0.02):
                               f"(oracle/_ref) through the oracle's fwdsurf1d, {cdt:.2f} s"}
                 disp[name]["max_rel_diff_vs_reference"] = worst
                 disp[name]["vs_cpu_core"] = round(disp[name]["columns_per_s"] / (len(cols) / cdt), 1)
+    # Rayleigh depth kernels of the same model (depthkernel, before combine_kernels)
+    ctx.depthkernel(vs3, depz, per3)
+    t0 = time.perf_counter()
+    dsv, _, _, dcp = ctx.depthkernel(vs3, depz, per3)
+    dwall = time.perf_counter() - t0
+    ddev = ctx.last_device_ms()
+    disp["rayleigh_depth_kernels"] = {"device_ms": round(ddev, 3), "wall_ms": round(dwall * 1e3, 3),
+                                      "columns_per_s": round(ncol / (ddev * 1e-3), 1)}
+    if cpu:
+        import oracle as O
+        if O.RefSolver.available():
+            ref = O.RefSolver()
+            cols = np.linspace(0, ncol - 1, 40).round().astype(int)
+            t0 = time.perf_counter()
+            worst = 0.0
+            for c in cols:
+                i, j = c % vs3.shape[0], c // vs3.shape[0]
+                thk, vp, vsr, rho = O.refinegrid(vs3[i, j], depz)
+                cpr = ref.surfdisp96(thk, vp, vsr, rho, 2, 0, per3)
+                osv, _, _ = O.depthkernel_rayleigh(vs3[i, j], depz, per3, cpr)
+                worst = max(worst, float(np.max(np.abs(dsv[:, i, j, :] - osv))
+                                         / np.max(np.abs(osv))))
+            cdt = time.perf_counter() - t0
+            disp["rayleigh_depth_kernels"]["cpu_baseline"] = {
+                "value": round(len(cols) / cdt, 1), "unit": "columns/s", "cores": 1, "kind": "port",
+                "sample": f"{len(cols)} of the {ncol} columns: the reference's surfdisp96 "
+                          f"(oracle/_ref) on refinegrid's layers, then the oracle's sregn96, "
+                          f"{cdt:.2f} s"}
+            disp["rayleigh_depth_kernels"]["max_rel_diff_vs_cpu"] = worst
+            disp["rayleigh_depth_kernels"]["vs_cpu_core"] = round(
+                disp["rayleigh_depth_kernels"]["columns_per_s"] / (len(cols) / cdt), 1)
     res["dispersion"] = {"workload": f"{vs3.shape[0]}x{vs3.shape[1]} columns x {vs3.shape[2]} "
                                      f"depths 0-60 km, Vs 3.0-4.5 km/s +-6% checkerboard, "
                                      f"{len(per3)} periods 4-40 s, spherical earth, fundamental mode",

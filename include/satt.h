@@ -32,6 +32,8 @@
  *   satt_kernel_postproc   att_acqui%post_processing_for_kernel
  *                          (src/acqui.f90:247-304)
  *   satt_correct_depth     correct_depth          src/surfker/surfker.f90:260
+ *   satt_depthkernel       depthkernel_decomp / depthkernel1d with sregn96
+ *                          (src/surfker/surfker.f90:37-103)
  *
  * Conventions
  *   - A grid field f(nx,ny) is column-major with x fastest: f[i + nx*j], as
@@ -212,6 +214,26 @@ int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
                    const double *depz, int nperiod, const double *periods,
                    int iwave, int igr, double *svel, int *col_status);
 
+/* The depth sensitivity kernels of att_acqui%depthkernel (src/acqui.f90:
+ * 163-195): depthkernel_decomp / depthkernel1d (src/surfker/surfker.f90:
+ * 37-103) for every column of vs3d(nx,ny,nz) on depths depz(nz): REAL(4) Vs,
+ * Vp and density by Brocher's relations, refinegrid's layers
+ * (:285-307: nz layers and a half-space copy of the last), surfdisp96's
+ * phase velocity at each period and CPS sregn96 (src/surfker/sregn96.f90)
+ * at that velocity with a spherical earth.  sen_vs, sen_vp, sen_rho are
+ * (nperiod, nx, ny, nz), as sen_vsRc/sen_vpRc/sen_rhoRc: dc/dVs, dc/dVp and
+ * dc/drho of each layer.  The Rayleigh phase-velocity branch (iwave 2,
+ * igr 0) is built; Love waves (slegn96) and group velocities (sregnpu,
+ * slegnpu) are SATT_E_ARG.  cphase (nperiod, nx, ny; may be NULL) receives
+ * the phase velocities the kernels were evaluated at (depthkernel1d's cp).
+ * A column whose dispersion search finds no root gets zero kernels and the
+ * call returns SATT_E_DISPERSION; col_status (nx*ny, may be NULL) holds each
+ * column's code.  nz + 1 <= SATT_DISP_MAX_NZ. */
+int satt_depthkernel(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
+                     const double *depz, int nperiod, const double *periods, int iwave,
+                     int igr, double *sen_vs, double *sen_vp, double *sen_rho,
+                     double *cphase, int *col_status);
+
 /* ---- 3-D kernel steps of the att_acqui path (after the stacking) ---- */
 
 /* combine_kernels (src/acqui.f90:197-245): the stacked period kernels
@@ -276,7 +298,7 @@ int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_
 
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
  * satt_gradient_geo2, satt_topo_metric, satt_steepest_descent,
- * satt_fwdsurf3d, satt_combine_kernels, satt_inv_grid_iso,
+ * satt_fwdsurf3d, satt_depthkernel, satt_combine_kernels, satt_inv_grid_iso,
  * satt_inv2fwd_iso, satt_kernel_postproc or satt_correct_depth call on the
  * context (HIP events on its stream; transfers excluded). */
 double satt_last_device_ms(const satt_ctx *ctx);

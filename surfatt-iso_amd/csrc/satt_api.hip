@@ -1007,6 +1007,87 @@ extern "C" int satt_fwdsurf3d(satt_ctx *ctx, const double *vs3d, int nx, int ny,
   return SATT_OK;
 }
 
+extern "C" int satt_depthkernel(satt_ctx *ctx, const double *vs3d, int nx, int ny, int nz,
+                                const double *depz, int nperiod, const double *periods,
+                                int iwave, int igr, double *sen_vs, double *sen_vp,
+                                double *sen_rho, double *cphase, int *col_status) {
+  if (!ctx) return SATT_E_ARG;
+  if (!vs3d || !depz || !periods || !sen_vs || !sen_vp || !sen_rho || nx < 1 || ny < 1 ||
+      nz < 2 || nz + 1 > SATT_DISP_MAX_NZ || nperiod < 1 ||
+      (size_t)nx * ny > (size_t)INT_MAX / (size_t)nperiod)
+    return fail(ctx, SATT_E_ARG, "satt_depthkernel: bad arguments");
+  if (iwave != 2 || igr != 0)
+    return fail(ctx, SATT_E_ARG,
+                "satt_depthkernel: only the Rayleigh phase-velocity kernels are built");
+  for (int k = 0; k < nz; ++k)
+    if (!std::isfinite(depz[k]) || (k > 0 && !(depz[k] > depz[k - 1])))
+      return fail(ctx, SATT_E_ARG, "satt_depthkernel: depths must increase");
+  for (int p = 0; p < nperiod; ++p)
+    if (!(periods[p] > 0) || !std::isfinite(periods[p]))
+      return fail(ctx, SATT_E_ARG, "satt_depthkernel: periods must be positive");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int ncol = nx * ny;
+  const size_t nv = (size_t)ncol * nz, no = (size_t)ncol * nperiod, ns = no * nz;
+  DBuf dvs, dsmall, dcp, dsen, dwork;
+  TRY(upload(ctx, dvs, vs3d, nv * 8));
+  std::vector<double> hs((size_t)nz + nperiod);
+  std::memcpy(hs.data(), depz, nz * 8);
+  std::memcpy(hs.data() + nz, periods, nperiod * 8);
+  TRY(upload(ctx, dsmall, hs.data(), hs.size() * 8));
+  ALLOC(ctx, dcp, no * 8 + (size_t)ncol * sizeof(int));
+  ALLOC(ctx, dsen, 3 * ns * 8);
+  ALLOC(ctx, dwork, depth_work_doubles(no, nz + 1) * 8);
+  // the phase velocities on the refinegrid model (surfdisp96, fwdsurf's
+  // kernel with refinegrid's layers), then sregn96 per (column, period)
+  DispJob J{};
+  J.ncol = ncol;
+  J.nz = nz;
+  J.kmax = nperiod;
+  J.iwave = 2;
+  J.igr = 0;
+  J.refine = 1;
+  J.vs = dvs.as<double>();
+  J.depz = dsmall.as<double>();
+  J.periods = dsmall.as<double>() + nz;
+  J.out = dcp.as<double>();
+  J.status = reinterpret_cast<int *>(dcp.as<double>() + no);
+  DepthJob K{};
+  K.ncol = ncol;
+  K.nz = nz;
+  K.kmax = nperiod;
+  K.vs = J.vs;
+  K.depz = J.depz;
+  K.periods = J.periods;
+  K.cp = J.out;
+  K.col_status = J.status;
+  K.sen_vs = dsen.as<double>();
+  K.sen_vp = K.sen_vs + ns;
+  K.sen_rho = K.sen_vp + ns;
+  K.work = dwork.as<double>();
+  GeoTimer t;
+  TRY(geo_timer_start(ctx, t));
+  launch_fwdsurf(ctx->stream, J);
+  launch_depthkernel(ctx->stream, K);
+  TRY(geo_timer_stop(ctx, t));
+  std::vector<int> st(ncol);
+  HIPCHK(ctx, hipMemcpyAsync(sen_vs, K.sen_vs, ns * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(sen_vp, K.sen_vp, ns * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(sen_rho, K.sen_rho, ns * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (cphase)
+    HIPCHK(ctx, hipMemcpyAsync(cphase, J.out, no * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(st.data(), J.status, (size_t)ncol * sizeof(int),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  TRY(sync_status(ctx));
+  TRY(geo_timer_read(ctx, t));
+  if (col_status) std::memcpy(col_status, st.data(), (size_t)ncol * sizeof(int));
+  for (int c = 0; c < ncol; ++c)
+    if (st[c] != SATT_OK)
+      return fail(ctx, SATT_E_DISPERSION,
+                  "satt_depthkernel: no root of the period equation in column (%d, %d)",
+                  c % nx, c / nx);
+  return SATT_OK;
+}
+
 extern "C" int satt_fwd_setup_fields(satt_ctx *ctx, const double *xx_deg,
                                      const double *yy_deg, int nx, int ny,
                                      const double *spha, const double *sphb,

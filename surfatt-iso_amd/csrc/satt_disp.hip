@@ -459,22 +459,30 @@ __device__ inline void brocher(float vs, float &vp, float &rho) {
 __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
   extern __shared__ double lds_terms[];
   const int col = blockIdx.x, lane = threadIdx.x;
-  const int nz = J.nz, kmax = J.kmax;
-  Layers L{reinterpret_cast<float *>(lds_terms + (size_t)nz * kTermWords), lds_terms, nz, 1};
-  // fwdsurf1d: Vs to REAL(4), Vp and density, layer thicknesses over a
-  // half-space at the bottom
-  for (int k = lane; k < nz; k += 64) {
-    const float vs = (float)J.vs[col + (size_t)J.ncol * k];
+  const int nz = J.nz, kmax = J.kmax, n = nz + (J.refine ? 1 : 0);
+  Layers L{reinterpret_cast<float *>(lds_terms + (size_t)n * kTermWords), lds_terms, n, 1};
+  // fwdsurf1d: Vs to REAL(4), Vp and density, layer thicknesses
+  // depz(k+1)-depz(k) in double over a half-space at the bottom; refinegrid:
+  // the same layers with thicknesses from the REAL(4) depths, the last one
+  // repeated, over a half-space copy of the last layer
+  for (int k = lane; k < n; k += 64) {
+    const int kk = k < nz ? k : nz - 1;
+    const float vs = (float)J.vs[col + (size_t)J.ncol * kk];
     float vp, rho;
     brocher(vs, vp, rho);
     L.b(k) = vs;
     L.a(k) = vp;
     L.rho(k) = rho;
-    L.d(k) = k < nz - 1 ? (float)(J.depz[k + 1] - J.depz[k]) : 0.0f;
+    if (!J.refine) {
+      L.d(k) = k < nz - 1 ? (float)(J.depz[k + 1] - J.depz[k]) : 0.0f;
+    } else {
+      const int kt = k < nz - 1 ? k : nz - 2;
+      L.d(k) = k < nz ? (float)J.depz[kt + 1] - (float)J.depz[kt] : 0.0f;
+    }
   }
   __syncthreads();
   float vmin = L.b(0);
-  for (int k = 1; k < nz; ++k) vmin = L.b(k) < vmin ? L.b(k) : vmin;
+  for (int k = 1; k < n; ++k) vmin = L.b(k) < vmin ? L.b(k) : vmin;
   // surfdisp96 with a spherical earth (iflsph 1) and the fundamental mode
   const float ddc = vmin < 1.0f ? 0.001f : (vmin < 2.0f ? 0.002f : 0.005f);
   const float sone = 1.5f, h = 0.005f;
@@ -485,7 +493,7 @@ __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
   __syncthreads();
   float betmx = -1.0e20f, betmn = 1.0e20f;
   int jmn = 0, jsol = 1;
-  for (int i = 0; i < nz; ++i) {
+  for (int i = 0; i < n; ++i) {
     const float bi = L.b(i), ai = L.a(i);
     if (bi > 0.01f && bi < betmn) {
       betmn = bi;
@@ -559,12 +567,13 @@ __global__ __launch_bounds__(64) void k_fwdsurf(DispJob J) {
 
 }  // namespace
 
-size_t fwdsurf_lds_bytes(int nz) {
-  return (size_t)nz * (kTermWords * sizeof(double) + kLayerFields * sizeof(float));
+size_t fwdsurf_lds_bytes(int n) {
+  return (size_t)n * (kTermWords * sizeof(double) + kLayerFields * sizeof(float));
 }
 
 void launch_fwdsurf(hipStream_t s, const DispJob &J) {
-  hipLaunchKernelGGL(k_fwdsurf, dim3((unsigned)J.ncol), dim3(64), fwdsurf_lds_bytes(J.nz), s, J);
+  hipLaunchKernelGGL(k_fwdsurf, dim3((unsigned)J.ncol), dim3(64),
+                     fwdsurf_lds_bytes(J.nz + (J.refine ? 1 : 0)), s, J);
 }
 
 }  // namespace satt

@@ -257,6 +257,8 @@ void launch_grad_update(hipStream_t s, size_t n, int nker, int q, const double *
 // ---- surface-wave dispersion per column (satt_disp.hip)
 struct DispJob {
   int ncol, nz, kmax, iwave, igr;
+  int refine;              // 0: fwdsurf1d's nz layers; 1: refinegrid's nz + 1
+                           // (depthkernel1d, src/surfker/surfker.f90:285-307)
   const double *vs;        // [nz][ncol]: the 3-D model vs3d(nx,ny,nz)
   const double *depz;      // [nz] km
   const double *periods;   // [kmax] s
@@ -266,8 +268,22 @@ struct DispJob {
 // Streaming read and copy bandwidth of the device (satt_probe.hip), GB/s
 int stream_probe(hipStream_t s, int cus, size_t nbytes, int reps, double &read_gbs,
                  double &copy_gbs);
-// LDS of one column's wave (the layered model and the per-layer terms)
-size_t fwdsurf_lds_bytes(int nz);
+// LDS of one column's wave (the layered model and the per-layer terms) for
+// n layers
+size_t fwdsurf_lds_bytes(int n);
 void launch_fwdsurf(hipStream_t s, const DispJob &J);
+// Rayleigh-wave depth kernels (satt_depth.hip): one thread per (column,
+// period), sregn96 at the column's phase velocity cp [col][kmax] on the
+// refinegrid model; sen_vs/vp/rho (kmax, ncol, nz) period-major, status per
+// column (kOk or the dispersion search's failure, which zeroes its kernels).
+struct DepthJob {
+  int ncol, nz, kmax;
+  const double *vs, *depz, *periods, *cp;
+  const int *col_status;
+  double *sen_vs, *sen_vp, *sen_rho;
+  double *work;            // depth_work_doubles(ncol*kmax, nz + 1)
+};
+size_t depth_work_doubles(size_t nthreads, int mmax);
+void launch_depthkernel(hipStream_t s, const DepthJob &J);
 
 }  // namespace satt

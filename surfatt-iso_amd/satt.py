@@ -392,6 +392,23 @@ class Context:
         self._check(st)
         return (out, cs) if status else out
 
+    def depthkernel(self, vs3d, depz, periods, iwave=2, igr=0, status=False):
+        """satt_depthkernel: depthkernel_decomp's sen_vs, sen_vp, sen_rho
+        (nperiod, nx, ny, nz) and the phase velocities cp (nperiod, nx, ny)
+        of vs3d (nx, ny, nz); with status also the per-column codes."""
+        vs = _f(vs3d)
+        nx, ny, nz = vs.shape
+        depz, per = _c(depz), _c(periods)
+        out = [np.zeros((len(per), nx, ny, nz), order="F") for _ in range(3)]
+        cp = np.zeros((len(per), nx, ny), order="F")
+        st = np.zeros(nx * ny, dtype=np.int32)
+        rc = self.lib.satt_depthkernel(self.ptr, _d(vs), nx, ny, nz, _d(depz), len(per), _d(per),
+                                       int(iwave), int(igr), *(_d(o) for o in out), _d(cp), _i(st))
+        if status:
+            return (*out, cp, st.reshape((nx, ny), order="F"), rc)
+        self._check(rc)
+        return (*out, cp)
+
     def hbm_probe(self, nbytes=1 << 30):
         """(read GB/s, copy GB/s) of streaming kernels over nbytes."""
         r, c = ctypes.c_double(), ctypes.c_double()
