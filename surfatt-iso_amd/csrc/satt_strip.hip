@@ -148,6 +148,14 @@ __device__ __forceinline__ size_t skidx(int c, int r, int l, int R) {
   return ((size_t)c * R + r) * 64 + l;
 }
 
+// The index unchanged, hidden from the optimizer: a load at a selected
+// index whose other choice is an index loaded anyway would otherwise become
+// that load plus a second one behind a branch.
+__device__ __forceinline__ size_t opaque_index(size_t x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
 // One cell of the forward sweep body (src/eikon_solver_2d.f90:910-1079) with
 // every operand in registers.  Every floating-point expression is the
 // reference's, in its order; what the four quadrants have in common is
@@ -415,8 +423,9 @@ __device__ __forceinline__ size_t skpos(int sk, int i, int j, int R) {
 // previous strip has finished, and the strip's own edge lane overwrites a
 // row after the strip's first lane has read it.  Either way the edge column
 // comes from LDS instead of one scattered global load per step.
-// A wait is bounded: past the bound the item gets kETeamWait and every later
-// wait of the team returns at once, so the kernel always ends.
+// A wait is bounded: past the bound it sets the team's stop flag, every
+// later wait of the team returns at once, so the kernel always ends, and the
+// kernel gives the item kETeamWait at its end.
 struct Team {
   int W, w;           // waves, this wave (wave-uniform)
   int ny, ns;
@@ -426,7 +435,6 @@ struct Team {
   volatile LMEM int *cnt;   // [2][W] rows published by each wave in the sweep
   volatile LMEM int *stop;  // set when a wait ran out
   LMEM double *part;        // [ns][3] per-strip residual terms
-  int *status;        // the item's status
   __device__ bool multi() const { return W > 1; }
 };
 
@@ -441,14 +449,13 @@ __host__ __device__ inline size_t team_lds_bytes(int W, int ny, int ns, int vbyt
 // TEAM false: the one-wave form, with W = 1 known at compile time so that
 // no team code is left in the kernel.
 template <bool TEAM>
-__device__ inline Team team_init(void *lds, int ny, int ns, int vbytes, int *status) {
+__device__ inline Team team_init(void *lds, int ny, int ns, int vbytes) {
   Team T;
   T.W = TEAM ? (int)(blockDim.x >> 6) : 1;
   T.w = TEAM ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
   T.ny = ny;
   T.ns = ns;
   T.sweep = 0;
-  T.status = status;
   LMEM char *base = (LMEM char *)lds;
   T.part = (LMEM double *)base;
   LMEM char *b = base + (TEAM ? (size_t)ns * 3 * 8 : 0);
@@ -484,10 +491,7 @@ __device__ inline void team_wait(const Team &T, int pw, int need) {
   int spins = 0;
   while (*c < need && !*T.stop) {
     __builtin_amdgcn_s_sleep(2);
-    if (++spins > kTeamSpinLimit) {
-      *T.stop = 1;
-      *T.status = kETeamWait;
-    }
+    if (++spins > kTeamSpinLimit) *T.stop = 1;
   }
 }
 
@@ -588,18 +592,31 @@ __device__ __forceinline__ int skew_row(int t, int ny, int L) {
   return YD > 0 ? t - L + 64 : ny + 62 - t;
 }
 
+// Where a lane loads its operands of step t in a strip pass: its own cell
+// while that cell is in the grid, else, in the same (clamped) row, the cell
+// of the nearest lane whose cell is.  Every load is then valid and none needs
+// a branch, and a step touches only the cache lines of its row that hold
+// cells of the step.
+template <int XD, int YD>
+__device__ __forceinline__ size_t step_pos(int c, int t, int lane, int ny, int L, int R) {
+  const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
+  const int lo = XD > 0 ? max(0, t - ny + 1) : max(0, L - 1 - t);
+  const int hi = XD > 0 ? min(L - 1, t) : min(L - 1, L - 2 + ny - t);
+  const int l = min(max(min(max(lane, lo), hi), 0), 63);
+  return skidx(c, r, l, R);
+}
+
 // The operands a lane needs for its cell of step t.  "Far" operands are the
 // ones other lanes read one step early (tau, T0) plus the strip-edge and
 // convergence values; they are fetched two steps ahead.  "Near" operands are
-// the cell's own coefficients, fetched one step ahead.  A lane loads only
-// while its cell of that step is in the grid, so the pipeline's fill and
-// drain steps and the padding lanes of a partial strip fetch nothing (whole
-// cache lines of a row are skipped); the values other lanes read from it
-// (lane shifts) are then never used for a cell in the grid.
+// the cell's own coefficients, fetched one or two steps ahead.  A lane whose
+// cell of that step is outside the grid (pipeline fill and drain, padding
+// lanes of a partial strip) loads a padding cell of the strip; the values
+// other lanes read from it (lane shifts) are never used for a cell in the
+// grid.
 template <typename V>
 struct FFar {
   V tau, t0;    // old tau and T0 of the lane's cell
-  V tup, zup;   // upstream neighbour column (edge lane): final tau, T0
   V tdn, zdn;   // downstream neighbour column (edge lane): old tau, T0
   V snap;       // tau at the start of the iteration (last sweep only)
   int j;
@@ -661,156 +678,106 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   const int base_out = (k / T.W) * ny;
   double acc_s = 0, acc_m = 0;
 
+  // Every lane loads at every step (at step_pos), and no global load sits
+  // behind a branch: a load that a branch may skip leaves the compiler
+  // unsure how many loads are in flight, and it then waits for all of them,
+  // the prefetched ones included.
   auto load_far = [&](int t, FFar<V> &q) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
     q.j = j;
     q.act = lv && u >= 0 && u < ny;
-    if (q.act) {
-      const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      q.t0 = is[e];
-      q.tau = tau[e];
-      if (up_ok) {  // k > 0: the upstream strip's edge
-        if (T.multi()) team_wait(T, pw, base_in + u + 1);
-        q.tup = team_read<V>(T, pw, u);
-        q.zup = team_read0<V>(T, pw, u);
-      }
-      if (kDown && dn_ok) {
-        q.tdn = tau[skpos(SK, id, j, R)];
-        q.zdn = is[skpos(SK, id, j, R)];
-      }
-      if (MODE == 2) q.snap = C.snap[e];
+    const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
+    q.t0 = is[e];
+    q.tau = tau[e];
+    if constexpr (kDown) {
+      const size_t ed = opaque_index(dn_ok ? skpos(SK, id, min(max(j, 0), ny - 1), R) : e);
+      q.tdn = tau[ed];
+      q.zdn = is[ed];
     }
+    if (MODE == 2) q.snap = C.snap[e];
   };
   auto load_near = [&](int t, FNear<V> &q) {
-    const int u = t - lag;
-    if (lv && u >= 0 && u < ny) {
-      const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      q.a = ps[e]; q.b = ps[SE + e]; q.c = ps[2 * SE + e]; q.f = ps[3 * SE + e];
-      q.dx_ = ps[4 * SE + e]; q.dy_ = ps[5 * SE + e];
-      q.t0x = is[SE + e]; q.t0y = is[2 * SE + e];
+    const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
+    q.a = ps[e]; q.b = ps[SE + e]; q.c = ps[2 * SE + e]; q.f = ps[3 * SE + e];
+    q.dx_ = ps[4 * SE + e]; q.dy_ = ps[5 * SE + e];
+    q.t0x = is[SE + e]; q.t0y = is[2 * SE + e];
+  };
+  // The upstream strip's edge (final tau and its T0) for the cell of step t,
+  // from the channel; read at the step itself, ahead of the step's global
+  // loads, so that the team's wait loop sits between none of them and their
+  // use.
+  auto load_up = [&](int t, bool act, V &tu, V &zu) {
+    if (up_ok && act) {  // k > 0
+      if (T.multi()) team_wait(T, pw, base_in + t - lag + 1);
+      tu = team_read<V>(T, pw, t - lag);
+      zu = team_read0<V>(T, pw, t - lag);
     }
   };
 
-  // Far operands are fetched two steps ahead, near ones two (float) or one
-  // (double) ahead.  The float loop cycles three operand sets, unrolled by
-  // three so that no set is copied (step t uses set t mod 3 and fills set
-  // (t+2) mod 3); up to two steps past the last run with every lane outside
-  // the grid, so they change nothing.  The double loop rotates its sets (a
-  // third near set would cost it a wave per SIMD of occupancy), written out
-  // plainly: the same loop through the step function ran 12% slower.
-  if constexpr (std::is_same<V, float>::value) {
-    FFar<V> fA{}, fB{}, fC{};
-    FNear<V> nA{}, nB{}, nC{};
-    load_far(0, fA);
-    load_far(1, fB);
-    load_near(0, nA);
-    load_near(1, nB);
-    V tprev = 0, zprev = 0;  // own result and T0 of the previous step
-    auto step = [&](int t, const FFar<V> &fc, const FFar<V> &fn, FFar<V> &f2, const FNear<V> &nc,
-                    FNear<V> &nx2, int tnear) {
-      load_far(t + 2, f2);
-      load_near(tnear, nx2);
-      // lane shifts read other lanes' registers: they run with every lane
-      // enabled, outside the per-cell branch
-      V tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
-      if (up_ok) { tUx = fc.tup; zUx = fc.zup; }
-      V tDx = 0, zDx = 0;
-      if constexpr (kDown) {
-        tDx = lane_shift<XD>(fn.tau);
-        zDx = lane_shift<XD>(fn.t0);
-        if (dn_ok) { tDx = fc.tdn; zDx = fc.zdn; }
-      }
-      const int jc = fc.j;
-      V tnew = fc.tau;
-      if (fc.act) {
-        const bool frozen = i >= C.fi0 && i <= C.fi1 && jc >= C.fj0 && jc <= C.fj1;
-        if constexpr (kDown) {
-          const V tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
-          const V zW = XD > 0 ? zUx : zDx, zE = XD > 0 ? zDx : zUx;
-          const V tS = YD > 0 ? tprev : fn.tau, tN = YD > 0 ? fn.tau : tprev;
-          const V zS = YD > 0 ? zprev : fn.t0, zN = YD > 0 ? fn.t0 : zprev;
-          if (!frozen)
-            tnew = fwd_cell_r(i, jc, nx, ny, C.dx, C.dy, nc.a, nc.b, nc.c, nc.f, nc.dx_, nc.dy_,
-                              fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
-        } else {
-          if (!frozen)
-            tnew = fwd_cell_up_r<V, XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
-                                        nc.dx_, nc.dy_, fc.t0, nc.t0x, nc.t0y, fc.tau, tUx, tprev,
-                                        zUx, zprev);
-        }
-        const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-        tau[e] = tnew;
-        if (pub) team_publish2<V>(T, t - lag, tnew, fc.t0, base_out + t - lag + 1);
-        if (MODE == 1) C.snap[e] = fc.tau;
-        if (MODE == 2) {
-          const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
-          acc_s += d;
-          acc_m = d > acc_m ? d : acc_m;
-        }
-      }
-      tprev = tnew;
-      zprev = fc.t0;
-    };
-    for (int t = 0; t < nsteps; t += 3) {
-      step(t, fA, fB, fC, nA, nC, t + 2);
-      step(t + 1, fB, fC, fA, nB, nA, t + 3);
-      step(t + 2, fC, fA, fB, nC, nB, t + 4);
+  // Far and near operands are fetched two steps ahead.  The loop cycles
+  // three operand sets, unrolled by three so that no set is copied (step t
+  // uses set t mod 3 and fills set (t+2) mod 3): a copy of a set still in
+  // flight would wait for its loads.  Up to two steps past the last run with
+  // every lane outside the grid, so they change nothing.
+  FFar<V> fA{}, fB{}, fC{};
+  FNear<V> nA{}, nB{}, nC{};
+  load_far(0, fA);
+  load_far(1, fB);
+  load_near(0, nA);
+  load_near(1, nB);
+  V tprev = 0, zprev = 0;  // own result and T0 of the previous step
+  auto step = [&](int t, const FFar<V> &fc, const FFar<V> &fn, FFar<V> &f2, const FNear<V> &nc,
+                  FNear<V> &nx2, int tnear) {
+    V tU = 0, zU = 0;
+    load_up(t, fc.act, tU, zU);
+    load_far(t + 2, f2);
+    load_near(tnear, nx2);
+    // lane shifts read other lanes' registers: they run with every lane
+    // enabled, outside the per-cell branch
+    V tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
+    if (up_ok) { tUx = tU; zUx = zU; }
+    V tDx = 0, zDx = 0;
+    if constexpr (kDown) {
+      tDx = lane_shift<XD>(fn.tau);
+      zDx = lane_shift<XD>(fn.t0);
+      if (dn_ok) { tDx = fc.tdn; zDx = fc.zdn; }
     }
-  } else {
-    FFar<V> fc{}, fn{}, f2{};
-    FNear<V> nc{}, nn{};
-    load_far(0, fc);
-    load_far(1, fn);
-    load_near(0, nc);
-    V tprev = 0, zprev = 0;  // own result and T0 of the previous step
-    for (int t = 0; t < nsteps; ++t) {
-      load_far(t + 2, f2);
-      load_near(t + 1, nn);
-      // lane shifts read other lanes' registers: they run with every lane
-      // enabled, outside the per-cell branch
-      V tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
-      if (up_ok) { tUx = fc.tup; zUx = fc.zup; }
-      V tDx = 0, zDx = 0;
+    const int jc = fc.j;
+    V tnew = fc.tau;
+    if (fc.act) {
+      const bool frozen = i >= C.fi0 && i <= C.fi1 && jc >= C.fj0 && jc <= C.fj1;
       if constexpr (kDown) {
-        tDx = lane_shift<XD>(fn.tau);
-        zDx = lane_shift<XD>(fn.t0);
-        if (dn_ok) { tDx = fc.tdn; zDx = fc.zdn; }
+        const V tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
+        const V zW = XD > 0 ? zUx : zDx, zE = XD > 0 ? zDx : zUx;
+        const V tS = YD > 0 ? tprev : fn.tau, tN = YD > 0 ? fn.tau : tprev;
+        const V zS = YD > 0 ? zprev : fn.t0, zN = YD > 0 ? fn.t0 : zprev;
+        if (!frozen)
+          tnew = fwd_cell_r(i, jc, nx, ny, C.dx, C.dy, nc.a, nc.b, nc.c, nc.f, nc.dx_, nc.dy_,
+                            fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
+      } else {
+        if (!frozen)
+          tnew = fwd_cell_up_r<V, XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
+                                      nc.dx_, nc.dy_, fc.t0, nc.t0x, nc.t0y, fc.tau, tUx, tprev,
+                                      zUx, zprev);
       }
-      const int jc = fc.j;
-      V tnew = fc.tau;
-      if (fc.act) {
-        const bool frozen = i >= C.fi0 && i <= C.fi1 && jc >= C.fj0 && jc <= C.fj1;
-        if constexpr (kDown) {
-          const V tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
-          const V zW = XD > 0 ? zUx : zDx, zE = XD > 0 ? zDx : zUx;
-          const V tS = YD > 0 ? tprev : fn.tau, tN = YD > 0 ? fn.tau : tprev;
-          const V zS = YD > 0 ? zprev : fn.t0, zN = YD > 0 ? fn.t0 : zprev;
-          if (!frozen)
-            tnew = fwd_cell_r(i, jc, nx, ny, C.dx, C.dy, nc.a, nc.b, nc.c, nc.f, nc.dx_, nc.dy_,
-                              fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
-        } else {
-          if (!frozen)
-            tnew = fwd_cell_up_r<V, XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
-                                        nc.dx_, nc.dy_, fc.t0, nc.t0x, nc.t0y, fc.tau, tUx, tprev,
-                                        zUx, zprev);
-        }
-        const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-        tau[e] = tnew;
-        if (pub) team_publish2<V>(T, t - lag, tnew, fc.t0, base_out + t - lag + 1);
-        if (MODE == 1) C.snap[e] = fc.tau;
-        if (MODE == 2) {
-          const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
-          acc_s += d;
-          acc_m = d > acc_m ? d : acc_m;
-        }
+      const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
+      tau[e] = tnew;
+      if (pub) team_publish2<V>(T, t - lag, tnew, fc.t0, base_out + t - lag + 1);
+      if (MODE == 1) C.snap[e] = fc.tau;
+      if (MODE == 2) {
+        const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
+        acc_s += d;
+        acc_m = d > acc_m ? d : acc_m;
       }
-      tprev = tnew;
-      zprev = fc.t0;
-      fc = fn;
-      fn = f2;
-      nc = nn;
     }
+    tprev = tnew;
+    zprev = fc.t0;
+  };
+  for (int t = 0; t < nsteps; t += 3) {
+    step(t, fA, fB, fC, nA, nC, t + 2);
+    step(t + 1, fB, fC, fA, nB, nA, t + 3);
+    step(t + 2, fC, fA, fB, nC, nB, t + 4);
   }
   if (MODE == 2) {
     ss = wave_sum(acc_s);
@@ -1035,7 +1002,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_strip(const S
   if (*D.status < 0) return;  // the setup failed
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
-  Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8, D.status);
+  Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8);
   fwd_finish64<false>(D, T, lane, rect);
 }
 
@@ -1051,7 +1018,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_mixed32(const
   const size_t SE = sg.skew_elems();
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
-  Team T = team_init<TEAM>(lds, D.g.ny, sg.ns, 4, D.status);
+  Team T = team_init<TEAM>(lds, D.g.ny, sg.ns, 4);
   double *sk0 = D.skew;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkF32 * SE);
   FwdCtx<float> C;
@@ -1084,7 +1051,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_polish(const 
   if (*D.status < 0) return;
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
-  Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8, D.status);
+  Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8);
   fwd_finish64<true>(D, T, lane, rect);
 }
 
@@ -1248,7 +1215,7 @@ __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count,
 
 template <typename V>
 struct AFar {
-  V ta, tup, tdn, snap;
+  V ta, tdn, snap;
   int j;
   bool act;
 };
@@ -1325,31 +1292,24 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
   const int base_out = (k / T.W) * ny;
   double acc_s = 0, acc_m = 0, acc_a = 0;
 
-  // The adjoint's step is a few operations, so its loads must not wait on
-  // a branch: every lane loads, at a clamped (always valid) row.
+  // Every lane loads at every step, at step_pos, with no load behind a
+  // branch (see the forward pass).
   auto load_far = [&](int t, AFar<V> &q) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
     q.j = j;
     q.act = lv && u >= 0 && u < ny;
     const int jc = min(max(j, 0), ny - 1);
-    const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
-    const size_t e = skidx(c, r, lane, R);
+    const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
     q.ta = ta[e];
-    if (up_ok && q.act) {  // k > 0: the upstream strip's edge
-      if (T.multi()) team_wait(T, pw, base_in + u + 1);
-      q.tup = team_read<V>(T, pw, u);
-    }
-    if (dn_ok) q.tdn = ta[skpos(SK, id, jc, R)];
+    q.tdn = ta[opaque_index(dn_ok ? skpos(SK, id, jc, R) : e)];
     if (MODE == 2) q.snap = C.snap[e];
   };
   auto load_near = [&](int t, ANear<V> &q) {
-    const int r = min(max(skew_row<XD, YD>(t, ny, L), 0), R - 1);
-    adj_load_near(is, SE, skidx(c, r, lane, R), q);
+    adj_load_near(is, SE, step_pos<XD, YD>(c, t, lane, ny, L, R), q);
   };
-  // operand sets as in the forward pass: the float loop unrolled by three
-  // over three sets (far and near two steps ahead), the double loop rotated
-  constexpr bool kUnroll = std::is_same<V, float>::value;
+  // operand sets as in the forward pass: unrolled by three over three sets
+  // (far and near two steps ahead)
   AFar<V> fA, fB, fC;
   ANear<V> nA, nB, nC;
   load_far(0, fA);
@@ -1359,11 +1319,18 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
   V tprev = 0;
   auto step = [&](int t, const AFar<V> &fc, const AFar<V> &fn, AFar<V> &f2, const ANear<V> &nc,
                   ANear<V> &nx2, int tnear) {
+    // the upstream strip's edge for this step, read ahead of the step's
+    // global loads (see the forward pass)
+    V tU = 0;
+    if (up_ok && fc.act) {  // k > 0
+      if (T.multi()) team_wait(T, pw, base_in + t - lag + 1);
+      tU = team_read<V>(T, pw, t - lag);
+    }
     load_far(t + 2, f2);
     load_near(tnear, nx2);
     V tUx = lane_shift<-XD>(tprev);
     V tDx = lane_shift<XD>(fn.ta);
-    if (up_ok) tUx = fc.tup;
+    if (up_ok) tUx = tU;
     if (dn_ok) tDx = fc.tdn;
     const V tW = XD > 0 ? tUx : tDx, tE = XD > 0 ? tDx : tUx;
     const V tS = YD > 0 ? tprev : fn.ta, tN = YD > 0 ? fn.ta : tprev;
@@ -1388,19 +1355,10 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
     }
     tprev = tnew;
   };
-  if constexpr (kUnroll) {
-    for (int t = 0; t < nsteps; t += 3) {
-      step(t, fA, fB, fC, nA, nC, t + 2);
-      step(t + 1, fB, fC, fA, nB, nA, t + 3);
-      step(t + 2, fC, fA, fB, nC, nB, t + 4);
-    }
-  } else {
-    for (int t = 0; t < nsteps; ++t) {
-      step(t, fA, fB, fC, nA, nB, t + 1);
-      fA = fB;
-      fB = fC;
-      nA = nB;
-    }
+  for (int t = 0; t < nsteps; t += 3) {
+    step(t, fA, fB, fC, nA, nC, t + 2);
+    step(t + 1, fB, fC, fA, nB, nA, t + 3);
+    step(t + 2, fC, fA, fB, nC, nB, t + 4);
   }
   if (MODE == 2) {
     ss = wave_sum(acc_s);
@@ -1472,7 +1430,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_strip(const S
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
-  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8, D.status);
+  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
@@ -1509,7 +1467,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
-  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 4, D.status);
+  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 4);
   double *sk0 = D.skew;
   float *f32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
   float *taP = f32 + 10 * SE, *taM = f32 + 11 * SE;
@@ -1550,7 +1508,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const 
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
-  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8, D.status);
+  Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
   AdjCtx<double> C;
