@@ -402,28 +402,45 @@ int satt_plan_set_waves(satt_plan *plan, int waves);
  * per launch group as above: pins a configuration the autotuner found.  No
  * counterpart in the reference. */
 int satt_plan_set_stage_waves(satt_plan *plan, const int *waves);
+/* How many steps ahead each sweep stage fetches its operands, depth[4] in
+ * satt_tune_result.stage_depth order, each 2 (the default) or 4.  Fetching
+ * further ahead hides more memory latency where few waves share a SIMD, at
+ * the price of registers.  The results do not depend on it.  No counterpart
+ * in the reference. */
+int satt_plan_set_stage_depth(satt_plan *plan, const int *depth);
 
-/* Autotuner of the team size: times a synthetic batch of `nitems` fwd+adj
- * solves on an nx x ny grid for each W in 1..8 (as far as the grid's strips
- * allow) and keeps, for each of the four sweep stages (forward FP32, forward
- * FP64, adjoint FP32, adjoint FP64), the fastest W in the context's table,
- * keyed by strips, rows, item count and precision, for plans
+/* Autotuner of the launch shape: times a synthetic batch of `nitems`
+ * fwd+adj solves on an nx x ny grid for each team size W in 1..8 (as far as
+ * the grid's strips allow) with the two-step prefetch, then with the deep
+ * (four-step) prefetch at the W each stage chose and at W = 1, and keeps, for
+ * each of the four sweep stages (forward FP32, forward FP64, adjoint FP32,
+ * adjoint FP64), the fastest (W, depth) in the context's table, keyed by
+ * strips, rows, item count and precision, for plans
  * (satt_plan_set_waves(plan, 0)) and servers (waves 0).  waves is the W of
- * the fastest whole run and ms[W-1] that run's time; stage_waves[k] is stage
- * k's choice and stage_ms[W-1][k] its time (0: not tried or not run in this
- * precision).  No counterpart in the reference. */
+ * the fastest whole two-step run and ms[W-1] that run's time;
+ * stage_ms[W-1][k] is stage k's two-step time at W (0: not tried or not run
+ * in this precision), stage_ms_deep its deep-prefetch times, and
+ * stage_waves[k], stage_depth[k] its choice.  No counterpart in the
+ * reference. */
 typedef struct {
   int waves;
   double ms[8];
   int stage_waves[4];
   double stage_ms[8][4];
+  int stage_depth[4];          /* prefetch depth chosen per stage: 2 or 4   */
+  double stage_ms_deep[2][4];  /* each stage's time with the deep prefetch:
+                                  [0] at the team sizes the two-step sweep
+                                  chose, [1] at one wave per item (0: not
+                                  tried)                                     */
 } satt_tune_result;
 int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int precision,
                     satt_tune_result *result);
 
 /* Autotunes the plan on its own inputs: runs it (istotable and isadj set) with
- * every W in 1..8, keeps each sweep stage's fastest W and sets those (as
- * satt_plan_set_stage_waves); result fields as satt_tune_waves.  The plan's
+ * every W in 1..8 and the two prefetch depths as satt_tune_waves does, keeps
+ * each sweep stage's fastest (W, depth) and sets those (as
+ * satt_plan_set_stage_waves and satt_plan_set_stage_depth); result fields as
+ * satt_tune_waves.  The plan's
  * results afterwards are those of a run (they do not depend on W).  No
  * counterpart in the reference. */
 int satt_plan_tune_waves(satt_plan *plan, satt_tune_result *result);

@@ -1227,6 +1227,7 @@ struct satt_plan {
   int graph_on = 0;
   hipGraphExec_t gexec[4] = {};  // replay form, per (istotable, isadj)
   int waves = 1;                 // waves per item asked for (each group caps it)
+  int depth[kStages] = {2, 2, 2, 2};  // prefetch depth per sweep stage
   void drop_graphs() {
     for (auto &x : gexec)
       if (x) {
@@ -1518,8 +1519,10 @@ int plan_groups(satt_plan *P) {
       gd.max_ny = std::max(gd.max_ny, Q.ny);
       gd.max_ns = std::max(gd.max_ns, Q.sg.ns);
     }
-    for (int st = 0; st < kStages; ++st)
+    for (int st = 0; st < kStages; ++st) {
       gd.waves[st] = team_waves(P->waves, gd.max_ny, gd.max_ns, st);
+      gd.depth[st] = P->depth[st];
+    }
     G.gd = gd;
     for (size_t q = 0; q < P->parts.size(); ++q) {
       const Part &Q = *P->parts[q];
@@ -1833,11 +1836,18 @@ extern "C" int satt_plan_set_waves(satt_plan *P, int waves) {
         satt_tune_result r{};
         const int st = satt_tune_waves(ctx, nx, G.gd.max_ny, G.gd.count, P->precision, &r);
         if (st < 0) return st;
-        for (int s = 0; s < kStages; ++s) tw.w[s] = r.stage_waves[s];
+        for (int s = 0; s < kStages; ++s) {
+          tw.w[s] = r.stage_waves[s];
+          tw.d[s] = r.stage_depth[s];
+        }
       }
+    } else {
+      for (int s = 0; s < kStages; ++s) tw.d[s] = P->depth[s];
     }
-    for (int s = 0; s < kStages; ++s)
+    for (int s = 0; s < kStages; ++s) {
       G.gd.waves[s] = team_waves(tw.w[s], G.gd.max_ny, G.gd.max_ns, s);
+      G.gd.depth[s] = tw.d[s];
+    }
   }
   P->drop_graphs();  // captured with the old launch shapes
   return SATT_OK;
@@ -1857,33 +1867,60 @@ extern "C" int satt_plan_set_stage_waves(satt_plan *P, const int *waves) {
   return SATT_OK;
 }
 
-extern "C" int satt_plan_tune_waves(satt_plan *P, satt_tune_result *out) {
-  if (!P) return SATT_E_ARG;
-  satt_tune_result r{};
-  double best = 0, best_stage[kStages] = {0, 0, 0, 0};
+extern "C" int satt_plan_set_stage_depth(satt_plan *P, const int *depth) {
+  if (!P || !depth) return SATT_E_ARG;
+  for (int s = 0; s < kStages; ++s)
+    if (depth[s] != 2 && depth[s] != kDeepPrefetch) return SATT_E_ARG;
+  satt_ctx *ctx = P->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < kStages; ++s) P->depth[s] = depth[s];
+  for (Group &G : P->groups)
+    for (int s = 0; s < kStages; ++s) G.gd.depth[s] = depth[s];
+  P->drop_graphs();
+  return SATT_OK;
+}
+
+// The launch shape of each sweep stage, measured on the plan's own inputs:
+// every team size W at the two-step prefetch, then the deep prefetch at each
+// stage's best W and at one wave per item; each stage keeps its fastest
+// (W, depth), which the plan is left set to.
+int satt::tune_plan_shapes(satt_plan *P, satt_tune_result &r) {
+  r = satt_tune_result{};
+  double best = 0;
   const int graph = P->graph_on;  // stage times are split in the eager form only
   P->graph_on = 0;
   int st = SATT_OK;
-  for (int W = 1; W <= kMaxTeam && st >= 0; ++W) {
+  auto run_stats = [&](const int *w, const int *d, satt_plan_stats &s) {
+    st = satt_plan_set_stage_waves(P, w);
+    if (st >= 0) st = satt_plan_set_stage_depth(P, d);
+    for (int rep = 0; rep < 2 && st >= 0; ++rep) {  // the first run warms up
+      st = satt_plan_run(P, 1, 1, SATT_LAUNCH_BATCHED);
+      if (st >= 0) st = satt_plan_wait(P);
+    }
+    if (st >= 0) st = satt_plan_get_stats(P, &s);
+    return st >= 0;
+  };
+  auto stage_times = [](const satt_plan_stats &s, double *sm) {
+    sm[0] = s.ms_fwd_fp32; sm[1] = s.ms_fwd_fp64; sm[2] = s.ms_adj_fp32; sm[3] = s.ms_adj_fp64;
+  };
+  const int d2[kStages] = {2, 2, 2, 2};
+  double best_stage[kStages] = {0, 0, 0, 0};
+  for (int W = 1; W <= kMaxTeam; ++W) {
     bool fresh = W == 1;  // a W that some group does not cap to a smaller one
     for (const Group &G : P->groups)
       fresh = fresh || team_waves(W, G.gd.max_ny, G.gd.max_ns, kStageFwd32) == W;
     if (!fresh) break;
     const int wW[kStages] = {W, W, W, W};
-    st = satt_plan_set_stage_waves(P, wW);
-    for (int rep = 0; rep < 2 && st >= 0; ++rep) {  // the first run warms up
-      st = satt_plan_run(P, 1, 1, SATT_LAUNCH_BATCHED);
-      if (st >= 0) st = satt_plan_wait(P);
-    }
     satt_plan_stats s{};
-    if (st >= 0) st = satt_plan_get_stats(P, &s);
-    if (st < 0) break;
+    if (!run_stats(wW, d2, s)) break;
     r.ms[W - 1] = s.ms_total;
     if (r.waves == 0 || s.ms_total < best) {
       best = s.ms_total;
       r.waves = W;
     }
-    const double sm[kStages] = {s.ms_fwd_fp32, s.ms_fwd_fp64, s.ms_adj_fp32, s.ms_adj_fp64};
+    double sm[kStages];
+    stage_times(s, sm);
     for (int k = 0; k < kStages; ++k) {
       r.stage_ms[W - 1][k] = sm[k];
       if (r.stage_waves[k] == 0 || sm[k] < best_stage[k]) {
@@ -1892,9 +1929,42 @@ extern "C" int satt_plan_tune_waves(satt_plan *P, satt_tune_result *out) {
       }
     }
   }
+  // the deep prefetch (the full FP64 forward has no deep form)
+  int dd[kStages];
+  for (int k = 0; k < kStages; ++k) {
+    r.stage_depth[k] = 2;
+    dd[k] = P->precision == SATT_PREC_FP64 && k == kStageFwd64 ? 2 : kDeepPrefetch;
+  }
+  const int one[kStages] = {1, 1, 1, 1};
+  bool all_one = true;
+  for (int k = 0; k < kStages; ++k) all_one = all_one && r.stage_waves[k] == 1;
+  satt_plan_stats s{};
+  if (st >= 0 && run_stats(r.stage_waves, dd, s)) stage_times(s, r.stage_ms_deep[0]);
+  if (st >= 0 && !all_one && run_stats(one, dd, s)) stage_times(s, r.stage_ms_deep[1]);
+  if (all_one)
+    for (int k = 0; k < kStages; ++k) r.stage_ms_deep[1][k] = r.stage_ms_deep[0][k];
   P->graph_on = graph;
   if (st < 0) return st;
+  for (int k = 0; k < kStages; ++k) {
+    if (dd[k] == 2) continue;
+    if (r.stage_ms_deep[0][k] > 0 && r.stage_ms_deep[0][k] < best_stage[k]) {
+      best_stage[k] = r.stage_ms_deep[0][k];
+      r.stage_depth[k] = kDeepPrefetch;
+    }
+    if (r.stage_ms_deep[1][k] > 0 && r.stage_ms_deep[1][k] < best_stage[k]) {
+      best_stage[k] = r.stage_ms_deep[1][k];
+      r.stage_waves[k] = 1;
+      r.stage_depth[k] = kDeepPrefetch;
+    }
+  }
   TRY(satt_plan_set_stage_waves(P, r.stage_waves));
+  return satt_plan_set_stage_depth(P, r.stage_depth);
+}
+
+extern "C" int satt_plan_tune_waves(satt_plan *P, satt_tune_result *out) {
+  if (!P) return SATT_E_ARG;
+  satt_tune_result r{};
+  TRY(tune_plan_shapes(P, r));
   if (out) *out = r;
   return SATT_OK;
 }

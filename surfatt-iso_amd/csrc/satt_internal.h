@@ -35,10 +35,11 @@ struct TuneKey {
     return precision < o.precision;
   }
 };
-// Team sizes of the four sweep stages (forward FP32, forward FP64, adjoint
-// FP32, adjoint FP64).
+// Team sizes and prefetch depths of the four sweep stages (forward FP32,
+// forward FP64, adjoint FP32, adjoint FP64).
 struct TuneWaves {
   int w[4];
+  int d[4] = {2, 2, 2, 2};
 };
 // The tuned team sizes of a key (false: not tuned), and storing them; both
 // thread-safe.

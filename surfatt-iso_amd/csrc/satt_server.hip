@@ -265,8 +265,10 @@ bool satt_server::launch(Batch &b) {
     const TuneKey key = TuneKey::of(64 * gd.max_ns, gd.max_ny, 1, precision);
     if (!ctx_tune_lookup(ctx, key, tw)) tw = TuneWaves{{4, 4, 4, 4}};
   }
-  for (int st = 0; st < kStages; ++st)
+  for (int st = 0; st < kStages; ++st) {
     gd.waves[st] = team_waves(tw.w[st], gd.max_ny, gd.max_ns, st);
+    gd.depth[st] = tw.d[st];
+  }
   std::memcpy(pin + b.desc_pin, desc.data(), desc.size() * sizeof(SlotDesc));
   SlotDesc *sd = reinterpret_cast<SlotDesc *>(dev + b.desc_dev);
   if (!hip_ok(hipMemcpyAsync(sd, pin + b.desc_pin, desc.size() * sizeof(SlotDesc),

@@ -57,7 +57,14 @@ struct GroupDims {
   int max_nxy;      // largest nx or ny
   int max_ny = 0, max_ns = 0;  // largest ny and strip count
   int waves[kStages] = {1, 1, 1, 1};  // waves per item (a team; at most kMaxTeam)
+  int depth[kStages] = {2, 2, 2, 2};  // steps the sweeps fetch their operands ahead
 };
+
+// The deeper of the two prefetch depths the sweep kernels are built with
+// (the other is two steps).  Fetching further ahead hides more memory
+// latency where few waves share a SIMD, at the price of one operand set of
+// registers per step.
+constexpr int kDeepPrefetch = 4;
 
 // Waves per item of the strip kernels: each strip of a sweep goes to one
 // wave of the item's team, the next strip's wave following at least one
@@ -70,6 +77,9 @@ static_assert(2 * 8 * (size_t)SATT_MAX_NY <= kTeamLdsMax, "SATT_MAX_NY exceeds t
 // The largest team (<= want) of a sweep stage whose LDS (edge channels of
 // the stage's value type) fits for grids up to ny rows, ns strips.
 int team_waves(int want, int ny, int ns, int stage);
+// Measures each sweep stage's fastest team size and prefetch depth on the
+// plan's own inputs and leaves the plan set to them (satt_api.hip).
+int tune_plan_shapes(satt_plan *P, satt_tune_result &r);
 
 // Batched kernels over the slots of a launch group (satt_strip.hip); all
 // asynchronous on `s`.  Forward: setup (per-item tables, then T0 skews and

@@ -645,8 +645,9 @@ struct FwdCtx {
 // in the sweep's own skew: the reference's update (UP false, FP64) or the
 // mixed mode's upwind update on dtau (UP true).  MODE 0: plain; 1: record
 // the snapshot (first sweep of an iteration); 2: the strip's sum and max of
-// |tau-snap|*T0 in double (last sweep) into ss, sm.
-template <typename V, int XD, int YD, int MODE, bool UP>
+// |tau-snap|*T0 in double (last sweep) into ss, sm.  PD: how many steps
+// ahead the operands are fetched.
+template <typename V, int XD, int YD, int MODE, bool UP, int PD>
 __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V *tau, double &ss,
                                double &sm) {
   constexpr int SK = XD == YD ? 0 : 1;
@@ -715,23 +716,26 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     }
   };
 
-  // Far and near operands are fetched two steps ahead.  The loop cycles
-  // three operand sets, unrolled by three so that no set is copied (step t
-  // uses set t mod 3 and fills set (t+2) mod 3): a copy of a set still in
-  // flight would wait for its loads.  Up to two steps past the last run with
-  // every lane outside the grid, so they change nothing.
-  FFar<V> fA{}, fB{}, fC{};
-  FNear<V> nA{}, nB{}, nC{};
-  load_far(0, fA);
-  load_far(1, fB);
-  load_near(0, nA);
-  load_near(1, nB);
+  // Far and near operands are fetched PD steps ahead.  The loop cycles
+  // S = PD+1 operand sets, unrolled by S so that no set is copied (step t
+  // uses set t mod S and fills set (t+PD) mod S): a copy of a set still in
+  // flight would wait for its loads.  Up to S-1 steps past the last run
+  // with every lane outside the grid, so they change nothing.
+  static_assert(PD >= 1, "operands are fetched at least one step ahead");
+  constexpr int S = PD + 1;
+  FFar<V> fs[S] = {};
+  FNear<V> ns[S] = {};
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    load_far(q, fs[q]);
+    load_near(q, ns[q]);
+  }
   V tprev = 0, zprev = 0;  // own result and T0 of the previous step
   auto step = [&](int t, const FFar<V> &fc, const FFar<V> &fn, FFar<V> &f2, const FNear<V> &nc,
                   FNear<V> &nx2, int tnear) {
     V tU = 0, zU = 0;
     load_up(t, fc.act, tU, zU);
-    load_far(t + 2, f2);
+    load_far(t + PD, f2);
     load_near(tnear, nx2);
     // lane shifts read other lanes' registers: they run with every lane
     // enabled, outside the per-cell branch
@@ -774,10 +778,10 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     tprev = tnew;
     zprev = fc.t0;
   };
-  for (int t = 0; t < nsteps; t += 3) {
-    step(t, fA, fB, fC, nA, nC, t + 2);
-    step(t + 1, fB, fC, fA, nB, nA, t + 3);
-    step(t + 2, fC, fA, fB, nC, nB, t + 4);
+  for (int t = 0; t < nsteps; t += S) {
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+      step(t + q, fs[q], fs[(q + 1) % S], fs[(q + PD) % S], ns[q], ns[(q + PD) % S], t + q + PD);
   }
   if (MODE == 2) {
     ss = wave_sum(acc_s);
@@ -786,12 +790,12 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   wave_fence();
 }
 
-template <typename V, int XD, int YD, int MODE, bool UP>
+template <typename V, int XD, int YD, int MODE, bool UP, int PD>
 __device__ void fwd_sweep(const FwdCtx<V> &C, Team &T, GMEM V *tau, StripAcc &acc) {
   team_sweep_begin(T);
   for (int k = T.w; k < T.ns; k += T.W) {
     double ss = 0, sm = 0;
-    fwd_strip_pass<V, XD, YD, MODE, UP>(C, T, k, tau, ss, sm);
+    fwd_strip_pass<V, XD, YD, MODE, UP, PD>(C, T, k, tau, ss, sm);
     if (MODE == 2) team_strip_term(T, k, ss, sm, 0.0, acc);
   }
   team_sweep_end(T);
@@ -802,15 +806,15 @@ __device__ void fwd_sweep(const FwdCtx<V> &C, Team &T, GMEM V *tau, StripAcc &ac
 // the first sweep in the plus skew, the middle two in the minus skew, the
 // last in the plus skew, with the field copied between the skews at the
 // two family changes.  Returns whether the stop rule (:1091-1115) holds.
-template <typename V, bool UP>
+template <typename V, bool UP, int PD>
 __device__ bool fwd_iteration(const FwdCtx<V> &C, Team &T, GMEM V *tP, GMEM V *tM) {
   StripAcc acc;
-  fwd_sweep<V, 1, 1, 1, UP>(C, T, tP, acc);
+  fwd_sweep<V, 1, 1, 1, UP, PD>(C, T, tP, acc);
   skew_copy<V, 0>(T, tP, tM, C.nx, C.ny, C.R, T.ns, C.lane);
-  fwd_sweep<V, 1, -1, 0, UP>(C, T, tM, acc);
-  fwd_sweep<V, -1, 1, 0, UP>(C, T, tM, acc);
+  fwd_sweep<V, 1, -1, 0, UP, PD>(C, T, tM, acc);
+  fwd_sweep<V, -1, 1, 0, UP, PD>(C, T, tM, acc);
   skew_copy<V, 1>(T, tM, tP, C.nx, C.ny, C.R, T.ns, C.lane);
-  fwd_sweep<V, -1, -1, 2, UP>(C, T, tP, acc);
+  fwd_sweep<V, -1, -1, 2, UP, PD>(C, T, tP, acc);
   const StripAcc r = team_strip_total(T, acc);
   const double l1 = r.s / (double)C.nx / (double)C.ny;
   // the FP32 phase stops at kMixFwdTolScale times the reference's tolerance:
@@ -939,7 +943,7 @@ __device__ void fwd_ctx_init(FwdCtx<V> &C, const Grid &g, const SkewGeom &sg, in
 // T = tau*T0 (src/eikon_solver_2d.f90:1124) in both skews and in natural
 // layout: the reference's update on tau (UP false), or the mixed mode's
 // upwind update on dtau, with T = (1 + dtau)*T0 (UP true).
-template <bool UP>
+template <bool UP, int PD>
 __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rect[4]) {
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
@@ -958,7 +962,7 @@ __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rec
   int iter = 1;
   bool conv = false;
   for (; iter <= kFwdMaxIter; ++iter)
-    if (fwd_iteration<double, UP>(C, T, gm(tauP), gm(tauM))) { conv = true; break; }
+    if (fwd_iteration<double, UP, PD>(C, T, gm(tauP), gm(tauM))) { conv = true; break; }
   if (!conv) iter = kFwdMaxIter;
   auto tau_of = [&](double v) { return UP ? 1.0 + v : v; };
   const GMEM double *gtau = gm(tauP);
@@ -993,8 +997,9 @@ __device__ bool fwd_item_rect(const SlotDesc &D, int lane, int rect[4]) {
 }
 
 // The strip kernels run one item per workgroup of W = blockDim.x/64 waves
-// (a team), with team_lds_bytes of dynamic LDS when W > 1.
-template <bool TEAM>
+// (a team), with team_lds_bytes of dynamic LDS when W > 1, fetching the
+// sweeps' operands PD steps ahead.
+template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_strip(const SlotDesc *sd) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
@@ -1003,12 +1008,12 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_strip(const S
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
   Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8);
-  fwd_finish64<false>(D, T, lane, rect);
+  fwd_finish64<false, PD>(D, T, lane, rect);
 }
 
 // Mixed forward, FP32 phase: the FP32 iterations (count to iters[3]), then
 // dtau widened into the FP64 plus skew for the polish.
-template <bool TEAM>
+template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_mixed32(const SlotDesc *sd) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
@@ -1031,7 +1036,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_mixed32(const
   float *dP = f32 + 6 * SE, *dM = f32 + 7 * SE;
   int iter = 1;
   for (; iter <= kFwdMaxIter; ++iter)
-    if (fwd_iteration<float, true>(C, T, gm(dP), gm(dM))) break;
+    if (fwd_iteration<float, true, PD>(C, T, gm(dP), gm(dM))) break;
   if (iter > kFwdMaxIter) iter = kFwdMaxIter;
   double *tauP = sk0 + kSkTauP * SE;
   for (size_t e = threadIdx.x; e < SE; e += blockDim.x) gm(tauP)[e] = (double)gm(dP)[e];
@@ -1043,7 +1048,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_mixed32(const
 
 // Mixed forward, FP64 polish: the upwind update in FP64 from the FP32 result
 // to the reference's stop rule, and the outputs.
-template <bool TEAM>
+template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_polish(const SlotDesc *sd) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
@@ -1052,7 +1057,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_polish(const 
   int rect[4];
   if (!fwd_item_rect(D, lane, rect)) return;
   Team T = team_init<TEAM>(lds, D.g.ny, D.sg.ns, 8);
-  fwd_finish64<true>(D, T, lane, rect);
+  fwd_finish64<true, PD>(D, T, lane, rect);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -1267,7 +1272,7 @@ struct AdjCtx {
 // One adjoint sweep over strip c, in place in the sweep's own skew; MODE as
 // for the forward, and MODE 2 also accumulates max |Ta| (acc_a) for the
 // FP32 stop rule.
-template <typename V, int XD, int YD, int MODE>
+template <typename V, int XD, int YD, int MODE, int PD>
 __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V *ta, double &ss,
                                double &sm, double &sa) {
   constexpr int SK = XD == YD ? 0 : 1;
@@ -1308,14 +1313,17 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
   auto load_near = [&](int t, ANear<V> &q) {
     adj_load_near(is, SE, step_pos<XD, YD>(c, t, lane, ny, L, R), q);
   };
-  // operand sets as in the forward pass: unrolled by three over three sets
-  // (far and near two steps ahead)
-  AFar<V> fA, fB, fC;
-  ANear<V> nA, nB, nC;
-  load_far(0, fA);
-  load_far(1, fB);
-  load_near(0, nA);
-  load_near(1, nB);
+  // operand sets as in the forward pass: unrolled by S = PD+1 over S sets
+  // (far and near PD steps ahead)
+  static_assert(PD >= 1, "operands are fetched at least one step ahead");
+  constexpr int S = PD + 1;
+  AFar<V> fs[S];
+  ANear<V> ns[S];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) {
+    load_far(q, fs[q]);
+    load_near(q, ns[q]);
+  }
   V tprev = 0;
   auto step = [&](int t, const AFar<V> &fc, const AFar<V> &fn, AFar<V> &f2, const ANear<V> &nc,
                   ANear<V> &nx2, int tnear) {
@@ -1326,7 +1334,7 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
       if (T.multi()) team_wait(T, pw, base_in + t - lag + 1);
       tU = team_read<V>(T, pw, t - lag);
     }
-    load_far(t + 2, f2);
+    load_far(t + PD, f2);
     load_near(tnear, nx2);
     V tUx = lane_shift<-XD>(tprev);
     V tDx = lane_shift<XD>(fn.ta);
@@ -1355,10 +1363,10 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
     }
     tprev = tnew;
   };
-  for (int t = 0; t < nsteps; t += 3) {
-    step(t, fA, fB, fC, nA, nC, t + 2);
-    step(t + 1, fB, fC, fA, nB, nA, t + 3);
-    step(t + 2, fC, fA, fB, nC, nB, t + 4);
+  for (int t = 0; t < nsteps; t += S) {
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+      step(t + q, fs[q], fs[(q + 1) % S], fs[(q + PD) % S], ns[q], ns[(q + PD) % S], t + q + PD);
   }
   if (MODE == 2) {
     ss = wave_sum(acc_s);
@@ -1368,12 +1376,12 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
   wave_fence();
 }
 
-template <typename V, int XD, int YD, int MODE>
+template <typename V, int XD, int YD, int MODE, int PD>
 __device__ void adj_sweep(const AdjCtx<V> &C, Team &T, GMEM V *ta, StripAcc &acc) {
   team_sweep_begin(T);
   for (int k = T.w; k < T.ns; k += T.W) {
     double ss = 0, sm = 0, sa = 0;
-    adj_strip_pass<V, XD, YD, MODE>(C, T, k, ta, ss, sm, sa);
+    adj_strip_pass<V, XD, YD, MODE, PD>(C, T, k, ta, ss, sm, sa);
     if (MODE == 2) team_strip_term(T, k, ss, sm, sa, acc);
   }
   team_sweep_end(T);
@@ -1382,16 +1390,16 @@ __device__ void adj_sweep(const AdjCtx<V> &C, Team &T, GMEM V *ta, StripAcc &acc
 // One iteration: (xdirec, ydirec) = (-1,-1), (-1,1), (1,-1), (1,1), x
 // ascending first (src/eikon_solver_2d.f90:1383-1390); with SNAP the
 // snapshot and the residuals (:1412-1434) are taken.
-template <typename V, bool SNAP>
+template <typename V, bool SNAP, int PD>
 __device__ void adj_iteration(const AdjCtx<V> &C, Team &T, GMEM V *tP, GMEM V *tM, double &s, double &m,
                               double &a) {
   StripAcc acc;
-  adj_sweep<V, 1, 1, SNAP ? 1 : 0>(C, T, tP, acc);
+  adj_sweep<V, 1, 1, SNAP ? 1 : 0, PD>(C, T, tP, acc);
   skew_copy<V, 0>(T, tP, tM, C.nx, C.ny, C.R, T.ns, C.lane);
-  adj_sweep<V, 1, -1, 0>(C, T, tM, acc);
-  adj_sweep<V, -1, 1, 0>(C, T, tM, acc);
+  adj_sweep<V, 1, -1, 0, PD>(C, T, tM, acc);
+  adj_sweep<V, -1, 1, 0, PD>(C, T, tM, acc);
   skew_copy<V, 1>(T, tM, tP, C.nx, C.ny, C.R, T.ns, C.lane);
-  adj_sweep<V, -1, -1, SNAP ? 2 : 0>(C, T, tP, acc);
+  adj_sweep<V, -1, -1, SNAP ? 2 : 0, PD>(C, T, tP, acc);
   const StripAcc r = team_strip_total(T, acc);
   s = r.s;
   m = r.m;
@@ -1421,7 +1429,7 @@ __device__ void adj_output(const Grid &g, const SkewGeom &sg, const double *TaP,
   }
 }
 
-template <bool TEAM>
+template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_strip(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
@@ -1442,7 +1450,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_strip(const S
   bool conv = false;
   for (; iter <= kAdjMaxIter; ++iter) {
     double s, m, a;
-    adj_iteration<double, true>(C, T, gm(TaP), gm(TaM), s, m, a);
+    adj_iteration<double, true, PD>(C, T, gm(TaP), gm(TaM), s, m, a);
     const double l1 = s / (double)(g.nx * g.ny);
     if (fabs(l1) < kAdjTol && m < kAdjTol) { conv = true; break; }
   }
@@ -1458,7 +1466,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_strip(const S
 // Mixed adjoint, FP32 phase: iterations on the float coefficients (float
 // skews from double skew 16: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
 // minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
-template <bool TEAM>
+template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
@@ -1480,7 +1488,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const
   bool done = false;
   for (; iter <= kAdjMaxIter; ++iter) {
     double s, m, a;
-    adj_iteration<float, true>(C, T, gm(taP), gm(taM), s, m, a);
+    adj_iteration<float, true, PD>(C, T, gm(taP), gm(taM), s, m, a);
     const double l1 = s / (double)(g.nx * g.ny);
     if ((fabs(l1) < kAdjTol && m < kAdjTol) || m <= (double)kMixRelStop * a) {
       done = true;
@@ -1499,7 +1507,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const
 
 // Mixed adjoint, FP64 polish: one iteration of the reference's update on the
 // FP64 coefficients, and the masked output.
-template <bool TEAM>
+template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
   const SlotDesc D = sd[blockIdx.x];
@@ -1517,7 +1525,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const 
   C.is[1] = gm(sk0 + kAF * SE);
   C.snap = nullptr;
   double s, m, a;
-  adj_iteration<double, false>(C, T, gm(TaP), gm(TaM), s, m, a);
+  adj_iteration<double, false, PD>(C, T, gm(TaP), gm(TaM), s, m, a);
   adj_output(g, sg, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
   if (threadIdx.x == 0) {
     if (team_failed(T)) *D.status = kETeamWait;
@@ -1577,6 +1585,16 @@ dim3 team_block(const GroupDims &gd, int st) {
 size_t team_lds(const GroupDims &gd, int st, int vbytes) {
   return team_lds_bytes(gd.waves[st], gd.max_ny, gd.max_ns, vbytes);
 }
+// The kernel form of a stage's launch: team or one wave, and the prefetch
+// depth (kDeepPrefetch or the default two steps).
+template <typename K>
+K pick_form(const GroupDims &gd, int st, K team2, K team_deep, K one2, K one_deep) {
+  const bool deep = gd.depth[st] >= kDeepPrefetch;
+  return gd.waves[st] > 1 ? (deep ? team_deep : team2) : (deep ? one_deep : one2);
+}
+dim3 stage_block(const GroupDims &gd, int st) {
+  return gd.waves[st] > 1 ? team_block(gd, st) : dim3(64);
+}
 }  // namespace
 
 int team_waves(int want, int ny, int ns, int stage) {
@@ -1588,30 +1606,25 @@ int team_waves(int want, int ny, int ns, int stage) {
 
 void launch_fwd_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
   const int st = kStageFwd64;
-  if (gd.waves[st] > 1)
-    hipLaunchKernelGGL(k_fwd_strip<true>, dim3(gd.count), team_block(gd, st), team_lds(gd, st, 8),
-                       s, sd);
-  else
-    hipLaunchKernelGGL(k_fwd_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd);
+  // the full FP64 cell reads more operands (both x neighbours and their
+  // T0): a third operand set would already spill its registers, so it has
+  // the two-step form only
+  auto k = gd.waves[st] > 1 ? k_fwd_strip<true, 2> : k_fwd_strip<false, 2>;
+  hipLaunchKernelGGL(k, dim3(gd.count), stage_block(gd, st), team_lds(gd, st, 8), s, sd);
 }
 
 void launch_fwd_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
   const int st = kStageFwd32;
-  if (gd.waves[st] > 1)
-    hipLaunchKernelGGL(k_fwd_mixed32<true>, dim3(gd.count), team_block(gd, st),
-                       team_lds(gd, st, 4), s, sd);
-  else
-    hipLaunchKernelGGL(k_fwd_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 4), s,
-                       sd);
+  auto k = pick_form(gd, st, k_fwd_mixed32<true, 2>, k_fwd_mixed32<true, kDeepPrefetch>,
+                     k_fwd_mixed32<false, 2>, k_fwd_mixed32<false, kDeepPrefetch>);
+  hipLaunchKernelGGL(k, dim3(gd.count), stage_block(gd, st), team_lds(gd, st, 4), s, sd);
 }
 
 void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
   const int st = kStageFwd64;
-  if (gd.waves[st] > 1)
-    hipLaunchKernelGGL(k_fwd_polish<true>, dim3(gd.count), team_block(gd, st),
-                       team_lds(gd, st, 8), s, sd);
-  else
-    hipLaunchKernelGGL(k_fwd_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd);
+  auto k = pick_form(gd, st, k_fwd_polish<true, 2>, k_fwd_polish<true, kDeepPrefetch>,
+                     k_fwd_polish<false, 2>, k_fwd_polish<false, kDeepPrefetch>);
+  hipLaunchKernelGGL(k, dim3(gd.count), stage_block(gd, st), team_lds(gd, st, 8), s, sd);
 }
 
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
@@ -1628,32 +1641,23 @@ void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
 
 void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
   const int st = kStageAdj64;
-  if (gd.waves[st] > 1)
-    hipLaunchKernelGGL(k_adj_strip<true>, dim3(gd.count), team_block(gd, st),
-                       team_lds(gd, st, 8), s, sd, which);
-  else
-    hipLaunchKernelGGL(k_adj_strip<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd,
-                       which);
+  auto k = pick_form(gd, st, k_adj_strip<true, 2>, k_adj_strip<true, kDeepPrefetch>,
+                     k_adj_strip<false, 2>, k_adj_strip<false, kDeepPrefetch>);
+  hipLaunchKernelGGL(k, dim3(gd.count), stage_block(gd, st), team_lds(gd, st, 8), s, sd, which);
 }
 
 void launch_adj_mixed32(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
   const int st = kStageAdj32;
-  if (gd.waves[st] > 1)
-    hipLaunchKernelGGL(k_adj_mixed32<true>, dim3(gd.count), team_block(gd, st),
-                       team_lds(gd, st, 4), s, sd, which);
-  else
-    hipLaunchKernelGGL(k_adj_mixed32<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 4), s,
-                       sd, which);
+  auto k = pick_form(gd, st, k_adj_mixed32<true, 2>, k_adj_mixed32<true, kDeepPrefetch>,
+                     k_adj_mixed32<false, 2>, k_adj_mixed32<false, kDeepPrefetch>);
+  hipLaunchKernelGGL(k, dim3(gd.count), stage_block(gd, st), team_lds(gd, st, 4), s, sd, which);
 }
 
 void launch_adj_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
   const int st = kStageAdj64;
-  if (gd.waves[st] > 1)
-    hipLaunchKernelGGL(k_adj_polish<true>, dim3(gd.count), team_block(gd, st),
-                       team_lds(gd, st, 8), s, sd, which);
-  else
-    hipLaunchKernelGGL(k_adj_polish<false>, dim3(gd.count), dim3(64), team_lds(gd, st, 8), s, sd,
-                       which);
+  auto k = pick_form(gd, st, k_adj_polish<true, 2>, k_adj_polish<true, kDeepPrefetch>,
+                     k_adj_polish<false, 2>, k_adj_polish<false, kDeepPrefetch>);
+  hipLaunchKernelGGL(k, dim3(gd.count), stage_block(gd, st), team_lds(gd, st, 8), s, sd, which);
 }
 
 }  // namespace satt

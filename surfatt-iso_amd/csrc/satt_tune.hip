@@ -1,14 +1,15 @@
 // Autotuner of the sweep kernels' launch shape: the team size W (waves per
-// item) for a grid shape and a launch-group size.
+// item) and the prefetch depth, for a grid shape and a launch-group size.
 //
 // One wave per item is the throughput form when the items fill the device;
 // a team of W waves cuts an item's latency on grids of several strips, at
 // the price of W wave slots and some waiting in the pipeline's fill.  Where
 // the balance lies depends on the grid (strips, rows), the number of items
 // launched together and the precision, so it is measured: a synthetic batch
-// of that shape runs once per candidate W, and the fastest is kept in the
-// context's table, which plans (satt_plan_set_waves(plan, 0)) and servers
-// (waves 0) consult.
+// of that shape runs once per candidate W, then with the deep prefetch (which
+// pays where few waves share a SIMD), and each stage's fastest shape is kept
+// in the context's table, which plans (satt_plan_set_waves(plan, 0)) and
+// servers (waves 0) consult.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -95,38 +96,14 @@ extern "C" int satt_tune_waves(satt_ctx *ctx, int nx, int ny, int nitems, int pr
   int st = satt_plan_create(ctx, &B, precision, &P);
   if (st < 0) return st;
   satt_tune_result r{};
-  const int ns = make_skew_geom(nx, ny).ns;
-  double best = 0, best_stage[kStages] = {0, 0, 0, 0};
-  for (int W = 1; W <= kMaxTeam && st >= 0; ++W) {
-    if (team_waves(W, ny, ns, kStageFwd32) != W) break;  // capped: as a smaller W
-    st = satt_plan_set_waves(P, W);
-    for (int rep = 0; rep < 2 && st >= 0; ++rep) {  // the first run warms up
-      st = satt_plan_run(P, 1, 1, SATT_LAUNCH_BATCHED);
-      if (st >= 0) st = satt_plan_wait(P);
-    }
-    satt_plan_stats s{};
-    if (st >= 0) st = satt_plan_get_stats(P, &s);
-    if (st < 0) break;
-    r.ms[W - 1] = s.ms_total;
-    if (r.waves == 0 || s.ms_total < best) {
-      best = s.ms_total;
-      r.waves = W;
-    }
-    // the stages run one after another on the stream, so each keeps its
-    // own fastest team size
-    const double sm[kStages] = {s.ms_fwd_fp32, s.ms_fwd_fp64, s.ms_adj_fp32, s.ms_adj_fp64};
-    for (int k = 0; k < kStages; ++k) {
-      r.stage_ms[W - 1][k] = sm[k];
-      if (r.stage_waves[k] == 0 || sm[k] < best_stage[k]) {
-        best_stage[k] = sm[k];
-        r.stage_waves[k] = W;
-      }
-    }
-  }
+  st = tune_plan_shapes(P, r);
   satt_plan_destroy(P);
   if (st < 0) return st;
   TuneWaves tw;
-  for (int k = 0; k < kStages; ++k) tw.w[k] = r.stage_waves[k];
+  for (int k = 0; k < kStages; ++k) {
+    tw.w[k] = r.stage_waves[k];
+    tw.d[k] = r.stage_depth[k];
+  }
   ctx_tune_store(ctx, TuneKey::of(nx, ny, nitems, precision), tw);
   if (out) *out = r;
   return SATT_OK;

@@ -138,16 +138,16 @@ class Context:
         self.ptr = p
 
     def tune_waves(self, nx, ny, nitems, precision=PREC_MIXED):
-        """Autotuner of the team size (waves per item) for an nx x ny grid and
-        a launch of nitems items; returns {"waves": W, "ms": [ms for W=1..8], "stage_waves":
-        [W per sweep stage], "stage_ms": [[ms per stage] for W=1..8]}
-        and keeps W in the context's table."""
+        """Autotuner of the launch shape (team size and prefetch depth per
+        sweep stage) for an nx x ny grid and a launch of nitems items; returns
+        {"waves": W, "ms": [ms for W=1..8], "stage_waves": [W per sweep stage],
+        "stage_ms": [[ms per stage] for W=1..8], "stage_depth": [2 or 4 per
+        stage], "stage_ms_deep": [[ms per stage] at the chosen W, at W=1]} and
+        keeps the choice in the context's table."""
         r = TuneResult()
         self._check(self.lib.satt_tune_waves(self.ptr, int(nx), int(ny), int(nitems),
                                              int(precision), ctypes.byref(r)))
-        return {"waves": r.waves, "ms": [float(v) for v in r.ms],
-                "stage_waves": [int(v) for v in r.stage_waves],
-                "stage_ms": [[float(v) for v in row] for row in r.stage_ms]}
+        return r.as_dict()
 
     def close(self):
         if self.ptr:
@@ -568,13 +568,20 @@ class Plan:
             raise SattError(E_ARG, "four stage team sizes expected")
         self.ctx._check(self.lib.satt_plan_set_stage_waves(self.ptr, _i(w)))
 
+    def set_stage_depth(self, depth):
+        """Prefetch depth (2 or 4 steps) per sweep stage, in the order of
+        set_stage_waves."""
+        d = np.ascontiguousarray(depth, dtype=np.int32)
+        if d.shape != (4,):
+            raise SattError(E_ARG, "four stage depths expected")
+        self.ctx._check(self.lib.satt_plan_set_stage_depth(self.ptr, _i(d)))
+
     def tune_waves(self):
-        """Autotune the team size of each sweep stage on this plan's inputs."""
+        """Autotune the team size and prefetch depth of each sweep stage on
+        this plan's inputs (as Context.tune_waves)."""
         r = TuneResult()
         self.ctx._check(self.lib.satt_plan_tune_waves(self.ptr, ctypes.byref(r)))
-        return {"waves": r.waves, "ms": [float(v) for v in r.ms],
-                "stage_waves": [int(v) for v in r.stage_waves],
-                "stage_ms": [[float(v) for v in row] for row in r.stage_ms]}
+        return r.as_dict()
 
     def set_graph(self, on=True):
         """Replay form: batched runs capture their launch chain into a HIP
@@ -609,7 +616,15 @@ class Plan:
 
 class TuneResult(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int), ("ms", ctypes.c_double * 8),
-                ("stage_waves", ctypes.c_int * 4), ("stage_ms", (ctypes.c_double * 4) * 8)]
+                ("stage_waves", ctypes.c_int * 4), ("stage_ms", (ctypes.c_double * 4) * 8),
+                ("stage_depth", ctypes.c_int * 4), ("stage_ms_deep", (ctypes.c_double * 4) * 2)]
+
+    def as_dict(self):
+        return {"waves": self.waves, "ms": [float(v) for v in self.ms],
+                "stage_waves": [int(v) for v in self.stage_waves],
+                "stage_ms": [[float(v) for v in row] for row in self.stage_ms],
+                "stage_depth": [int(v) for v in self.stage_depth],
+                "stage_ms_deep": [[float(v) for v in row] for row in self.stage_ms_deep]}
 
 
 class ServerConfig(ctypes.Structure):

@@ -2,7 +2,9 @@
 item -- strips of a sweep shared by the waves, edge columns handed over in
 LDS -- give the one-wave results bit for bit, in both precisions, on grids
 of one, two, three and seven strips (full and partial), with the FP64
-results also within the FP64 bar of the oracle."""
+results also within the FP64 bar of the oracle.  So does the deep (four-step)
+prefetch, alone and with teams, and the autotuner keeps each stage's fastest
+(team size, prefetch depth)."""
 import numpy as np
 import pytest
 
@@ -52,16 +54,30 @@ def test_team_multi_grid_and_graph(ctx):
                 same(p.results(k), ref[k], 2, (rep, k))
 
 
+def check_choice(r):
+    """Each sweep stage keeps its fastest (team size, depth): the first
+    fastest W of the two-step sweep, replaced by a deep-prefetch trial (at that
+    W, then at W = 1) only where one is faster."""
+    tried = [w for w in range(8) if r["ms"][w] > 0]
+    for k in range(4):
+        two = [r["stage_ms"][w][k] for w in tried]
+        assert all(t > 0 for t in two), (k, two)
+        best, choice = min(two), (tried[int(np.argmin(two))] + 1, 2)
+        d0, d1 = r["stage_ms_deep"][0][k], r["stage_ms_deep"][1][k]
+        assert d0 > 0 and d1 > 0, (k, d0, d1)
+        if d0 < best:
+            best, choice = d0, (choice[0], 4)
+        if d1 < best:
+            best, choice = d1, (1, 4)
+        assert (r["stage_waves"][k], r["stage_depth"][k]) == choice, (k, r)
+
+
 def test_tuner_and_tuned_plan(ctx):
     r = ctx.tune_waves(260, 60, 2, satt.PREC_MIXED)
     assert 1 <= r["waves"] <= 8 and r["ms"][0] > 0
     tried = [m for m in r["ms"] if m > 0]
     assert r["ms"][r["waves"] - 1] == min(tried)
-    # each sweep stage keeps its own fastest team size
-    for k in range(4):
-        times = [r["stage_ms"][w][k] for w in range(len(tried))]
-        assert all(t > 0 for t in times), (k, times)
-        assert r["stage_ms"][r["stage_waves"][k] - 1][k] == min(times)
+    check_choice(r)
     b = shaped_batch(260, 60, 3, 2, seed=33)
     with ctx.plan(b, satt.PREC_MIXED) as p:
         p.run(True, True)
@@ -78,9 +94,7 @@ def test_plan_tuned_on_its_own_inputs(ctx):
         ref = p.results()
         r = p.tune_waves()
         assert all(1 <= w <= 8 for w in r["stage_waves"])
-        for k in range(4):
-            tried = [r["stage_ms"][w][k] for w in range(8) if r["ms"][w] > 0]
-            assert r["stage_ms"][r["stage_waves"][k] - 1][k] == min(tried)
+        check_choice(r)
         same(p.results(), ref, 2, "after tuning")
         p.set_graph(True)
         p.run(True, True)
@@ -97,6 +111,25 @@ def test_stage_team_sizes_pinned_equal_one_wave(ctx):
         same(p.results(), ref, 2, "stage waves")
         with pytest.raises(satt.SattError):
             p.set_stage_waves([1, 0, 1, 1])
+
+
+@pytest.mark.parametrize("prec", [satt.PREC_FP64, satt.PREC_MIXED])
+@pytest.mark.parametrize("nx,ny,dim", [(41, 41, 3), (150, 97, 2), (449, 50, 2)])
+def test_deep_prefetch_equals_two_step(ctx, prec, nx, ny, dim):
+    b = shaped_batch(nx, ny, 4, dim, seed=37)
+    with ctx.plan(b, prec) as p:
+        p.run(True, True)
+        ref = p.results()
+        for waves in ([1, 1, 1, 1], [3, 2, 4, 3]):
+            p.set_stage_waves(waves)
+            p.set_stage_depth([4, 4, 4, 4])
+            p.run(True, True)
+            same(p.results(), ref, dim, ("deep", waves))
+            p.set_stage_depth([2, 4, 2, 4])
+            p.run(True, True)
+            same(p.results(), ref, dim, ("mixed depths", waves))
+        with pytest.raises(satt.SattError):
+            p.set_stage_depth([2, 3, 2, 2])
 
 
 def test_server_team_results_equal_one_wave(ctx):
