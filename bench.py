@@ -83,19 +83,60 @@ STAGE_OF = {"k_fwd_strip": "ms_fwd_fp64", "k_adj_strip": "ms_adj_fp64",
             "k_adj_mixed32": "ms_adj_fp32", "k_adj_polish": "ms_adj_fp64"}
 
 
-def cpu_baseline(parts, nits, nsample, threads):
+def host_info():
+    """What the host offers this process: its CPU model, the cores the OS
+    reports, the cores this process may run on, and OMP_NUM_THREADS."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"cpu_model": model, "nproc_os": os.cpu_count(), "cores_this_process": affinity,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def scatter_block(count, size, rank):
+    """The reference's static partition of `count` items over `size` ranks
+    (scatter_all_i, src/shared/parallel.f90:1877-1903), 0-based [start, end)."""
+    if count < size:
+        return (rank, rank + 1) if rank < count else (0, 0)
+    per, rem = divmod(count, size)
+    if rank < rem:
+        start = rank * (per + 1)
+        return start, start + per + 1
+    start = rank * per + rem
+    return start, start + per
+
+
+def cpu_baseline(parts, nits, nsample, threads, min_seconds=5.0):
     """The reference's own solver file (oracle/_ref, built from
-    src/eikon_solver_2d.f90) driving the loop body over a sample of items
-    (spread over the parts in proportion) on `threads` host threads;
-    interpolation, misfit and stacking by the oracle.  Falls back to the
-    oracle alone when the reference build is absent."""
+    src/eikon_solver_2d.f90 at the -O3 of its release build) driving the loop
+    body over evenly spaced items of the workload (spread over the parts in
+    proportion) on `threads` host threads, each thread a contiguous block of
+    them as the reference's ranks take theirs; interpolation, misfit and
+    stacking by the oracle.  A first pass of `nsample` items sets the sample
+    size so that the timed pass runs at least `min_seconds`.  Falls back to
+    the oracle alone when the reference build is absent."""
     import oracle as O
     nit = sum(nits)
-    picks = []
-    for q, n in enumerate(nits):
-        k = max(1, round(nsample * n / nit))
-        picks += [(q, int(i)) for i in np.linspace(0, n - 1, k).round().astype(int)]
+
+    def pick(nsamp):
+        out = []
+        for q, n in enumerate(nits):
+            k = max(1, min(n, round(nsamp * n / nit)))
+            out += [(q, int(i)) for i in np.unique(np.linspace(0, n - 1, k).round().astype(int))]
+        return out
+
     if not O.RefSolver.available():
+        picks = pick(nsample)
         t0 = time.perf_counter()
         for q in range(len(parts)):
             sel = [i for qq, i in picks if qq == q]
@@ -103,8 +144,9 @@ def cpu_baseline(parts, nits, nsample, threads):
         dt = time.perf_counter() - t0
         return dict(value=len(picks) / dt, unit="solves/s", cores=threads, kind="port",
                     sample=f"{len(picks)} items of the workload, oracle loop, {dt:.1f} s wall")
-    ref = O.RefSolver()
-    blocks = [picks[k::threads] for k in range(threads)]
+    path = O.REF_O3_PATH if O.RefSolver.available(O.REF_O3_PATH) else O.REF_PATH
+    build = "-O3" if path == O.REF_O3_PATH else "-O2"
+    ref = O.RefSolver(path)
     err = []
 
     def work(items):
@@ -134,24 +176,40 @@ def cpu_baseline(parts, nits, nsample, threads):
         except BaseException as e:  # reported below
             err.append(e)
 
-    old = threading.stack_size()
-    threading.stack_size(ref.STACK)
-    try:
-        ths = [threading.Thread(target=work, args=(blk,)) for blk in blocks]
-        t0 = time.perf_counter()
-        for t in ths:
-            t.start()
-        for t in ths:
-            t.join()
-        dt = time.perf_counter() - t0
-    finally:
-        threading.stack_size(old)
-    if err:
-        raise err[0]
-    return dict(value=len(picks) / dt, unit="solves/s", cores=threads, kind="reference",
-                sample=f"{len(picks)} of the workload's items (evenly spaced), reference "
-                       f"src/eikon_solver_2d.f90 solvers built -O2, {threads} threads, "
-                       f"{dt:.2f} s wall, {dt * threads:.1f} thread-s")
+    def timed(picks):
+        blocks = [picks[slice(*scatter_block(len(picks), threads, r))] for r in range(threads)]
+        old = threading.stack_size()
+        threading.stack_size(ref.STACK)
+        try:
+            ths = [threading.Thread(target=work, args=(blk,)) for blk in blocks]
+            t0 = time.perf_counter()
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join()
+            dt = time.perf_counter() - t0
+        finally:
+            threading.stack_size(old)
+        if err:
+            raise err[0]
+        return dt
+
+    picks = pick(nsample)
+    dt = timed(picks)
+    if dt < min_seconds and len(picks) < nit:      # scale the sample up to the time floor
+        picks = pick(min(nit, int(len(picks) * 1.2 * min_seconds / dt) + threads))
+        dt = timed(picks)
+    passes = 1
+    if dt < min_seconds:                           # every item: repeat the passes
+        passes = int(np.ceil(1.2 * min_seconds / dt))
+        dt = timed(picks * passes)
+    return dict(value=passes * len(picks) / dt, unit="solves/s", cores=threads, kind="reference",
+                sample=f"{len(picks)} of the workload's {nit} items (evenly spaced)"
+                       f"{f', {passes} passes' if passes > 1 else ''}, reference "
+                       f"src/eikon_solver_2d.f90 solvers built {build} (amdflang), {threads} "
+                       f"threads each running a contiguous block of the items as the "
+                       f"reference's ranks do, {dt:.2f} s wall, {dt * threads:.1f} thread-s",
+                host=host_info())
 
 
 def _ref_fwd(ref, xg, yg, a, b, c, fun, x0, y0):
@@ -218,11 +276,27 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
-def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
+def _stream(srv, reqs, gaps):
+    """Submit reqs at the given inter-arrival gaps; wait for every result."""
+    tickets, t0 = [], time.perf_counter()
+    due = t0
+    for r, gap in zip(reqs, gaps):
+        due += gap
+        wait = due - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        tickets.append(srv.submit(r))
+    outs = [srv.wait(t) for t in tickets]
+    return outs, time.perf_counter() - t0
+
+
+def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6, sweep=(), sweep_requests=120):
     """The serving mix (grid sizes 101/201/401/801 squared with probabilities
     0.4/0.3/0.2/0.1, 100 receivers) as a Poisson stream of `nreq` requests at
     `qps`: latency percentiles from submit to results, throughput, and the
-    plain FP64 one-solve-per-launch time of a sample of the same requests."""
+    plain FP64 one-solve-per-launch time of a sample of the same requests.
+    `sweep`: further arrival rates, `sweep_requests` requests each, on the same
+    server, for the latency/throughput curve up to saturation."""
     reqs = W.serving_requests(nreq, seed=seed)
     rng = np.random.default_rng(seed + 100)
     gaps = rng.exponential(1.0 / qps, nreq)
@@ -230,22 +304,29 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
     tuned = {}
     for m in sorted(set(int(r["nx"]) for r in reqs)):
         t = ctx.tune_waves(m, m, 1, prec)
-        tuned[m] = {"waves": t["waves"], "alone_ms_by_waves": [round(v, 2) for v in t["ms"]]}
-    log(f"[bench] tuned team sizes (one request): {tuned}")
+        tuned[m] = {"waves": t["waves"], "stage_waves": t["stage_waves"],
+                    "stage_prefetch_depth": t["stage_depth"],
+                    "alone_ms_by_waves": [round(v, 2) for v in t["ms"]]}
+    log(f"[bench] tuned launch shapes (one request): {tuned}")
+    curve = []
     with satt.Server(ctx, prec) as srv:
         for r in reqs[:4]:                      # warm the kernels and pools
             srv.wait(srv.submit(r))
-        tickets, t0 = [], time.perf_counter()
-        due = t0
-        for r, gap in zip(reqs, gaps):
-            due += gap
-            wait = due - time.perf_counter()
-            if wait > 0:
-                time.sleep(wait)
-            tickets.append(srv.submit(r))
-        outs = [srv.wait(t) for t in tickets]
-        wall = time.perf_counter() - t0
+        outs, wall = _stream(srv, reqs, gaps)
         st = srv.stats()
+        for k, q in enumerate(sweep):
+            sreqs = W.serving_requests(sweep_requests, seed=seed + 11 + k)
+            sg = np.random.default_rng(seed + 200 + k).exponential(1.0 / q, sweep_requests)
+            so, sw = _stream(srv, sreqs, sg)
+            sl = np.array([o["latency_ms"] for o in so])
+            sq = np.array([o["queue_ms"] for o in so])
+            curve.append({"qps": q, "requests": sweep_requests,
+                          "p50_ms": round(float(np.percentile(sl, 50)), 2),
+                          "p99_ms": round(float(np.percentile(sl, 99)), 2),
+                          "queue_p99_ms": round(float(np.percentile(sq, 99)), 2),
+                          "solves_per_s": round(sweep_requests / sw, 2),
+                          "all_ok": bool(all(o["status"] == 0 for o in so))})
+            log(f"[bench] serving at {q} requests/s: {curve[-1]}")
     lat = np.array([o["latency_ms"] for o in outs])
     queue = np.array([o["queue_ms"] for o in outs])
     sizes = np.array([r["nx"] for r in reqs])
@@ -272,9 +353,10 @@ def serve(ctx, prec, nreq, qps, seed=4, plain_sample=6):
             "queue_p99_ms": round(float(np.percentile(queue, 99)), 2),
             "solves_per_s": round(nreq / wall, 2), "wall_s": round(wall, 2),
             "launch_groups": int(st["launches"]), "max_in_flight": int(st["max_in_flight"]),
-            "team_waves": tuned,
+            "launch_shapes": tuned,
             "all_ok": bool(all(o["status"] == 0 for o in outs)),
-            "plain_fp64_per_solve_solves_per_s": None if plain is None else round(plain, 3)}
+            "plain_fp64_per_solve_solves_per_s": None if plain is None else round(plain, 3),
+            "rate_sweep": curve}
 
 
 def dropin_leg(ctx, part, prec, calls=3):
@@ -518,11 +600,13 @@ def ker3d_leg(ctx, cpu, n_inv=(8, 9, 8), nset=5, chi0=2.5, weight=1.0):
 
 
 def host_threads():
+    """The CPU threads this job is given: the cores it may run on, capped by
+    OMP_NUM_THREADS where the host sets it (the share of a shared host)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n)), 16))
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
 
 
 def main():
@@ -534,6 +618,8 @@ def main():
                     help="skip the HIP-graph replay timing of the same step")
     ap.add_argument("--no-tune", action="store_true",
                     help="one wave per item, no team-size tuning (and no comparison run)")
+    ap.add_argument("--stage-depth", default=None, metavar="D,D,D,D",
+                    help="with --stage-waves: prefetch depth per stage (2 or 4; default 2)")
     ap.add_argument("--stage-waves", default=None, metavar="W,W,W,W",
                     help="team sizes of the four sweep stages instead of tuning them "
                          "(forward FP32, forward FP64, adjoint FP32, adjoint FP64)")
@@ -544,8 +630,13 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving stream (0: skip the serving measurement)")
     ap.add_argument("--serve-qps", type=float, default=40.0)
+    ap.add_argument("--serve-sweep", default="20,40,80,160,320",
+                    help="further arrival rates (requests/s) for the serving curve; '' for none")
+    ap.add_argument("--serve-sweep-requests", type=int, default=120)
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="mixed mode: skip the FP64 batched run and the error report")
+    ap.add_argument("--plain-team-items", type=int, default=4,
+                    help="items timed one at a time through a one-item tuned strip plan")
     ap.add_argument("--plain-items", type=int, default=48,
                     help="items timed in the one-solve-per-launch FP64 form")
     ap.add_argument("--no-model-side", action="store_true",
@@ -591,7 +682,9 @@ def main():
     # serving first, on an otherwise idle device
     serving = None
     if args.serve_requests > 0:
-        serving = serve(ctx, prec, args.serve_requests, args.serve_qps)
+        sweep = [float(v) for v in args.serve_sweep.split(",")] if args.serve_sweep else []
+        serving = serve(ctx, prec, args.serve_requests, args.serve_qps, sweep=sweep,
+                        sweep_requests=args.serve_sweep_requests)
         log(f"[bench] serving: {serving}")
     with ctx.plan_multi([bt for bt, _ in pairs]) as p:   # observed: FP64 forward, true model
         p.run(True, False)
@@ -606,10 +699,13 @@ def main():
         ttune, tuned_w = None, None
         if args.stage_waves:
             sw = [int(v) for v in args.stage_waves.split(",")]
+            sd = [int(v) for v in args.stage_depth.split(",")] if args.stage_depth else [2] * 4
             plan.set_stage_waves(sw)
-            tuned_w = {"stage_waves": sw, "stage_ms": [[0.0] * 4] * 8, "pinned": True}
+            plan.set_stage_depth(sd)
+            tuned_w = {"stage_waves": sw, "stage_depth": sd, "stage_ms": [[0.0] * 4] * 8,
+                       "pinned": True}
             ttune = 0.0
-            log(f"[bench] team sizes pinned: {sw}")
+            log(f"[bench] launch shape pinned: waves {sw}, prefetch depth {sd}")
         elif not args.no_tune:
             t1 = time.perf_counter()
             tuned_w = plan.tune_waves()     # on the plan's own inputs
@@ -647,6 +743,7 @@ def main():
         one_wave = None
         if not args.no_tune and not args.stage_waves:
             plan.set_waves(1)
+            plan.set_stage_depth([2, 2, 2, 2])
             plan.run(True, True)
             plan.wait()
             t1 = time.perf_counter()
@@ -659,8 +756,13 @@ def main():
                         "results_same_as_one_wave": bool(same)}
             if tuned_w is not None:
                 one_wave["stage_waves"] = dict(zip(STAGES, tuned_w["stage_waves"]))
+                one_wave["stage_prefetch_depth"] = dict(zip(STAGES, tuned_w["stage_depth"]))
                 one_wave["tune_stage_ms_by_waves"] = {
                     st: [round(tuned_w["stage_ms"][w][k], 2) for w in range(len(tuned_w["stage_ms"]))]
+                    for k, st in enumerate(STAGES)}
+                one_wave["tune_stage_ms_deep_prefetch"] = {
+                    st: {"at_chosen_waves": round(tuned_w["stage_ms_deep"][0][k], 2),
+                         "at_one_wave": round(tuned_w["stage_ms_deep"][1][k], 2)}
                     for k, st in enumerate(STAGES)}
             log(f"[bench] one wave per item: {one_wave}")
     log(f"[bench] timed {args.steps} steps in {dt:.3f}s")
@@ -687,9 +789,13 @@ def main():
 
     # mixed mode: the same step in FP64 (throughput and the error of the mixed
     # results against it), and a sample of items against the FP64 oracle
-    fp64_value, mixed_err = None, None
+    fp64_value, mixed_err, fp64_shape, stage_cmp = None, None, None, None
     if mixed and not args.no_fp64_compare:
         with ctx.plan_multi(parts, satt.PREC_FP64) as p64:
+            # the FP64 form gets the same launch-shape tuning as the mixed one
+            t1 = time.perf_counter()
+            tw64 = p64.tune_waves() if not args.no_tune else None
+            t64 = time.perf_counter() - t1
             p64.run(True, True)
             p64.wait()
             t1 = time.perf_counter()
@@ -697,7 +803,26 @@ def main():
             p64.wait()
             fp64_value = nit / (time.perf_counter() - t1)
             o64 = [p64.results(k) for k in range(len(parts))]
-        log(f"[bench] batched FP64: {fp64_value:.1f} solves/s")
+            s64 = p64.stats()
+        if tw64 is not None:
+            fp64_shape = {"tuning_s": round(t64, 2),
+                          "stage_waves": dict(zip(STAGES, tw64["stage_waves"])),
+                          "stage_prefetch_depth": dict(zip(STAGES, tw64["stage_depth"]))}
+        # per-stage device times of one step, mixed against FP64, both tuned
+        stage_cmp = {
+            "forward_ms": {"mixed_fp32_phase": round(st["ms_fwd_fp32"], 3),
+                           "mixed_fp64_polish": round(st["ms_fwd_fp64"], 3),
+                           "mixed_total": round(st["ms_fwd_fp32"] + st["ms_fwd_fp64"], 3),
+                           "fp64": round(s64["ms_fwd_fp64"], 3)},
+            "adjoint_ms": {"mixed_fp32_phase": round(st["ms_adj_fp32"], 3),
+                           "mixed_fp64_polish": round(st["ms_adj_fp64"], 3),
+                           "mixed_total": round(st["ms_adj_fp32"] + st["ms_adj_fp64"], 3),
+                           "fp64": round(s64["ms_adj_fp64"], 3)},
+            "setup_ms": {"mixed": round(st["ms_fwd_setup"] + st["ms_adj_setup"], 3),
+                         "fp64": round(s64["ms_fwd_setup"] + s64["ms_adj_setup"], 3)},
+        }
+        log(f"[bench] batched FP64 (tuned {fp64_shape}): {fp64_value:.1f} solves/s; "
+            f"stages {stage_cmp}")
         cat = lambda key, ol: np.concatenate([o[key].ravel() for o in ol])
         d = np.abs(cat("tt_fwd", outs) - cat("tt_fwd", o64))
         mixed_err = {
@@ -751,6 +876,28 @@ def main():
             tot += k
         plain = tot / pdt
         log(f"[bench] plain per-solve FP64: {plain:.1f} solves/s")
+    # the same one-solve-at-a-time use through the strip kernels: a plan of
+    # one item, its team size and prefetch depth tuned, one launch per stage
+    plain_team = None
+    if args.plain_team_items > 0:
+        tot, pdt, shapes = 0, 0.0, []
+        for b, n in zip(parts, nits):
+            k = max(1, round(args.plain_team_items * n / nit))
+            for it in np.unique(np.linspace(0, n - 1, k).round().astype(int)):
+                with ctx.plan(_subset(b, np.array([it])), satt.PREC_FP64) as pp:
+                    if not args.no_tune:
+                        tw = pp.tune_waves()
+                        shapes.append(tw["stage_waves"][1:4:2])
+                    pp.run(True, True)
+                    pp.wait()
+                    t1 = time.perf_counter()
+                    pp.run(True, True)
+                    pp.wait()
+                    pdt += time.perf_counter() - t1
+                tot += 1
+        plain_team = {"solves_per_s": round(tot / pdt, 3), "items": tot,
+                      "fwd_adj_team_waves": shapes}
+        log(f"[bench] one-item team-strip FP64: {plain_team}")
 
     cpu = None
     if not args.no_cpu:
@@ -784,9 +931,13 @@ def main():
         "config": {"workload": args.workload, "description": desc, "items": nit,
                    "grid": grids[0] if len(grids) == 1 else grids, "precision": args.precision,
                    "launch": "batched", "chunk": st["chunk"],
-                   "teams": ("one wave per item" if args.no_tune else
-                             f"pinned per stage {args.stage_waves}" if args.stage_waves else
-                             "autotuned per launch group and stage"),
+                   "launch_shape": ("one wave per item, two-step prefetch" if args.no_tune else
+                                    f"pinned per stage: waves {args.stage_waves}, prefetch depth "
+                                    f"{args.stage_depth or '2,2,2,2'}" if args.stage_waves else
+                                    "team size and prefetch depth autotuned per stage"),
+                   "stage_waves": dict(zip(STAGES, tuned_w["stage_waves"])) if tuned_w else None,
+                   "stage_prefetch_depth": (dict(zip(STAGES, tuned_w["stage_depth"]))
+                                            if tuned_w else None),
                    "mean_iters_fwd": float(np.concatenate([o["iters"][:, 0] for o in outs]).mean()),
                    "mean_iters_adj": float(np.concatenate([o["iters"][:, 1] for o in outs]).mean()),
                    "mean_iters_fwd_fp32": float(np.concatenate([o["iters_fp32"][:, 0] for o in outs]).mean()),
@@ -804,9 +955,12 @@ def main():
         "graph_replay": graph,
         "team_tuning": one_wave,
         "batched_fp64": None if fp64_value is None else round(fp64_value, 3),
+        "batched_fp64_launch_shape": fp64_shape,
+        "mixed_vs_fp64_stages": stage_cmp,
         "mixed_error": mixed_err,
         "dropin_end_to_end": dropin,
         "plain_fp64_per_solve": None if plain is None else round(plain, 3),
+        "plain_fp64_team_strip_per_solve": plain_team,
         "batched_vs_plain": None if plain is None else round(value / plain, 2),
         "serving": serving,
         "cpu_baseline": cpu,

@@ -21,6 +21,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libsatt_ref.so")
+# the same solver file at the reference's release optimisation (-O3), for timing
+REF_O3_PATH = os.path.join(HERE, "_ref", "libsatt_ref_o3.so")
 
 OK, W_MAXITER, E_ARG, E_SRC_ON_BOUNDARY, E_REC_OUT_OF_GRID, E_NOMEM = 0, 1, -1, -2, -3, -4
 
