@@ -906,8 +906,10 @@ def main():
         log(f"[bench] cpu baseline: {cpu['value']:.2f} solves/s on {th} threads")
 
     hbm = None if args.no_hbm_probe else ctx.hbm_probe(1 << 31)
+    lds = None if args.no_hbm_probe else ctx.lds_probe()
     if hbm is not None:
-        log(f"[bench] streaming kernels: read {hbm[0]:.0f} GB/s, copy {hbm[1]:.0f} GB/s")
+        log(f"[bench] streaming kernels: read {hbm[0]:.0f} GB/s, copy {hbm[1]:.0f} GB/s; "
+            f"LDS read {lds[0]:.0f} GB/s of {lds[1]:.0f}")
     side = None
     if not args.no_model_side:
         side = model_side(ctx, cpu=not args.no_cpu)
@@ -947,6 +949,12 @@ def main():
                      "traffic": traffic,
                      "achievable_GBs": None if hbm is None else
                      {"stream_read": round(hbm[0], 1), "stream_copy": round(hbm[1], 1)},
+                     # the strip kernels keep their operands in registers and pass
+                     # neighbours by DPP; LDS carries only a strip's edge column
+                     # (one value and its T0 per row and sweep), so it bounds nothing
+                     "lds": None if lds is None else
+                     {"probe_read_GBs": round(lds[0], 1), "peak_GBs": round(lds[1], 1),
+                      "kernel_lds_bytes_per_strip_row_sweep": 16 if mixed else 32},
                      "measured_traffic_GBs": None if traffic is None else
                      round(traffic / (kms / launches * 1e-3) / 1e9, 1),
                      "alg_bytes_per_launch": kbytes / launches,

@@ -295,6 +295,11 @@ int satt_kernel_postproc(satt_ctx *ctx, int nker, int nx, int ny, int nz,
  * (the copy counts bytes read and written).  A measurement aid for the
  * roofline of the kernels above; no counterpart in the reference. */
 int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, double *copy_gbs);
+/* Achievable LDS read bandwidth (best of five runs of 16-byte LDS reads by
+ * every thread of four workgroups per CU), and the peak it is held against,
+ * 256 bytes per clock per CU (gfx950) at the device's clock, both in GB/s.  A
+ * measurement aid; no counterpart in the reference. */
+int satt_lds_probe(satt_ctx *ctx, double *read_gbs, double *peak_gbs);
 
 /* Device time in ms of the kernels of the last satt_smooth_geo2,
  * satt_gradient_geo2, satt_topo_metric, satt_steepest_descent,

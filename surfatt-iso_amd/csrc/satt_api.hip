@@ -554,6 +554,19 @@ extern "C" int satt_hbm_probe(satt_ctx *ctx, double nbytes, double *read_gbs, do
   return SATT_OK;
 }
 
+extern "C" int satt_lds_probe(satt_ctx *ctx, double *read_gbs, double *peak_gbs) {
+  if (!ctx || !read_gbs || !peak_gbs) return SATT_E_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int cus = 0, khz = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  HIPCHK(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, ctx->device));
+  // gfx950's LDS returns up to 256 bytes per clock per CU
+  *peak_gbs = 256.0 * cus * (khz * 1e3) / 1e9;
+  const int st = lds_probe(ctx->stream, cus, 5, *read_gbs);
+  if (st < 0) return fail(ctx, st, "satt_lds_probe: %s", st == SATT_E_NOMEM ? "no memory" : "HIP error");
+  return SATT_OK;
+}
+
 extern "C" double satt_topo_sigma(double vel, double period, double wavelen_factor) {
   const double pi = (double)3.14159265359f;
   const double km2deg = 1.0 / (6371.0 * pi / 180.0);

@@ -269,6 +269,7 @@ struct DispJob {
 // Streaming read and copy bandwidth of the device (satt_probe.hip), GB/s
 int stream_probe(hipStream_t s, int cus, size_t nbytes, int reps, double &read_gbs,
                  double &copy_gbs);
+int lds_probe(hipStream_t s, int cus, int reps, double &read_gbs);
 // LDS of one column's wave (the layered model and the per-layer terms) for
 // n layers
 size_t fwdsurf_lds_bytes(int n);

@@ -416,6 +416,12 @@ class Context:
                                             ctypes.byref(c)))
         return r.value, c.value
 
+    def lds_probe(self):
+        """(LDS read GB/s, LDS peak GB/s at 256 bytes per clock per CU)."""
+        r, pk = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.satt_lds_probe(self.ptr, ctypes.byref(r), ctypes.byref(pk)))
+        return r.value, pk.value
+
     def last_device_ms(self):
         """Device time of the last model-side call's kernels (ms)."""
         return float(self.lib.satt_last_device_ms(self.ptr))
