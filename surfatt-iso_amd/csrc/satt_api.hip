@@ -1601,6 +1601,11 @@ extern "C" int satt_plan_create_multi(satt_ctx *ctx, int nbatch, const satt_batc
     return fail(ctx, SATT_E_ARG, "satt_plan_create: unknown precision %d", precision);
   for (int k = 0; k < nbatch; ++k) {
     TRY(check_batch(ctx, &batches[k], k));
+    const size_t se = make_skew_geom(batches[k].nx, batches[k].ny).skew_elems();
+    if (precision == SATT_PREC_MIXED && se > kMaxSkewElems32)
+      return fail(ctx, SATT_E_ARG,
+                  "satt_plan_create: batch %d: %zu skewed elements per field, more than the mixed "
+                  "mode's %zu", k, se, (size_t)kMaxSkewElems32);
     if (batches[k].dim != batches[0].dim)
       return fail(ctx, SATT_E_ARG, "satt_plan_create: batches of one plan share dim");
   }

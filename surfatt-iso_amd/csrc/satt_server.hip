@@ -516,6 +516,9 @@ extern "C" int satt_server_submit(satt_server *S, const satt_request *q, satt_re
                                   long long *ticket) {
   if (!S || !q || !resp) return SATT_E_ARG;
   const bool no_metric = !q->m11 && !q->m12 && !q->m22;
+  if (S->precision == SATT_PREC_MIXED && q->nx >= 3 && q->ny >= 3 &&
+      make_skew_geom(q->nx, q->ny).skew_elems() > kMaxSkewElems32)
+    return SATT_E_ARG;
   if (q->nx < 3 || q->ny < 3 || q->ny > SATT_MAX_NY || !q->xgrids || !q->ygrids || !q->svel || q->nrec < 0 ||
       (!no_metric && (!q->m11 || !q->m12 || !q->m22)) ||
       (q->nrec > 0 && (!q->recx || !q->recy || !q->tt || !q->weight)))

@@ -9,6 +9,9 @@ namespace satt {
 constexpr int kPF = 6;  // per-period skew fields: a, b, c, fun, dis_x, dis_y
 constexpr int kFF = 3;  // per-item forward skew fields: T0, T0x, T0y
 constexpr int kAF = 6;  // per-item adjoint skew fields: a1, a2, b1, b2, d, delta
+// The FP32 strip passes address a set of up to kPF fields of one skew with
+// 32-bit byte offsets: plans in mixed precision keep a skew within this.
+constexpr size_t kMaxSkewElems32 = 0xFFFFFFFFull / (kPF * sizeof(float));
 
 // Per-slot scratch of the batched path, in units of one skewed field (SE
 // doubles) and one natural field (N doubles).

@@ -560,6 +560,38 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const GMEM T *p, u
 constexpr int kOutOfRange = INT_MIN;  // a buffer offset past any resource
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
+// A set of n skewed fields of SE elements each (field k at base + k*SE), as
+// the strip passes read and write them at element index e.  FP64: global
+// accesses at 64-bit addresses.  FP32: one buffer resource over the whole
+// set, the element as a 32-bit byte offset and the field as a scalar offset,
+// so that the loads of a step share one offset register and need no address
+// arithmetic of their own (plans keep a set within 4 GiB, kMaxSkewElems32).
+template <typename V>
+struct FieldSet;
+template <>
+struct FieldSet<double> {
+  GMEM double *p;
+  size_t SE;
+  __device__ FieldSet(const GMEM double *base, size_t se, int) : p((GMEM double *)base), SE(se) {}
+  __device__ double ld(int k, size_t e) const { return p[k * SE + e]; }
+  __device__ void st(int k, size_t e, double v) const { p[k * SE + e] = v; }
+};
+template <>
+struct FieldSet<float> {
+  __amdgpu_buffer_rsrc_t r;
+  unsigned fb;  // bytes of one field
+  __device__ FieldSet(const GMEM float *base, size_t se, int n)
+      : r(buffer_over(base, (unsigned)(n * se * sizeof(float)))), fb((unsigned)(se * sizeof(float))) {}
+  __device__ float ld(int k, size_t e) const {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)((unsigned)e * 4u), (int)(k * fb), 0));
+  }
+  __device__ void st(int k, size_t e, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r,
+                                          (int)((unsigned)e * 4u), (int)(k * fb), 0);
+  }
+};
+
 // Copies the grid cells of a field from skew FROM into the other skew, one
 // whole destination row per step (coalesced stores).  The loads of one row
 // gather from 64 source rows; lane l's source rows are its own column shifted
@@ -690,7 +722,8 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  const GMEM V *ps = C.ps[SK], *is = C.is[SK];
+  const FieldSet<V> fps(C.ps[SK], SE, kPF), fis(C.is[SK], SE, kFF), ftau(tau, SE, 1),
+      fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -725,20 +758,20 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     q.j = j;
     q.act = lv && u >= 0 && u < ny;
     const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
-    q.t0 = is[e];
-    q.tau = tau[e];
+    q.t0 = fis.ld(0, e);
+    q.tau = ftau.ld(0, e);
     if constexpr (kDown) {
       const size_t ed = opaque_index(dn_ok ? skpos(SK, id, min(max(j, 0), ny - 1), R) : e);
-      q.tdn = tau[ed];
-      q.zdn = is[ed];
+      q.tdn = ftau.ld(0, ed);
+      q.zdn = fis.ld(0, ed);
     }
-    if (MODE == 2) q.snap = C.snap[e];
+    if (MODE == 2) q.snap = fsnap.ld(0, e);
   };
   auto load_near = [&](int t, FNear<V> &q) {
     const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
-    q.a = ps[e]; q.b = ps[SE + e]; q.c = ps[2 * SE + e]; q.f = ps[3 * SE + e];
-    q.dx_ = ps[4 * SE + e]; q.dy_ = ps[5 * SE + e];
-    q.t0x = is[SE + e]; q.t0y = is[2 * SE + e];
+    q.a = fps.ld(0, e); q.b = fps.ld(1, e); q.c = fps.ld(2, e); q.f = fps.ld(3, e);
+    q.dx_ = fps.ld(4, e); q.dy_ = fps.ld(5, e);
+    q.t0x = fis.ld(1, e); q.t0y = fis.ld(2, e);
   };
   // The upstream strip's edge (final tau and its T0) for the cell of step t,
   // from the channel; read at the step itself, ahead of the step's global
@@ -802,9 +835,9 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
                                       zUx, zprev);
       }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      tau[e] = tnew;
+      ftau.st(0, e, tnew);
       if (pub) team_publish2<V>(T, t - lag, tnew, fc.t0, base_out + t - lag + 1);
-      if (MODE == 1) C.snap[e] = fc.tau;
+      if (MODE == 1) fsnap.st(0, e, fc.tau);
       if (MODE == 2) {
         const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
         acc_s += d;
@@ -1271,15 +1304,15 @@ struct ANear<float> {
   float k0, kW, kE, kS, kN;
 };
 
-__device__ __forceinline__ void adj_load_near(const GMEM double *is, size_t SE, size_t e,
+__device__ __forceinline__ void adj_load_near(const FieldSet<double> &f, size_t e,
                                               ANear<double> &q) {
-  q.a1 = is[e]; q.a2 = is[SE + e]; q.b1 = is[2 * SE + e]; q.b2 = is[3 * SE + e];
-  q.d = is[4 * SE + e]; q.dl = is[5 * SE + e];
+  q.a1 = f.ld(0, e); q.a2 = f.ld(1, e); q.b1 = f.ld(2, e); q.b2 = f.ld(3, e);
+  q.d = f.ld(4, e); q.dl = f.ld(5, e);
 }
-__device__ __forceinline__ void adj_load_near(const GMEM float *is, size_t SE, size_t e,
+__device__ __forceinline__ void adj_load_near(const FieldSet<float> &f, size_t e,
                                               ANear<float> &q) {
-  q.k0 = is[e]; q.kW = is[SE + e]; q.kE = is[2 * SE + e]; q.kS = is[3 * SE + e];
-  q.kN = is[4 * SE + e];
+  q.k0 = f.ld(0, e); q.kW = f.ld(1, e); q.kE = f.ld(2, e); q.kS = f.ld(3, e);
+  q.kN = f.ld(4, e);
 }
 
 // The interior update (src/eikon_solver_2d.f90:1392-1402), and its FP32 form.
@@ -1315,7 +1348,9 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  const GMEM V *is = C.is[SK];
+  // the coefficient set: kAF FP64 fields, or 5 FP32 ones
+  const FieldSet<V> fis(C.is[SK], SE, sizeof(V) == 8 ? kAF : 5), fta(ta, SE, 1),
+      fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -1342,12 +1377,12 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
     q.act = lv && u >= 0 && u < ny;
     const int jc = min(max(j, 0), ny - 1);
     const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
-    q.ta = ta[e];
-    q.tdn = ta[opaque_index(dn_ok ? skpos(SK, id, jc, R) : e)];
-    if (MODE == 2) q.snap = C.snap[e];
+    q.ta = fta.ld(0, e);
+    q.tdn = fta.ld(0, opaque_index(dn_ok ? skpos(SK, id, jc, R) : e));
+    if (MODE == 2) q.snap = fsnap.ld(0, e);
   };
   auto load_near = [&](int t, ANear<V> &q) {
-    adj_load_near(is, SE, step_pos<XD, YD>(c, t, lane, ny, L, R), q);
+    adj_load_near(fis, step_pos<XD, YD>(c, t, lane, ny, L, R), q);
   };
   // operand sets as in the forward pass: unrolled by S = PD+1 over S sets
   // (far and near PD steps ahead)
@@ -1384,9 +1419,9 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
       if (interior_col && jc >= 1 && jc <= ny - 2)
         tnew = adj_update(nc, tW, tE, tS, tN, C.dx, C.dy);
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      ta[e] = tnew;
+      fta.st(0, e, tnew);
       if (pub) team_publish<V>(T, t - lag, tnew, base_out + t - lag + 1);
-      if (MODE == 1) C.snap[e] = fc.ta;
+      if (MODE == 1) fsnap.st(0, e, fc.ta);
       if (MODE == 2) {
         const double dd = fabs((double)tnew - (double)fc.snap);
         acc_s += dd;
