@@ -471,7 +471,27 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
       return it32;
     }
   }
-  /* mixed mode: exactly one FP64 polish iteration after the FP32 phase */
+  /* mixed mode: exactly one FP64 polish iteration after the FP32 phase, on
+   * the coefficients rounded to float once (the device's float set: a1, a2,
+   * b1, b2 -- whose +/- parts round as they do -- d, zero where |d| < eps,
+   * and the source), widened */
+  double *dpol = NULL;
+  if (mixed) {
+    dpol = malloc(sizeof(double) * n);
+    if (!dpol) {
+      free(xx); free(yy); free(buf); free(bb);
+      return ORC_E_NOMEM;
+    }
+    for (size_t k = 0; k < n; ++k) {
+      const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
+      dpol[k] = fabs(d) < ADJ_EPS ? 0.0 : (double)(float)d;
+      a1m[k] = (double)(float)a1m[k]; a1p[k] = (double)(float)a1p[k];
+      a2m[k] = (double)(float)a2m[k]; a2p[k] = (double)(float)a2p[k];
+      b1m[k] = (double)(float)b1m[k]; b1p[k] = (double)(float)b1p[k];
+      b2m[k] = (double)(float)b2m[k]; b2p[k] = (double)(float)b2p[k];
+      delta[k] = (double)(float)delta[k];
+    }
+  }
   const int maxit = mixed ? 1 : ADJ_MAXITER;
   int iter, converged = 0;
   for (iter = 1; iter <= maxit; ++iter) {
@@ -485,8 +505,8 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
           for (int jj = 1; jj < ny - 1; ++jj) {
             const int j = ydir < 0 ? jj : ny - 1 - jj;
             const size_t k = IDX(i, j);
-            const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
-            if (fabs(d) < ADJ_EPS) {
+            const double d = mixed ? dpol[k] : (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
+            if (mixed ? d == 0.0 : fabs(d) < ADJ_EPS) {
               Ta[k] = 0;
             } else {
               const double e = (Ta[IDX(i - 1, j)] * a1p[k] - Ta[IDX(i + 1, j)] * a2m[k]) / dx +
@@ -517,7 +537,7 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
     if (!converged) iter = ADJ_MAXITER;
     if (iters) *iters = iter;
   }
-  free(xx); free(yy); free(buf); free(bb);
+  free(xx); free(yy); free(buf); free(bb); free(dpol);
   return converged ? ORC_OK : ORC_W_MAXITER;
 }
 
@@ -1037,19 +1057,14 @@ static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
     int c32 = 0, c64 = 0;
     it32 = fwd_iterate_up_f32(nx, ny, (float)(1.0 / dx), (float)(1.0 / dy), f32, isc, tau32,
                               old32, &c32);
-    /* polish on dtau in double: the same fields in FP64 */
+    /* polish on dtau in double: the period fields as the FP32 phase has them
+     * (rounded to float once, widened), T0 and its gradients in FP64 */
     double *f64 = malloc(sizeof(double) * n * 9);
     if (!f64) {
       st = ORC_E_NOMEM;
     } else {
       for (size_t k = 0; k < n; ++k) {
-        const double ak = a[k], bk = b[k], ck = c[k], fk = fun[k];
-        f64[k] = ak;
-        f64[n + k] = bk;
-        f64[2 * n + k] = ck;
-        f64[3 * n + k] = fk;
-        f64[4 * n + k] = sqrt(fk * fk * (bk / (ak * bk - ck * ck)));
-        f64[5 * n + k] = sqrt(fk * fk * (ak / (ak * bk - ck * ck)));
+        for (int q = 0; q < 6; ++q) f64[q * n + k] = (double)f32[q * n + k];
         f64[6 * n + k] = T0[k];
         f64[7 * n + k] = T0x[k];
         f64[8 * n + k] = T0y[k];
@@ -1113,7 +1128,8 @@ static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
       const size_t k = IDX(i, j);
       const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
       if (fabs(d) < ADJ_EPS) continue;
-      k0[k] = (float)(delta[k] / d);
+      /* the source term divides by d as the polish's float set holds it */
+      k0[k] = (float)(delta[k] / (double)(float)d);
       kW[k] = (float)(a1p[k] / dx / d);
       kE[k] = (float)(-a2m[k] / dx / d);
       kS[k] = (float)(b1p[k] / dy / d);

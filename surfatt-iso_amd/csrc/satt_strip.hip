@@ -703,7 +703,11 @@ struct FwdCtx {
   double dx, dy;           // FP64 update
   V rdx, rdy;              // upwind update: 1/dx, 1/dy in V
   int fi0, fi1, fj0, fj1;  // frozen rectangle
-  const GMEM V *ps[2];     // period skews (plus, minus), kPF fields each
+  // period skews (plus, minus), kPF fields each: in V, except for the mixed
+  // mode's FP64 polish (UP in double), which reads the float fields of its
+  // FP32 phase: the polish's arithmetic is FP64 on coefficients rounded once
+  // to float, half the bytes of FP64 ones
+  const GMEM void *ps[2];
   const GMEM V *is[2];     // item T0 skews (plus, minus), kFF fields each
   GMEM V *snap;            // plus skew
 };
@@ -722,8 +726,9 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  const FieldSet<V> fps(C.ps[SK], SE, kPF), fis(C.is[SK], SE, kFF), ftau(tau, SE, 1),
-      fsnap(C.snap, SE, 1);
+  using PS = typename std::conditional<UP && sizeof(V) == 8, float, V>::type;
+  const FieldSet<PS> fps((const GMEM PS *)C.ps[SK], SE, kPF);
+  const FieldSet<V> fis(C.is[SK], SE, kFF), ftau(tau, SE, 1), fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -1023,8 +1028,13 @@ __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rec
   double *TP = sk0 + kSkTP * SE, *TM = sk0 + kSkTM * SE;
   FwdCtx<double> C;
   fwd_ctx_init(C, g, sg, lane, rect);
-  C.ps[0] = gm(D.pskew);
-  C.ps[1] = gm(D.pskew + (size_t)kPF * SE);
+  if (UP) {  // the mixed mode's polish: the float period fields
+    C.ps[0] = gm(D.pskew32);
+    C.ps[1] = gm(D.pskew32 + (size_t)kPF * SE);
+  } else {
+    C.ps[0] = gm(D.pskew);
+    C.ps[1] = gm(D.pskew + (size_t)kPF * SE);
+  }
   C.is[0] = gm(sk0);
   C.is[1] = gm(sk0 + kFF * SE);
   C.snap = gm(sk0 + kSkSnapF * SE);
@@ -1187,9 +1197,11 @@ __global__ __launch_bounds__(256) void k_adj_zero(const SlotDesc *sd) {
 // The source field is left 0 here and scattered in by k_adj_splat.
 //   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
 //           the boundary (:1368-1380), in the plus skew
-//   KIND 1: the FP64 set without the Ta init, plus (mixed mode) the float set
-//           k0 (0 here), kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
-//           kN = -b2m/dy/d and the float Ta init, from double skew 16
+//   KIND 1 (mixed mode): the same six coefficients rounded to float, for
+//           the FP64 polish, at the start of each skew's coefficient region;
+//           the FP32 phase's set k0 (0 here), kW = a1p/dx/d, kE = -a2m/dx/d,
+//           kS = b1p/dy/d, kN = -b2m/dy/d from the FP64 values; and the
+//           float Ta init, from double skew 16
 // Padding elements get zeros.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
@@ -1221,8 +1233,14 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
                pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
   }
   GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
+  if (KIND == 0) {
 #pragma unroll
-  for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
+    for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
+  } else {
+    GMEM float *b0f = (GMEM float *)b0;
+#pragma unroll
+    for (int f = 0; f < kAF; ++f) b0f[f * SE + e] = (float)v[f];
+  }
   if (KIND == 0) {
     if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
   } else {
@@ -1247,8 +1265,9 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
 // ... then, one thread per item: splat the receivers in order into the
 // natural field (the reference's sums, in its order), and scatter the
 // source of every touched interior cell into the coefficient fields of both
-// skews (mixed: also k0 = delta/d rounded to float, where d != 0).  The
-// right-hand side is w*(t - tobs) (which 0) or -w (which 1).
+// skews (mixed: into the float set, and k0 = delta/d rounded to float with
+// the set's d, where d != 0).  The right-hand side is w*(t - tobs) (which
+// 0) or -w (which 1).
 __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count, int which,
                                                   int mixed) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1277,10 +1296,13 @@ __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count,
       for (int sk = 0; sk < 2; ++sk) {
         const size_t e = skpos(sk, i, j, R);
         double *b0 = sk0 + (size_t)sk * kAF * SE;
-        b0[5 * SE + e] = dl;
         if (mixed) {
-          const double d = b0[4 * SE + e];
+          float *b0f = reinterpret_cast<float *>(b0);
+          b0f[5 * SE + e] = (float)dl;
+          const double d = b0f[4 * SE + e];
           k32[(size_t)sk * 5 * SE + e] = d != 0.0 ? (float)(dl / d) : 0.0f;
+        } else {
+          b0[5 * SE + e] = dl;
         }
       }
     }
@@ -1310,6 +1332,11 @@ __device__ __forceinline__ void adj_load_near(const FieldSet<double> &f, size_t 
   q.d = f.ld(4, e); q.dl = f.ld(5, e);
 }
 __device__ __forceinline__ void adj_load_near(const FieldSet<float> &f, size_t e,
+                                              ANear<double> &q) {
+  q.a1 = f.ld(0, e); q.a2 = f.ld(1, e); q.b1 = f.ld(2, e); q.b2 = f.ld(3, e);
+  q.d = f.ld(4, e); q.dl = f.ld(5, e);
+}
+__device__ __forceinline__ void adj_load_near(const FieldSet<float> &f, size_t e,
                                               ANear<float> &q) {
   q.k0 = f.ld(0, e); q.kW = f.ld(1, e); q.kE = f.ld(2, e); q.kS = f.ld(3, e);
   q.kN = f.ld(4, e);
@@ -1329,28 +1356,29 @@ __device__ __forceinline__ float adj_update(const ANear<float> &n, float tW, flo
   return fmaf(n.kN, tN, fmaf(n.kS, tS, fmaf(n.kE, tE, fmaf(n.kW, tW, n.k0))));
 }
 
-template <typename V>
+template <typename V, typename S = V>
 struct AdjCtx {
   int nx, ny, R, lane;
   size_t SE;
   double dx, dy;
-  const GMEM V *is[2];  // item coefficient skews
+  const GMEM S *is[2];  // item coefficient skews, stored in S
   GMEM V *snap;         // plus skew
 };
 
 // One adjoint sweep over strip c, in place in the sweep's own skew; MODE as
 // for the forward, and MODE 2 also accumulates max |Ta| (acc_a) for the
 // FP32 stop rule.
-template <typename V, int XD, int YD, int MODE, int PD>
-__device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V *ta, double &ss,
+template <typename V, int XD, int YD, int MODE, int PD, typename CS = V>
+__device__ void adj_strip_pass(const AdjCtx<V, CS> &C, const Team &T, int k, GMEM V *ta, double &ss,
                                double &sm, double &sa) {
   constexpr int SK = XD == YD ? 0 : 1;
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  // the coefficient set: kAF FP64 fields, or 5 FP32 ones
-  const FieldSet<V> fis(C.is[SK], SE, sizeof(V) == 8 ? kAF : 5), fta(ta, SE, 1),
-      fsnap(C.snap, SE, 1);
+  // the coefficient set: kAF fields (a1, a2, b1, b2, d, delta; FP64, or
+  // float for the mixed mode's polish), or the FP32 phase's 5
+  const FieldSet<CS> fis(C.is[SK], SE, sizeof(V) == 8 ? kAF : 5);
+  const FieldSet<V> fta(ta, SE, 1), fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
   const bool lv = lane < L;
@@ -1447,12 +1475,12 @@ __device__ void adj_strip_pass(const AdjCtx<V> &C, const Team &T, int k, GMEM V 
   wave_fence();
 }
 
-template <typename V, int XD, int YD, int MODE, int PD>
-__device__ void adj_sweep(const AdjCtx<V> &C, Team &T, GMEM V *ta, StripAcc &acc) {
+template <typename V, int XD, int YD, int MODE, int PD, typename CS = V>
+__device__ void adj_sweep(const AdjCtx<V, CS> &C, Team &T, GMEM V *ta, StripAcc &acc) {
   team_sweep_begin(T);
   for (int k = T.w; k < T.ns; k += T.W) {
     double ss = 0, sm = 0, sa = 0;
-    adj_strip_pass<V, XD, YD, MODE, PD>(C, T, k, ta, ss, sm, sa);
+    adj_strip_pass<V, XD, YD, MODE, PD, CS>(C, T, k, ta, ss, sm, sa);
     if (MODE == 2) team_strip_term(T, k, ss, sm, sa, acc);
   }
   team_sweep_end(T);
@@ -1461,24 +1489,24 @@ __device__ void adj_sweep(const AdjCtx<V> &C, Team &T, GMEM V *ta, StripAcc &acc
 // One iteration: (xdirec, ydirec) = (-1,-1), (-1,1), (1,-1), (1,1), x
 // ascending first (src/eikon_solver_2d.f90:1383-1390); with SNAP the
 // snapshot and the residuals (:1412-1434) are taken.
-template <typename V, bool SNAP, int PD>
-__device__ void adj_iteration(const AdjCtx<V> &C, Team &T, GMEM V *tP, GMEM V *tM, double &s, double &m,
-                              double &a) {
+template <typename V, bool SNAP, int PD, typename CS = V>
+__device__ void adj_iteration(const AdjCtx<V, CS> &C, Team &T, GMEM V *tP, GMEM V *tM, double &s,
+                              double &m, double &a) {
   StripAcc acc;
-  adj_sweep<V, 1, 1, SNAP ? 1 : 0, PD>(C, T, tP, acc);
+  adj_sweep<V, 1, 1, SNAP ? 1 : 0, PD, CS>(C, T, tP, acc);
   skew_copy<V, 0>(T, tP, tM, C.nx, C.ny, C.R, T.ns, C.lane);
-  adj_sweep<V, 1, -1, 0, PD>(C, T, tM, acc);
-  adj_sweep<V, -1, 1, 0, PD>(C, T, tM, acc);
+  adj_sweep<V, 1, -1, 0, PD, CS>(C, T, tM, acc);
+  adj_sweep<V, -1, 1, 0, PD, CS>(C, T, tM, acc);
   skew_copy<V, 1>(T, tM, tP, C.nx, C.ny, C.R, T.ns, C.lane);
-  adj_sweep<V, -1, -1, SNAP ? 2 : 0, PD>(C, T, tP, acc);
+  adj_sweep<V, -1, -1, SNAP ? 2 : 0, PD, CS>(C, T, tP, acc);
   const StripAcc r = team_strip_total(T, acc);
   s = r.s;
   m = r.m;
   a = r.a;
 }
 
-template <typename V>
-__device__ void adj_ctx_init(AdjCtx<V> &C, const Grid &g, const SkewGeom &sg, int lane) {
+template <typename V, typename S>
+__device__ void adj_ctx_init(AdjCtx<V, S> &C, const Grid &g, const SkewGeom &sg, int lane) {
   C.nx = g.nx; C.ny = g.ny; C.R = sg.R; C.lane = lane; C.SE = sg.skew_elems();
   C.dx = g.dx; C.dy = g.dy;
 }
@@ -1590,13 +1618,15 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const 
   Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
-  AdjCtx<double> C;
+  // the float coefficient set k_adj_build<1> wrote at the start of each
+  // skew's coefficient region
+  AdjCtx<double, float> C;
   adj_ctx_init(C, g, sg, lane);
-  C.is[0] = gm(sk0);
-  C.is[1] = gm(sk0 + kAF * SE);
+  C.is[0] = gm(reinterpret_cast<const float *>(sk0));
+  C.is[1] = gm(reinterpret_cast<const float *>(sk0 + kAF * SE));
   C.snap = nullptr;
   double s, m, a;
-  adj_iteration<double, false, PD>(C, T, gm(TaP), gm(TaM), s, m, a);
+  adj_iteration<double, false, PD, float>(C, T, gm(TaP), gm(TaM), s, m, a);
   adj_output(g, sg, TaP, D.x0, D.y0, 1, which ? D.Td : D.Ta);
   if (threadIdx.x == 0) {
     if (team_failed(T)) *D.status = kETeamWait;
