@@ -358,10 +358,11 @@ int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
 
 /* ------------------------------------------------------------------ adjoint */
 
-static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
-                       const double *a1p, const double *a2m, const double *a2p,
-                       const double *b1m, const double *b1p, const double *b2m,
-                       const double *b2p, const double *delta, double *Ta, int *done);
+static void adj_kform(int nx, int ny, double dx, double dy, const double *a1m,
+                      const double *a1p, const double *a2m, const double *a2p,
+                      const double *b1m, const double *b1p, const double *b2m,
+                      const double *b2p, const double *delta, float *kk);
+static int adj_phase32(int nx, int ny, const float *kk, double *Ta, int *done);
 
 static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
                       int ny, const double *spha, const double *sphb,
@@ -463,33 +464,20 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
   for (int j = 0; j < ny; ++j) { Ta[IDX(0, j)] = 0; Ta[IDX(nx - 1, j)] = 0; }
 
   int it32 = 0, done32 = 0;
+  /* mixed mode: the float k fields, the FP32 phase on them, then exactly
+   * one FP64 polish iteration on the same fields, widened */
+  float *kk = NULL;
   if (mixed) {
-    it32 = adj_phase32(nx, ny, dx, dy, a1m, a1p, a2m, a2p, b1m, b1p, b2m, b2p, delta, Ta,
-                       &done32);
-    if (it32 < 0) {
-      free(xx); free(yy); free(buf); free(bb);
-      return it32;
-    }
-  }
-  /* mixed mode: exactly one FP64 polish iteration after the FP32 phase, on
-   * the coefficients rounded to float once (the device's float set: a1, a2,
-   * b1, b2 -- whose +/- parts round as they do -- d, zero where |d| < eps,
-   * and the source), widened */
-  double *dpol = NULL;
-  if (mixed) {
-    dpol = malloc(sizeof(double) * n);
-    if (!dpol) {
+    kk = malloc(sizeof(float) * n * 5);
+    if (!kk) {
       free(xx); free(yy); free(buf); free(bb);
       return ORC_E_NOMEM;
     }
-    for (size_t k = 0; k < n; ++k) {
-      const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
-      dpol[k] = fabs(d) < ADJ_EPS ? 0.0 : (double)(float)d;
-      a1m[k] = (double)(float)a1m[k]; a1p[k] = (double)(float)a1p[k];
-      a2m[k] = (double)(float)a2m[k]; a2p[k] = (double)(float)a2p[k];
-      b1m[k] = (double)(float)b1m[k]; b1p[k] = (double)(float)b1p[k];
-      b2m[k] = (double)(float)b2m[k]; b2p[k] = (double)(float)b2p[k];
-      delta[k] = (double)(float)delta[k];
+    adj_kform(nx, ny, dx, dy, a1m, a1p, a2m, a2p, b1m, b1p, b2m, b2p, delta, kk);
+    it32 = adj_phase32(nx, ny, kk, Ta, &done32);
+    if (it32 < 0) {
+      free(xx); free(yy); free(buf); free(bb); free(kk);
+      return it32;
     }
   }
   const int maxit = mixed ? 1 : ADJ_MAXITER;
@@ -505,8 +493,18 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
           for (int jj = 1; jj < ny - 1; ++jj) {
             const int j = ydir < 0 ? jj : ny - 1 - jj;
             const size_t k = IDX(i, j);
-            const double d = mixed ? dpol[k] : (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
-            if (mixed ? d == 0.0 : fabs(d) < ADJ_EPS) {
+            if (mixed) {
+              /* k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN, one rounding per
+               * operation, in the FP32 phase's order */
+              double t = (double)kk[k];
+              t = (double)kk[n + k] * Ta[IDX(i - 1, j)] + t;
+              t = (double)kk[2 * n + k] * Ta[IDX(i + 1, j)] + t;
+              t = (double)kk[3 * n + k] * Ta[IDX(i, j - 1)] + t;
+              Ta[k] = (double)kk[4 * n + k] * Ta[IDX(i, j + 1)] + t;
+              continue;
+            }
+            const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
+            if (fabs(d) < ADJ_EPS) {
               Ta[k] = 0;
             } else {
               const double e = (Ta[IDX(i - 1, j)] * a1p[k] - Ta[IDX(i + 1, j)] * a2m[k]) / dx +
@@ -537,7 +535,7 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
     if (!converged) iter = ADJ_MAXITER;
     if (iters) *iters = iter;
   }
-  free(xx); free(yy); free(buf); free(bb); free(dpol);
+  free(xx); free(yy); free(buf); free(bb); free(kk);
   return converged ? ORC_OK : ORC_W_MAXITER;
 }
 
@@ -858,15 +856,19 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
  * rules, sweep order, tau init, frozen set and the stop rule (with the
  * residual |dtau|*T0 accumulated in double) are the reference's.  FP64
  * polish: the same upwind update in double on dtau = (double)dtau32, with
- * the same stop rule (at least one iteration); T = (1 + dtau)*T0.
+ * the period fields, T0x and T0y as the FP32 phase reads them (float,
+ * widened) and T0 in FP64, and the same stop rule (at least one
+ * iteration); T = (1 + dtau)*T0.
  *
  * Adjoint.  The face coefficients, d and delta are computed in FP64 as the
- * reference does.  FP32 phase: Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN
- * with k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
- * kN = -b2m/dy/d rounded to float (all zero where |d| < eps), iterated until
- * the reference's absolute rule holds or the largest change is at most
- * 2^-21 of the largest |Ta| (float resolution).  FP64 polish: one
- * iteration of the reference's update from Ta = (double)Ta32.
+ * reference does, and the update (:1392-1402) is taken in the form
+ * Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN with k0 = delta/d,
+ * kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d computed in
+ * double and rounded to float (all zero where |d| < eps).  FP32 phase: that
+ * update in float, iterated until the reference's absolute rule holds or
+ * the largest change is at most 2^-21 of the largest |Ta| (float
+ * resolution).  FP64 polish: one iteration of the same update in double on
+ * the same float coefficients, from Ta = (double)Ta32.
  *
  * The FP32 upwind cell and the FP32 adjoint update are written with
  * explicit fused multiply-adds (fmaf, correctly rounded), as the device
@@ -1057,8 +1059,8 @@ static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
     int c32 = 0, c64 = 0;
     it32 = fwd_iterate_up_f32(nx, ny, (float)(1.0 / dx), (float)(1.0 / dy), f32, isc, tau32,
                               old32, &c32);
-    /* polish on dtau in double: the period fields as the FP32 phase has them
-     * (rounded to float once, widened), T0 and its gradients in FP64 */
+    /* polish on dtau in double: the period fields and T0x, T0y as the FP32
+     * phase has them (rounded to float once, widened), T0 in FP64 */
     double *f64 = malloc(sizeof(double) * n * 9);
     if (!f64) {
       st = ORC_E_NOMEM;
@@ -1066,8 +1068,8 @@ static int fwd_mixed_common(const double *xx_deg, const double *yy_deg, int nx,
       for (size_t k = 0; k < n; ++k) {
         for (int q = 0; q < 6; ++q) f64[q * n + k] = (double)f32[q * n + k];
         f64[6 * n + k] = T0[k];
-        f64[7 * n + k] = T0x[k];
-        f64[8 * n + k] = T0y[k];
+        f64[7 * n + k] = (double)f32[7 * n + k];
+        f64[8 * n + k] = (double)f32[8 * n + k];
         tau[k] = (double)tau32[k];
       }
       it64 = fwd_iterate_up_f64(nx, ny, 1.0 / dx, 1.0 / dy, f64, isc, tau, tau_old, &c64);
@@ -1106,35 +1108,39 @@ int orc_fwd_mixed_pre(const double *xx_deg, const double *yy_deg, int nx, int ny
   return st;
 }
 
-/* FP32 phase of the mixed adjoint (see the note above).  Ta holds the
- * reference's initial field on entry and the FP32 result, widened, on exit.
- * Returns the iteration count or a negative status; *done tells whether the
- * stop rule held. */
-static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
-                       const double *a1p, const double *a2m, const double *a2p,
-                       const double *b1m, const double *b1p, const double *b2m,
-                       const double *b2p, const double *delta, double *Ta, int *done_out) {
+/* The mixed adjoint's float coefficients k0, kW, kE, kS, kN (n each, in
+ * kk; see the note above). */
+static void adj_kform(int nx, int ny, double dx, double dy, const double *a1m,
+                      const double *a1p, const double *a2m, const double *a2p,
+                      const double *b1m, const double *b1p, const double *b2m,
+                      const double *b2p, const double *delta, float *kk) {
   const size_t n = (size_t)nx * ny;
-  float *kk = malloc(sizeof(float) * n * 7);
-  if (!kk) return ORC_E_NOMEM;
-  float *k0 = kk, *kW = kk + n, *kE = kk + 2 * n, *kS = kk + 3 * n, *kN = kk + 4 * n,
-        *ta = kk + 5 * n, *old = kk + 6 * n;
-  for (size_t k = 0; k < n; ++k) {
-    k0[k] = kW[k] = kE[k] = kS[k] = kN[k] = 0;
-    ta[k] = (float)Ta[k];
-  }
+  float *k0 = kk, *kW = kk + n, *kE = kk + 2 * n, *kS = kk + 3 * n, *kN = kk + 4 * n;
+  for (size_t k = 0; k < n; ++k) k0[k] = kW[k] = kE[k] = kS[k] = kN[k] = 0;
   for (int i = 1; i < nx - 1; ++i)
     for (int j = 1; j < ny - 1; ++j) {
       const size_t k = IDX(i, j);
       const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
       if (fabs(d) < ADJ_EPS) continue;
-      /* the source term divides by d as the polish's float set holds it */
-      k0[k] = (float)(delta[k] / (double)(float)d);
+      k0[k] = (float)(delta[k] / d);
       kW[k] = (float)(a1p[k] / dx / d);
       kE[k] = (float)(-a2m[k] / dx / d);
       kS[k] = (float)(b1p[k] / dy / d);
       kN[k] = (float)(-b2m[k] / dy / d);
     }
+}
+
+/* FP32 phase of the mixed adjoint (see the note above) on the float
+ * coefficients kk.  Ta holds the reference's initial field on entry and the
+ * FP32 result, widened, on exit.  Returns the iteration count or a negative
+ * status; *done tells whether the stop rule held. */
+static int adj_phase32(int nx, int ny, const float *kk, double *Ta, int *done_out) {
+  const size_t n = (size_t)nx * ny;
+  float *tt = malloc(sizeof(float) * n * 2);
+  if (!tt) return ORC_E_NOMEM;
+  const float *k0 = kk, *kW = kk + n, *kE = kk + 2 * n, *kS = kk + 3 * n, *kN = kk + 4 * n;
+  float *ta = tt, *old = tt + n;
+  for (size_t k = 0; k < n; ++k) ta[k] = (float)Ta[k];
   int iter, done = 0;
   for (iter = 1; iter <= ADJ_MAXITER; ++iter) {
     memcpy(old, ta, n * sizeof(float));
@@ -1168,7 +1174,7 @@ static int adj_phase32(int nx, int ny, double dx, double dy, const double *a1m,
   }
   if (!done) iter = ADJ_MAXITER;
   for (size_t k = 0; k < n; ++k) Ta[k] = (double)ta[k];
-  free(kk);
+  free(tt);
   *done_out = done;
   return iter;
 }

@@ -29,8 +29,10 @@ enum : int { kSkTauP = 6, kSkTauM = 7, kSkSnapF = 8, kSkTP = 12, kSkTM = 13,
              kSkTaP = 14, kSkTaM = 15, kSkSnapA = 16 };
 enum : int { kNatT = 0, kNatTa = 1, kNatTd = 2, kNatDelta = 3 };
 // The mixed mode adds 6 skewed fields per slot: the adjoint's float fields
-// (coefficients k0..kN in both skews, Ta plus and minus, snapshot) from
-// double skew 16, beside the FP64 coefficients of its polish (0..11).
+// (coefficients k0..kN in both skews, which both its FP32 phase and its FP64
+// polish read, Ta plus and minus, snapshot) from double skew 16.  Of the
+// forward's FP64 fields it keeps T0 (0, 3), tau and the snapshot: its passes
+// read the period fields and T0x, T0y in float.
 constexpr int kSlotSkewsMixed = 23;
 constexpr int kSkA32 = 16;
 

@@ -155,6 +155,14 @@ __device__ __forceinline__ size_t opaque_index(size_t x) {
   asm("" : "+v"(x));
   return x;
 }
+// A copy of x in a register of its own (early-clobber output): a value kept
+// from one step to the next then never shares a register with an operand
+// set that the next step refills.
+__device__ __forceinline__ double own_reg(double x) {
+  double y;
+  asm("v_mov_b64 %0, %1" : "=&v"(y) : "v"(x));
+  return y;
+}
 
 // One cell of the forward sweep body (src/eikon_solver_2d.f90:910-1079) with
 // every operand in registers.  Every floating-point expression is the
@@ -690,9 +698,13 @@ struct FFar {
   int j;
   bool act;
 };
-template <typename V>
+// The cell's coefficients, the period fields and T0x, T0y, in their stored
+// type P: float for the mixed mode's FP64 polish, converted where the update
+// reads them (a conversion at the load would wait for the load, steps
+// early).
+template <typename P>
 struct FNear {
-  V a, b, c, f, dx_, dy_, t0x, t0y;
+  P a, b, c, f, dx_, dy_, t0x, t0y;
 };
 
 // Everything one forward solve's strip passes need.
@@ -703,12 +715,15 @@ struct FwdCtx {
   double dx, dy;           // FP64 update
   V rdx, rdy;              // upwind update: 1/dx, 1/dy in V
   int fi0, fi1, fj0, fj1;  // frozen rectangle
-  // period skews (plus, minus), kPF fields each: in V, except for the mixed
-  // mode's FP64 polish (UP in double), which reads the float fields of its
-  // FP32 phase: the polish's arithmetic is FP64 on coefficients rounded once
-  // to float, half the bytes of FP64 ones
+  // period skews (plus, minus), kPF fields each, and item T0 skews (plus,
+  // minus), kFF fields each (T0, T0x, T0y), whose T0x and T0y the passes
+  // read: in V, except for the mixed mode's FP64 polish (UP in double),
+  // which reads the float fields of its FP32 phase: the polish's arithmetic
+  // is FP64 on coefficients rounded once to float, half the bytes of FP64
+  // ones
   const GMEM void *ps[2];
-  const GMEM V *is[2];     // item T0 skews (plus, minus), kFF fields each
+  const GMEM void *gs[2];
+  const GMEM V *is[2];     // item T0 skews (plus, minus) whose T0 the passes read
   GMEM V *snap;            // plus skew
 };
 
@@ -727,7 +742,7 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
   using PS = typename std::conditional<UP && sizeof(V) == 8, float, V>::type;
-  const FieldSet<PS> fps((const GMEM PS *)C.ps[SK], SE, kPF);
+  const FieldSet<PS> fps((const GMEM PS *)C.ps[SK], SE, kPF), fgs((const GMEM PS *)C.gs[SK], SE, kFF);
   const FieldSet<V> fis(C.is[SK], SE, kFF), ftau(tau, SE, 1), fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
@@ -772,11 +787,11 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     }
     if (MODE == 2) q.snap = fsnap.ld(0, e);
   };
-  auto load_near = [&](int t, FNear<V> &q) {
+  auto load_near = [&](int t, FNear<PS> &q) {
     const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
     q.a = fps.ld(0, e); q.b = fps.ld(1, e); q.c = fps.ld(2, e); q.f = fps.ld(3, e);
     q.dx_ = fps.ld(4, e); q.dy_ = fps.ld(5, e);
-    q.t0x = fis.ld(1, e); q.t0y = fis.ld(2, e);
+    q.t0x = fgs.ld(1, e); q.t0y = fgs.ld(2, e);
   };
   // The upstream strip's edge (final tau and its T0) for the cell of step t,
   // from the channel; read at the step itself, ahead of the step's global
@@ -790,23 +805,27 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     }
   };
 
-  // Far and near operands are fetched PD steps ahead.  The loop cycles
-  // S = PD+1 operand sets, unrolled by S so that no set is copied (step t
-  // uses set t mod S and fills set (t+PD) mod S): a copy of a set still in
-  // flight would wait for its loads.  Up to S-1 steps past the last run
-  // with every lane outside the grid, so they change nothing.
-  static_assert(PD >= 1, "operands are fetched at least one step ahead");
-  constexpr int S = PD + 1;
+  // Far and near operands are fetched PD steps ahead, into S = PD+2
+  // operand sets that the loop cycles, unrolled by S (step t uses set
+  // (t+PD) mod S and fills set (t+2PD) mod S), so that no set is copied: a
+  // copy of a set still in flight waits for its loads.  With PD+1 sets the
+  // set that a step refills is the previous step's, whose tau and T0 are
+  // still live as that step's result, and the compiler rotates registers
+  // with copies of fresh loads at the loop's end, which wait for them.
+  // The loop starts PD steps early; those steps only fetch (every lane is
+  // outside the grid), so the loop has a single entry with nothing in
+  // flight: a separate prologue of loads, issued in whatever order the
+  // scheduler picks, makes the wait counts derived for the loop assume the
+  // worst order, and the loop then waits for all but the newest step's
+  // loads.  Up to S-1 steps past the last run with every lane outside the
+  // grid too, so they change nothing.
+  static_assert(PD >= 2, "the next step's far set is read while this step fills one");
+  constexpr int S = PD + (UP ? 2 : 1);
   FFar<V> fs[S] = {};
-  FNear<V> ns[S] = {};
-#pragma unroll
-  for (int q = 0; q < PD; ++q) {
-    load_far(q, fs[q]);
-    load_near(q, ns[q]);
-  }
+  FNear<PS> ns[S] = {};
   V tprev = 0, zprev = 0;  // own result and T0 of the previous step
-  auto step = [&](int t, const FFar<V> &fc, const FFar<V> &fn, FFar<V> &f2, const FNear<V> &nc,
-                  FNear<V> &nx2, int tnear) {
+  auto step = [&](int t, const FFar<V> &fc, const FFar<V> &fn, FFar<V> &f2,
+                  const FNear<PS> &nc, FNear<PS> &nx2, int tnear) {
     V tU = 0, zU = 0;
     load_up(t, fc.act, tU, zU);
     load_far(t + PD, f2);
@@ -835,9 +854,9 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
                             fc.t0, nc.t0x, nc.t0y, fc.tau, tW, tE, tS, tN, zW, zE, zS, zN);
       } else {
         if (!frozen)
-          tnew = fwd_cell_up_r<V, XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, nc.a, nc.b, nc.c, nc.f,
-                                      nc.dx_, nc.dy_, fc.t0, nc.t0x, nc.t0y, fc.tau, tUx, tprev,
-                                      zUx, zprev);
+          tnew = fwd_cell_up_r<V, XD, YD>(i, jc, nx, ny, C.rdx, C.rdy, (V)nc.a, (V)nc.b, (V)nc.c,
+                                          (V)nc.f, (V)nc.dx_, (V)nc.dy_, fc.t0, (V)nc.t0x,
+                                          (V)nc.t0y, fc.tau, tUx, tprev, zUx, zprev);
       }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       ftau.st(0, e, tnew);
@@ -849,10 +868,16 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
         acc_m = d > acc_m ? d : acc_m;
       }
     }
-    tprev = tnew;
-    zprev = fc.t0;
+    if constexpr (kDown) {
+      // PD+1 sets: the set refilled next step is this step's
+      tprev = own_reg(tnew);
+      zprev = own_reg(fc.t0);
+    } else {
+      tprev = tnew;
+      zprev = fc.t0;
+    }
   };
-  for (int t = 0; t < nsteps; t += S) {
+  for (int t = -PD; t < nsteps; t += S) {
 #pragma unroll
     for (int q = 0; q < S; ++q)
       step(t + q, fs[q], fs[(q + 1) % S], fs[(q + PD) % S], ns[q], ns[(q + PD) % S], t + q + PD);
@@ -993,7 +1018,11 @@ __global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd) {
   }
   GMEM double *sk0 = gm(D.skew);
   GMEM double *b0 = sk0 + (size_t)sk * kFF * SE;
-  b0[e] = t0; b0[SE + e] = t0x; b0[2 * SE + e] = t0y;
+  b0[e] = t0;
+  if (!MIXED) {  // the mixed mode's passes read the gradients in float
+    b0[SE + e] = t0x;
+    b0[2 * SE + e] = t0y;
+  }
   if (MIXED) {
     GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkF32 * SE);
     GMEM float *b32 = f32 + (size_t)sk * kFF * SE;
@@ -1037,6 +1066,14 @@ __device__ void fwd_finish64(const SlotDesc &D, Team &T, int lane, const int rec
   }
   C.is[0] = gm(sk0);
   C.is[1] = gm(sk0 + kFF * SE);
+  if (UP) {  // and the float T0 gradients
+    const GMEM float *f32 = reinterpret_cast<const GMEM float *>(gm(sk0) + (size_t)kSkF32 * SE);
+    C.gs[0] = f32;
+    C.gs[1] = f32 + (size_t)kFF * SE;
+  } else {
+    C.gs[0] = C.is[0];
+    C.gs[1] = C.is[1];
+  }
   C.snap = gm(sk0 + kSkSnapF * SE);
   int iter = 1;
   bool conv = false;
@@ -1111,6 +1148,8 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_fwd_mixed32(const
   C.ps[1] = gm(D.pskew32 + (size_t)kPF * SE);
   C.is[0] = gm(f32);
   C.is[1] = gm(f32 + (size_t)kFF * SE);
+  C.gs[0] = C.is[0];
+  C.gs[1] = C.is[1];
   C.snap = gm(f32 + 8 * SE);
   float *dP = f32 + 6 * SE, *dM = f32 + 7 * SE;
   int iter = 1;
@@ -1194,14 +1233,15 @@ __global__ __launch_bounds__(256) void k_adj_zero(const SlotDesc *sd) {
 
 // Coefficients of every item of a launch group, one thread per skew element;
 // the neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
-// The source field is left 0 here and scattered in by k_adj_splat.
+// The source field is left 0 here and scattered in by k_adj_splat (FP64)
+// or k_adj_source32 (mixed).
 //   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
 //           the boundary (:1368-1380), in the plus skew
-//   KIND 1 (mixed mode): the same six coefficients rounded to float, for
-//           the FP64 polish, at the start of each skew's coefficient region;
-//           the FP32 phase's set k0 (0 here), kW = a1p/dx/d, kE = -a2m/dx/d,
-//           kS = b1p/dy/d, kN = -b2m/dy/d from the FP64 values; and the
-//           float Ta init, from double skew 16
+//   KIND 1 (mixed mode): the update in the form Ta = k0 + kW*TaW + kE*TaE
+//           + kS*TaS + kN*TaN, its coefficients k0 (0 here), kW = a1p/dx/d,
+//           kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d computed in FP64
+//           and stored in float, which both the FP32 phase and the FP64
+//           polish read; and the float Ta init, from double skew 16
 // Padding elements get zeros.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
@@ -1232,14 +1272,10 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
                Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
                pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
   }
-  GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
   if (KIND == 0) {
+    GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
 #pragma unroll
     for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
-  } else {
-    GMEM float *b0f = (GMEM float *)b0;
-#pragma unroll
-    for (int f = 0; f < kAF; ++f) b0f[f * SE + e] = (float)v[f];
   }
   if (KIND == 0) {
     if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
@@ -1265,9 +1301,8 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
 // ... then, one thread per item: splat the receivers in order into the
 // natural field (the reference's sums, in its order), and scatter the
 // source of every touched interior cell into the coefficient fields of both
-// skews (mixed: into the float set, and k0 = delta/d rounded to float with
-// the set's d, where d != 0).  The right-hand side is w*(t - tobs) (which
-// 0) or -w (which 1).
+// skews (FP64 mode; the mixed mode's k0 = delta/d is k_adj_source32's).
+// The right-hand side is w*(t - tobs) (which 0) or -w (which 1).
 __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count, int which,
                                                   int mixed) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1283,8 +1318,8 @@ __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count,
     D.iters[1 + which] = 0;
     return;
   }
+  if (mixed) return;
   double *sk0 = D.skew;
-  float *k32 = reinterpret_cast<float *>(sk0 + (size_t)kSkA32 * SE);
   for (int q = 0; q < D.nr; ++q) {
     const double xr = deg2rad_s(D.recx[q]), yr = deg2rad_s(D.recy[q]);
     const int i0 = (int)floor((xr - g.xx[0]) / g.dx + 1) - 1;
@@ -1293,19 +1328,40 @@ __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count,
       const int i = i0 + (c >> 1), j = j0 + (c & 1);
       if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
       const double dl = D.delta[i + (size_t)nx * j];
-      for (int sk = 0; sk < 2; ++sk) {
-        const size_t e = skpos(sk, i, j, R);
-        double *b0 = sk0 + (size_t)sk * kAF * SE;
-        if (mixed) {
-          float *b0f = reinterpret_cast<float *>(b0);
-          b0f[5 * SE + e] = (float)dl;
-          const double d = b0f[4 * SE + e];
-          k32[(size_t)sk * 5 * SE + e] = d != 0.0 ? (float)(dl / d) : 0.0f;
-        } else {
-          b0[5 * SE + e] = dl;
-        }
-      }
+      for (int sk = 0; sk < 2; ++sk) sk0[(size_t)sk * kAF * SE + 5 * SE + skpos(sk, i, j, R)] = dl;
     }
+  }
+}
+
+// The mixed mode's source coefficient k0 = delta/d (float; 0 where d is 0)
+// of every interior cell a receiver touches, in both skews, with d the
+// denominator k_adj_build<1> computed, recomputed here from the same
+// operands.  One workgroup per item, a thread per (receiver, corner); cells
+// that several receivers touch are written the same value each time.
+__global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
+  const SlotDesc D = sd[blockIdx.x];
+  if (*D.status < 0) return;  // an earlier stage failed (receivers included)
+  const Grid &g = D.g;
+  const int nx = g.nx, ny = g.ny, R = D.sg.R;
+  const size_t SE = D.sg.skew_elems();
+  const GMEM double *Ts = gm(D.skew) + kSkTP * SE;
+  const GMEM double *pa = gm(D.pskew), *pb = pa + SE, *pc = pa + 2 * SE;
+  GMEM float *k32 = reinterpret_cast<GMEM float *>(gm(D.skew) + (size_t)kSkA32 * SE);
+  for (int w = threadIdx.x; w < 4 * D.nr; w += blockDim.x) {
+    const int q = w >> 2, cn = w & 3;
+    const double xr = deg2rad_s(D.recx[q]), yr = deg2rad_s(D.recy[q]);
+    const int i = (int)floor((xr - g.xx[0]) / g.dx + 1) - 1 + (cn >> 1);
+    const int j = (int)floor((yr - g.yy[0]) / g.dy + 1) - 1 + (cn & 1);
+    if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
+    auto at = [&](int di, int dj) { return skpos(0, i + di, j + dj, R); };
+    const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
+    double v[kAF];
+    adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
+               Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
+               pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], g.dx, g.dy, v);
+    const double dl = D.delta[i + (size_t)nx * j];
+    const float kk = v[4] != 0.0 ? (float)(dl / v[4]) : 0.0f;
+    for (int sk = 0; sk < 2; ++sk) k32[(size_t)sk * 5 * SE + skpos(sk, i, j, R)] = kk;
   }
 }
 
@@ -1326,12 +1382,8 @@ struct ANear<float> {
   float k0, kW, kE, kS, kN;
 };
 
+
 __device__ __forceinline__ void adj_load_near(const FieldSet<double> &f, size_t e,
-                                              ANear<double> &q) {
-  q.a1 = f.ld(0, e); q.a2 = f.ld(1, e); q.b1 = f.ld(2, e); q.b2 = f.ld(3, e);
-  q.d = f.ld(4, e); q.dl = f.ld(5, e);
-}
-__device__ __forceinline__ void adj_load_near(const FieldSet<float> &f, size_t e,
                                               ANear<double> &q) {
   q.a1 = f.ld(0, e); q.a2 = f.ld(1, e); q.b1 = f.ld(2, e); q.b2 = f.ld(3, e);
   q.d = f.ld(4, e); q.dl = f.ld(5, e);
@@ -1355,6 +1407,16 @@ __device__ __forceinline__ float adj_update(const ANear<float> &n, float tW, flo
                                             float tN, double, double) {
   return fmaf(n.kN, tN, fmaf(n.kS, tS, fmaf(n.kE, tE, fmaf(n.kW, tW, n.k0))));
 }
+// The mixed mode's FP64 polish: the same form in double on the float
+// coefficients, accumulated in the same order with a rounding per operation.
+__device__ __forceinline__ double adj_update_k64(const ANear<float> &n, double tW, double tE,
+                                                 double tS, double tN) {
+  double t = (double)n.k0;
+  t = (double)n.kW * tW + t;
+  t = (double)n.kE * tE + t;
+  t = (double)n.kS * tS + t;
+  return (double)n.kN * tN + t;
+}
 
 template <typename V, typename S = V>
 struct AdjCtx {
@@ -1375,9 +1437,10 @@ __device__ void adj_strip_pass(const AdjCtx<V, CS> &C, const Team &T, int k, GME
   const int nx = C.nx, ny = C.ny, R = C.R, lane = C.lane;
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
-  // the coefficient set: kAF fields (a1, a2, b1, b2, d, delta; FP64, or
-  // float for the mixed mode's polish), or the FP32 phase's 5
-  const FieldSet<CS> fis(C.is[SK], SE, sizeof(V) == 8 ? kAF : 5);
+  // the coefficient set: kAF FP64 fields (a1, a2, b1, b2, d, delta), or
+  // the mixed mode's 5 float ones (k0, kW, kE, kS, kN)
+  constexpr bool kK = sizeof(CS) == 4;
+  const FieldSet<CS> fis(C.is[SK], SE, kK ? 5 : kAF);
   const FieldSet<V> fta(ta, SE, 1), fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
@@ -1409,23 +1472,19 @@ __device__ void adj_strip_pass(const AdjCtx<V, CS> &C, const Team &T, int k, GME
     q.tdn = fta.ld(0, opaque_index(dn_ok ? skpos(SK, id, jc, R) : e));
     if (MODE == 2) q.snap = fsnap.ld(0, e);
   };
-  auto load_near = [&](int t, ANear<V> &q) {
+  using NS = ANear<CS>;
+  auto load_near = [&](int t, NS &q) {
     adj_load_near(fis, step_pos<XD, YD>(c, t, lane, ny, L, R), q);
   };
-  // operand sets as in the forward pass: unrolled by S = PD+1 over S sets
-  // (far and near PD steps ahead)
-  static_assert(PD >= 1, "operands are fetched at least one step ahead");
-  constexpr int S = PD + 1;
-  AFar<V> fs[S];
-  ANear<V> ns[S];
-#pragma unroll
-  for (int q = 0; q < PD; ++q) {
-    load_far(q, fs[q]);
-    load_near(q, ns[q]);
-  }
+  // operand sets as in the forward pass: S = PD+2 sets, far and near
+  // operands PD steps ahead, the loop starting PD fetch-only steps early
+  static_assert(PD >= 2, "the next step's far set is read while this step fills one");
+  constexpr int S = PD + 2;
+  AFar<V> fs[S] = {};
+  NS ns[S] = {};
   V tprev = 0;
-  auto step = [&](int t, const AFar<V> &fc, const AFar<V> &fn, AFar<V> &f2, const ANear<V> &nc,
-                  ANear<V> &nx2, int tnear) {
+  auto step = [&](int t, const AFar<V> &fc, const AFar<V> &fn, AFar<V> &f2, const NS &nc,
+                  NS &nx2, int tnear) {
     // the upstream strip's edge for this step, read ahead of the step's
     // global loads (see the forward pass)
     V tU = 0;
@@ -1444,8 +1503,12 @@ __device__ void adj_strip_pass(const AdjCtx<V, CS> &C, const Team &T, int k, GME
     const int jc = fc.j;
     V tnew = fc.ta;
     if (fc.act) {
-      if (interior_col && jc >= 1 && jc <= ny - 2)
-        tnew = adj_update(nc, tW, tE, tS, tN, C.dx, C.dy);
+      if (interior_col && jc >= 1 && jc <= ny - 2) {
+        if constexpr (kK && sizeof(V) == 8)
+          tnew = adj_update_k64(nc, tW, tE, tS, tN);
+        else
+          tnew = adj_update(nc, tW, tE, tS, tN, C.dx, C.dy);
+      }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
       fta.st(0, e, tnew);
       if (pub) team_publish<V>(T, t - lag, tnew, base_out + t - lag + 1);
@@ -1462,7 +1525,7 @@ __device__ void adj_strip_pass(const AdjCtx<V, CS> &C, const Team &T, int k, GME
     }
     tprev = tnew;
   };
-  for (int t = 0; t < nsteps; t += S) {
+  for (int t = -PD; t < nsteps; t += S) {
 #pragma unroll
     for (int q = 0; q < S; ++q)
       step(t + q, fs[q], fs[(q + 1) % S], fs[(q + PD) % S], ns[q], ns[(q + PD) % S], t + q + PD);
@@ -1604,8 +1667,8 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const
   }
 }
 
-// Mixed adjoint, FP64 polish: one iteration of the reference's update on the
-// FP64 coefficients, and the masked output.
+// Mixed adjoint, FP64 polish: one iteration of the update in double on the
+// FP32 phase's float coefficients, and the masked output.
 template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
@@ -1618,12 +1681,12 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const 
   Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
-  // the float coefficient set k_adj_build<1> wrote at the start of each
-  // skew's coefficient region
+  // the float k set of the FP32 phase (k_adj_build<1>, k_adj_source32)
+  const float *f32 = reinterpret_cast<const float *>(sk0 + (size_t)kSkA32 * SE);
   AdjCtx<double, float> C;
   adj_ctx_init(C, g, sg, lane);
-  C.is[0] = gm(reinterpret_cast<const float *>(sk0));
-  C.is[1] = gm(reinterpret_cast<const float *>(sk0 + kAF * SE));
+  C.is[0] = gm(f32);
+  C.is[1] = gm(f32 + 5 * SE);
   C.snap = nullptr;
   double s, m, a;
   adj_iteration<double, false, PD, float>(C, T, gm(TaP), gm(TaM), s, m, a);
@@ -1738,6 +1801,7 @@ void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
     hipLaunchKernelGGL(k_adj_build<0>, skew_grid(gd), dim3(256), 0, s, sd);
   hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((gd.count + 63) / 64)), dim3(64), 0, s, sd,
                      gd.count, which, (int)mixed);
+  if (mixed) hipLaunchKernelGGL(k_adj_source32, dim3(gd.count), dim3(256), 0, s, sd);
 }
 
 void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
