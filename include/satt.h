@@ -79,8 +79,10 @@ extern "C" {
 #define SATT_PREC_FP64 0  /* the reference's FP64 arithmetic, sweep for sweep */
 #define SATT_PREC_MIXED 1 /* FP32 sweeps, then FP64 polish: forward to the
                              reference's stopping rule, adjoint one iteration;
-                             T0, residuals, interpolation and kernel
-                             accumulation in FP64.  Batched form only. */
+                             the polish computes in FP64 on coefficients
+                             stored in float; T0, residuals, interpolation
+                             and kernel accumulation in FP64.  Batched form
+                             only. */
 
 /* launch forms for satt_plan_run */
 #define SATT_LAUNCH_BATCHED 0   /* all items of a chunk in one launch per stage */

@@ -358,10 +358,9 @@ int orc_fwd_t0(const double *xx_deg, const double *yy_deg, int nx, int ny,
 
 /* ------------------------------------------------------------------ adjoint */
 
-static void adj_kform(int nx, int ny, double dx, double dy, const double *a1m,
-                      const double *a1p, const double *a2m, const double *a2p,
-                      const double *b1m, const double *b1p, const double *b2m,
-                      const double *b2p, const double *delta, float *kk);
+static void adj_kform(int nx, int ny, double dx, double dy, const double *T,
+                      const double *a, const double *b, const double *c,
+                      const double *delta, float *kk);
 static int adj_phase32(int nx, int ny, const float *kk, double *Ta, int *done);
 
 static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
@@ -473,7 +472,7 @@ static int adj_common(const double *xx_deg, const double *yy_deg, int nx,
       free(xx); free(yy); free(buf); free(bb);
       return ORC_E_NOMEM;
     }
-    adj_kform(nx, ny, dx, dy, a1m, a1p, a2m, a2p, b1m, b1p, b2m, b2p, delta, kk);
+    adj_kform(nx, ny, dx, dy, T, a, b, c, delta, kk);
     it32 = adj_phase32(nx, ny, kk, Ta, &done32);
     if (it32 < 0) {
       free(xx); free(yy); free(buf); free(bb); free(kk);
@@ -860,11 +859,13 @@ int orc_forward_simulate(int dim, int nx, int ny, const double *xgrids,
  * widened) and T0 in FP64, and the same stop rule (at least one
  * iteration); T = (1 + dtau)*T0.
  *
- * Adjoint.  The face coefficients, d and delta are computed in FP64 as the
- * reference does, and the update (:1392-1402) is taken in the form
- * Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN with k0 = delta/d,
- * kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d computed in
- * double and rounded to float (all zero where |d| < eps).  FP32 phase: that
+ * Adjoint.  The face coefficients and d are computed in FP64 by the
+ * reference's formulas (:1348-1370) on the metric rounded to float, with
+ * products by the reciprocals of dx, dy, 4dx, 4dy, and the update
+ * (:1392-1402) is taken in the form Ta = k0 + kW*TaW + kE*TaE + kS*TaS +
+ * kN*TaN with k0 = delta/d, kW = a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d,
+ * kN = -b2m/dy/d computed in double (as products with 1/d) and rounded to
+ * float (all zero where |d| < eps).  FP32 phase: that
  * update in float, iterated until the reference's absolute rule holds or
  * the largest change is at most 2^-21 of the largest |Ta| (float
  * resolution).  FP64 polish: one iteration of the same update in double on
@@ -1109,25 +1110,48 @@ int orc_fwd_mixed_pre(const double *xx_deg, const double *yy_deg, int nx, int ny
 }
 
 /* The mixed adjoint's float coefficients k0, kW, kE, kS, kN (n each, in
- * kk; see the note above). */
-static void adj_kform(int nx, int ny, double dx, double dy, const double *a1m,
-                      const double *a1p, const double *a2m, const double *a2p,
-                      const double *b1m, const double *b1p, const double *b2m,
-                      const double *b2p, const double *delta, float *kk) {
+ * kk; see the note above): the face coefficients of :1348-1370 on the
+ * metric a, b, c rounded to float (the float period fields the device
+ * reads), with the divisions by dx, dy, 4dx, 4dy taken as products with
+ * their reciprocals and the halvings as products with 0.5, then
+ * kW = a1p*rdx*(1/d) and so on, in the device's order. */
+static void adj_kform(int nx, int ny, double dx, double dy, const double *T,
+                      const double *a, const double *b, const double *c,
+                      const double *delta, float *kk) {
   const size_t n = (size_t)nx * ny;
   float *k0 = kk, *kW = kk + n, *kE = kk + 2 * n, *kS = kk + 3 * n, *kN = kk + 4 * n;
   for (size_t k = 0; k < n; ++k) k0[k] = kW[k] = kE[k] = kS[k] = kN[k] = 0;
+  const double rdx = 1.0 / dx, rdy = 1.0 / dy, rdx4 = 0.25 * rdx, rdy4 = 0.25 * rdy;
+#define F(p, ii, jj) ((double)(float)p[IDX(ii, jj)])
   for (int i = 1; i < nx - 1; ++i)
     for (int j = 1; j < ny - 1; ++j) {
       const size_t k = IDX(i, j);
-      const double d = (a2p[k] - a1m[k]) / dx + (b2p[k] - b1m[k]) / dy;
+      const double Tk = T[k], TW = T[IDX(i - 1, j)], TE = T[IDX(i + 1, j)];
+      const double TS = T[IDX(i, j - 1)], TN = T[IDX(i, j + 1)];
+      const double TWS = T[IDX(i - 1, j - 1)], TWN = T[IDX(i - 1, j + 1)];
+      const double TES = T[IDX(i + 1, j - 1)], TEN = T[IDX(i + 1, j + 1)];
+      const double ak = F(a, i, j), aW = F(a, i - 1, j), aE = F(a, i + 1, j);
+      const double bk = F(b, i, j), bS = F(b, i, j - 1), bN = F(b, i, j + 1);
+      const double ck = F(c, i, j), cW = F(c, i - 1, j), cE = F(c, i + 1, j);
+      const double cS = F(c, i, j - 1), cN = F(c, i, j + 1);
+      const double a1 = -((Tk - TW) * rdx * (ak + aW) * 0.5) + (TN - TS + TWN - TWS) * rdy4 * (ck + cW) * 0.5;
+      const double a2 = -((TE - Tk) * rdx * (aE + ak) * 0.5) + (TEN - TES + TN - TS) * rdy4 * (cE + ck) * 0.5;
+      const double b1 = -((Tk - TS) * rdy * (bk + bS) * 0.5) + (TE - TW + TES - TWS) * rdx4 * (ck + cS) * 0.5;
+      const double b2 = -((TN - Tk) * rdy * (bN + bk) * 0.5) + (TEN - TWN + TE - TW) * rdx4 * (cN + ck) * 0.5;
+      const double a1p = (a1 + fabs(a1)) * 0.5, a1m = (a1 - fabs(a1)) * 0.5;
+      const double a2p = (a2 + fabs(a2)) * 0.5, a2m = (a2 - fabs(a2)) * 0.5;
+      const double b1p = (b1 + fabs(b1)) * 0.5, b1m = (b1 - fabs(b1)) * 0.5;
+      const double b2p = (b2 + fabs(b2)) * 0.5, b2m = (b2 - fabs(b2)) * 0.5;
+      const double d = (a2p - a1m) * rdx + (b2p - b1m) * rdy;
       if (fabs(d) < ADJ_EPS) continue;
-      k0[k] = (float)(delta[k] / d);
-      kW[k] = (float)(a1p[k] / dx / d);
-      kE[k] = (float)(-a2m[k] / dx / d);
-      kS[k] = (float)(b1p[k] / dy / d);
-      kN[k] = (float)(-b2m[k] / dy / d);
+      const double rd = 1.0 / d;
+      k0[k] = (float)(delta[k] * rd);
+      kW[k] = (float)(a1p * rdx * rd);
+      kE[k] = (float)(-a2m * rdx * rd);
+      kS[k] = (float)(b1p * rdy * rd);
+      kN[k] = (float)(-b2m * rdy * rd);
     }
+#undef F
 }
 
 /* FP32 phase of the mixed adjoint (see the note above) on the float

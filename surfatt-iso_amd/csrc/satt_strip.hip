@@ -1200,6 +1200,39 @@ __device__ __forceinline__ void adj_coef_v(double Tk, double TW, double TE, doub
   out[4] = fabs(d) < kAdjEps ? 0.0 : d;
 }
 
+// The mixed mode's update weights of one interior cell, for the form
+// Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN: the face coefficients of
+// adj_coef_v on the metric as stored in float (widened), with the divisions
+// by dx, dy, 4dx and 4dy taken as products with their reciprocals and the
+// halvings as products with 0.5; out = {kW, kE, kS, kN} in double (before
+// the rounding to float) and 1/d, all 0 where abs(d) < eps.
+__device__ __forceinline__ void adj_coef_k(double Tk, double TW, double TE, double TS, double TN,
+                                           double TWS, double TWN, double TES, double TEN,
+                                           double ak, double aW, double aE, double bk, double bS,
+                                           double bN, double ck, double cW, double cE, double cS,
+                                           double cN, double rdx, double rdy, double *out) {
+  const double rdx4 = 0.25 * rdx, rdy4 = 0.25 * rdy;
+  const double a1 = -((Tk - TW) * rdx * (ak + aW) * 0.5) + (TN - TS + TWN - TWS) * rdy4 * (ck + cW) * 0.5;
+  const double a2 = -((TE - Tk) * rdx * (aE + ak) * 0.5) + (TEN - TES + TN - TS) * rdy4 * (cE + ck) * 0.5;
+  const double b1 = -((Tk - TS) * rdy * (bk + bS) * 0.5) + (TE - TW + TES - TWS) * rdx4 * (ck + cS) * 0.5;
+  const double b2 = -((TN - Tk) * rdy * (bN + bk) * 0.5) + (TEN - TWN + TE - TW) * rdx4 * (cN + ck) * 0.5;
+  const double a1p = (a1 + fabs(a1)) * 0.5, a1m = (a1 - fabs(a1)) * 0.5;
+  const double a2p = (a2 + fabs(a2)) * 0.5, a2m = (a2 - fabs(a2)) * 0.5;
+  const double b1p = (b1 + fabs(b1)) * 0.5, b1m = (b1 - fabs(b1)) * 0.5;
+  const double b2p = (b2 + fabs(b2)) * 0.5, b2m = (b2 - fabs(b2)) * 0.5;
+  const double d = (a2p - a1m) * rdx + (b2p - b1m) * rdy;
+  if (fabs(d) < kAdjEps) {
+    out[0] = out[1] = out[2] = out[3] = out[4] = 0.0;
+    return;
+  }
+  const double rd = 1.0 / d;
+  out[0] = a1p * rdx * rd;
+  out[1] = -a2m * rdx * rd;
+  out[2] = b1p * rdy * rd;
+  out[3] = -b2m * rdy * rd;
+  out[4] = rd;
+}
+
 // Delta source of src/eikon_solver_2d.f90:1320-1336, receivers in order.
 __device__ int adj_splat_s(const Grid &g, const double *xr_deg, const double *yr_deg,
                            const double *rhs, int nr, double *delta) {
@@ -1240,8 +1273,9 @@ __global__ __launch_bounds__(256) void k_adj_zero(const SlotDesc *sd) {
 //   KIND 1 (mixed mode): the update in the form Ta = k0 + kW*TaW + kE*TaE
 //           + kS*TaS + kN*TaN, its coefficients k0 (0 here), kW = a1p/dx/d,
 //           kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d computed in FP64
-//           and stored in float, which both the FP32 phase and the FP64
-//           polish read; and the float Ta init, from double skew 16
+//           (adj_coef_k, on the metric as stored in float) and stored in
+//           float, which both the FP32 phase and the FP64 polish read; and
+//           the float Ta init, from double skew 16
 // Padding elements get zeros.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
@@ -1256,39 +1290,39 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
   const double dx = g.dx, dy = g.dy;
   GMEM double *sk0 = gm(D.skew);
   const GMEM double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
-  const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE;
-  const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
   const int l = (int)(e & 63);
   const int r = (int)((e >> 6) % R);
   const int cc = (int)((e >> 6) / R);
   const int i = 64 * cc + l;
   const int j = sk == 0 ? r - l : r + l - 63;
-  double v[kAF] = {0, 0, 0, 0, 0, 0};
   const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  if (interior) {
-    auto at = [&](int di, int dj) { return skpos(sk, i + di, j + dj, R); };
-    const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
-    adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
-               Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
-               pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
-  }
+  auto at = [&](int di, int dj) { return skpos(sk, i + di, j + dj, R); };
   if (KIND == 0) {
+    const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE;
+    const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
+    double v[kAF] = {0, 0, 0, 0, 0, 0};
+    if (interior) {
+      const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
+      adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
+                 Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
+                 pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
+    }
     GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
 #pragma unroll
     for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
-  }
-  if (KIND == 0) {
     if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
   } else {
+    const GMEM float *pa = gm(D.pskew32) + (size_t)sk * kPF * SE;
+    const GMEM float *pb = pa + SE, *pc = pa + 2 * SE;
     float k[5] = {0, 0, 0, 0, 0};
-    if (interior && v[4] != 0.0) {
-      const double d = v[4];
-      const double a1p = (v[0] + fabs(v[0])) / 2, a2m = (v[1] - fabs(v[1])) / 2;
-      const double b1p = (v[2] + fabs(v[2])) / 2, b2m = (v[3] - fabs(v[3])) / 2;
-      k[1] = (float)(a1p / dx / d);
-      k[2] = (float)(-a2m / dx / d);
-      k[3] = (float)(b1p / dy / d);
-      k[4] = (float)(-b2m / dy / d);
+    if (interior) {
+      const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
+      double w[5];
+      adj_coef_k(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
+                 Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
+                 pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], 1.0 / dx, 1.0 / dy, w);
+#pragma unroll
+      for (int f = 0; f < 4; ++f) k[1 + f] = (float)w[f];
     }
     GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
     GMEM float *k32 = f32 + (size_t)sk * 5 * SE;
@@ -1334,9 +1368,8 @@ __global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count,
 }
 
 // The mixed mode's source coefficient k0 = delta/d (float; 0 where d is 0)
-// of every interior cell a receiver touches, in both skews, with d the
-// denominator k_adj_build<1> computed, recomputed here from the same
-// operands.  One workgroup per item, a thread per (receiver, corner); cells
+// of every interior cell a receiver touches, in both skews, with 1/d as
+// k_adj_build<1> computed it, recomputed here from the same operands.  One workgroup per item, a thread per (receiver, corner); cells
 // that several receivers touch are written the same value each time.
 __global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
   const SlotDesc D = sd[blockIdx.x];
@@ -1345,7 +1378,7 @@ __global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
   const int nx = g.nx, ny = g.ny, R = D.sg.R;
   const size_t SE = D.sg.skew_elems();
   const GMEM double *Ts = gm(D.skew) + kSkTP * SE;
-  const GMEM double *pa = gm(D.pskew), *pb = pa + SE, *pc = pa + 2 * SE;
+  const GMEM float *pa = gm(D.pskew32), *pb = pa + SE, *pc = pa + 2 * SE;
   GMEM float *k32 = reinterpret_cast<GMEM float *>(gm(D.skew) + (size_t)kSkA32 * SE);
   for (int w = threadIdx.x; w < 4 * D.nr; w += blockDim.x) {
     const int q = w >> 2, cn = w & 3;
@@ -1355,12 +1388,12 @@ __global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
     if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
     auto at = [&](int di, int dj) { return skpos(0, i + di, j + dj, R); };
     const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
-    double v[kAF];
-    adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
+    double wk[5];
+    adj_coef_k(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
                Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
-               pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], g.dx, g.dy, v);
+               pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], 1.0 / g.dx, 1.0 / g.dy, wk);
     const double dl = D.delta[i + (size_t)nx * j];
-    const float kk = v[4] != 0.0 ? (float)(dl / v[4]) : 0.0f;
+    const float kk = wk[4] != 0.0 ? (float)(dl * wk[4]) : 0.0f;
     for (int sk = 0; sk < 2; ++sk) k32[(size_t)sk * 5 * SE + skpos(sk, i, j, R)] = kk;
   }
 }
