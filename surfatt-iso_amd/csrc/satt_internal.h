@@ -92,7 +92,9 @@ struct SlotDesc {
   Grid g;
   SkewGeom sg;
   const double *pskew;    // the period's skewed fields, [2 skews][kPF][SE]
-  const float *pskew32;   // the same in float (mixed mode), or null
+  const float *pskew32;   // the same in float (mixed mode), or null: per skew
+                          // a, b, c and fun, dis_x, dis_y as two sets of
+                          // three stored element by element
   const double *fun;      // the period's 1/svel, natural layout
   const double *vel;      // the period's svel, natural layout
   double *kern;           // the item's own kernel Ta/svel**2 (serving), or null

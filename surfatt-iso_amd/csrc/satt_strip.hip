@@ -342,10 +342,10 @@ __global__ void k_period_skew(Grid g, SkewGeom sg, int np, const double *a, cons
   double *base = out + ((size_t)p * 2 + sk) * kPF * per_skew;
 #pragma unroll
   for (int f = 0; f < kPF; ++f) base[f * per_skew + e] = v[f];
-  if (out32) {
+  if (out32) {  // two sets of three, element by element (CellSet<3>): a, b, c; fun, dis_x, dis_y
     float *b32 = out32 + ((size_t)p * 2 + sk) * kPF * per_skew;
 #pragma unroll
-    for (int f = 0; f < kPF; ++f) b32[f * per_skew + e] = (float)v[f];
+    for (int f = 0; f < kPF; ++f) b32[(f / 3) * 3 * per_skew + e * 3 + f % 3] = (float)v[f];
   }
 }
 
@@ -600,6 +600,35 @@ struct FieldSet<float> {
   }
 };
 
+// A set of N float fields of one skew stored element by element: element
+// e's N values at e*N .. e*N+N-1, so that one buffer load reads several
+// fields (the mixed mode's period, T0 and adjoint coefficient sets).  A
+// step's loads of a set share one 32-bit offset (plans keep a set within
+// 4 GiB, kMaxSkewElems32); the loads of several consecutive fields are one
+// 64-, 96- or 128-bit load.
+typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <int N>
+struct CellSet {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ CellSet(const GMEM void *base, size_t se)
+      : r(buffer_over((const GMEM float *)base, base ? (unsigned)(N * se * sizeof(float)) : 0u)) {}
+  __device__ static int off(size_t e, int k) { return (int)(((unsigned)e * N + k) * 4u); }
+  __device__ float ld1(size_t e, int k) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off(e, k), 0, 0));
+  }
+  __device__ u32x2 ld2(size_t e, int k) const {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, off(e, k), 0, 0);
+  }
+  __device__ u32x3 ld3(size_t e, int k) const {
+    return __builtin_amdgcn_raw_buffer_load_b96(r, off(e, k), 0, 0);
+  }
+  __device__ u32x4 ld4(size_t e, int k) const {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off(e, k), 0, 0);
+  }
+};
+__device__ __forceinline__ float as_f(unsigned u) { return __builtin_bit_cast(float, u); }
+
 // Copies the grid cells of a field from skew FROM into the other skew, one
 // whole destination row per step (coalesced stores).  The loads of one row
 // gather from 64 source rows; lane l's source rows are its own column shifted
@@ -742,7 +771,16 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   const int c = XD > 0 ? k : T.ns - 1 - k;
   const size_t SE = C.SE;
   using PS = typename std::conditional<UP && sizeof(V) == 8, float, V>::type;
-  const FieldSet<PS> fps((const GMEM PS *)C.ps[SK], SE, kPF), fgs((const GMEM PS *)C.gs[SK], SE, kFF);
+  // float period and T0 sets are stored element by element (CellSet), FP64
+  // ones field by field
+  constexpr bool kF = sizeof(PS) == 4;
+  const FieldSet<PS> fps((const GMEM PS *)C.ps[SK], SE, kF ? 0 : kPF),
+      fgs((const GMEM PS *)C.gs[SK], SE, kF ? 0 : kFF);
+  // the float period fields as two sets of three, a, b, c and fun, dis_x,
+  // dis_y (the adjoint's coefficient build reads the first only)
+  const GMEM float *ps32 = (const GMEM float *)C.ps[SK];
+  const CellSet<3> cpa(kF ? ps32 : nullptr, SE), cpf(kF ? ps32 + 3 * SE : nullptr, SE);
+  const CellSet<kFF> cgs(kF ? C.gs[SK] : nullptr, SE);
   const FieldSet<V> fis(C.is[SK], SE, kFF), ftau(tau, SE, 1), fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
@@ -772,13 +810,30 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
   // behind a branch: a load that a branch may skip leaves the compiler
   // unsure how many loads are in flight, and it then waits for all of them,
   // the prefetched ones included.
-  auto load_far = [&](int t, FFar<V> &q) {
+  // The far and near operands of step t (both fetched PD steps ahead).
+  auto load_ops = [&](int t, FFar<V> &q, FNear<PS> &n) {
     const int u = t - lag;
     const int j = YD > 0 ? u : ny - 1 - u;
     q.j = j;
     q.act = lv && u >= 0 && u < ny;
     const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
-    q.t0 = fis.ld(0, e);
+    if constexpr (kF) {
+      const u32x3 pa = cpa.ld3(e, 0);
+      const u32x3 pf = cpf.ld3(e, 0);
+      const u32x3 g3 = cgs.ld3(e, 0);
+      n.a = as_f(pa.x); n.b = as_f(pa.y); n.c = as_f(pa.z);
+      n.f = as_f(pf.x); n.dx_ = as_f(pf.y); n.dy_ = as_f(pf.z);
+      n.t0x = as_f(g3.y); n.t0y = as_f(g3.z);
+      if constexpr (sizeof(V) == 4)
+        q.t0 = as_f(g3.x);
+      else
+        q.t0 = fis.ld(0, e);  // the polish's FP64 T0
+    } else {
+      q.t0 = fis.ld(0, e);
+      n.a = fps.ld(0, e); n.b = fps.ld(1, e); n.c = fps.ld(2, e); n.f = fps.ld(3, e);
+      n.dx_ = fps.ld(4, e); n.dy_ = fps.ld(5, e);
+      n.t0x = fgs.ld(1, e); n.t0y = fgs.ld(2, e);
+    }
     q.tau = ftau.ld(0, e);
     if constexpr (kDown) {
       const size_t ed = opaque_index(dn_ok ? skpos(SK, id, min(max(j, 0), ny - 1), R) : e);
@@ -786,12 +841,6 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
       q.zdn = fis.ld(0, ed);
     }
     if (MODE == 2) q.snap = fsnap.ld(0, e);
-  };
-  auto load_near = [&](int t, FNear<PS> &q) {
-    const size_t e = step_pos<XD, YD>(c, t, lane, ny, L, R);
-    q.a = fps.ld(0, e); q.b = fps.ld(1, e); q.c = fps.ld(2, e); q.f = fps.ld(3, e);
-    q.dx_ = fps.ld(4, e); q.dy_ = fps.ld(5, e);
-    q.t0x = fgs.ld(1, e); q.t0y = fgs.ld(2, e);
   };
   // The upstream strip's edge (final tau and its T0) for the cell of step t,
   // from the channel; read at the step itself, ahead of the step's global
@@ -828,8 +877,8 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
                   const FNear<PS> &nc, FNear<PS> &nx2, int tnear) {
     V tU = 0, zU = 0;
     load_up(t, fc.act, tU, zU);
-    load_far(t + PD, f2);
-    load_near(tnear, nx2);
+    load_ops(t + PD, f2, nx2);
+    (void)tnear;
     // lane shifts read other lanes' registers: they run with every lane
     // enabled, outside the per-cell branch
     V tUx = lane_shift<-XD>(tprev), zUx = lane_shift<-XD>(zprev);
@@ -923,9 +972,10 @@ __device__ bool fwd_iteration(const FwdCtx<V> &C, Team &T, GMEM V *tP, GMEM V *t
 }
 
 // Per-slot FP32 region of the mixed forward, in float skews from double
-// skew 9: T0, T0x, T0y plus (0..2) and minus (3..5), dtau plus (6) and
-// minus (7), snapshot (8).  Double skews 9..13 are unused by the forward
-// until the output T skews (12, 13) are written after the FP32 phase.
+// skew 9: T0, T0x, T0y plus (0..2) and minus (3..5), each skew's three
+// stored element by element (CellSet<3>), dtau plus (6) and minus (7),
+// snapshot (8).  Double skews 9..13 are unused by the forward until the
+// output T skews (12, 13) are written after the FP32 phase.
 constexpr int kSkF32 = 9;
 
 // sin and cos of the grid's latitudes, once per grid.
@@ -1025,8 +1075,8 @@ __global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd) {
   }
   if (MIXED) {
     GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkF32 * SE);
-    GMEM float *b32 = f32 + (size_t)sk * kFF * SE;
-    b32[e] = (float)t0; b32[SE + e] = (float)t0x; b32[2 * SE + e] = (float)t0y;
+    GMEM float *b32 = f32 + ((size_t)sk * SE + e) * kFF;  // element by element
+    b32[0] = (float)t0; b32[1] = (float)t0x; b32[2] = (float)t0y;
     if (sk == 0) f32[6 * SE + e] = frz ? 0.0f : 9.0f;
   } else if (sk == 0) {
     sk0[kSkTauP * SE + e] = frz ? 1.0 : 10.0;
@@ -1312,22 +1362,25 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
     for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
     if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
   } else {
-    const GMEM float *pa = gm(D.pskew32) + (size_t)sk * kPF * SE;
-    const GMEM float *pb = pa + SE, *pc = pa + 2 * SE;
+    // the float period fields a, b, c, element by element (at e*3)
+    const GMEM float *ps = gm(D.pskew32) + (size_t)sk * kPF * SE;
+    auto pa = [&](size_t q) { return ps[q * 3]; };
+    auto pb = [&](size_t q) { return ps[q * 3 + 1]; };
+    auto pc = [&](size_t q) { return ps[q * 3 + 2]; };
     float k[5] = {0, 0, 0, 0, 0};
     if (interior) {
       const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
       double w[5];
       adj_coef_k(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
-                 Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
-                 pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], 1.0 / dx, 1.0 / dy, w);
+                 Ts[at(1, -1)], Ts[at(1, 1)], pa(k0), pa(kW), pa(kE), pb(k0), pb(kS), pb(kN),
+                 pc(k0), pc(kW), pc(kE), pc(kS), pc(kN), 1.0 / dx, 1.0 / dy, w);
 #pragma unroll
       for (int f = 0; f < 4; ++f) k[1 + f] = (float)w[f];
     }
     GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
-    GMEM float *k32 = f32 + (size_t)sk * 5 * SE;
+    GMEM float *k32 = f32 + ((size_t)sk * SE + e) * 5;  // element by element
 #pragma unroll
-    for (int f = 0; f < 5; ++f) k32[f * SE + e] = k[f];
+    for (int f = 0; f < 5; ++f) k32[f] = k[f];
     if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
   }
 }
@@ -1378,7 +1431,10 @@ __global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
   const int nx = g.nx, ny = g.ny, R = D.sg.R;
   const size_t SE = D.sg.skew_elems();
   const GMEM double *Ts = gm(D.skew) + kSkTP * SE;
-  const GMEM float *pa = gm(D.pskew32), *pb = pa + SE, *pc = pa + 2 * SE;
+  const GMEM float *ps = gm(D.pskew32);  // plus skew: a, b, c element by element
+  auto pa = [&](size_t q) { return ps[q * 3]; };
+  auto pb = [&](size_t q) { return ps[q * 3 + 1]; };
+  auto pc = [&](size_t q) { return ps[q * 3 + 2]; };
   GMEM float *k32 = reinterpret_cast<GMEM float *>(gm(D.skew) + (size_t)kSkA32 * SE);
   for (int w = threadIdx.x; w < 4 * D.nr; w += blockDim.x) {
     const int q = w >> 2, cn = w & 3;
@@ -1390,11 +1446,11 @@ __global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
     const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
     double wk[5];
     adj_coef_k(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
-               Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
-               pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], 1.0 / g.dx, 1.0 / g.dy, wk);
+               Ts[at(1, -1)], Ts[at(1, 1)], pa(k0), pa(kW), pa(kE), pb(k0), pb(kS), pb(kN),
+               pc(k0), pc(kW), pc(kE), pc(kS), pc(kN), 1.0 / g.dx, 1.0 / g.dy, wk);
     const double dl = D.delta[i + (size_t)nx * j];
     const float kk = wk[4] != 0.0 ? (float)(dl * wk[4]) : 0.0f;
-    for (int sk = 0; sk < 2; ++sk) k32[(size_t)sk * 5 * SE + skpos(sk, i, j, R)] = kk;
+    for (int sk = 0; sk < 2; ++sk) k32[((size_t)sk * SE + skpos(sk, i, j, R)) * 5] = kk;
   }
 }
 
@@ -1421,10 +1477,10 @@ __device__ __forceinline__ void adj_load_near(const FieldSet<double> &f, size_t 
   q.a1 = f.ld(0, e); q.a2 = f.ld(1, e); q.b1 = f.ld(2, e); q.b2 = f.ld(3, e);
   q.d = f.ld(4, e); q.dl = f.ld(5, e);
 }
-__device__ __forceinline__ void adj_load_near(const FieldSet<float> &f, size_t e,
-                                              ANear<float> &q) {
-  q.k0 = f.ld(0, e); q.kW = f.ld(1, e); q.kE = f.ld(2, e); q.kS = f.ld(3, e);
-  q.kN = f.ld(4, e);
+__device__ __forceinline__ void adj_load_near(const CellSet<5> &f, size_t e, ANear<float> &q) {
+  const u32x4 k4 = f.ld4(e, 0);
+  q.k0 = as_f(k4.x); q.kW = as_f(k4.y); q.kE = as_f(k4.z); q.kS = as_f(k4.w);
+  q.kN = f.ld1(e, 4);
 }
 
 // The interior update (src/eikon_solver_2d.f90:1392-1402), and its FP32 form.
@@ -1451,6 +1507,15 @@ __device__ __forceinline__ double adj_update_k64(const ANear<float> &n, double t
   return (double)n.kN * tN + t;
 }
 
+template <bool K>
+__device__ __forceinline__ typename std::conditional<K, CellSet<5>, FieldSet<double>>::type
+make_coef_set(const GMEM void *base, size_t SE) {
+  if constexpr (K)
+    return CellSet<5>(base, SE);
+  else
+    return FieldSet<double>((const GMEM double *)base, SE, kAF);
+}
+
 template <typename V, typename S = V>
 struct AdjCtx {
   int nx, ny, R, lane;
@@ -1473,7 +1538,9 @@ __device__ void adj_strip_pass(const AdjCtx<V, CS> &C, const Team &T, int k, GME
   // the coefficient set: kAF FP64 fields (a1, a2, b1, b2, d, delta), or
   // the mixed mode's 5 float ones (k0, kW, kE, kS, kN)
   constexpr bool kK = sizeof(CS) == 4;
-  const FieldSet<CS> fis(C.is[SK], SE, kK ? 5 : kAF);
+  // the k set element by element (CellSet), the FP64 set field by field
+  using CSet = typename std::conditional<kK, CellSet<5>, FieldSet<double>>::type;
+  const CSet fis = make_coef_set<kK>(C.is[SK], SE);
   const FieldSet<V> fta(ta, SE, 1), fsnap(C.snap, SE, 1);
   const int L = min(64, nx - 64 * c);
   const int i = 64 * c + lane;
@@ -1659,8 +1726,9 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_strip(const S
 }
 
 // Mixed adjoint, FP32 phase: iterations on the float coefficients (float
-// skews from double skew 16: k0..kN plus 0..4 and minus 5..9, Ta plus 10 and
-// minus 11, snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
+// skews from double skew 16: k0..kN plus 0..4 and minus 5..9, each skew's
+// five stored element by element (CellSet<5>), Ta plus 10 and minus 11,
+// snapshot 12), then Ta = (double)Ta32 into the FP64 plus skew.
 template <bool TEAM, int PD>
 __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_mixed32(const SlotDesc *sd, int which) {
   extern __shared__ double lds[];
