@@ -1283,41 +1283,9 @@ __device__ __forceinline__ void adj_coef_k(double Tk, double TW, double TE, doub
   out[4] = rd;
 }
 
-// Delta source of src/eikon_solver_2d.f90:1320-1336, receivers in order.
-__device__ int adj_splat_s(const Grid &g, const double *xr_deg, const double *yr_deg,
-                           const double *rhs, int nr, double *delta) {
-  const int nx = g.nx, ny = g.ny;
-  for (int r = 0; r < nr; ++r) {
-    const double xr = deg2rad_s(xr_deg[r]), yr = deg2rad_s(yr_deg[r]);
-    const int idx0 = (int)floor((xr - g.xx[0]) / g.dx + 1);
-    const int idy0 = (int)floor((yr - g.yy[0]) / g.dy + 1);
-    if (idx0 < 1 || idy0 < 1 || idx0 + 1 > nx || idy0 + 1 > ny) return kERecOutOfGrid;
-    const int i0 = idx0 - 1, j0 = idy0 - 1;
-    const float r1 = weight_r4_s((xr - g.xx[i0]) / g.dx);
-    const float r2 = weight_r4_s((yr - g.yy[j0]) / g.dy);
-    const double sa = rhs[r];
-    const double den = g.dx * kREarth * cos(yr) * g.dy * kREarth;
-    const size_t k00 = i0 + (size_t)nx * j0, k01 = k00 + nx, k10 = k00 + 1, k11 = k00 + nx + 1;
-    delta[k00] = delta[k00] + sa * (double)(1.0f - r1) * (double)(1.0f - r2) / den;
-    delta[k01] = delta[k01] + sa * (double)(1.0f - r1) * (double)r2 / den;
-    delta[k10] = delta[k10] + sa * (double)r1 * (double)(1.0f - r2) / den;
-    delta[k11] = delta[k11] + sa * (double)r1 * (double)r2 / den;
-  }
-  return kOk;
-}
-
-// Delta sources of a launch group (src/eikon_solver_2d.f90:1320-1336): zero
-// the natural fields, one thread per cell ...
-__global__ __launch_bounds__(256) void k_adj_zero(const SlotDesc *sd) {
-  const SlotDesc D = sd[blockIdx.y];
-  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (k < (size_t)D.g.nx * D.g.ny) D.delta[k] = 0;
-}
-
 // Coefficients of every item of a launch group, one thread per skew element;
 // the neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
-// The source field is left 0 here and scattered in by k_adj_splat (FP64)
-// or k_adj_source32 (mixed).
+// The source field is left 0 here and scattered in by k_adj_source.
 //   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
 //           the boundary (:1368-1380), in the plus skew
 //   KIND 1 (mixed mode): the update in the form Ta = k0 + kW*TaW + kE*TaE
@@ -1385,72 +1353,123 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
   }
 }
 
-// ... then, one thread per item: splat the receivers in order into the
-// natural field (the reference's sums, in its order), and scatter the
-// source of every touched interior cell into the coefficient fields of both
-// skews (FP64 mode; the mixed mode's k0 = delta/d is k_adj_source32's).
-// The right-hand side is w*(t - tobs) (which 0) or -w (which 1).
-__global__ __launch_bounds__(64) void k_adj_splat(const SlotDesc *sd, int count, int which,
-                                                  int mixed) {
-  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= count) return;
-  const SlotDesc D = sd[slot];
+// The delta source of a launch group's items (src/eikon_solver_2d.f90:
+// 1320-1336) into their coefficient fields, one workgroup per item: each
+// thread forms the terms of receiver corners (entry 4r + c, in the
+// reference's order), then the first entry of every touched cell sums that
+// cell's terms in entry order, from 0 as the reference's zeroed field does,
+// and writes the cell's source, interior cells only: delta itself in both
+// skews (FP64 mode), or k0 = delta/d in the float k set with 1/d as
+// k_adj_build<1> computes it (mixed).  Only the touched cells are read
+// later, so no field is cleared first.  An item with more receivers than
+// the workgroup's entries hold is done by one thread in receiver order.
+constexpr int kSrcEntries = 2048;
+__global__ __launch_bounds__(256) void k_adj_source(const SlotDesc *sd, int which, int mixed) {
+  __shared__ int cell[kSrcEntries];
+  __shared__ double term[kSrcEntries];
+  __shared__ int bad;
+  const SlotDesc D = sd[blockIdx.x];
   if (*D.status < 0) return;  // an earlier stage failed
   const Grid &g = D.g;
   const int nx = g.nx, ny = g.ny, R = D.sg.R;
   const size_t SE = D.sg.skew_elems();
-  const int err = adj_splat_s(g, D.recx, D.recy, which ? D.rhs_den : D.rhs_adj, D.nr, D.delta);
-  if (err != kOk) {
-    *D.status = err;
-    D.iters[1 + which] = 0;
-    return;
-  }
-  if (mixed) return;
-  double *sk0 = D.skew;
-  for (int q = 0; q < D.nr; ++q) {
-    const double xr = deg2rad_s(D.recx[q]), yr = deg2rad_s(D.recy[q]);
-    const int i0 = (int)floor((xr - g.xx[0]) / g.dx + 1) - 1;
-    const int j0 = (int)floor((yr - g.yy[0]) / g.dy + 1) - 1;
-    for (int c = 0; c < 4; ++c) {
-      const int i = i0 + (c >> 1), j = j0 + (c & 1);
-      if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
-      const double dl = D.delta[i + (size_t)nx * j];
+  const double *rhs = which ? D.rhs_den : D.rhs_adj;
+  const int n = 4 * D.nr;
+  GMEM double *sk0 = gm(D.skew);
+  // the source of interior cell (i, j) into the item's coefficient fields
+  auto put = [&](int i, int j, double dl) {
+    if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) return;
+    if (!mixed) {
       for (int sk = 0; sk < 2; ++sk) sk0[(size_t)sk * kAF * SE + 5 * SE + skpos(sk, i, j, R)] = dl;
+      return;
     }
-  }
-}
-
-// The mixed mode's source coefficient k0 = delta/d (float; 0 where d is 0)
-// of every interior cell a receiver touches, in both skews, with 1/d as
-// k_adj_build<1> computed it, recomputed here from the same operands.  One workgroup per item, a thread per (receiver, corner); cells
-// that several receivers touch are written the same value each time.
-__global__ __launch_bounds__(256) void k_adj_source32(const SlotDesc *sd) {
-  const SlotDesc D = sd[blockIdx.x];
-  if (*D.status < 0) return;  // an earlier stage failed (receivers included)
-  const Grid &g = D.g;
-  const int nx = g.nx, ny = g.ny, R = D.sg.R;
-  const size_t SE = D.sg.skew_elems();
-  const GMEM double *Ts = gm(D.skew) + kSkTP * SE;
-  const GMEM float *ps = gm(D.pskew32);  // plus skew: a, b, c element by element
-  auto pa = [&](size_t q) { return ps[q * 3]; };
-  auto pb = [&](size_t q) { return ps[q * 3 + 1]; };
-  auto pc = [&](size_t q) { return ps[q * 3 + 2]; };
-  GMEM float *k32 = reinterpret_cast<GMEM float *>(gm(D.skew) + (size_t)kSkA32 * SE);
-  for (int w = threadIdx.x; w < 4 * D.nr; w += blockDim.x) {
-    const int q = w >> 2, cn = w & 3;
-    const double xr = deg2rad_s(D.recx[q]), yr = deg2rad_s(D.recy[q]);
-    const int i = (int)floor((xr - g.xx[0]) / g.dx + 1) - 1 + (cn >> 1);
-    const int j = (int)floor((yr - g.yy[0]) / g.dy + 1) - 1 + (cn & 1);
-    if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) continue;
+    const GMEM double *Ts = sk0 + kSkTP * SE;
+    const GMEM float *ps = gm(D.pskew32);  // plus skew: a, b, c element by element
+    auto pa = [&](size_t q) { return ps[q * 3]; };
+    auto pb = [&](size_t q) { return ps[q * 3 + 1]; };
+    auto pc = [&](size_t q) { return ps[q * 3 + 2]; };
     auto at = [&](int di, int dj) { return skpos(0, i + di, j + dj, R); };
     const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
     double wk[5];
     adj_coef_k(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
                Ts[at(1, -1)], Ts[at(1, 1)], pa(k0), pa(kW), pa(kE), pb(k0), pb(kS), pb(kN),
                pc(k0), pc(kW), pc(kE), pc(kS), pc(kN), 1.0 / g.dx, 1.0 / g.dy, wk);
-    const double dl = D.delta[i + (size_t)nx * j];
     const float kk = wk[4] != 0.0 ? (float)(dl * wk[4]) : 0.0f;
+    GMEM float *k32 = reinterpret_cast<GMEM float *>(sk0 + (size_t)kSkA32 * SE);
     for (int sk = 0; sk < 2; ++sk) k32[((size_t)sk * SE + skpos(sk, i, j, R)) * 5] = kk;
+  };
+  // receiver q's bracket (i0, j0), its weights and its term's scale
+  auto bracket = [&](int q, int &i0, int &j0, float &r1, float &r2, double &sa, double &den) {
+    const double xr = deg2rad_s(D.recx[q]), yr = deg2rad_s(D.recy[q]);
+    const int idx0 = (int)floor((xr - g.xx[0]) / g.dx + 1);
+    const int idy0 = (int)floor((yr - g.yy[0]) / g.dy + 1);
+    if (idx0 < 1 || idy0 < 1 || idx0 + 1 > nx || idy0 + 1 > ny) return false;
+    i0 = idx0 - 1;
+    j0 = idy0 - 1;
+    r1 = weight_r4_s((xr - g.xx[i0]) / g.dx);
+    r2 = weight_r4_s((yr - g.yy[j0]) / g.dy);
+    sa = rhs[q];
+    den = g.dx * kREarth * cos(yr) * g.dy * kREarth;
+    return true;
+  };
+  auto corner = [&](int c, double sa, float r1, float r2, double den) {
+    const float wa = c < 2 ? 1.0f - r1 : r1, wb = (c & 1) ? r2 : 1.0f - r2;
+    return sa * (double)wa * (double)wb / den;
+  };
+  if (n > kSrcEntries) {  // one thread, receivers in order, on the natural field
+    if (threadIdx.x != 0) return;
+    double *delta = D.delta;
+    for (int pass = 0; pass < 3; ++pass)  // clear, accumulate, scatter
+      for (int q = 0; q < D.nr; ++q) {
+        int i0, j0;
+        float r1, r2;
+        double sa, den;
+        if (!bracket(q, i0, j0, r1, r2, sa, den)) {
+          *D.status = kERecOutOfGrid;
+          D.iters[1 + which] = 0;
+          return;
+        }
+        for (int c = 0; c < 4; ++c) {
+          const size_t k = i0 + (c >> 1) + (size_t)nx * (j0 + (c & 1));
+          if (pass == 0) delta[k] = 0;
+          else if (pass == 1) delta[k] = delta[k] + corner(c, sa, r1, r2, den);
+          else put(i0 + (c >> 1), j0 + (c & 1), delta[k]);
+        }
+      }
+    return;
+  }
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int w = threadIdx.x; w < n; w += blockDim.x) {
+    int i0, j0;
+    float r1, r2;
+    double sa, den;
+    const int q = w >> 2, c = w & 3;
+    if (!bracket(q, i0, j0, r1, r2, sa, den)) {
+      bad = 1;
+      cell[w] = -1;
+      continue;
+    }
+    cell[w] = i0 + (c >> 1) + nx * (j0 + (c & 1));
+    term[w] = corner(c, sa, r1, r2, den);
+  }
+  __syncthreads();
+  if (bad) {
+    if (threadIdx.x == 0) {
+      *D.status = kERecOutOfGrid;
+      D.iters[1 + which] = 0;
+    }
+    return;
+  }
+  for (int w = threadIdx.x; w < n; w += blockDim.x) {
+    const int k = cell[w];
+    bool first = true;
+    for (int v = 0; v < w && first; ++v) first = cell[v] != k;
+    if (!first) continue;
+    double dl = 0;
+    for (int v = w; v < n; ++v)
+      if (cell[v] == k) dl = dl + term[v];
+    put(k % nx, k / nx, dl);
   }
 }
 
@@ -1782,7 +1801,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const 
   Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
-  // the float k set of the FP32 phase (k_adj_build<1>, k_adj_source32)
+  // the float k set of the FP32 phase (k_adj_build<1>, k_adj_source)
   const float *f32 = reinterpret_cast<const float *>(sk0 + (size_t)kSkA32 * SE);
   AdjCtx<double, float> C;
   adj_ctx_init(C, g, sg, lane);
@@ -1894,15 +1913,11 @@ void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
 
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
                       bool mixed) {
-  hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((gd.max_n + 255) / 256), gd.count), dim3(256),
-                     0, s, sd);
   if (mixed)
     hipLaunchKernelGGL(k_adj_build<1>, skew_grid(gd), dim3(256), 0, s, sd);
   else
     hipLaunchKernelGGL(k_adj_build<0>, skew_grid(gd), dim3(256), 0, s, sd);
-  hipLaunchKernelGGL(k_adj_splat, dim3((unsigned)((gd.count + 63) / 64)), dim3(64), 0, s, sd,
-                     gd.count, which, (int)mixed);
-  if (mixed) hipLaunchKernelGGL(k_adj_source32, dim3(gd.count), dim3(256), 0, s, sd);
+  hipLaunchKernelGGL(k_adj_source, dim3(gd.count), dim3(256), 0, s, sd, which, (int)mixed);
 }
 
 void launch_adj_strip(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which) {
