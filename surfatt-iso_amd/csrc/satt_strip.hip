@@ -566,6 +566,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const GMEM T *p, u
   return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)n, 0x00020000);
 }
 constexpr int kOutOfRange = INT_MIN;  // a buffer offset past any resource
+constexpr int kNT = 2;  // a buffer access's cache-policy operand: the non-temporal hint
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 // A set of n skewed fields of SE elements each (field k at base + k*SE), as
@@ -583,6 +584,9 @@ struct FieldSet<double> {
   __device__ FieldSet(const GMEM double *base, size_t se, int) : p((GMEM double *)base), SE(se) {}
   __device__ double ld(int k, size_t e) const { return p[k * SE + e]; }
   __device__ void st(int k, size_t e, double v) const { p[k * SE + e] = v; }
+  // the same with the non-temporal hint (streamed data the L2 need not keep)
+  __device__ double ldn(int k, size_t e) const { return __builtin_nontemporal_load(&p[k * SE + e]); }
+  __device__ void stn(int k, size_t e, double v) const { __builtin_nontemporal_store(v, &p[k * SE + e]); }
 };
 template <>
 struct FieldSet<float> {
@@ -597,6 +601,15 @@ struct FieldSet<float> {
   __device__ void st(int k, size_t e, float v) const {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r,
                                           (int)((unsigned)e * 4u), (int)(k * fb), 0);
+  }
+  // the same with the non-temporal hint (streamed data the L2 need not keep)
+  __device__ float ldn(int k, size_t e) const {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)((unsigned)e * 4u), (int)(k * fb), kNT));
+  }
+  __device__ void stn(int k, size_t e, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r,
+                                          (int)((unsigned)e * 4u), (int)(k * fb), kNT);
   }
 };
 
@@ -622,6 +635,9 @@ struct CellSet {
   }
   __device__ u32x3 ld3(size_t e, int k) const {
     return __builtin_amdgcn_raw_buffer_load_b96(r, off(e, k), 0, 0);
+  }
+  __device__ u32x3 ld3n(size_t e, int k) const {
+    return __builtin_amdgcn_raw_buffer_load_b96(r, off(e, k), 0, kNT);
   }
   __device__ u32x4 ld4(size_t e, int k) const {
     return __builtin_amdgcn_raw_buffer_load_b128(r, off(e, k), 0, 0);
@@ -820,27 +836,32 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
     if constexpr (kF) {
       const u32x3 pa = cpa.ld3(e, 0);
       const u32x3 pf = cpf.ld3(e, 0);
-      const u32x3 g3 = cgs.ld3(e, 0);
+      const u32x3 g3 = cgs.ld3n(e, 0);
       n.a = as_f(pa.x); n.b = as_f(pa.y); n.c = as_f(pa.z);
       n.f = as_f(pf.x); n.dx_ = as_f(pf.y); n.dy_ = as_f(pf.z);
       n.t0x = as_f(g3.y); n.t0y = as_f(g3.z);
       if constexpr (sizeof(V) == 4)
         q.t0 = as_f(g3.x);
       else
-        q.t0 = fis.ld(0, e);  // the polish's FP64 T0
+        q.t0 = fis.ldn(0, e);  // the polish's FP64 T0
     } else {
       q.t0 = fis.ld(0, e);
       n.a = fps.ld(0, e); n.b = fps.ld(1, e); n.c = fps.ld(2, e); n.f = fps.ld(3, e);
       n.dx_ = fps.ld(4, e); n.dy_ = fps.ld(5, e);
       n.t0x = fgs.ld(1, e); n.t0y = fgs.ld(2, e);
     }
-    q.tau = ftau.ld(0, e);
+    if constexpr (kF) {
+      q.tau = ftau.ldn(0, e);
+      if (MODE == 2) q.snap = fsnap.ldn(0, e);
+    } else {
+      q.tau = ftau.ld(0, e);
+      if (MODE == 2) q.snap = fsnap.ld(0, e);
+    }
     if constexpr (kDown) {
       const size_t ed = opaque_index(dn_ok ? skpos(SK, id, min(max(j, 0), ny - 1), R) : e);
       q.tdn = ftau.ld(0, ed);
       q.zdn = fis.ld(0, ed);
     }
-    if (MODE == 2) q.snap = fsnap.ld(0, e);
   };
   // The upstream strip's edge (final tau and its T0) for the cell of step t,
   // from the channel; read at the step itself, ahead of the step's global
@@ -908,9 +929,14 @@ __device__ void fwd_strip_pass(const FwdCtx<V> &C, const Team &T, int k, GMEM V 
                                           (V)nc.t0y, fc.tau, tUx, tprev, zUx, zprev);
       }
       const size_t e = skidx(c, skew_row<XD, YD>(t, ny, L), lane, R);
-      ftau.st(0, e, tnew);
+      if constexpr (kF) {
+        ftau.stn(0, e, tnew);
+        if (MODE == 1) fsnap.stn(0, e, fc.tau);
+      } else {
+        ftau.st(0, e, tnew);
+        if (MODE == 1) fsnap.st(0, e, fc.tau);
+      }
       if (pub) team_publish2<V>(T, t - lag, tnew, fc.t0, base_out + t - lag + 1);
-      if (MODE == 1) fsnap.st(0, e, fc.tau);
       if (MODE == 2) {
         const double d = fabs((double)tnew - (double)fc.snap) * (double)fc.t0;
         acc_s += d;
