@@ -653,7 +653,9 @@ __device__ __forceinline__ float as_f(unsigned u) { return __builtin_bit_cast(fl
 // a line's uses fall within few batches and the batch's latency is paid
 // once.  Loads and stores go through buffer resources over the strip: rows
 // outside it read 0 and stores to padding cells are dropped, with no branch.
-// Padding elements of the destination are left as they are.
+// Padding elements of the destination are left as they are.  The stores
+// carry the non-temporal hint (the loads do not: a source line is read again
+// by the next batches).
 // Rows per batch: 16 in FP32, 8 in FP64 (measured at 401^2 x 4000 items and
 // 41^2 x 1600: 32 rows cost the FP32 kernels an occupancy step on the small
 // grid and gained nothing on the large one).
@@ -696,9 +698,9 @@ __device__ void skew_copy(const Team &T, const GMEM V *__restrict__ src, GMEM V 
         const int off = col && j >= 0 && j < ny ? ((r0 + u) * 64 + lane) * (int)sizeof(V)
                                                 : kOutOfRange;
         if constexpr (sizeof(V) == 4)
-          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[u]), rd, off, 0, 0);
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[u]), rd, off, 0, kNT);
         else
-          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[u]), rd, off, 0, 0);
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[u]), rd, off, 0, kNT);
       }
     }
   }
