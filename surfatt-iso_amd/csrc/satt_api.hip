@@ -1438,7 +1438,6 @@ SlotDesc make_desc(const Part &Q, int k, double *base, int slot_skews, int mixed
 // Items that run together on one XCD then stream the same rows of their
 // period's fields through its L2.  Every item still has its own slot and
 // scratch, so the order changes no result.
-constexpr int kXcds = 8;
 void xcd_order(const satt_plan *P, std::vector<std::pair<int, int>> &slots) {
   const size_t n = slots.size();
   if (n < 2 * (size_t)kXcds) return;

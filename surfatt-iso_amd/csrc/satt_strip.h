@@ -71,6 +71,10 @@ struct GroupDims {
 // registers per step.
 constexpr int kDeepPrefetch = 4;
 
+// XCDs (chiplets, each with its own L2) of the device: workgroup w of a
+// launch runs on XCD w mod kXcds.
+constexpr int kXcds = 8;
+
 // Waves per item of the strip kernels: each strip of a sweep goes to one
 // wave of the item's team, the next strip's wave following at least one
 // strip width behind.  More waves cut an item's latency on wide grids; one

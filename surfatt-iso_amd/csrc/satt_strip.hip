@@ -1062,53 +1062,107 @@ __device__ __forceinline__ void t0_cell_tab(double sy, double cy, double sdx, do
   }
 }
 
-// Forward setup of every item of a chunk, one thread per skew element:
+// Tiled per-element launches (the forward setup, the adjoint coefficient
+// build): a workgroup does a tile of kTileElems consecutive elements of one
+// skew of one slot (32 rows of a strip), so that the rows above and below a
+// row, which the build's cells read, are mostly the tile's own, and the row
+// of each element follows from the tile's first row without a division per
+// element.  The workgroups are dealt to the XCDs as the launch group's slots
+// are (workgroup w runs on XCD w mod kXcds and slot s belongs to XCD s mod
+// kXcds), tile by tile: the items that share an XCD then read each tile of
+// their period's fields through its L2 one after another.
+constexpr int kTileElems = 2048;
+
+unsigned tiles_per_skew(const GroupDims &gd) {
+  return (unsigned)((gd.max_se + kTileElems - 1) / kTileElems);
+}
+dim3 tile_grid(const GroupDims &gd) {
+  return dim3((unsigned)(kXcds * ((gd.count + kXcds - 1) / kXcds) * 2 * tiles_per_skew(gd)));
+}
+
+// The slot and tile (skew-major: tile t is tile t mod tps of skew t / tps)
+// of this workgroup; false if it has none.
+__device__ __forceinline__ bool tile_slot(int count, unsigned tps, int &slot, unsigned &t) {
+  const int x = (int)(blockIdx.x % kXcds);
+  const unsigned k = blockIdx.x / kXcds;
+  const int cnt = (count - x + kXcds - 1) / kXcds;  // slots of XCD x
+  if (cnt <= 0) return false;
+  t = k / (unsigned)cnt;
+  if (t >= 2 * tps) return false;
+  slot = x + kXcds * (int)(k % (unsigned)cnt);
+  return true;
+}
+
+// f(e, cc, r, l) for this thread's elements (strip cc, row r, lane l) of
+// tile ts of a skew; a tile's 32 rows cross into the next strip at most once
+// (R > 64).
+template <typename F>
+__device__ __forceinline__ void tile_elems(const SkewGeom &sg, unsigned ts, F &&f) {
+  const size_t SE = sg.skew_elems();
+  const int R = sg.R;
+  const unsigned row0 = ts * (unsigned)(kTileElems / 64);
+  const int cc0 = (int)(row0 / (unsigned)R), r0 = (int)(row0 % (unsigned)R);
+  const int l = (int)(threadIdx.x & 63);
+  for (int q = 0; q < kTileElems / 256; ++q) {
+    int r = r0 + 4 * q + (int)(threadIdx.x >> 6), cc = cc0;
+    if (r >= R) {
+      r -= R;
+      ++cc;
+    }
+    const size_t e = ((size_t)cc * R + r) * 64 + l;
+    if (e >= SE) break;
+    f(e, cc, r, l);
+  }
+}
+
+// Forward setup of every item of a launch group, in XCD-dealt tiles:
 // T0, T0x, T0y in both skews (and, MIXED, rounded to float), and the tau
 // init in the plus skew (:859-872: 1 on the frozen cells, 10 elsewhere;
 // MIXED: dtau = tau - 1 in float).  Padding elements get zeros.  A frozen
 // cell on the boundary is the reference's 'source on the boundary' error
 // (:863-867).
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd) {
-  const SlotDesc D = sd[blockIdx.z];
+__global__ __launch_bounds__(256) void k_fwd_setup(const SlotDesc *sd, int count, unsigned tps) {
+  int slot;
+  unsigned t;
+  if (!tile_slot(count, tps, slot, t)) return;
+  const SlotDesc D = sd[slot];
+  if (*D.status < 0) return;
   const Grid &g = D.g;
   const SkewGeom &sg = D.sg;
   const size_t SE = sg.skew_elems();
-  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const int sk = blockIdx.y;
-  if (e >= SE || *D.status < 0) return;
+  const int sk = (int)(t / tps);
   const int nx = g.nx, ny = g.ny;
   const GMEM double *aux = gm(D.aux);
-  const int l = (int)(e & 63);
-  const int r = (int)((e >> 6) % sg.R);
-  const int cc = (int)((e >> 6) / sg.R);
-  const int i = 64 * cc + l;
-  const int j = sk == 0 ? r - l : r + l - 63;
-  const bool valid = i < nx && j >= 0 && j < ny;
-  double t0 = 0, t0x = 0, t0y = 0;
-  bool frz = false;
-  if (valid) {
-    const GMEM double *tail = aux + 3 * nx + ny;
-    t0_cell_tab(gm(g.sy)[j], gm(g.cy)[j], aux[i], aux[nx + i], tail[1], tail[2], tail[0], t0,
-                t0x, t0y);
-    frz = aux[2 * nx + i] != 0.0 && aux[3 * nx + j] != 0.0;
-    if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) *D.status = kESrcOnBoundary;
-  }
+  const GMEM double *tail = aux + 3 * nx + ny;
+  const double fun0 = tail[0], sy0 = tail[1], cy0 = tail[2];
   GMEM double *sk0 = gm(D.skew);
-  GMEM double *b0 = sk0 + (size_t)sk * kFF * SE;
-  b0[e] = t0;
-  if (!MIXED) {  // the mixed mode's passes read the gradients in float
-    b0[SE + e] = t0x;
-    b0[2 * SE + e] = t0y;
-  }
-  if (MIXED) {
-    GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkF32 * SE);
-    GMEM float *b32 = f32 + ((size_t)sk * SE + e) * kFF;  // element by element
-    b32[0] = (float)t0; b32[1] = (float)t0x; b32[2] = (float)t0y;
-    if (sk == 0) f32[6 * SE + e] = frz ? 0.0f : 9.0f;
-  } else if (sk == 0) {
-    sk0[kSkTauP * SE + e] = frz ? 1.0 : 10.0;
-  }
+  tile_elems(sg, t % tps, [&](size_t e, int cc, int r, int l) {
+    const int i = 64 * cc + l;
+    const int j = sk == 0 ? r - l : r + l - 63;
+    const bool valid = i < nx && j >= 0 && j < ny;
+    double t0 = 0, t0x = 0, t0y = 0;
+    bool frz = false;
+    if (valid) {
+      t0_cell_tab(gm(g.sy)[j], gm(g.cy)[j], aux[i], aux[nx + i], sy0, cy0, fun0, t0, t0x, t0y);
+      frz = aux[2 * nx + i] != 0.0 && aux[3 * nx + j] != 0.0;
+      if (frz && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) *D.status = kESrcOnBoundary;
+    }
+    GMEM double *b0 = sk0 + (size_t)sk * kFF * SE;
+    b0[e] = t0;
+    if (!MIXED) {  // the mixed mode's passes read the gradients in float
+      b0[SE + e] = t0x;
+      b0[2 * SE + e] = t0y;
+    }
+    if (MIXED) {
+      GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkF32 * SE);
+      GMEM float *b32 = f32 + ((size_t)sk * SE + e) * kFF;  // element by element
+      b32[0] = (float)t0; b32[1] = (float)t0x; b32[2] = (float)t0y;
+      if (sk == 0) f32[6 * SE + e] = frz ? 0.0f : 9.0f;
+    } else if (sk == 0) {
+      sk0[kSkTauP * SE + e] = frz ? 1.0 : 10.0;
+    }
+  });
 }
 
 template <typename V>
@@ -1311,7 +1365,7 @@ __device__ __forceinline__ void adj_coef_k(double Tk, double TW, double TE, doub
   out[4] = rd;
 }
 
-// Coefficients of every item of a launch group, one thread per skew element;
+// Coefficients of every item of a launch group, per skew element;
 // the neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
 // The source field is left 0 here and scattered in by k_adj_source.
 //   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
@@ -1323,35 +1377,46 @@ __device__ __forceinline__ void adj_coef_k(double Tk, double TW, double TE, doub
 //           float, which both the FP32 phase and the FP64 polish read; and
 //           the float Ta init, from double skew 16
 // Padding elements get zeros.
+// Offsets, in elements of one skew, from the element of an interior cell
+// (i, j) at lane l to those of (i - 1, j) and (i + 1, j); the cells
+// (i -+ 1, j + dj) are dj rows (64 dj elements) on from those.  A row of a
+// strip holds j + l (plus skew) or j - l + 63 (minus skew), and the strips
+// are R rows apart, so the neighbour in the strip before (lane 0) or after
+// (lane 63) is about a strip away.
+__device__ __forceinline__ int nb_west(int sk, int l, int R) {
+  const int s = sk == 0 ? 1 : -1;
+  return l == 0 ? -64 * R + 63 * 64 * s + 63 : -64 * s - 1;
+}
+__device__ __forceinline__ int nb_east(int sk, int l, int R) {
+  const int s = sk == 0 ? 1 : -1;
+  return l == 63 ? 64 * R - 63 * 64 * s - 63 : 64 * s + 1;
+}
+
+// One element e (strip cc, row r, lane l) of the build.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
-  const SlotDesc D = sd[blockIdx.z];
+__device__ __forceinline__ void adj_build_elem(const SlotDesc &D, int sk, size_t e, int cc, int r,
+                                               int l) {
   const Grid &g = D.g;
-  const SkewGeom &sg = D.sg;
-  const size_t SE = sg.skew_elems();
-  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const int sk = blockIdx.y;
-  if (e >= SE || *D.status < 0) return;
-  const int nx = g.nx, ny = g.ny, R = sg.R;
+  const int nx = g.nx, ny = g.ny, R = D.sg.R;
+  const size_t SE = D.sg.skew_elems();
   const double dx = g.dx, dy = g.dy;
   GMEM double *sk0 = gm(D.skew);
-  const GMEM double *Ts = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
-  const int l = (int)(e & 63);
-  const int r = (int)((e >> 6) % R);
-  const int cc = (int)((e >> 6) / R);
   const int i = 64 * cc + l;
   const int j = sk == 0 ? r - l : r + l - 63;
   const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  auto at = [&](int di, int dj) { return skpos(sk, i + di, j + dj, R); };
+  const int ow = nb_west(sk, l, R), oe = nb_east(sk, l, R);
+  // T at the cell and its eight neighbours: tc, tw, te point at (i, j),
+  // (i-1, j), (i+1, j), and row dj of each is [64*dj]
+  const GMEM double *tc = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE + e;
+  const GMEM double *tw = tc + ow, *te = tc + oe;
   if (KIND == 0) {
-    const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE;
+    const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE + e;
     const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
     double v[kAF] = {0, 0, 0, 0, 0, 0};
     if (interior) {
-      const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
-      adj_coef_v(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
-                 Ts[at(1, -1)], Ts[at(1, 1)], pa[k0], pa[kW], pa[kE], pb[k0], pb[kS], pb[kN],
-                 pc[k0], pc[kW], pc[kE], pc[kS], pc[kN], dx, dy, v);
+      adj_coef_v(tc[0], tw[0], te[0], tc[-64], tc[64], tw[-64], tw[64], te[-64], te[64], pa[0],
+                 pa[ow], pa[oe], pb[0], pb[-64], pb[64], pc[0], pc[ow], pc[oe], pc[-64], pc[64],
+                 dx, dy, v);
     }
     GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
 #pragma unroll
@@ -1359,17 +1424,14 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
     if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
   } else {
     // the float period fields a, b, c, element by element (at e*3)
-    const GMEM float *ps = gm(D.pskew32) + (size_t)sk * kPF * SE;
-    auto pa = [&](size_t q) { return ps[q * 3]; };
-    auto pb = [&](size_t q) { return ps[q * 3 + 1]; };
-    auto pc = [&](size_t q) { return ps[q * 3 + 2]; };
+    const GMEM float *ps = gm(D.pskew32) + (size_t)sk * kPF * SE + e * 3;
+    const GMEM float *pw = ps + 3 * ow, *pe = ps + 3 * oe;
     float k[5] = {0, 0, 0, 0, 0};
     if (interior) {
-      const size_t k0 = at(0, 0), kW = at(-1, 0), kE = at(1, 0), kS = at(0, -1), kN = at(0, 1);
       double w[5];
-      adj_coef_k(Ts[k0], Ts[kW], Ts[kE], Ts[kS], Ts[kN], Ts[at(-1, -1)], Ts[at(-1, 1)],
-                 Ts[at(1, -1)], Ts[at(1, 1)], pa(k0), pa(kW), pa(kE), pb(k0), pb(kS), pb(kN),
-                 pc(k0), pc(kW), pc(kE), pc(kS), pc(kN), 1.0 / dx, 1.0 / dy, w);
+      adj_coef_k(tc[0], tw[0], te[0], tc[-64], tc[64], tw[-64], tw[64], te[-64], te[64], ps[0],
+                 pw[0], pe[0], ps[1], ps[1 - 192], ps[1 + 192], ps[2], pw[2], pe[2], ps[2 - 192],
+                 ps[2 + 192], 1.0 / dx, 1.0 / dy, w);
 #pragma unroll
       for (int f = 0; f < 4; ++f) k[1 + f] = (float)w[f];
     }
@@ -1379,6 +1441,21 @@ __global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd) {
     for (int f = 0; f < 5; ++f) k32[f] = k[f];
     if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
   }
+}
+
+// The build of a launch group's items in XCD-dealt tiles.  The build of the
+// mixed mode fits 8 waves per SIMD.
+template <int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 1 ? 8 : 6)))
+void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
+  int slot;
+  unsigned t;
+  if (!tile_slot(count, tps, slot, t)) return;
+  const SlotDesc D = sd[slot];
+  if (*D.status < 0) return;
+  const int sk = (int)(t / tps);
+  tile_elems(D.sg, t % tps,
+             [&](size_t e, int cc, int r, int l) { adj_build_elem<KIND>(D, sk, e, cc, r, l); });
 }
 
 // The delta source of a launch group's items (src/eikon_solver_2d.f90:
@@ -1874,18 +1951,17 @@ void launch_grid_trig(hipStream_t s, const Grid &g, double *sy, double *cy) {
 }
 
 namespace {
-dim3 skew_grid(const GroupDims &gd) {
-  return dim3((unsigned)((gd.max_se + 255) / 256), 2, (unsigned)gd.count);
-}
 }  // namespace
 
 void launch_fwd_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, bool mixed) {
   hipLaunchKernelGGL(k_fwd_prep, dim3((unsigned)((gd.max_nxy + 63) / 64), gd.count), dim3(64), 0,
                      s, sd);
   if (mixed)
-    hipLaunchKernelGGL(k_fwd_setup<true>, skew_grid(gd), dim3(256), 0, s, sd);
+    hipLaunchKernelGGL(k_fwd_setup<true>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
+                       tiles_per_skew(gd));
   else
-    hipLaunchKernelGGL(k_fwd_setup<false>, skew_grid(gd), dim3(256), 0, s, sd);
+    hipLaunchKernelGGL(k_fwd_setup<false>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
+                       tiles_per_skew(gd));
 }
 
 namespace {
@@ -1942,9 +2018,11 @@ void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
                       bool mixed) {
   if (mixed)
-    hipLaunchKernelGGL(k_adj_build<1>, skew_grid(gd), dim3(256), 0, s, sd);
+    hipLaunchKernelGGL(k_adj_build<1>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
+                       tiles_per_skew(gd));
   else
-    hipLaunchKernelGGL(k_adj_build<0>, skew_grid(gd), dim3(256), 0, s, sd);
+    hipLaunchKernelGGL(k_adj_build<0>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
+                       tiles_per_skew(gd));
   hipLaunchKernelGGL(k_adj_source, dim3(gd.count), dim3(256), 0, s, sd, which, (int)mixed);
 }
 
