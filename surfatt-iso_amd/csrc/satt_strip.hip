@@ -1458,6 +1458,137 @@ void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
              [&](size_t e, int cc, int r, int l) { adj_build_elem<KIND>(D, sk, e, cc, r, l); });
 }
 
+// The mixed mode's build (KIND 1 of adj_build_elem, the same arithmetic)
+// with each tile staged in LDS first: T over the tile and kBuildHalo
+// elements either side, a, b, c over the tile and kBuildHaloP either side
+// (both copied with 16-byte loads, all of a thread's issued before any is
+// waited for), and the columns of the strips either side that lanes 0 and 63
+// read.  The tile's cells then read their neighbours from LDS: the loads of
+// a tile are in flight together, where a thread loading each cell's own
+// neighbours waits for them cell by cell.  Elements of the staging windows
+// outside the skew are staged as 0 (a load past the buffer's range returns
+// 0): no interior cell reads them.
+constexpr int kBuildHalo = 192, kBuildHaloP = 128;
+constexpr int kBuildT = kTileElems + 2 * kBuildHalo;          // staged T (doubles)
+constexpr int kBuildP = 3 * (kTileElems + 2 * kBuildHaloP);  // staged a, b, c (floats)
+constexpr int kTileRows = kTileElems / 64;
+__global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int count,
+                                                      unsigned tps) {
+  __shared__ double sT[kBuildT];
+  __shared__ float sP[kBuildP];
+  // T west of lane 0 and east of lane 63 for the tile's rows -1..kTileRows;
+  // a and c there for rows 0..kTileRows-1
+  __shared__ double sTw[kTileRows + 2], sTe[kTileRows + 2];
+  __shared__ float sAw[kTileRows], sCw[kTileRows], sAe[kTileRows], sCe[kTileRows];
+  int slot;
+  unsigned t;
+  if (!tile_slot(count, tps, slot, t)) return;  // the same for the whole workgroup
+  const SlotDesc D = sd[slot];
+  if (*D.status < 0) return;
+  const Grid &g = D.g;
+  const int nx = g.nx, ny = g.ny, R = D.sg.R;
+  const long SE = (long)D.sg.skew_elems();
+  const int sk = (int)(t / tps);
+  const unsigned ts = t % tps;
+  const long e0 = (long)ts * kTileElems;
+  GMEM double *sk0 = gm(D.skew);
+  const GMEM double *Tf = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
+  const GMEM float *Pf = gm(D.pskew32) + (size_t)sk * kPF * SE;  // a, b, c at 3e
+  const int tid = (int)threadIdx.x;
+  // SE is a multiple of 64 and the windows start at multiples of 64
+  // elements, so no 16-byte chunk straddles the skew's end
+  {
+    constexpr int kCT = kBuildT / 2, kCP = kBuildP / 4;
+    constexpr int nT = (kCT + 255) / 256, nP = (kCP + 255) / 256;
+    const __amdgpu_buffer_rsrc_t rT = buffer_over(Tf, (unsigned)(SE * 8));
+    const __amdgpu_buffer_rsrc_t rP = buffer_over(Pf, (unsigned)(SE * 12));
+    u32x4 vt[nT], vp[nP];
+#pragma unroll
+    for (int q = 0; q < nT; ++q) {
+      const int c = tid + 256 * q;
+      const long x = e0 - kBuildHalo + 2 * c;
+      const bool in = c < kCT && x >= 0 && x < SE;
+      vt[q] = __builtin_amdgcn_raw_buffer_load_b128(rT, in ? (int)(x * 8) : kOutOfRange, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < nP; ++q) {
+      const int c = tid + 256 * q;
+      const long x = 3 * (e0 - kBuildHaloP) + 4 * c;
+      const bool in = c < kCP && x >= 0 && x < 3 * SE;
+      vp[q] = __builtin_amdgcn_raw_buffer_load_b128(rP, in ? (int)(x * 4) : kOutOfRange, 0, 0);
+    }
+    const int ow0 = nb_west(sk, 0, R), oe63 = nb_east(sk, 63, R);
+    if (tid < kTileRows + 2) {
+      const long x = e0 + (long)(tid - 1) * 64 + ow0;
+      sTw[tid] = x >= 0 && x < SE ? Tf[x] : 0.0;
+    } else if (tid >= 64 && tid < 64 + kTileRows + 2) {
+      const long x = e0 + (long)(tid - 65) * 64 + 63 + oe63;
+      sTe[tid - 64] = x >= 0 && x < SE ? Tf[x] : 0.0;
+    } else if (tid >= 128 && tid < 128 + kTileRows) {
+      const long x = e0 + (long)(tid - 128) * 64 + ow0;
+      const bool in = x >= 0 && x < SE;
+      sAw[tid - 128] = in ? Pf[3 * x] : 0.0f;
+      sCw[tid - 128] = in ? Pf[3 * x + 2] : 0.0f;
+    } else if (tid >= 192 && tid < 192 + kTileRows) {
+      const long x = e0 + (long)(tid - 192) * 64 + 63 + oe63;
+      const bool in = x >= 0 && x < SE;
+      sAe[tid - 192] = in ? Pf[3 * x] : 0.0f;
+      sCe[tid - 192] = in ? Pf[3 * x + 2] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < nT; ++q) {
+      const int c = tid + 256 * q;
+      if (c < kCT) *(u32x4 *)&sT[2 * c] = vt[q];
+    }
+#pragma unroll
+    for (int q = 0; q < nP; ++q) {
+      const int c = tid + 256 * q;
+      if (c < kCP) *(u32x4 *)&sP[4 * c] = vp[q];
+    }
+  }
+  __syncthreads();
+  const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
+  GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
+  const int l = tid & 63;
+  const int ow = nb_west(sk, l, R), oe = nb_east(sk, l, R);
+  const unsigned row0 = ts * (unsigned)kTileRows;
+  const int cc0 = (int)(row0 / (unsigned)R), r0 = (int)(row0 % (unsigned)R);
+  // one cell at a time: its operands come from LDS, and registers held for
+  // several would cost the workgroup its third slot on a CU
+#pragma unroll 1
+  for (int q = 0; q < kTileElems / 256; ++q) {
+    const int ro = 4 * q + (tid >> 6);  // row within the tile
+    int r = r0 + ro, cc = cc0;
+    if (r >= R) {
+      r -= R;
+      ++cc;
+    }
+    const long e = e0 + ro * 64 + l;
+    if (e >= SE) break;
+    const int i = 64 * cc + l;
+    const int j = sk == 0 ? r - l : r + l - 63;
+    const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+    float k[5] = {0, 0, 0, 0, 0};
+    if (interior) {
+      const int xt = kBuildHalo + ro * 64 + l, xp = 3 * (kBuildHaloP + ro * 64 + l);
+      auto tw = [&](int dj) { return l == 0 ? sTw[ro + 1 + dj] : sT[xt + ow + 64 * dj]; };
+      auto te = [&](int dj) { return l == 63 ? sTe[ro + 1 + dj] : sT[xt + oe + 64 * dj]; };
+      const float aW = l == 0 ? sAw[ro] : sP[xp + 3 * ow], cW = l == 0 ? sCw[ro] : sP[xp + 3 * ow + 2];
+      const float aE = l == 63 ? sAe[ro] : sP[xp + 3 * oe], cE = l == 63 ? sCe[ro] : sP[xp + 3 * oe + 2];
+      double w[5];
+      adj_coef_k(sT[xt], tw(0), te(0), sT[xt - 64], sT[xt + 64], tw(-1), tw(1), te(-1), te(1),
+                 sP[xp], aW, aE, sP[xp + 1], sP[xp + 1 - 192], sP[xp + 1 + 192], sP[xp + 2], cW,
+                 cE, sP[xp + 2 - 192], sP[xp + 2 + 192], rdx, rdy, w);
+#pragma unroll
+      for (int f = 0; f < 4; ++f) k[1 + f] = (float)w[f];
+    }
+    GMEM float *k32 = f32 + ((size_t)sk * SE + e) * 5;  // element by element
+#pragma unroll
+    for (int f = 0; f < 5; ++f) k32[f] = k[f];
+    if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
+  }
+}
+
 // The delta source of a launch group's items (src/eikon_solver_2d.f90:
 // 1320-1336) into their coefficient fields, one workgroup per item: each
 // thread forms the terms of receiver corners (entry 4r + c, in the
@@ -2018,7 +2149,7 @@ void launch_fwd_polish(hipStream_t s, const SlotDesc *sd, const GroupDims &gd) {
 void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, int which,
                       bool mixed) {
   if (mixed)
-    hipLaunchKernelGGL(k_adj_build<1>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
+    hipLaunchKernelGGL(k_adj_build32, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
                        tiles_per_skew(gd));
   else
     hipLaunchKernelGGL(k_adj_build<0>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
