@@ -1365,18 +1365,6 @@ __device__ __forceinline__ void adj_coef_k(double Tk, double TW, double TE, doub
   out[4] = rd;
 }
 
-// Coefficients of every item of a launch group, per skew element;
-// the neighbours come from the same skew (src/eikon_solver_2d.f90:1348-1370).
-// The source field is left 0 here and scattered in by k_adj_source.
-//   KIND 0: FP64 a1, a2, b1, b2, d, delta (0..11) and Ta = 100 inside, 0 on
-//           the boundary (:1368-1380), in the plus skew
-//   KIND 1 (mixed mode): the update in the form Ta = k0 + kW*TaW + kE*TaE
-//           + kS*TaS + kN*TaN, its coefficients k0 (0 here), kW = a1p/dx/d,
-//           kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d computed in FP64
-//           (adj_coef_k, on the metric as stored in float) and stored in
-//           float, which both the FP32 phase and the FP64 polish read; and
-//           the float Ta init, from double skew 16
-// Padding elements get zeros.
 // Offsets, in elements of one skew, from the element of an interior cell
 // (i, j) at lane l to those of (i - 1, j) and (i + 1, j); the cells
 // (i -+ 1, j + dj) are dj rows (64 dj elements) on from those.  A row of a
@@ -1392,14 +1380,18 @@ __device__ __forceinline__ int nb_east(int sk, int l, int R) {
   return l == 63 ? 64 * R - 63 * 64 * s - 63 : 64 * s + 1;
 }
 
-// One element e (strip cc, row r, lane l) of the build.
-template <int KIND>
+// The FP64 mode's adjoint coefficients of one element e (strip cc, row r,
+// lane l) of one skew; the neighbours come from the same skew
+// (src/eikon_solver_2d.f90:1348-1370): a1, a2, b1, b2, d, delta in skew
+// fields 0..5 (plus skew) and 6..11 (minus skew), and Ta = 100 inside, 0 on
+// the boundary (:1368-1380), in the plus skew.  delta is left 0 here and
+// written where the receivers touch it by k_adj_source.  Padding elements
+// get zeros.
 __device__ __forceinline__ void adj_build_elem(const SlotDesc &D, int sk, size_t e, int cc, int r,
                                                int l) {
   const Grid &g = D.g;
   const int nx = g.nx, ny = g.ny, R = D.sg.R;
   const size_t SE = D.sg.skew_elems();
-  const double dx = g.dx, dy = g.dy;
   GMEM double *sk0 = gm(D.skew);
   const int i = 64 * cc + l;
   const int j = sk == 0 ? r - l : r + l - 63;
@@ -1409,45 +1401,22 @@ __device__ __forceinline__ void adj_build_elem(const SlotDesc &D, int sk, size_t
   // (i-1, j), (i+1, j), and row dj of each is [64*dj]
   const GMEM double *tc = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE + e;
   const GMEM double *tw = tc + ow, *te = tc + oe;
-  if (KIND == 0) {
-    const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE + e;
-    const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
-    double v[kAF] = {0, 0, 0, 0, 0, 0};
-    if (interior) {
-      adj_coef_v(tc[0], tw[0], te[0], tc[-64], tc[64], tw[-64], tw[64], te[-64], te[64], pa[0],
-                 pa[ow], pa[oe], pb[0], pb[-64], pb[64], pc[0], pc[ow], pc[oe], pc[-64], pc[64],
-                 dx, dy, v);
-    }
-    GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
-#pragma unroll
-    for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
-    if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
-  } else {
-    // the float period fields a, b, c, element by element (at e*3)
-    const GMEM float *ps = gm(D.pskew32) + (size_t)sk * kPF * SE + e * 3;
-    const GMEM float *pw = ps + 3 * ow, *pe = ps + 3 * oe;
-    float k[5] = {0, 0, 0, 0, 0};
-    if (interior) {
-      double w[5];
-      adj_coef_k(tc[0], tw[0], te[0], tc[-64], tc[64], tw[-64], tw[64], te[-64], te[64], ps[0],
-                 pw[0], pe[0], ps[1], ps[1 - 192], ps[1 + 192], ps[2], pw[2], pe[2], ps[2 - 192],
-                 ps[2 + 192], 1.0 / dx, 1.0 / dy, w);
-#pragma unroll
-      for (int f = 0; f < 4; ++f) k[1 + f] = (float)w[f];
-    }
-    GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
-    GMEM float *k32 = f32 + ((size_t)sk * SE + e) * 5;  // element by element
-#pragma unroll
-    for (int f = 0; f < 5; ++f) k32[f] = k[f];
-    if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
+  const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE + e;
+  const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
+  double v[kAF] = {0, 0, 0, 0, 0, 0};
+  if (interior) {
+    adj_coef_v(tc[0], tw[0], te[0], tc[-64], tc[64], tw[-64], tw[64], te[-64], te[64], pa[0],
+               pa[ow], pa[oe], pb[0], pb[-64], pb[64], pc[0], pc[ow], pc[oe], pc[-64], pc[64], g.dx,
+               g.dy, v);
   }
+  GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
+#pragma unroll
+  for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
+  if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
 }
 
-// The build of a launch group's items in XCD-dealt tiles.  The build of the
-// mixed mode fits 8 waves per SIMD.
-template <int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 1 ? 8 : 6)))
-void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
+// The FP64 mode's build of a launch group's items in XCD-dealt tiles.
+__global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
   int slot;
   unsigned t;
   if (!tile_slot(count, tps, slot, t)) return;
@@ -1455,19 +1424,24 @@ void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
   if (*D.status < 0) return;
   const int sk = (int)(t / tps);
   tile_elems(D.sg, t % tps,
-             [&](size_t e, int cc, int r, int l) { adj_build_elem<KIND>(D, sk, e, cc, r, l); });
+             [&](size_t e, int cc, int r, int l) { adj_build_elem(D, sk, e, cc, r, l); });
 }
 
-// The mixed mode's build (KIND 1 of adj_build_elem, the same arithmetic)
-// with each tile staged in LDS first: T over the tile and kBuildHalo
-// elements either side, a, b, c over the tile and kBuildHaloP either side
-// (both copied with 16-byte loads, all of a thread's issued before any is
-// waited for), and the columns of the strips either side that lanes 0 and 63
-// read.  The tile's cells then read their neighbours from LDS: the loads of
-// a tile are in flight together, where a thread loading each cell's own
-// neighbours waits for them cell by cell.  Elements of the staging windows
-// outside the skew are staged as 0 (a load past the buffer's range returns
-// 0): no interior cell reads them.
+// The mixed mode's build: the adjoint update in the form Ta = k0 + kW*TaW +
+// kE*TaE + kS*TaS + kN*TaN, its coefficients kW = a1p/dx/d, kE = -a2m/dx/d,
+// kS = b1p/dy/d, kN = -b2m/dy/d computed in FP64 (adj_coef_k, on the metric as
+// stored in float) and stored in float, element by element, which both the
+// FP32 phase and the FP64 polish read; k0 is left 0 here and written where the
+// receivers touch it by k_adj_source; and the float Ta init (100 inside, 0 on
+// the boundary, plus skew).  Each tile is staged in LDS first: T over the tile
+// and kBuildHalo elements either side, a, b, c over the tile and kBuildHaloP
+// either side (both copied with 16-byte loads, all of a thread's issued before
+// any is waited for), and the columns of the strips either side that lanes 0
+// and 63 read.  The tile's cells then read their neighbours from LDS: the
+// loads of a tile are in flight together, where a thread loading each cell's
+// own neighbours waits for them cell by cell.  Elements of the staging windows
+// outside the skew are staged as 0 (a load past the buffer's range returns 0):
+// no interior cell reads them.
 constexpr int kBuildHalo = 192, kBuildHaloP = 128;
 constexpr int kBuildT = kTileElems + 2 * kBuildHalo;          // staged T (doubles)
 constexpr int kBuildP = 3 * (kTileElems + 2 * kBuildHaloP);  // staged a, b, c (floats)
@@ -1596,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int cou
 // cell's terms in entry order, from 0 as the reference's zeroed field does,
 // and writes the cell's source, interior cells only: delta itself in both
 // skews (FP64 mode), or k0 = delta/d in the float k set with 1/d as
-// k_adj_build<1> computes it (mixed).  Only the touched cells are read
+// k_adj_build32 computes it (mixed).  Only the touched cells are read
 // later, so no field is cleared first.  An item with more receivers than
 // the workgroup's entries hold is done by one thread in receiver order.
 constexpr int kSrcEntries = 2048;
@@ -2037,7 +2011,7 @@ __global__ __launch_bounds__(TEAM ? 64 * kMaxTeam : 64) void k_adj_polish(const 
   Team T = team_init<TEAM>(lds, g.ny, sg.ns, 8);
   double *sk0 = D.skew;
   double *TaP = sk0 + kSkTaP * SE, *TaM = sk0 + kSkTaM * SE;
-  // the float k set of the FP32 phase (k_adj_build<1>, k_adj_source)
+  // the float k set of the FP32 phase (k_adj_build32, k_adj_source)
   const float *f32 = reinterpret_cast<const float *>(sk0 + (size_t)kSkA32 * SE);
   AdjCtx<double, float> C;
   adj_ctx_init(C, g, sg, lane);
@@ -2152,7 +2126,7 @@ void launch_adj_setup(hipStream_t s, const SlotDesc *sd, const GroupDims &gd, in
     hipLaunchKernelGGL(k_adj_build32, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
                        tiles_per_skew(gd));
   else
-    hipLaunchKernelGGL(k_adj_build<0>, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
+    hipLaunchKernelGGL(k_adj_build, tile_grid(gd), dim3(256), 0, s, sd, gd.count,
                        tiles_per_skew(gd));
   hipLaunchKernelGGL(k_adj_source, dim3(gd.count), dim3(256), 0, s, sd, which, (int)mixed);
 }
