@@ -1380,80 +1380,41 @@ __device__ __forceinline__ int nb_east(int sk, int l, int R) {
   return l == 63 ? 64 * R - 63 * 64 * s - 63 : 64 * s + 1;
 }
 
-// The FP64 mode's adjoint coefficients of one element e (strip cc, row r,
-// lane l) of one skew; the neighbours come from the same skew
-// (src/eikon_solver_2d.f90:1348-1370): a1, a2, b1, b2, d, delta in skew
-// fields 0..5 (plus skew) and 6..11 (minus skew), and Ta = 100 inside, 0 on
-// the boundary (:1368-1380), in the plus skew.  delta is left 0 here and
-// written where the receivers touch it by k_adj_source.  Padding elements
-// get zeros.
-__device__ __forceinline__ void adj_build_elem(const SlotDesc &D, int sk, size_t e, int cc, int r,
-                                               int l) {
-  const Grid &g = D.g;
-  const int nx = g.nx, ny = g.ny, R = D.sg.R;
-  const size_t SE = D.sg.skew_elems();
-  GMEM double *sk0 = gm(D.skew);
-  const int i = 64 * cc + l;
-  const int j = sk == 0 ? r - l : r + l - 63;
-  const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  const int ow = nb_west(sk, l, R), oe = nb_east(sk, l, R);
-  // T at the cell and its eight neighbours: tc, tw, te point at (i, j),
-  // (i-1, j), (i+1, j), and row dj of each is [64*dj]
-  const GMEM double *tc = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE + e;
-  const GMEM double *tw = tc + ow, *te = tc + oe;
-  const GMEM double *pa = gm(D.pskew) + (size_t)sk * kPF * SE + e;
-  const GMEM double *pb = pa + SE, *pc = pa + 2 * SE;
-  double v[kAF] = {0, 0, 0, 0, 0, 0};
-  if (interior) {
-    adj_coef_v(tc[0], tw[0], te[0], tc[-64], tc[64], tw[-64], tw[64], te[-64], te[64], pa[0],
-               pa[ow], pa[oe], pb[0], pb[-64], pb[64], pc[0], pc[ow], pc[oe], pc[-64], pc[64], g.dx,
-               g.dy, v);
-  }
-  GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
-#pragma unroll
-  for (int f = 0; f < kAF; ++f) b0[f * SE + e] = v[f];
-  if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
-}
-
-// The FP64 mode's build of a launch group's items in XCD-dealt tiles.
-__global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
-  int slot;
-  unsigned t;
-  if (!tile_slot(count, tps, slot, t)) return;
-  const SlotDesc D = sd[slot];
-  if (*D.status < 0) return;
-  const int sk = (int)(t / tps);
-  tile_elems(D.sg, t % tps,
-             [&](size_t e, int cc, int r, int l) { adj_build_elem(D, sk, e, cc, r, l); });
-}
-
-// The mixed mode's build: the adjoint update in the form Ta = k0 + kW*TaW +
-// kE*TaE + kS*TaS + kN*TaN, its coefficients kW = a1p/dx/d, kE = -a2m/dx/d,
-// kS = b1p/dy/d, kN = -b2m/dy/d computed in FP64 (adj_coef_k, on the metric as
-// stored in float) and stored in float, element by element, which both the
-// FP32 phase and the FP64 polish read; k0 is left 0 here and written where the
-// receivers touch it by k_adj_source; and the float Ta init (100 inside, 0 on
-// the boundary, plus skew).  Each tile is staged in LDS first: T over the tile
-// and kBuildHalo elements either side, a, b, c over the tile and kBuildHaloP
-// either side (both copied with 16-byte loads, all of a thread's issued before
-// any is waited for), and the columns of the strips either side that lanes 0
-// and 63 read.  The tile's cells then read their neighbours from LDS: the
-// loads of a tile are in flight together, where a thread loading each cell's
-// own neighbours waits for them cell by cell.  Elements of the staging windows
-// outside the skew are staged as 0 (a load past the buffer's range returns 0):
-// no interior cell reads them.
+// The adjoint coefficients of a launch group's items, in XCD-dealt tiles
+// (src/eikon_solver_2d.f90:1348-1370), a cell's neighbours from its own skew.
+// FP64 mode (k_adj_build): a1, a2, b1, b2, d, delta in skew fields 0..5 (plus
+// skew) and 6..11 (minus skew), with delta left 0 for k_adj_source to write
+// where the receivers touch it, and Ta = 100 inside, 0 on the boundary
+// (:1368-1380), in the plus skew. Mixed mode (k_adj_build32): the update in
+// the form Ta = k0 + kW*TaW + kE*TaE + kS*TaS + kN*TaN, its coefficients kW =
+// a1p/dx/d, kE = -a2m/dx/d, kS = b1p/dy/d, kN = -b2m/dy/d computed in FP64
+// (adj_coef_k, on the metric as stored in float) and stored in float, element
+// by element, which both the FP32 phase and the FP64 polish read; k0 left 0
+// for k_adj_source; and the float Ta init. Padding elements get zeros. Each
+// tile is staged in LDS first: T over the tile and kBuildHalo elements either
+// side, a, b, c over the tile and kBuildHaloP either side (both copied with
+// 16-byte loads, all of a thread's issued before any is waited for), and the
+// columns of the strips either side that lanes 0 and 63 read. The tile's cells
+// then read their neighbours from LDS: the loads of a tile are in flight
+// together, where a thread loading each cell's own neighbours waits for them
+// cell by cell. Elements of the staging windows outside the skew are staged as
+// 0 (a load past the buffer's range returns 0): no interior cell reads them.
 constexpr int kBuildHalo = 192, kBuildHaloP = 128;
-constexpr int kBuildT = kTileElems + 2 * kBuildHalo;          // staged T (doubles)
-constexpr int kBuildP = 3 * (kTileElems + 2 * kBuildHaloP);  // staged a, b, c (floats)
+constexpr int kBuildT = kTileElems + 2 * kBuildHalo;   // staged T (doubles)
+constexpr int kBuildPE = kTileElems + 2 * kBuildHaloP;  // staged metric elements
 constexpr int kTileRows = kTileElems / 64;
-__global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int count,
-                                                      unsigned tps) {
+// MIXED: the float k set from the float metric (a, b, c element by element);
+// else the FP64 mode's coefficients from the FP64 metric (field by field).
+template <bool MIXED>
+__device__ __forceinline__ void adj_build_staged(const SlotDesc *sd, int count, unsigned tps) {
+  using PT = typename std::conditional<MIXED, float, double>::type;
   __shared__ double sT[kBuildT];
-  __shared__ float sP[kBuildP];
+  // a, b, c: element by element (MIXED) or field by field
+  __shared__ PT sP[3 * kBuildPE];
   // T west of lane 0 and east of lane 63 for the tile's rows -1..kTileRows;
   // a and c there for rows 0..kTileRows-1
   __shared__ double sTw[kTileRows + 2], sTe[kTileRows + 2];
-  __shared__ float sAw[kTileRows], sCw[kTileRows], sAe[kTileRows], sCe[kTileRows];
+  __shared__ PT sAw[kTileRows], sCw[kTileRows], sAe[kTileRows], sCe[kTileRows];
   int slot;
   unsigned t;
   if (!tile_slot(count, tps, slot, t)) return;  // the same for the whole workgroup
@@ -1467,15 +1428,26 @@ __global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int cou
   const long e0 = (long)ts * kTileElems;
   GMEM double *sk0 = gm(D.skew);
   const GMEM double *Tf = sk0 + (sk == 0 ? kSkTP : kSkTM) * SE;
-  const GMEM float *Pf = gm(D.pskew32) + (size_t)sk * kPF * SE;  // a, b, c at 3e
+  // the metric of the skew: MIXED a, b, c at 3e (floats), else a at e,
+  // b at SE + e, c at 2 SE + e (doubles)
+  const GMEM PT *Pf;
+  if constexpr (MIXED)
+    Pf = gm(D.pskew32) + (size_t)sk * kPF * SE;
+  else
+    Pf = gm(D.pskew) + (size_t)sk * kPF * SE;
+  // position of field f of element x in the staged metric and in memory
+  auto sp = [](int f, long x) { return MIXED ? 3 * x + f : f * (long)kBuildPE + x; };
+  auto gp = [&](int f, long x) { return MIXED ? 3 * x + f : f * SE + x; };
   const int tid = (int)threadIdx.x;
   // SE is a multiple of 64 and the windows start at multiples of 64
   // elements, so no 16-byte chunk straddles the skew's end
   {
-    constexpr int kCT = kBuildT / 2, kCP = kBuildP / 4;
+    constexpr int kCT = kBuildT / 2;                            // 16-byte chunks of T
+    constexpr int kCF = kBuildPE * (int)sizeof(PT) / 16;        // of one metric field
+    constexpr int kCP = MIXED ? 3 * kBuildPE / 4 : 3 * kCF;     // of the metric
     constexpr int nT = (kCT + 255) / 256, nP = (kCP + 255) / 256;
     const __amdgpu_buffer_rsrc_t rT = buffer_over(Tf, (unsigned)(SE * 8));
-    const __amdgpu_buffer_rsrc_t rP = buffer_over(Pf, (unsigned)(SE * 12));
+    const __amdgpu_buffer_rsrc_t rP = buffer_over(Pf, (unsigned)(SE * 3 * sizeof(PT)));
     u32x4 vt[nT], vp[nP];
 #pragma unroll
     for (int q = 0; q < nT; ++q) {
@@ -1487,9 +1459,19 @@ __global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int cou
 #pragma unroll
     for (int q = 0; q < nP; ++q) {
       const int c = tid + 256 * q;
-      const long x = 3 * (e0 - kBuildHaloP) + 4 * c;
-      const bool in = c < kCP && x >= 0 && x < 3 * SE;
-      vp[q] = __builtin_amdgcn_raw_buffer_load_b128(rP, in ? (int)(x * 4) : kOutOfRange, 0, 0);
+      bool in;
+      long off;  // byte offset
+      if constexpr (MIXED) {
+        const long x = 3 * (e0 - kBuildHaloP) + 4 * c;
+        in = c < kCP && x >= 0 && x < 3 * SE;
+        off = x * 4;
+      } else {
+        const int f = c / kCF;
+        const long x = e0 - kBuildHaloP + 2 * (c % kCF);
+        in = c < kCP && x >= 0 && x < SE;
+        off = (f * SE + x) * 8;
+      }
+      vp[q] = __builtin_amdgcn_raw_buffer_load_b128(rP, in ? (int)off : kOutOfRange, 0, 0);
     }
     const int ow0 = nb_west(sk, 0, R), oe63 = nb_east(sk, 63, R);
     if (tid < kTileRows + 2) {
@@ -1501,13 +1483,13 @@ __global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int cou
     } else if (tid >= 128 && tid < 128 + kTileRows) {
       const long x = e0 + (long)(tid - 128) * 64 + ow0;
       const bool in = x >= 0 && x < SE;
-      sAw[tid - 128] = in ? Pf[3 * x] : 0.0f;
-      sCw[tid - 128] = in ? Pf[3 * x + 2] : 0.0f;
+      sAw[tid - 128] = in ? Pf[gp(0, x)] : PT(0);
+      sCw[tid - 128] = in ? Pf[gp(2, x)] : PT(0);
     } else if (tid >= 192 && tid < 192 + kTileRows) {
       const long x = e0 + (long)(tid - 192) * 64 + 63 + oe63;
       const bool in = x >= 0 && x < SE;
-      sAe[tid - 192] = in ? Pf[3 * x] : 0.0f;
-      sCe[tid - 192] = in ? Pf[3 * x + 2] : 0.0f;
+      sAe[tid - 192] = in ? Pf[gp(0, x)] : PT(0);
+      sCe[tid - 192] = in ? Pf[gp(2, x)] : PT(0);
     }
 #pragma unroll
     for (int q = 0; q < nT; ++q) {
@@ -1517,18 +1499,16 @@ __global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int cou
 #pragma unroll
     for (int q = 0; q < nP; ++q) {
       const int c = tid + 256 * q;
-      if (c < kCP) *(u32x4 *)&sP[4 * c] = vp[q];
+      if (c < kCP) *(u32x4 *)((char *)sP + 16 * c) = vp[q];
     }
   }
   __syncthreads();
-  const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
-  GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
   const int l = tid & 63;
   const int ow = nb_west(sk, l, R), oe = nb_east(sk, l, R);
   const unsigned row0 = ts * (unsigned)kTileRows;
   const int cc0 = (int)(row0 / (unsigned)R), r0 = (int)(row0 % (unsigned)R);
   // one cell at a time: its operands come from LDS, and registers held for
-  // several would cost the workgroup its third slot on a CU
+  // several would cost the workgroup a slot on a CU
 #pragma unroll 1
   for (int q = 0; q < kTileElems / 256; ++q) {
     const int ro = 4 * q + (tid >> 6);  // row within the tile
@@ -1542,25 +1522,45 @@ __global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int cou
     const int i = 64 * cc + l;
     const int j = sk == 0 ? r - l : r + l - 63;
     const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-    float k[5] = {0, 0, 0, 0, 0};
+    double w[kAF] = {0, 0, 0, 0, 0, 0};
     if (interior) {
-      const int xt = kBuildHalo + ro * 64 + l, xp = 3 * (kBuildHaloP + ro * 64 + l);
+      const int xt = kBuildHalo + ro * 64 + l, xp = kBuildHaloP + ro * 64 + l;
       auto tw = [&](int dj) { return l == 0 ? sTw[ro + 1 + dj] : sT[xt + ow + 64 * dj]; };
       auto te = [&](int dj) { return l == 63 ? sTe[ro + 1 + dj] : sT[xt + oe + 64 * dj]; };
-      const float aW = l == 0 ? sAw[ro] : sP[xp + 3 * ow], cW = l == 0 ? sCw[ro] : sP[xp + 3 * ow + 2];
-      const float aE = l == 63 ? sAe[ro] : sP[xp + 3 * oe], cE = l == 63 ? sCe[ro] : sP[xp + 3 * oe + 2];
-      double w[5];
-      adj_coef_k(sT[xt], tw(0), te(0), sT[xt - 64], sT[xt + 64], tw(-1), tw(1), te(-1), te(1),
-                 sP[xp], aW, aE, sP[xp + 1], sP[xp + 1 - 192], sP[xp + 1 + 192], sP[xp + 2], cW,
-                 cE, sP[xp + 2 - 192], sP[xp + 2 + 192], rdx, rdy, w);
-#pragma unroll
-      for (int f = 0; f < 4; ++f) k[1 + f] = (float)w[f];
+      auto P = [&](int f, int x) { return sP[sp(f, x)]; };
+      const PT aW = l == 0 ? sAw[ro] : P(0, xp + ow), cW = l == 0 ? sCw[ro] : P(2, xp + ow);
+      const PT aE = l == 63 ? sAe[ro] : P(0, xp + oe), cE = l == 63 ? sCe[ro] : P(2, xp + oe);
+      if constexpr (MIXED)
+        adj_coef_k(sT[xt], tw(0), te(0), sT[xt - 64], sT[xt + 64], tw(-1), tw(1), te(-1), te(1),
+                   P(0, xp), aW, aE, P(1, xp), P(1, xp - 64), P(1, xp + 64), P(2, xp), cW, cE,
+                   P(2, xp - 64), P(2, xp + 64), 1.0 / g.dx, 1.0 / g.dy, w);
+      else
+        adj_coef_v(sT[xt], tw(0), te(0), sT[xt - 64], sT[xt + 64], tw(-1), tw(1), te(-1), te(1),
+                   P(0, xp), aW, aE, P(1, xp), P(1, xp - 64), P(1, xp + 64), P(2, xp), cW, cE,
+                   P(2, xp - 64), P(2, xp + 64), g.dx, g.dy, w);
     }
-    GMEM float *k32 = f32 + ((size_t)sk * SE + e) * 5;  // element by element
+    if constexpr (MIXED) {
+      GMEM float *f32 = (GMEM float *)(sk0 + (size_t)kSkA32 * SE);
+      GMEM float *k32 = f32 + ((size_t)sk * SE + e) * 5;  // element by element
+      k32[0] = 0.0f;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) k32[f] = k[f];
-    if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
+      for (int f = 0; f < 4; ++f) k32[1 + f] = interior ? (float)w[f] : 0.0f;
+      if (sk == 0) f32[10 * SE + e] = interior ? 100.0f : 0.0f;
+    } else {
+      GMEM double *b0 = sk0 + (size_t)sk * kAF * SE;
+#pragma unroll
+      for (int f = 0; f < kAF; ++f) b0[f * SE + e] = w[f];
+      if (sk == 0) sk0[kSkTaP * SE + e] = interior ? 100.0 : 0.0;
+    }
   }
+}
+
+__global__ __launch_bounds__(256) void k_adj_build32(const SlotDesc *sd, int count,
+                                                      unsigned tps) {
+  adj_build_staged<true>(sd, count, tps);
+}
+__global__ __launch_bounds__(256) void k_adj_build(const SlotDesc *sd, int count, unsigned tps) {
+  adj_build_staged<false>(sd, count, tps);
 }
 
 // The delta source of a launch group's items (src/eikon_solver_2d.f90:
