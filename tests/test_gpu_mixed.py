@@ -23,8 +23,8 @@ TOL_T_S = 1e-3      # travel-time error bar of the mixed mode: 1 ms
 TOL_KERNEL = 1e-3   # kernel relative L2 bar of the mixed mode
 # what the FP32 phase (tau offsets, residual-sized quadratic) plus the FP64
 # polish delivers on these grids; a regression guard well inside the bar
-DESIGN_T_S = 1e-4
-DESIGN_KERNEL = 1e-4
+GUARD_T_S = 1e-4
+GUARD_KERNEL = 1e-4
 
 
 def check_mixed(out, ref, dim):
@@ -32,9 +32,9 @@ def check_mixed(out, ref, dim):
     err_k = rel_l2(out["adj_s"], ref["adj_s"])
     print(f"mixed: max travel-time error {err_t * 1e3:.3g} ms, kernel rel L2 {err_k:.3g}")
     assert err_t <= TOL_T_S and err_k <= TOL_KERNEL
-    assert err_t <= DESIGN_T_S and err_k <= DESIGN_KERNEL
+    assert err_t <= GUARD_T_S and err_k <= GUARD_KERNEL
     if dim == 3:
-        assert rel_l2(out["adj_density"], ref["adj_density"]) <= DESIGN_KERNEL
+        assert rel_l2(out["adj_density"], ref["adj_density"]) <= GUARD_KERNEL
     assert abs(out["chi"] - ref["chi"]) <= 1e-4 * abs(ref["chi"])
     assert np.all(out["item_status"] == 0)
     assert np.all(out["iters_fp32"][:, 0] >= 1) and np.all(out["iters"][:, 0] >= 1)
